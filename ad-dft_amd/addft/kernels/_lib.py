@@ -1,0 +1,98 @@
+"""ctypes binding of libaddft_hip.so (the C ABI declared in include/addft_hip.h).
+
+The library is looked up at $ADDFT_NATIVE_LIB or next to this package
+(addft/_lib/libaddft_hip.so), mirroring the reference loader
+(src/addft/kernels/pack.py:51-55).  Unlike the reference there is no silent
+fallback: if the library is missing every native entry point raises
+NativeUnavailable, so a GPU run can never pass on a substitute path.
+Tensors are torch device tensors; pointers and the current HIP stream are
+passed through as plain integers.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+_LIB = None
+_PATH = None
+
+C64, C128 = 0, 1
+
+
+class NativeUnavailable(RuntimeError):
+    pass
+
+
+class AddftError(RuntimeError):
+    pass
+
+
+def lib_path() -> Path:
+    env = os.environ.get("ADDFT_NATIVE_LIB")
+    if env:
+        return Path(env)
+    return Path(__file__).resolve().parents[1] / "_lib" / "libaddft_hip.so"
+
+
+_VP, _I64, _I32, _D = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_double
+
+_SIGS = {
+    "addft_last_error": ([], ctypes.c_char_p),
+    "addft_version": ([], _I32),
+    "addft_pack_scatter": ([_I32, _VP, _VP, _I64, _I64, _I64, _VP, _VP], _I32),
+    "addft_pack_gather": ([_I32, _VP, _VP, _I64, _I64, _I64, _VP, _VP], _I32),
+    "addft_xc_work_bytes": ([_I64], _I64),
+    "addft_xc_lda": ([_VP, _I64, _VP, _VP, _VP, _VP], _I32),
+    "addft_xc_lda_fxc": ([_VP, _VP, _I64, _VP, _VP], _I32),
+    "addft_sphere_plan_create": ([_I32, _I32, _I32, _I64, _VP, _I32, ctypes.POINTER(_VP)], _I32),
+    "addft_sphere_plan_destroy": ([_VP], _I32),
+    "addft_sphere_plan_info": ([_VP, ctypes.POINTER(_I64), ctypes.POINTER(_I64), _VP], _I32),
+    "addft_sphere_work_bytes": ([_VP, _I64], _I64),
+    "addft_sphere_permute": ([_VP, _I32, _I32, _VP, _I64, _VP, _VP], _I32),
+    "addft_grid_transpose": ([_I32, _I32, _I32, _I32, _VP, _VP, _VP], _I32),
+    "addft_grid_stage": ([_VP, _VP, _D, _VP, _VP], _I32),
+    "addft_hpsi_local": ([_VP, _VP, _VP, _VP, _I64, _VP, _VP, _VP], _I32),
+    "addft_density_accumulate": ([_VP, _VP, _VP, _I64, _VP, _I32, _VP, _VP], _I32),
+    "addft_sphere_to_real": ([_VP, _VP, _D, _VP, _VP, _VP], _I32),
+    "addft_real_to_sphere": ([_VP, _VP, _D, _VP, _VP, _VP], _I32),
+}
+
+
+def exported_symbols():
+    return tuple(_SIGS)
+
+
+def lib():
+    """Load (once) and return the library, or raise NativeUnavailable."""
+    global _LIB, _PATH
+    if _LIB is not None:
+        return _LIB
+    path = lib_path()
+    if not path.exists():
+        raise NativeUnavailable(
+            f"libaddft_hip.so not found at {path}; build it with "
+            "`python -c 'import __graft_entry__ as g; g.build()'` or set ADDFT_NATIVE_LIB")
+    handle = ctypes.CDLL(str(path))
+    for name, (args, res) in _SIGS.items():
+        fn = getattr(handle, name)
+        fn.argtypes = args
+        fn.restype = res
+    _LIB, _PATH = handle, path
+    return handle
+
+
+def check(rc: int, what: str = "") -> None:
+    if rc != 0:
+        msg = lib().addft_last_error().decode(errors="replace")
+        raise AddftError(f"{what} failed (code {rc}): {msg}")
+
+
+def stream() -> int:
+    import torch
+    return torch.cuda.current_stream().cuda_stream
+
+
+def ptr(t) -> int:
+    return 0 if t is None else int(t.data_ptr())

@@ -1,0 +1,215 @@
+"""Tensor-level wrappers of the HIP entry points.
+
+Every function takes and returns torch tensors on the current CUDA (HIP)
+device, checks shapes/dtypes on the host, and launches on torch's current
+stream.  Nothing here computes on the host.
+"""
+
+from __future__ import annotations
+
+import weakref
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import C64, C128, check, lib, ptr, stream
+
+_CDT = {torch.complex64: C64, torch.complex128: C128}
+
+
+def _dtype_code(t):
+    try:
+        return _CDT[t.dtype]
+    except KeyError:
+        raise TypeError(f"expected complex64/complex128, got {t.dtype}") from None
+
+
+def _require(t, dtype=None, device=True):
+    if device and not t.is_cuda:
+        raise ValueError("tensor must live on the GPU")
+    if dtype is not None and t.dtype != dtype:
+        raise TypeError(f"expected {dtype}, got {t.dtype}")
+    if not t.is_contiguous():
+        raise ValueError("tensor must be contiguous")
+
+
+# ------------------------------------------------------------------ pack
+def pack_scatter(c: torch.Tensor, idx: torch.Tensor, nfft: int) -> torch.Tensor:
+    """(..., nG) -> zero-filled (..., nfft) (reference pack.cu:16-26)."""
+    _require(c)
+    _require(idx, torch.int32)
+    ng = c.shape[-1]
+    if idx.numel() != ng:
+        raise ValueError("idx length must equal nG (pack.cu:50-51)")
+    batch = int(np.prod(c.shape[:-1])) if c.dim() > 1 else 1
+    out = torch.empty(c.shape[:-1] + (nfft,), dtype=c.dtype, device=c.device)
+    check(lib().addft_pack_scatter(_dtype_code(c), ptr(c), ptr(idx), batch, ng, nfft, ptr(out), stream()),
+          "addft_pack_scatter")
+    return out
+
+
+def pack_gather(grid: torch.Tensor, idx: torch.Tensor) -> torch.Tensor:
+    """(..., nfft) -> (..., nG) (reference pack.cu:28-38)."""
+    _require(grid)
+    _require(idx, torch.int32)
+    ng, nfft = idx.numel(), grid.shape[-1]
+    batch = int(np.prod(grid.shape[:-1])) if grid.dim() > 1 else 1
+    out = torch.empty(grid.shape[:-1] + (ng,), dtype=grid.dtype, device=grid.device)
+    check(lib().addft_pack_gather(_dtype_code(grid), ptr(grid), ptr(idx), batch, ng, nfft, ptr(out), stream()),
+          "addft_pack_gather")
+    return out
+
+
+# -------------------------------------------------------------------- XC
+def xc_lda(rho: torch.Tensor, want_energy: bool = True):
+    """(v_xc, raw sum n eps) for a float64 density (reference xc_lda.cu)."""
+    _require(rho, torch.float64)
+    n = rho.numel()
+    vxc = torch.empty_like(rho)
+    work = torch.empty(int(lib().addft_xc_work_bytes(n)) // 8 + 1, dtype=torch.float64, device=rho.device)
+    exc = torch.empty((), dtype=torch.float64, device=rho.device) if want_energy else None
+    check(lib().addft_xc_lda(ptr(rho), n, ptr(vxc), ptr(exc), ptr(work), stream()), "addft_xc_lda")
+    return vxc, exc
+
+
+def xc_fxc(rho: torch.Tensor, w: torch.Tensor | None = None) -> torch.Tensor:
+    """f_xc (or f_xc * w) (reference xc.py:181-191, 205-208)."""
+    _require(rho, torch.float64)
+    if w is not None:
+        _require(w, torch.float64)
+    out = torch.empty_like(rho)
+    check(lib().addft_xc_lda_fxc(ptr(rho), ptr(w), rho.numel(), ptr(out), stream()), "addft_xc_lda_fxc")
+    return out
+
+
+# ------------------------------------------------------------- real grids
+def to_planar(grid: torch.Tensor, shape) -> torch.Tensor:
+    """row-major (i1,i2,i3) float64 -> plane-major (i3,i1,i2)."""
+    _require(grid, torch.float64)
+    n1, n2, n3 = shape
+    out = torch.empty(n1 * n2 * n3, dtype=torch.float64, device=grid.device)
+    check(lib().addft_grid_transpose(n1, n2, n3, 0, ptr(grid), ptr(out), stream()), "addft_grid_transpose")
+    return out
+
+
+def from_planar(grid: torch.Tensor, shape) -> torch.Tensor:
+    _require(grid, torch.float64)
+    n1, n2, n3 = shape
+    out = torch.empty(n1 * n2 * n3, dtype=torch.float64, device=grid.device)
+    check(lib().addft_grid_transpose(n1, n2, n3, 1, ptr(grid), ptr(out), stream()), "addft_grid_transpose")
+    return out
+
+
+# ------------------------------------------------------------- sphere plan
+class SpherePlan:
+    """One G-sphere on one FFT box, with its stick tables and twiddles on
+    the device.  `precision` is torch.complex64 or torch.complex128."""
+
+    def __init__(self, fft_shape, miller, precision=torch.complex128, device=None):
+        self.shape = tuple(int(x) for x in fft_shape)
+        self.nfft = int(np.prod(self.shape))
+        m = np.ascontiguousarray(np.asarray(miller, dtype=np.int32).reshape(-1, 3))
+        self.ng = int(m.shape[0])
+        self.cdtype = precision
+        self.rdtype = torch.float32 if precision == torch.complex64 else torch.float64
+        self.device = torch.device("cuda") if device is None else torch.device(device)
+        L = lib()
+        handle = _lib.ctypes.c_void_p()
+        with torch.cuda.device(self.device):
+            check(L.addft_sphere_plan_create(self.shape[0], self.shape[1], self.shape[2], self.ng,
+                                             m.ctypes.data, _CDT[precision], _lib.ctypes.byref(handle)),
+                  "addft_sphere_plan_create")
+        self._h = handle
+        self._fin = weakref.finalize(self, L.addft_sphere_plan_destroy, handle)
+        nst, nra = _lib.ctypes.c_int64(), _lib.ctypes.c_int64()
+        perm = np.empty(self.ng, dtype=np.int32)
+        check(L.addft_sphere_plan_info(handle, _lib.ctypes.byref(nst), _lib.ctypes.byref(nra),
+                                       perm.ctypes.data), "addft_sphere_plan_info")
+        self.n_sticks, self.n_rows = int(nst.value), int(nra.value)
+        self.perm = perm                    # stick position -> reference index
+        self.iperm = np.empty_like(perm)
+        self.iperm[perm] = np.arange(self.ng, dtype=np.int32)
+        self._work = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    def work(self, nb: int) -> torch.Tensor:
+        need = int(lib().addft_sphere_work_bytes(self._h, max(1, nb)))
+        if self._work is None or self._work.numel() < need:
+            self._work = torch.empty(need, dtype=torch.uint8, device=self.device)
+        return self._work
+
+    # coefficient orders -------------------------------------------------
+    def to_stick(self, c: torch.Tensor) -> torch.Tensor:
+        return self._permute(c, 0)
+
+    def to_reference(self, c: torch.Tensor) -> torch.Tensor:
+        return self._permute(c, 1)
+
+    def _permute(self, c, direction):
+        _require(c)
+        if c.shape[-1] != self.ng:
+            raise ValueError(f"last dim must be nG={self.ng}")
+        nb = int(np.prod(c.shape[:-1])) if c.dim() > 1 else 1
+        out = torch.empty_like(c)
+        check(lib().addft_sphere_permute(self._h, _dtype_code(c), direction, ptr(c), nb, ptr(out), stream()),
+              "addft_sphere_permute")
+        return out
+
+    # real-space staging -------------------------------------------------
+    def stage(self, v_planar: torch.Tensor, scale: float) -> torch.Tensor:
+        _require(v_planar, torch.float64)
+        out = torch.empty(self.nfft, dtype=self.rdtype, device=v_planar.device)
+        check(lib().addft_grid_stage(self._h, ptr(v_planar), float(scale), ptr(out), stream()),
+              "addft_grid_stage")
+        return out
+
+    # fused operators ----------------------------------------------------
+    def hpsi_local(self, psi: torch.Tensor, tkin: torch.Tensor, v_staged: torch.Tensor,
+                   out: torch.Tensor | None = None) -> torch.Tensor:
+        """T psi + V psi for (nb, nG) stick-order psi."""
+        _require(psi, self.cdtype)
+        _require(tkin, self.rdtype)
+        _require(v_staged, self.rdtype)
+        nb = psi.shape[0] if psi.dim() == 2 else 1
+        if psi.shape[-1] != self.ng or tkin.numel() != self.ng or v_staged.numel() != self.nfft:
+            raise ValueError("shape mismatch in hpsi_local")
+        if out is None:
+            out = torch.empty_like(psi)
+        check(lib().addft_hpsi_local(self._h, ptr(psi), ptr(tkin), ptr(v_staged), nb, ptr(out),
+                                     ptr(self.work(nb)), stream()), "addft_hpsi_local")
+        return out
+
+    def density(self, psi: torch.Tensor, w: torch.Tensor, out: torch.Tensor | None = None,
+                accumulate: bool = False) -> torch.Tensor:
+        """sum_b w_b |IFFT(psi_b)|^2, plane-major float64."""
+        _require(psi, self.cdtype)
+        _require(w, torch.float64)
+        nb = psi.shape[0] if psi.dim() == 2 else 1
+        if w.numel() != nb or psi.shape[-1] != self.ng:
+            raise ValueError("shape mismatch in density")
+        if out is None:
+            out = torch.empty(self.nfft, dtype=torch.float64, device=psi.device)
+            accumulate = False
+        check(lib().addft_density_accumulate(self._h, ptr(psi), ptr(w), nb, ptr(out), int(bool(accumulate)),
+                                             ptr(self.work(nb)), stream()), "addft_density_accumulate")
+        return out
+
+    def to_real(self, c: torch.Tensor, scale: float = 1.0) -> torch.Tensor:
+        """scale * Re sum_G c_G e^{iGr}, plane-major float64 (stick-order c)."""
+        _require(c, torch.complex128)
+        out = torch.empty(self.nfft, dtype=torch.float64, device=c.device)
+        check(lib().addft_sphere_to_real(self._h, ptr(c), float(scale), ptr(out), ptr(self.work(1)), stream()),
+              "addft_sphere_to_real")
+        return out
+
+    def to_sphere(self, grid_planar: torch.Tensor, scale: float = 1.0) -> torch.Tensor:
+        """scale * sum_r f(r) e^{-iGr} in stick order."""
+        _require(grid_planar, torch.float64)
+        out = torch.empty(self.ng, dtype=torch.complex128, device=grid_planar.device)
+        check(lib().addft_real_to_sphere(self._h, ptr(grid_planar), float(scale), ptr(out), ptr(self.work(1)),
+                                         stream()), "addft_real_to_sphere")
+        return out

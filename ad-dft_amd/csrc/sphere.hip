@@ -1,0 +1,748 @@
+// Sphere plans and the stick-pruned box transforms: fused local Hpsi,
+// density accumulation, sphere <-> real grid.
+//
+// Data path of one band (box n1 x n2 x n3, row-major (i1, i2, i3)):
+//   z pass   : each FFT column ("stick", a distinct (i1, i2) of the sphere)
+//              is expanded from its few sphere coefficients to n3 points and
+//              transformed along i3 in LDS; written as T[b][i3][stick].
+//   plane    : one workgroup per (z-plane, band).  The plane is held in LDS
+//              compressed to the rows i1 that carry sticks (n_rows x n2),
+//              transformed along i2, then column chunks are expanded to n1,
+//              transformed along i1, multiplied by V(r), transformed back and
+//              re-compressed; the row transform is undone and only the
+//              stick columns are written back.
+//   z pass   : sticks are transformed back along i3 and gathered onto the
+//              sphere, with the kinetic term fused in.
+// The 3-D box is never materialised in HBM.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "fft_device.h"
+
+using addft::cpx;
+using addft::FftFactors;
+
+struct addft_sphere_plan_s {
+  int n1, n2, n3;
+  int64_t ng;
+  int precision;
+  int64_t nst;
+  int nra;
+  FftFactors f1, f2, f3;
+  int32_t* d_perm;    // stick-order position -> reference index
+  int32_t* d_iperm;   // reference index -> stick-order position
+  int32_t* d_sid;     // stick id of each stick-order position
+  int32_t* d_iz;      // i3 of each stick-order position
+  int32_t* d_soff;    // (nst + 1) CSR offsets into stick order
+  int32_t* d_srow;    // compressed row of each stick
+  int32_t* d_scol;    // i2 of each stick
+  int32_t* d_rowval;  // i1 of each compressed row
+  void* d_tw1;
+  void* d_tw2;
+  void* d_tw3;
+  std::vector<int32_t> h_perm;
+  int spb;     // sticks per z-pass block
+  int cchunk;  // columns per plane-pass chunk
+  int ept_p;   // elements per thread of the plane pass
+  size_t lds_plane;
+};
+
+namespace {
+
+constexpr int kZThreads = 256;
+constexpr int kZEpt = 8;
+constexpr int kPThreads = 512;
+constexpr size_t kLdsMax = 160 * 1024;
+
+template <typename R>
+size_t csize() {
+  return sizeof(cpx<R>);
+}
+
+// ------------------------------------------------------------------ z pass
+// Sphere (stick order) -> sticks T[b][i3][s], inverse transform along i3.
+template <typename R, typename RIN>
+__global__ void __launch_bounds__(kZThreads)
+k_z_inverse(const cpx<RIN>* __restrict__ psi, int64_t ng, const int32_t* __restrict__ soff,
+            const int32_t* __restrict__ sid, const int32_t* __restrict__ iz, int64_t nst,
+            int spb, FftFactors f3, const cpx<R>* __restrict__ tw3, cpx<R>* __restrict__ T) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+  cpx<R>* L = reinterpret_cast<cpx<R>*>(smem_raw);
+  const int n3 = f3.n;
+  const int64_t s0 = (int64_t)blockIdx.x * spb;
+  const int cnt = (int)min((int64_t)spb, nst - s0);
+  const int64_t b = blockIdx.y;
+  for (int i = threadIdx.x; i < cnt * n3; i += blockDim.x) L[i] = addft::mk<R>(0, 0);
+  __syncthreads();
+  const int p0 = soff[s0], p1 = soff[s0 + cnt];
+  const cpx<RIN>* src = psi + b * ng;
+  for (int p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
+    cpx<RIN> c = src[p];
+    L[(sid[p] - s0) * n3 + iz[p]] = addft::mk<R>((R)c.x, (R)c.y);
+  }
+  __syncthreads();
+  addft::lds_fft<R, kZEpt, true>(L, f3, cnt, n3, 1, tw3);
+  cpx<R>* dst = T + (b * n3) * nst + s0;
+  for (int i = threadIdx.x; i < cnt * n3; i += blockDim.x) {
+    int z = i / cnt, j = i - z * cnt;
+    dst[(int64_t)z * nst + j] = L[j * n3 + z];
+  }
+}
+
+// Sticks -> sphere: forward transform along i3, gather, scale, and the
+// optional kinetic epilogue out = scale*F + tkin*psi.
+template <typename R, typename ROUT>
+__global__ void __launch_bounds__(kZThreads)
+k_z_forward(const cpx<R>* __restrict__ T, int64_t ng, const int32_t* __restrict__ soff,
+            const int32_t* __restrict__ sid, const int32_t* __restrict__ iz, int64_t nst,
+            int spb, FftFactors f3, const cpx<R>* __restrict__ tw3, R scale,
+            const R* __restrict__ tkin, const cpx<R>* __restrict__ psi, cpx<ROUT>* __restrict__ out) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+  cpx<R>* L = reinterpret_cast<cpx<R>*>(smem_raw);
+  const int n3 = f3.n;
+  const int64_t s0 = (int64_t)blockIdx.x * spb;
+  const int cnt = (int)min((int64_t)spb, nst - s0);
+  const int64_t b = blockIdx.y;
+  const cpx<R>* src = T + (b * n3) * nst + s0;
+  for (int i = threadIdx.x; i < cnt * n3; i += blockDim.x) {
+    int z = i / cnt, j = i - z * cnt;
+    L[j * n3 + z] = src[(int64_t)z * nst + j];
+  }
+  __syncthreads();
+  addft::lds_fft<R, kZEpt, false>(L, f3, cnt, n3, 1, tw3);
+  const int p0 = soff[s0], p1 = soff[s0 + cnt];
+  cpx<ROUT>* o = out + b * ng;
+  for (int p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
+    cpx<R> v = L[(sid[p] - s0) * n3 + iz[p]] * scale;
+    if (tkin != nullptr) {
+      cpx<R> x = psi[b * ng + p];
+      R t = tkin[p];
+      v.x += t * x.x;
+      v.y += t * x.y;
+    }
+    o[p] = addft::mk<ROUT>((ROUT)v.x, (ROUT)v.y);
+  }
+}
+
+// ------------------------------------------------------------- plane pass
+// Load sticks of plane z into the compressed plane P (nra x n2).
+template <typename R>
+__device__ __forceinline__ void load_plane(cpx<R>* P, const cpx<R>* __restrict__ Tp, int nra, int n2,
+                                           int64_t nst, const int32_t* __restrict__ srow,
+                                           const int32_t* __restrict__ scol) {
+  for (int i = threadIdx.x; i < nra * n2; i += blockDim.x) P[i] = addft::mk<R>(0, 0);
+  __syncthreads();
+  for (int64_t j = threadIdx.x; j < nst; j += blockDim.x) P[srow[j] * n2 + scol[j]] = Tp[j];
+  __syncthreads();
+}
+
+template <typename R>
+__device__ __forceinline__ void store_plane(const cpx<R>* P, cpx<R>* __restrict__ Tp, int n2, int64_t nst,
+                                            const int32_t* __restrict__ srow,
+                                            const int32_t* __restrict__ scol) {
+  for (int64_t j = threadIdx.x; j < nst; j += blockDim.x) Tp[j] = P[srow[j] * n2 + scol[j]];
+}
+
+// Column chunk W is laid out [i1][c] (c fastest) so that V(r) reads and the
+// row copies are contiguous; the column transforms then run with fs = 1,
+// es = cc.
+template <typename R>
+__device__ __forceinline__ void expand_columns(cpx<R>* W, const cpx<R>* P, int n1, int n2, int nra,
+                                               int c0, int cc, const int32_t* __restrict__ rowval) {
+  for (int i = threadIdx.x; i < cc * n1; i += blockDim.x) W[i] = addft::mk<R>(0, 0);
+  __syncthreads();
+  for (int i = threadIdx.x; i < cc * nra; i += blockDim.x) {
+    int r = i / cc, c = i - r * cc;
+    W[rowval[r] * cc + c] = P[r * n2 + c0 + c];
+  }
+  __syncthreads();
+}
+
+template <typename R>
+__device__ __forceinline__ void compress_columns(cpx<R>* P, const cpx<R>* W, int n2, int nra, int c0,
+                                                 int cc, const int32_t* __restrict__ rowval) {
+  for (int i = threadIdx.x; i < cc * nra; i += blockDim.x) {
+    int r = i / cc, c = i - r * cc;
+    P[r * n2 + c0 + c] = W[rowval[r] * cc + c];
+  }
+  __syncthreads();
+}
+
+// Fused:  T <- FFT_plane( v * IFFT_plane(T) )  for plane z = blockIdx.x,
+// band b = blockIdx.y.  v is plane-major in precision R, pre-scaled.
+template <typename R, int EPT>
+__global__ void __launch_bounds__(kPThreads)
+k_plane_v(cpx<R>* __restrict__ T, int64_t nst, int nra, const int32_t* __restrict__ srow,
+          const int32_t* __restrict__ scol, const int32_t* __restrict__ rowval, FftFactors f1,
+          FftFactors f2, const cpx<R>* __restrict__ tw1, const cpx<R>* __restrict__ tw2,
+          const R* __restrict__ v, int cchunk) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+  const int n1 = f1.n, n2 = f2.n, n3 = gridDim.x;
+  cpx<R>* P = reinterpret_cast<cpx<R>*>(smem_raw);
+  cpx<R>* W = P + nra * n2;
+  const int z = blockIdx.x;
+  const int64_t b = blockIdx.y;
+  cpx<R>* Tp = T + (b * n3 + z) * nst;
+  load_plane(P, Tp, nra, n2, nst, srow, scol);
+  addft::lds_fft<R, EPT, true>(P, f2, nra, n2, 1, tw2);
+  const R* vp = v + (int64_t)z * n1 * n2;
+  for (int c0 = 0; c0 < n2; c0 += cchunk) {
+    const int cc = min(cchunk, n2 - c0);
+    expand_columns(W, P, n1, n2, nra, c0, cc, rowval);
+    addft::lds_fft<R, EPT, true>(W, f1, cc, 1, cc, tw1);
+    for (int i = threadIdx.x; i < cc * n1; i += blockDim.x) {
+      int x = i / cc, c = i - x * cc;
+      W[i] = W[i] * vp[x * n2 + c0 + c];
+    }
+    __syncthreads();
+    addft::lds_fft<R, EPT, false>(W, f1, cc, 1, cc, tw1);
+    compress_columns(P, W, n2, nra, c0, cc, rowval);
+  }
+  addft::lds_fft<R, EPT, false>(P, f2, nra, n2, 1, tw2);
+  store_plane(P, Tp, n2, nst, srow, scol);
+}
+
+// Density: for plane z and band group g (blockIdx.y), accumulate
+// part[g][z][x][y] = sum_{b in group} w_b |IFFT_plane(T_b)|^2.
+template <typename R, int EPT>
+__global__ void __launch_bounds__(kPThreads)
+k_plane_density(const cpx<R>* __restrict__ T, int64_t nst, int nra, const int32_t* __restrict__ srow,
+                const int32_t* __restrict__ scol, const int32_t* __restrict__ rowval, FftFactors f1,
+                FftFactors f2, const cpx<R>* __restrict__ tw1, const cpx<R>* __restrict__ tw2,
+                const double* __restrict__ w, int64_t nb, int bpg, int cchunk,
+                double* __restrict__ part) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+  const int n1 = f1.n, n2 = f2.n, n3 = gridDim.x;
+  cpx<R>* P = reinterpret_cast<cpx<R>*>(smem_raw);
+  cpx<R>* W = P + nra * n2;
+  const int z = blockIdx.x;
+  const int64_t g = blockIdx.y;
+  const int64_t nfft = (int64_t)n1 * n2 * n3;
+  double* out = part + g * nfft + (int64_t)z * n1 * n2;
+  for (int i = threadIdx.x; i < n1 * n2; i += blockDim.x) out[i] = 0.0;
+  const int64_t b0 = g * bpg, b1 = min(nb, b0 + bpg);
+  for (int64_t b = b0; b < b1; ++b) {
+    __syncthreads();
+    const cpx<R>* Tp = T + (b * n3 + z) * nst;
+    for (int i = threadIdx.x; i < nra * n2; i += blockDim.x) P[i] = addft::mk<R>(0, 0);
+    __syncthreads();
+    for (int64_t j = threadIdx.x; j < nst; j += blockDim.x) P[srow[j] * n2 + scol[j]] = Tp[j];
+    __syncthreads();
+    addft::lds_fft<R, EPT, true>(P, f2, nra, n2, 1, tw2);
+    const double wb = w[b];
+    for (int c0 = 0; c0 < n2; c0 += cchunk) {
+      const int cc = min(cchunk, n2 - c0);
+      expand_columns(W, P, n1, n2, nra, c0, cc, rowval);
+      addft::lds_fft<R, EPT, true>(W, f1, cc, 1, cc, tw1);
+      for (int i = threadIdx.x; i < cc * n1; i += blockDim.x) {
+        int x = i / cc, c = i - x * cc;
+        cpx<R> a = W[i];
+        out[x * n2 + c0 + c] += wb * ((double)a.x * (double)a.x + (double)a.y * (double)a.y);
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// Sum the band-group partials in a fixed order (deterministic).
+__global__ void k_reduce_parts(const double* __restrict__ part, int64_t ngroups, int64_t nfft,
+                               double* __restrict__ rho, int accumulate) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nfft;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    double s = accumulate ? rho[i] : 0.0;
+    for (int64_t g = 0; g < ngroups; ++g) s += part[g * nfft + i];
+    rho[i] = s;
+  }
+}
+
+// Sticks -> full real plane (plane-major, double), real part times scale.
+template <typename R, int EPT>
+__global__ void __launch_bounds__(kPThreads)
+k_plane_to_real(const cpx<R>* __restrict__ T, int64_t nst, int nra, const int32_t* __restrict__ srow,
+                const int32_t* __restrict__ scol, const int32_t* __restrict__ rowval, FftFactors f1,
+                FftFactors f2, const cpx<R>* __restrict__ tw1, const cpx<R>* __restrict__ tw2,
+                double scale, int cchunk, double* __restrict__ grid) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+  const int n1 = f1.n, n2 = f2.n;
+  cpx<R>* P = reinterpret_cast<cpx<R>*>(smem_raw);
+  cpx<R>* W = P + nra * n2;
+  const int z = blockIdx.x;
+  load_plane(P, T + (int64_t)z * nst, nra, n2, nst, srow, scol);
+  addft::lds_fft<R, EPT, true>(P, f2, nra, n2, 1, tw2);
+  double* out = grid + (int64_t)z * n1 * n2;
+  for (int c0 = 0; c0 < n2; c0 += cchunk) {
+    const int cc = min(cchunk, n2 - c0);
+    expand_columns(W, P, n1, n2, nra, c0, cc, rowval);
+    addft::lds_fft<R, EPT, true>(W, f1, cc, 1, cc, tw1);
+    for (int i = threadIdx.x; i < cc * n1; i += blockDim.x) {
+      int x = i / cc, c = i - x * cc;
+      out[x * n2 + c0 + c] = scale * (double)W[i].x;
+    }
+    __syncthreads();
+  }
+}
+
+// Full real plane -> sticks: forward transform along i1 (full columns),
+// keep the active rows, forward along i2, keep the stick columns.
+template <typename R, int EPT>
+__global__ void __launch_bounds__(kPThreads)
+k_real_to_plane(const double* __restrict__ grid, int64_t nst, int nra, const int32_t* __restrict__ srow,
+                const int32_t* __restrict__ scol, const int32_t* __restrict__ rowval, FftFactors f1,
+                FftFactors f2, const cpx<R>* __restrict__ tw1, const cpx<R>* __restrict__ tw2,
+                int cchunk, cpx<R>* __restrict__ T) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+  const int n1 = f1.n, n2 = f2.n;
+  cpx<R>* P = reinterpret_cast<cpx<R>*>(smem_raw);
+  cpx<R>* W = P + nra * n2;
+  const int z = blockIdx.x;
+  const double* in = grid + (int64_t)z * n1 * n2;
+  for (int c0 = 0; c0 < n2; c0 += cchunk) {
+    const int cc = min(cchunk, n2 - c0);
+    for (int i = threadIdx.x; i < cc * n1; i += blockDim.x) {
+      int x = i / cc, c = i - x * cc;
+      W[i] = addft::mk<R>((R)in[x * n2 + c0 + c], (R)0);
+    }
+    __syncthreads();
+    addft::lds_fft<R, EPT, false>(W, f1, cc, 1, cc, tw1);
+    compress_columns(P, W, n2, nra, c0, cc, rowval);
+  }
+  addft::lds_fft<R, EPT, false>(P, f2, nra, n2, 1, tw2);
+  store_plane(P, T + (int64_t)z * nst, n2, nst, srow, scol);
+}
+
+// ------------------------------------------------------------ small kernels
+template <typename RIN, typename ROUT>
+__global__ void k_permute(const cpx<RIN>* __restrict__ in, const int32_t* __restrict__ map, int64_t ng,
+                          int64_t nb, cpx<ROUT>* __restrict__ out) {
+  // out[b][p] = in[b][map[p]]
+  const int64_t total = nb * ng;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    int64_t b = i / ng, p = i - b * ng;
+    cpx<RIN> v = in[b * ng + map[p]];
+    out[i] = addft::mk<ROUT>((ROUT)v.x, (ROUT)v.y);
+  }
+}
+
+// (A x B) -> (B x A) transpose of doubles with an LDS tile.
+__global__ void k_transpose(const double* __restrict__ in, int64_t rows, int64_t cols,
+                            double* __restrict__ out) {
+  __shared__ double tile[32][33];
+  const int64_t c0 = (int64_t)blockIdx.x * 32, r0 = (int64_t)blockIdx.y * 32;
+  for (int k = threadIdx.y; k < 32; k += blockDim.y) {
+    int64_t r = r0 + k, c = c0 + threadIdx.x;
+    if (r < rows && c < cols) tile[k][threadIdx.x] = in[r * cols + c];
+  }
+  __syncthreads();
+  for (int k = threadIdx.y; k < 32; k += blockDim.y) {
+    int64_t c = c0 + k, r = r0 + threadIdx.x;
+    if (r < rows && c < cols) out[c * rows + r] = tile[threadIdx.x][k];
+  }
+}
+
+template <typename R>
+__global__ void k_stage(const double* __restrict__ in, int64_t n, double scale, R* __restrict__ out) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = (R)(in[i] * scale);
+}
+
+// ------------------------------------------------------------------ host
+template <typename R>
+int upload_twiddles(int n, void** dst) {
+  std::vector<cpx<R>> h(n);
+  for (int k = 0; k < n; ++k) {
+    double a = -2.0 * M_PI * (double)k / (double)n;
+    h[k].x = (R)std::cos(a);
+    h[k].y = (R)std::sin(a);
+  }
+  ADDFT_HIP(hipMalloc(dst, sizeof(cpx<R>) * n));
+  ADDFT_HIP(hipMemcpy(*dst, h.data(), sizeof(cpx<R>) * n, hipMemcpyHostToDevice));
+  return ADDFT_OK;
+}
+
+int upload_i32(const std::vector<int32_t>& v, int32_t** dst) {
+  size_t bytes = std::max<size_t>(1, v.size()) * sizeof(int32_t);
+  ADDFT_HIP(hipMalloc(dst, bytes));
+  if (!v.empty()) ADDFT_HIP(hipMemcpy(*dst, v.data(), v.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+  return ADDFT_OK;
+}
+
+int grid_blocks(int64_t n, int threads = 256) {
+  int64_t b = (n + threads - 1) / threads;
+  return (int)std::max<int64_t>(1, std::min<int64_t>(b, 65536));
+}
+
+template <typename R, int EPT>
+int set_plane_attrs(size_t lds) {
+  ADDFT_HIP(hipFuncSetAttribute((const void*)k_plane_v<R, EPT>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  ADDFT_HIP(hipFuncSetAttribute((const void*)k_plane_density<R, EPT>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  ADDFT_HIP(hipFuncSetAttribute((const void*)k_plane_to_real<R, EPT>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  ADDFT_HIP(hipFuncSetAttribute((const void*)k_real_to_plane<R, EPT>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  return ADDFT_OK;
+}
+
+template <typename R>
+int set_plane_attrs_any(int ept, size_t lds) {
+  switch (ept) {
+    case 4: return set_plane_attrs<R, 4>(lds);
+    case 8: return set_plane_attrs<R, 8>(lds);
+    case 16: return set_plane_attrs<R, 16>(lds);
+    default: return set_plane_attrs<R, 32>(lds);
+  }
+}
+
+int64_t t_bytes(const addft_sphere_plan_s* p, int64_t nb) {
+  size_t cs = p->precision == ADDFT_C64 ? 8 : 16;
+  return nb * (int64_t)p->n3 * p->nst * (int64_t)cs;
+}
+
+constexpr int kBandsPerGroup = 8;
+
+int64_t density_groups(int64_t nb) { return (nb + kBandsPerGroup - 1) / kBandsPerGroup; }
+
+template <typename R>
+int launch_hpsi(addft_sphere_plan_s* p, const void* psi, const void* tkin, const void* v, int64_t nb,
+                void* out, void* work, hipStream_t st) {
+  cpx<R>* T = reinterpret_cast<cpx<R>*>(work);
+  const size_t zlds = (size_t)p->spb * p->n3 * sizeof(cpx<R>);
+  dim3 zg((unsigned)((p->nst + p->spb - 1) / p->spb), (unsigned)nb);
+  hipLaunchKernelGGL((k_z_inverse<R, R>), zg, dim3(kZThreads), zlds, st,
+                     reinterpret_cast<const cpx<R>*>(psi), p->ng, p->d_soff, p->d_sid, p->d_iz, p->nst,
+                     p->spb, p->f3, reinterpret_cast<const cpx<R>*>(p->d_tw3), T);
+  ADDFT_LAUNCH_CHECK();
+  dim3 pg((unsigned)p->n3, (unsigned)nb);
+  auto tw1 = reinterpret_cast<const cpx<R>*>(p->d_tw1);
+  auto tw2 = reinterpret_cast<const cpx<R>*>(p->d_tw2);
+  auto vv = reinterpret_cast<const R*>(v);
+#define ADDFT_PV(E)                                                                              \
+  hipLaunchKernelGGL((k_plane_v<R, E>), pg, dim3(kPThreads), p->lds_plane, st, T, p->nst, p->nra, \
+                     p->d_srow, p->d_scol, p->d_rowval, p->f1, p->f2, tw1, tw2, vv, p->cchunk)
+  switch (p->ept_p) {
+    case 4: ADDFT_PV(4); break;
+    case 8: ADDFT_PV(8); break;
+    case 16: ADDFT_PV(16); break;
+    default: ADDFT_PV(32); break;
+  }
+#undef ADDFT_PV
+  ADDFT_LAUNCH_CHECK();
+  hipLaunchKernelGGL((k_z_forward<R, R>), zg, dim3(kZThreads), zlds, st, T, p->ng, p->d_soff, p->d_sid,
+                     p->d_iz, p->nst, p->spb, p->f3, reinterpret_cast<const cpx<R>*>(p->d_tw3), (R)1,
+                     reinterpret_cast<const R*>(tkin), reinterpret_cast<const cpx<R>*>(psi),
+                     reinterpret_cast<cpx<R>*>(out));
+  ADDFT_LAUNCH_CHECK();
+  return ADDFT_OK;
+}
+
+template <typename R>
+int launch_density(addft_sphere_plan_s* p, const void* psi, const double* w, int64_t nb, double* rho,
+                   int accumulate, void* work, hipStream_t st) {
+  cpx<R>* T = reinterpret_cast<cpx<R>*>(work);
+  double* part = reinterpret_cast<double*>(reinterpret_cast<char*>(work) + t_bytes(p, nb));
+  const size_t zlds = (size_t)p->spb * p->n3 * sizeof(cpx<R>);
+  dim3 zg((unsigned)((p->nst + p->spb - 1) / p->spb), (unsigned)nb);
+  hipLaunchKernelGGL((k_z_inverse<R, R>), zg, dim3(kZThreads), zlds, st,
+                     reinterpret_cast<const cpx<R>*>(psi), p->ng, p->d_soff, p->d_sid, p->d_iz, p->nst,
+                     p->spb, p->f3, reinterpret_cast<const cpx<R>*>(p->d_tw3), T);
+  ADDFT_LAUNCH_CHECK();
+  const int64_t ngr = density_groups(nb);
+  dim3 pg((unsigned)p->n3, (unsigned)ngr);
+  auto tw1 = reinterpret_cast<const cpx<R>*>(p->d_tw1);
+  auto tw2 = reinterpret_cast<const cpx<R>*>(p->d_tw2);
+#define ADDFT_PD(E)                                                                                     \
+  hipLaunchKernelGGL((k_plane_density<R, E>), pg, dim3(kPThreads), p->lds_plane, st, T, p->nst, p->nra, \
+                     p->d_srow, p->d_scol, p->d_rowval, p->f1, p->f2, tw1, tw2, w, nb, kBandsPerGroup,  \
+                     p->cchunk, part)
+  switch (p->ept_p) {
+    case 4: ADDFT_PD(4); break;
+    case 8: ADDFT_PD(8); break;
+    case 16: ADDFT_PD(16); break;
+    default: ADDFT_PD(32); break;
+  }
+#undef ADDFT_PD
+  ADDFT_LAUNCH_CHECK();
+  const int64_t nfft = (int64_t)p->n1 * p->n2 * p->n3;
+  hipLaunchKernelGGL(k_reduce_parts, dim3(grid_blocks(nfft)), dim3(256), 0, st, part, ngr, nfft, rho,
+                     accumulate);
+  ADDFT_LAUNCH_CHECK();
+  return ADDFT_OK;
+}
+
+}  // namespace
+
+namespace addft {
+thread_local std::string g_last_error;
+void set_error(const std::string& msg) { g_last_error = msg; }
+}  // namespace addft
+
+extern "C" {
+
+const char* addft_last_error(void) { return addft::g_last_error.c_str(); }
+int addft_version(void) { return 1; }
+
+int addft_sphere_plan_create(int n1, int n2, int n3, int64_t ng, const int32_t* h_miller, int precision,
+                             addft_sphere_plan* out) {
+  ADDFT_CHECK_ARG(out != nullptr && h_miller != nullptr, "null argument");
+  ADDFT_CHECK_ARG(ng > 0 && ng < (int64_t)INT32_MAX, "ng out of range");
+  ADDFT_CHECK_ARG(precision == ADDFT_C64 || precision == ADDFT_C128, "bad precision");
+  auto* p = new addft_sphere_plan_s();
+  std::memset((void*)&p->f1, 0, sizeof(FftFactors) * 3);
+  p->n1 = n1;
+  p->n2 = n2;
+  p->n3 = n3;
+  p->ng = ng;
+  p->precision = precision;
+  if (!addft::factorize(n1, &p->f1) || !addft::factorize(n2, &p->f2) || !addft::factorize(n3, &p->f3)) {
+    delete p;
+    addft::set_error("box dimensions must be 2*3*5-smooth");
+    return ADDFT_EINVAL;
+  }
+  // flat index in the reference convention and the stick-order sort
+  std::vector<int64_t> flat(ng);
+  for (int64_t g = 0; g < ng; ++g) {
+    int64_t m0 = h_miller[3 * g], m1 = h_miller[3 * g + 1], m2 = h_miller[3 * g + 2];
+    int64_t i1 = ((m0 % n1) + n1) % n1, i2 = ((m1 % n2) + n2) % n2, i3 = ((m2 % n3) + n3) % n3;
+    flat[g] = (i1 * n2 + i2) * n3 + i3;
+  }
+  std::vector<int32_t> perm(ng);
+  std::iota(perm.begin(), perm.end(), 0);
+  std::stable_sort(perm.begin(), perm.end(), [&](int32_t a, int32_t b) { return flat[a] < flat[b]; });
+  for (int64_t i = 1; i < ng; ++i) {
+    if (flat[perm[i]] == flat[perm[i - 1]]) {
+      delete p;
+      addft::set_error("duplicate G vectors in the sphere (aliased on this box)");
+      return ADDFT_EINVAL;
+    }
+  }
+  std::vector<int32_t> iperm(ng), sid(ng), iz(ng), soff, srow, scol, rowval;
+  std::vector<int> row_of_i1(n1, -1);
+  int64_t prev_col = -1;
+  for (int64_t q = 0; q < ng; ++q) {
+    int64_t f = flat[perm[q]];
+    int64_t col = f / n3;  // i1*n2 + i2
+    if (col != prev_col) {
+      soff.push_back((int32_t)q);
+      int i1 = (int)(col / n2), i2 = (int)(col % n2);
+      if (row_of_i1[i1] < 0) {
+        row_of_i1[i1] = (int)rowval.size();
+        rowval.push_back(i1);
+      }
+      srow.push_back(row_of_i1[i1]);
+      scol.push_back(i2);
+      prev_col = col;
+    }
+    sid[q] = (int32_t)(soff.size() - 1);
+    iz[q] = (int32_t)(f % n3);
+    iperm[perm[q]] = (int32_t)q;
+  }
+  soff.push_back((int32_t)ng);
+  p->nst = (int64_t)srow.size();
+  p->nra = (int)rowval.size();
+  p->h_perm = perm;
+  // launch shapes
+  const size_t cs = precision == ADDFT_C64 ? 8 : 16;
+  p->spb = std::max(1, std::min(64, kZEpt * kZThreads / n3));
+  const int64_t plane_elems = (int64_t)p->nra * n2;
+  int ept = 4;
+  while (ept < 32 && plane_elems > (int64_t)ept * kPThreads) ept *= 2;
+  if (plane_elems > (int64_t)ept * kPThreads || n1 > ept * kPThreads) {
+    delete p;
+    addft::set_error("plane too large for the LDS plane pass");
+    return ADDFT_EUNSUPPORTED;
+  }
+  int cch = std::min(n2, (ept * kPThreads) / n1);
+  size_t base = (size_t)plane_elems * cs;
+  while (cch > 1 && base + (size_t)cch * n1 * cs > kLdsMax) --cch;
+  if (base + (size_t)cch * n1 * cs > kLdsMax) {
+    delete p;
+    addft::set_error("plane does not fit in LDS");
+    return ADDFT_EUNSUPPORTED;
+  }
+  p->ept_p = ept;
+  p->cchunk = cch;
+  p->lds_plane = base + (size_t)cch * n1 * cs;
+  int rc = ADDFT_OK;
+  if ((rc = upload_i32(perm, &p->d_perm)) || (rc = upload_i32(iperm, &p->d_iperm)) ||
+      (rc = upload_i32(sid, &p->d_sid)) || (rc = upload_i32(iz, &p->d_iz)) ||
+      (rc = upload_i32(soff, &p->d_soff)) || (rc = upload_i32(srow, &p->d_srow)) ||
+      (rc = upload_i32(scol, &p->d_scol)) || (rc = upload_i32(rowval, &p->d_rowval))) {
+    delete p;
+    return rc;
+  }
+  if (precision == ADDFT_C64) {
+    if ((rc = upload_twiddles<float>(n1, &p->d_tw1)) || (rc = upload_twiddles<float>(n2, &p->d_tw2)) ||
+        (rc = upload_twiddles<float>(n3, &p->d_tw3)) || (rc = set_plane_attrs_any<float>(ept, p->lds_plane)))
+      return rc;
+  } else {
+    if ((rc = upload_twiddles<double>(n1, &p->d_tw1)) || (rc = upload_twiddles<double>(n2, &p->d_tw2)) ||
+        (rc = upload_twiddles<double>(n3, &p->d_tw3)) ||
+        (rc = set_plane_attrs_any<double>(ept, p->lds_plane)))
+      return rc;
+  }
+  *out = p;
+  return ADDFT_OK;
+}
+
+int addft_sphere_plan_destroy(addft_sphere_plan p) {
+  if (!p) return ADDFT_OK;
+  void* ptrs[] = {p->d_perm, p->d_iperm, p->d_sid, p->d_iz, p->d_soff, p->d_srow,
+                  p->d_scol, p->d_rowval, p->d_tw1, p->d_tw2, p->d_tw3};
+  for (void* q : ptrs)
+    if (q) (void)hipFree(q);
+  delete p;
+  return ADDFT_OK;
+}
+
+int addft_sphere_plan_info(addft_sphere_plan p, int64_t* n_sticks, int64_t* n_rows, int32_t* h_perm) {
+  ADDFT_CHECK_ARG(p != nullptr, "null plan");
+  if (n_sticks) *n_sticks = p->nst;
+  if (n_rows) *n_rows = p->nra;
+  if (h_perm) std::memcpy(h_perm, p->h_perm.data(), sizeof(int32_t) * p->ng);
+  return ADDFT_OK;
+}
+
+int64_t addft_sphere_work_bytes(addft_sphere_plan p, int64_t nb) {
+  if (!p || nb < 1) return 0;
+  const int64_t nfft = (int64_t)p->n1 * p->n2 * p->n3;
+  return t_bytes(p, nb) + density_groups(nb) * nfft * (int64_t)sizeof(double) + 256;
+}
+
+int addft_sphere_permute(addft_sphere_plan p, int dtype, int direction, const void* in, int64_t nb,
+                         void* out, void* stream) {
+  ADDFT_CHECK_ARG(p && in && out && nb >= 0, "bad argument");
+  ADDFT_CHECK_ARG(in != out, "permute cannot run in place");
+  ADDFT_CHECK_ARG(direction == 0 || direction == 1, "bad direction");
+  if (nb == 0) return ADDFT_OK;
+  // stick order p <- reference index perm[p];  reference g <- stick iperm[g]
+  const int32_t* map = direction == 0 ? p->d_perm : p->d_iperm;
+  hipStream_t st = addft::as_stream(stream);
+  int blocks = grid_blocks(nb * p->ng);
+  if (dtype == ADDFT_C64)
+    hipLaunchKernelGGL((k_permute<float, float>), dim3(blocks), dim3(256), 0, st,
+                       reinterpret_cast<const cpx<float>*>(in), map, p->ng, nb, reinterpret_cast<cpx<float>*>(out));
+  else if (dtype == ADDFT_C128)
+    hipLaunchKernelGGL((k_permute<double, double>), dim3(blocks), dim3(256), 0, st,
+                       reinterpret_cast<const cpx<double>*>(in), map, p->ng, nb,
+                       reinterpret_cast<cpx<double>*>(out));
+  else
+    ADDFT_CHECK_ARG(false, "bad dtype");
+  ADDFT_LAUNCH_CHECK();
+  return ADDFT_OK;
+}
+
+int addft_grid_transpose(int n1, int n2, int n3, int direction, const double* in, double* out, void* stream) {
+  ADDFT_CHECK_ARG(in && out && in != out && n1 > 0 && n2 > 0 && n3 > 0, "bad argument");
+  // row-major (n1*n2) x n3  <->  plane-major n3 x (n1*n2)
+  int64_t rows = direction == 0 ? (int64_t)n1 * n2 : n3;
+  int64_t cols = direction == 0 ? n3 : (int64_t)n1 * n2;
+  dim3 grid((unsigned)((cols + 31) / 32), (unsigned)((rows + 31) / 32));
+  hipLaunchKernelGGL(k_transpose, grid, dim3(32, 8), 0, addft::as_stream(stream), in, rows, cols, out);
+  ADDFT_LAUNCH_CHECK();
+  return ADDFT_OK;
+}
+
+int addft_grid_stage(addft_sphere_plan p, const double* in, double scale, void* out, void* stream) {
+  ADDFT_CHECK_ARG(p && in && out, "bad argument");
+  const int64_t n = (int64_t)p->n1 * p->n2 * p->n3;
+  hipStream_t st = addft::as_stream(stream);
+  if (p->precision == ADDFT_C64)
+    hipLaunchKernelGGL(k_stage<float>, dim3(grid_blocks(n)), dim3(256), 0, st, in, n, scale,
+                       reinterpret_cast<float*>(out));
+  else
+    hipLaunchKernelGGL(k_stage<double>, dim3(grid_blocks(n)), dim3(256), 0, st, in, n, scale,
+                       reinterpret_cast<double*>(out));
+  ADDFT_LAUNCH_CHECK();
+  return ADDFT_OK;
+}
+
+int addft_hpsi_local(addft_sphere_plan p, const void* psi, const void* tkin, const void* v, int64_t nb,
+                     void* out, void* work, void* stream) {
+  ADDFT_CHECK_ARG(p && psi && tkin && v && out && work, "null argument");
+  ADDFT_CHECK_ARG(out != psi, "out may not alias psi");
+  ADDFT_CHECK_ARG(nb >= 0 && nb < 65536, "nb out of range");
+  if (nb == 0) return ADDFT_OK;
+  hipStream_t st = addft::as_stream(stream);
+  if (p->precision == ADDFT_C64) return launch_hpsi<float>(p, psi, tkin, v, nb, out, work, st);
+  return launch_hpsi<double>(p, psi, tkin, v, nb, out, work, st);
+}
+
+int addft_density_accumulate(addft_sphere_plan p, const void* psi, const double* w, int64_t nb, double* rho,
+                             int accumulate, void* work, void* stream) {
+  ADDFT_CHECK_ARG(p && psi && w && rho && work, "null argument");
+  ADDFT_CHECK_ARG(nb >= 0 && nb < 65536, "nb out of range");
+  hipStream_t st = addft::as_stream(stream);
+  if (nb == 0) {
+    if (!accumulate) ADDFT_HIP(hipMemsetAsync(rho, 0, sizeof(double) * p->n1 * p->n2 * p->n3, st));
+    return ADDFT_OK;
+  }
+  if (p->precision == ADDFT_C64) return launch_density<float>(p, psi, w, nb, rho, accumulate, work, st);
+  return launch_density<double>(p, psi, w, nb, rho, accumulate, work, st);
+}
+
+int addft_sphere_to_real(addft_sphere_plan p, const double* c, double scale, double* grid, void* work,
+                         void* stream) {
+  ADDFT_CHECK_ARG(p && c && grid && work, "null argument");
+  ADDFT_CHECK_ARG(p->precision == ADDFT_C128, "sphere_to_real needs a C128 plan");
+  hipStream_t st = addft::as_stream(stream);
+  cpx<double>* T = reinterpret_cast<cpx<double>*>(work);
+  const size_t zlds = (size_t)p->spb * p->n3 * sizeof(cpx<double>);
+  dim3 zg((unsigned)((p->nst + p->spb - 1) / p->spb), 1u);
+  hipLaunchKernelGGL((k_z_inverse<double, double>), zg, dim3(kZThreads), zlds, st,
+                     reinterpret_cast<const cpx<double>*>(c), p->ng, p->d_soff, p->d_sid, p->d_iz, p->nst,
+                     p->spb, p->f3, reinterpret_cast<const cpx<double>*>(p->d_tw3), T);
+  ADDFT_LAUNCH_CHECK();
+  auto tw1 = reinterpret_cast<const cpx<double>*>(p->d_tw1);
+  auto tw2 = reinterpret_cast<const cpx<double>*>(p->d_tw2);
+#define ADDFT_PR(E)                                                                                  \
+  hipLaunchKernelGGL((k_plane_to_real<double, E>), dim3(p->n3), dim3(kPThreads), p->lds_plane, st, T, \
+                     p->nst, p->nra, p->d_srow, p->d_scol, p->d_rowval, p->f1, p->f2, tw1, tw2, scale, \
+                     p->cchunk, grid)
+  switch (p->ept_p) {
+    case 4: ADDFT_PR(4); break;
+    case 8: ADDFT_PR(8); break;
+    case 16: ADDFT_PR(16); break;
+    default: ADDFT_PR(32); break;
+  }
+#undef ADDFT_PR
+  ADDFT_LAUNCH_CHECK();
+  return ADDFT_OK;
+}
+
+int addft_real_to_sphere(addft_sphere_plan p, const double* grid, double scale, double* c, void* work,
+                         void* stream) {
+  ADDFT_CHECK_ARG(p && c && grid && work, "null argument");
+  ADDFT_CHECK_ARG(p->precision == ADDFT_C128, "real_to_sphere needs a C128 plan");
+  hipStream_t st = addft::as_stream(stream);
+  cpx<double>* T = reinterpret_cast<cpx<double>*>(work);
+  auto tw1 = reinterpret_cast<const cpx<double>*>(p->d_tw1);
+  auto tw2 = reinterpret_cast<const cpx<double>*>(p->d_tw2);
+#define ADDFT_RP(E)                                                                                  \
+  hipLaunchKernelGGL((k_real_to_plane<double, E>), dim3(p->n3), dim3(kPThreads), p->lds_plane, st, grid, \
+                     p->nst, p->nra, p->d_srow, p->d_scol, p->d_rowval, p->f1, p->f2, tw1, tw2,       \
+                     p->cchunk, T)
+  switch (p->ept_p) {
+    case 4: ADDFT_RP(4); break;
+    case 8: ADDFT_RP(8); break;
+    case 16: ADDFT_RP(16); break;
+    default: ADDFT_RP(32); break;
+  }
+#undef ADDFT_RP
+  ADDFT_LAUNCH_CHECK();
+  const size_t zlds = (size_t)p->spb * p->n3 * sizeof(cpx<double>);
+  dim3 zg((unsigned)((p->nst + p->spb - 1) / p->spb), 1u);
+  hipLaunchKernelGGL((k_z_forward<double, double>), zg, dim3(kZThreads), zlds, st, T, p->ng, p->d_soff,
+                     p->d_sid, p->d_iz, p->nst, p->spb, p->f3, reinterpret_cast<const cpx<double>*>(p->d_tw3),
+                     scale, (const double*)nullptr, (const cpx<double>*)nullptr,
+                     reinterpret_cast<cpx<double>*>(c));
+  ADDFT_LAUNCH_CHECK();
+  return ADDFT_OK;
+}
+
+}  // extern "C"

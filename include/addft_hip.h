@@ -1,0 +1,147 @@
+/*
+ * addft_hip.h -- C ABI of the MI355X (gfx950) Kohn-Sham Hpsi / density /
+ * adjoint kernels.
+ *
+ * Conventions
+ *   - All pointers named d_* are device pointers; h_* are host pointers.
+ *   - Dimensions are 64-bit.  Every call takes a hipStream_t (passed as
+ *     void*) and is stream ordered; no call synchronises the host except
+ *     the plan constructors.
+ *   - Return value: ADDFT_OK (0) or a negative ADDFT_E* code;
+ *     addft_last_error() returns a thread-local message for the last
+ *     failure on the calling thread.
+ *   - complex data are interleaved (re, im) pairs of float (c64) or
+ *     double (c128).
+ *   - A "sphere plan" owns device copies of the index maps for one G-sphere
+ *     on one FFT box: the reference-order flat index, the stick tables and
+ *     the FFT twiddles.  Plans are immutable after creation and may be used
+ *     from several streams at once.  Workspace is supplied by the caller.
+ *
+ * Coefficient orders
+ *   - "reference order": the lexsort(g2, m0, m1, m2) order of the
+ *     reference's frozen sphere (geometry/basis.py:89-121).
+ *   - "stick order": the plan's internal order, sorted by FFT column
+ *     (i1, i2) and then i3.  Hpsi, density and the nonlocal kernels work in
+ *     stick order; addft_sphere_permute converts between the two.
+ *
+ * Real-space grids
+ *   - "row-major": the reference layout, index (i1*n2 + i2)*n3 + i3.
+ *   - "plane-major": index (i3*n1 + i1)*n2 + i2, used internally so that
+ *     one z-plane is contiguous.  addft_grid_transpose converts.
+ */
+#ifndef ADDFT_HIP_H
+#define ADDFT_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  ADDFT_OK = 0,
+  ADDFT_EINVAL = -1,    /* invalid argument or shape mismatch */
+  ADDFT_EINTERNAL = -2, /* HIP runtime / launch failure */
+  ADDFT_EUNSUPPORTED = -3
+};
+
+enum { ADDFT_C64 = 0, ADDFT_C128 = 1 };
+
+typedef struct addft_sphere_plan_s* addft_sphere_plan;
+
+const char* addft_last_error(void);
+int addft_version(void);
+
+/* ------------------------------------------------------------ pack
+ * Zero-filled scatter (batch, nG) -> (batch, nfft) and gather back, in
+ * reference order, with the int32 flat index of geometry/basis.py:196-202.
+ * Replaces reference native/pack.cu:16-38,105-127 (FFI targets
+ * addft_scatter_c128 / addft_gather_c128 / _c64, pack.cu:88-100,177-189).
+ * 64-bit offsets; errors: nG/nfft <= 0.  Indices outside [0, nfft) are
+ * skipped (scatter) or read as 0 (gather) instead of faulting.          */
+int addft_pack_scatter(int dtype, const void* d_coeffs, const int32_t* d_idx,
+                       int64_t batch, int64_t ng, int64_t nfft, void* d_grid,
+                       void* stream);
+int addft_pack_gather(int dtype, const void* d_grid, const int32_t* d_idx,
+                      int64_t batch, int64_t ng, int64_t nfft, void* d_coeffs,
+                      void* stream);
+
+/* ------------------------------------------------------------ LDA XC
+ * Per point n = max(rho, 1e-14): v_xc = d(n eps_xc)/dn (Dirac + PZ81),
+ * and the raw sum  sum_i n_i eps_xc(n_i)  written to *d_exc (one double),
+ * reduced deterministically (per-block partials, fixed-order second pass).
+ * Replaces reference native/xc_lda.cu:28-75 (FFI addft_xc_lda).
+ * d_work: at least addft_xc_work_bytes(n) bytes.  d_exc may be NULL.   */
+int64_t addft_xc_work_bytes(int64_t n);
+int addft_xc_lda(const double* d_rho, int64_t n, double* d_vxc, double* d_exc,
+                 void* d_work, void* stream);
+/* f_xc = d^2(n eps)/dn^2 times an optional vector: d_out = fxc * d_w (d_w
+ * NULL -> fxc).  The backward of v_xc (reference xc.py:195-210).       */
+int addft_xc_lda_fxc(const double* d_rho, const double* d_w, int64_t n,
+                     double* d_out, void* stream);
+
+/* ------------------------------------------------------------ sphere plan
+ * h_miller: (ng, 3) int32 host array of integer G in reference order.
+ * precision: ADDFT_C64 or ADDFT_C128 (of the Hpsi / density transforms).
+ * Errors: box dims not 2*3*5-smooth, ng <= 0, duplicate columns.   */
+int addft_sphere_plan_create(int n1, int n2, int n3, int64_t ng,
+                             const int32_t* h_miller, int precision,
+                             addft_sphere_plan* out);
+int addft_sphere_plan_destroy(addft_sphere_plan plan);
+/* Query: number of sticks; host copy of the stick-order permutation
+ * (h_perm[p] = reference index of stick-order position p), may be NULL. */
+int addft_sphere_plan_info(addft_sphere_plan plan, int64_t* n_sticks,
+                           int64_t* n_rows, int32_t* h_perm);
+/* Bytes of workspace for Hpsi / density / transforms on nb bands.   */
+int64_t addft_sphere_work_bytes(addft_sphere_plan plan, int64_t nb);
+
+/* Permute (nb, ng) complex coefficients: direction 0 reference->stick,
+ * 1 stick->reference.  dtype ADDFT_C64/C128 (may differ from the plan's). */
+int addft_sphere_permute(addft_sphere_plan plan, int dtype, int direction,
+                         const void* d_in, int64_t nb, void* d_out, void* stream);
+
+/* Real grid layout change: direction 0 row-major -> plane-major,
+ * 1 plane-major -> row-major.  double in, double out.              */
+int addft_grid_transpose(int n1, int n2, int n3, int direction, const double* d_in,
+                         double* d_out, void* stream);
+/* plane-major double grid -> plane-major grid in the plan's precision,
+ * times scale (used to stage V(r)/N for Hpsi).                      */
+int addft_grid_stage(addft_sphere_plan plan, const double* d_in, double scale,
+                     void* d_out, void* stream);
+
+/* ------------------------------------------------------------ local Hpsi
+ * Fused  out = T .* psi + (1/N) FFT( v .* IFFT( psi ) )  on the sphere:
+ * sticks (z pass) -> z-planes in LDS (row pass, column pass with the V(r)
+ * multiply, and back) -> sticks -> sphere, with the kinetic epilogue.
+ * Equals reference ops/hpsi.py:18-32 (kinetic_apply + potential_apply),
+ * psi_to_real / real_to_psi conventions of geometry/basis.py:237-254.
+ * psi, out: (nb, ng) stick order, plan precision.  d_tkin: (ng) real in
+ * plan precision (stick order).  d_vstaged: output of addft_grid_stage
+ * with scale 1/N.  d_work: addft_sphere_work_bytes(plan, nb).
+ * d_out may not alias d_psi.                                          */
+int addft_hpsi_local(addft_sphere_plan plan, const void* d_psi, const void* d_tkin,
+                     const void* d_vstaged, int64_t nb, void* d_out, void* d_work,
+                     void* stream);
+
+/* Density accumulation rho(r) += sum_b w_b |IFFT(psi_b)(r)|^2 into a
+ * plane-major double grid (reference physics/density.py:18-23 with
+ * w_b = w_k f_b g_s / Omega).  d_w: (nb) double.  accumulate 0 overwrites.
+ * Deterministic: bands are summed in a fixed order.                  */
+int addft_density_accumulate(addft_sphere_plan plan, const void* d_psi,
+                             const double* d_w, int64_t nb, double* d_rho,
+                             int accumulate, void* d_work, void* stream);
+
+/* Sphere <-> full real grid (plan precision internally, double I/O):
+ *   s2r: grid(r) = scale * Re sum_G c_G e^{iG.r}   (plane-major, double)
+ *        c: (ng) complex double, stick order.
+ *   r2s: c_G = scale * sum_r f(r) e^{-iG.r}        (stick order)
+ * Used for Hartree (psi-sphere), V_loc (rho-sphere), Kerker.          */
+int addft_sphere_to_real(addft_sphere_plan plan, const double* d_c, double scale,
+                         double* d_grid, void* d_work, void* stream);
+int addft_real_to_sphere(addft_sphere_plan plan, const double* d_grid, double scale,
+                         double* d_c, void* d_work, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
