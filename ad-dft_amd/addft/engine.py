@@ -1,0 +1,461 @@
+"""Device engine: everything the SCF, energy, forces and stress need on one GPU.
+
+One `Engine` per (Static, device).  It owns the sphere plans, the per-k
+G-vector tables, the projector matrices and V_loc(r) for the current
+geometry, and exposes the operators the solvers use:
+
+  effective_potential(rho)  v_loc + v_H + v_xc   (reference solver/scf.py:81-92)
+  hpsi(ik, psi, vs)         T + V + V_NL          (reference ops/hpsi.py:35-50)
+  density(psis, occ)        rho(r)                (reference physics/density.py:18-23)
+  energy(...)               variational / explicit energies (observables/energy.py)
+  backward(...)             envelope dE/dR and dE/deps (api.py:103-113)
+
+Layouts: coefficients in the plan's stick order, real-space grids
+plane-major (i3, i1, i2).  Conversions to the reference's orders happen only
+at the API boundary.  All numerical work runs in the HIP library; torch is
+used for device memory, the library GEMMs (rocBLAS) of the nonlocal and
+subspace products, small dense linear algebra, and elementwise glue.
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import torch
+from scipy.special import erf
+
+from .geometry.cell import Cell, positions, reciprocal, volume
+from .geometry.psp import lm_table
+from .kernels import _lib
+from .kernels import native
+from .kernels._lib import check, lib, ptr, stream
+
+
+def _simpson_weights(n):
+    """psp.py:66-70."""
+    w = np.ones((n,))
+    w[1:n - 1:2] = 4.0
+    w[2:n - 1:2] = 2.0
+    return w / 3.0
+
+
+class Engine:
+    def __init__(self, static, device=None):
+        self.st = static
+        self.dev = torch.device("cuda") if device is None else torch.device(device)
+        b = static.basis
+        self.shape = tuple(b.fft_shape)
+        self.nfft = int(np.prod(self.shape))
+        self.ng = b.nG
+        self.nk = int(static.kpts.shape[0])
+        self.nat = int(static.spec_idx.shape[0])
+        self.mixed = static.precision == "mixed"
+        with torch.cuda.device(self.dev):
+            self.plan64 = native.SpherePlan(self.shape, b.miller, torch.complex128, self.dev)
+            self.plan32 = native.SpherePlan(self.shape, b.miller, torch.complex64, self.dev) if self.mixed else None
+            self.plan_rho = native.SpherePlan(self.shape, b.miller_rho, torch.complex128, self.dev)
+        self.perm = self.plan64.perm
+        self.iperm = self.plan64.iperm
+        self.perm_d = torch.as_tensor(self.perm, device=self.dev)
+        self.iperm_d = torch.as_tensor(self.iperm.astype(np.int64), device=self.dev)
+        self.miller_d = torch.as_tensor(np.ascontiguousarray(b.miller, dtype=np.int32), device=self.dev)
+        self.miller_rho_d = torch.as_tensor(np.ascontiguousarray(b.miller_rho, dtype=np.int32), device=self.dev)
+        self.perm_rho_d = torch.as_tensor(self.plan_rho.perm, device=self.dev)
+        self._prep_psp()
+        self._work = {}
+        self.geo = None
+
+    # ------------------------------------------------------------- helpers
+    def work(self, key, nbytes):
+        w = self._work.get(key)
+        if w is None or w.numel() < nbytes:
+            w = torch.empty(int(nbytes), dtype=torch.uint8, device=self.dev)
+            self._work[key] = w
+        return w
+
+    def _f64(self, a):
+        return torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64), device=self.dev)
+
+    def _i32(self, a):
+        return torch.as_tensor(np.ascontiguousarray(a, dtype=np.int32), device=self.dev)
+
+    def _prep_psp(self):
+        psp, s = self.st.psp, int(self.st.spec_idx[0])
+        self.kind = int(psp.kind)
+        self.nrad = int(psp.nrad)
+        self.l_d = self._i32(psp.proj_l[s])
+        self.zval = float(psp.zval[s])
+        if self.kind == 0:
+            self.pw_d = self._f64(psp.proj_w[s])
+            self.pb_d = self._f64(psp.proj_b[s])
+            self.mesh = 0
+            self.r_d = self.f_d = self.aux_d = None
+            self.vamp, self.vsig, self.g0 = float(psp.vloc_amp[s]), float(psp.vloc_sigma[s]), 0.0
+        else:
+            r, rab = psp.r[s], psp.rab[s]
+            w = _simpson_weights(r.shape[0])
+            v = psp.vloc_at[s] / 2.0
+            z = self.zval
+            self.mesh = int(r.shape[0])
+            self.r_d = self._f64(r)
+            self.f_d = self._f64(np.stack([w * rab * psp.betar[s][ib] * r for ib in range(self.nrad)]))
+            self.aux_d = self._f64(w * rab * (r * v + z * erf(r)))
+            self.g0 = float(4.0 * np.pi * np.sum(w * rab * r * (r * v + z)))
+            self.pw_d = self.pb_d = None
+            self.vamp = self.vsig = 0.0
+        rows = []
+        for i in range(self.nat):
+            for ib, l, m in lm_table(psp.proj_l[s]):
+                rows.append((i, ib, l * l + m))
+        rows = np.asarray(rows, dtype=np.int32).reshape(-1, 3)
+        self.nproj = rows.shape[0]
+        self.nh = int(psp.nh[s])
+        self.patom_d = self._i32(rows[:, 0])
+        self.prad_d = self._i32(rows[:, 1])
+        self.plm_d = self._i32(rows[:, 2])
+        self.patom_l = torch.as_tensor(rows[:, 0].astype(np.int64), device=self.dev)
+        # block-diagonal D laid out (i, p, i, q) (nonlocal_op.py:25-42)
+        lm = lm_table(psp.proj_l[s])
+        dblk = np.zeros((self.nh, self.nh))
+        for p, (ib, l, m) in enumerate(lm):
+            for q, (jb, l2, m2) in enumerate(lm):
+                if l == l2 and m == m2:
+                    dblk[p, q] = psp.dion[s, ib, jb]
+        self.dblk = dblk
+        D = np.kron(np.eye(self.nat), dblk)
+        self.D64 = torch.as_tensor(D, dtype=torch.complex128, device=self.dev)
+        self.D32 = self.D64.to(torch.complex64)
+        self.z_d = self._f64(np.asarray(self.st.psp.zval)[self.st.spec_idx])
+
+    # ------------------------------------------------------------ geometry
+    def _radial(self, qn, n):
+        beta = torch.empty((self.nrad, n), dtype=torch.float64, device=self.dev)
+        dbeta = torch.empty_like(beta)
+        check(lib().addft_beta_table(self.kind, self.nrad, ptr(self.l_d), ptr(self.pw_d), ptr(self.pb_d),
+                                     self.mesh, ptr(self.r_d), ptr(self.f_d), ptr(qn), n, ptr(beta), ptr(dbeta),
+                                     stream()), "addft_beta_table")
+        return beta, dbeta
+
+    def set_geometry(self, cell: Cell):
+        """Build G tables, projectors and V_loc(r) for this geometry."""
+        self.cell = cell
+        lat = np.asarray(cell.lattice, dtype=np.float64)
+        self.lattice = lat
+        self.omega = volume(lat)
+        self.bmat = reciprocal(lat)
+        self.pos = positions(cell)
+        self.bmat_d = self._f64(self.bmat)
+        self.lat_d = self._f64(lat)
+        self.pos_d = self._f64(self.pos)
+        L = lib()
+        ng = self.ng
+        # psi-sphere, k = 0 (Hartree)
+        self.g0_d = torch.empty((ng, 3), dtype=torch.float64, device=self.dev)
+        qn0 = torch.empty(ng, dtype=torch.float64, device=self.dev)
+        check(L.addft_gvectors(ptr(self.miller_d), ptr(self.perm_d), ng, ptr(self.bmat_d), None, ptr(self.g0_d),
+                               None, ptr(qn0), stream()), "addft_gvectors")
+        self.kq, self.kqn, self.tkin64, self.tkin32, self.beta, self.dbeta = [], [], [], [], [], []
+        self.P64, self.P32 = [], []
+        isq = 1.0 / math.sqrt(self.omega)
+        for k in self.st.kpts:
+            kk = np.ascontiguousarray(k, dtype=np.float64)
+            g = torch.empty((ng, 3), dtype=torch.float64, device=self.dev)
+            q = torch.empty_like(g)
+            qn = torch.empty(ng, dtype=torch.float64, device=self.dev)
+            check(L.addft_gvectors(ptr(self.miller_d), ptr(self.perm_d), ng, ptr(self.bmat_d), kk.ctypes.data,
+                                   ptr(g), ptr(q), ptr(qn), stream()), "addft_gvectors")
+            t = 0.5 * (q * q).sum(dim=1)
+            beta, dbeta = self._radial(qn, ng)
+            P = torch.empty((self.nproj, ng), dtype=torch.complex128, device=self.dev)
+            check(L.addft_projectors(_lib.C128, self.nproj, ng, ptr(self.patom_d), ptr(self.prad_d),
+                                     ptr(self.plm_d), ptr(q), ptr(beta), ptr(self.pos_d), isq, ptr(P), stream()),
+                  "addft_projectors")
+            self.kq.append(q)
+            self.kqn.append(qn)
+            self.tkin64.append(t.contiguous())
+            self.tkin32.append(t.to(torch.float32).contiguous() if self.mixed else None)
+            self.beta.append(beta)
+            self.dbeta.append(dbeta)
+            self.P64.append(P)
+            self.P32.append(P.to(torch.complex64) if self.mixed else None)
+        # local potential on the density sphere
+        ngr = self.st.basis.nG_rho
+        self.grho_d = torch.empty((ngr, 3), dtype=torch.float64, device=self.dev)
+        self.qnrho_d = torch.empty(ngr, dtype=torch.float64, device=self.dev)
+        check(L.addft_gvectors(ptr(self.miller_rho_d), ptr(self.perm_rho_d), ngr, ptr(self.bmat_d), None,
+                               ptr(self.grho_d), None, ptr(self.qnrho_d), stream()), "addft_gvectors")
+        self.vq_d = torch.empty(ngr, dtype=torch.float64, device=self.dev)
+        self.dvq_d = torch.empty_like(self.vq_d)
+        check(L.addft_vloc_table(self.kind, self.vamp, self.vsig, self.zval, self.g0, self.mesh, ptr(self.r_d),
+                                 ptr(self.aux_d), ptr(self.qnrho_d), ngr, ptr(self.vq_d), ptr(self.dvq_d), stream()),
+              "addft_vloc_table")
+        vg = torch.empty(ngr, dtype=torch.complex128, device=self.dev)
+        check(L.addft_vloc_g(ngr, self.nat, ptr(self.grho_d), ptr(self.vq_d), ptr(self.pos_d), 1.0 / self.omega,
+                             ptr(vg), stream()), "addft_vloc_g")
+        self.vloc_pl = self.plan_rho.to_real(vg, 1.0)
+        self.ewald_cache = None
+        self.geo = cell
+
+    # ---------------------------------------------------- potentials
+    def hartree(self, rho_pl, want_strain=False):
+        """(v_H planar, E_H, strain sums) on the psi-sphere (hartree.py:21-48)."""
+        n_g = self.plan64.to_sphere(rho_pl, 1.0 / self.nfft)
+        vh_g = torch.empty_like(n_g)
+        out = torch.empty(10, dtype=torch.float64, device=self.dev)
+        check(lib().addft_hartree_g(self.ng, ptr(self.g0_d), ptr(n_g), ptr(vh_g), ptr(out),
+                                    ptr(self.work("red", 64 * 10 * 8 + 256)), stream()), "addft_hartree_g")
+        v_h = self.plan64.to_real(vh_g, 1.0)
+        return v_h, out
+
+    def effective_potential(self, rho_pl):
+        v_h, hout = self.hartree(rho_pl)
+        vxc, exc = native.xc_lda(rho_pl)
+        v = self.vloc_pl + v_h + vxc
+        return v, {"v_h": v_h, "v_xc": vxc, "E_H": 0.5 * self.omega * hout[0], "exc_sum": exc}
+
+    def stage(self, v_pl, mixed_stage=False):
+        plan = self.plan32 if mixed_stage else self.plan64
+        return plan.stage(v_pl, 1.0 / self.nfft)
+
+    # ---------------------------------------------------------- operators
+    def hpsi(self, ik, psi, vs, fp32=False):
+        """T + V_loc,eff + V_NL on (nb, nG) stick-order psi."""
+        plan = self.plan32 if fp32 else self.plan64
+        tk = self.tkin32[ik] if fp32 else self.tkin64[ik]
+        P = self.P32[ik] if fp32 else self.P64[ik]
+        D = self.D32 if fp32 else self.D64
+        out = plan.hpsi_local(psi, tk, vs)
+        becp_t = torch.matmul(psi, P.conj().T)          # (nb, nproj) = <beta|psi>^T
+        out.add_(torch.matmul(torch.matmul(becp_t, D), P))
+        return out
+
+    def becp(self, ik, psi):
+        """(nproj, nb) <beta|psi> in FP64 (nonlocal_op.py:67-69)."""
+        return torch.matmul(self.P64[ik].conj(), psi.to(torch.complex128).T)
+
+    def density(self, psis, occ, fp32=False):
+        """rho = sum_k w_k sum_n f g_s |psi(r)|^2, plane-major (density.py:18-23)."""
+        plan = self.plan32 if fp32 else self.plan64
+        rho = None
+        for ik in range(self.nk):
+            w = (self.st.kweights[ik] * self.st.spin_degeneracy / self.omega) * occ[ik]
+            psi = psis[ik].to(plan.cdtype) if psis[ik].dtype != plan.cdtype else psis[ik]
+            rho = plan.density(psi, w.contiguous(), out=rho, accumulate=ik > 0)
+        return rho
+
+    def occupations(self, eig):
+        """(f, mu) for eig (nk, nb) float64 (occupation.py:25-49)."""
+        st = self.st
+        if st.sigma < 1e-6:
+            return self._integer_occupation(eig), 0.0
+        occ = torch.empty_like(eig)
+        mu = torch.empty((), dtype=torch.float64, device=self.dev)
+        w = self._f64(st.kweights)
+        check(lib().addft_occupations(ptr(eig.contiguous()), ptr(w), self.nk, eig.shape[1], st.nelec, st.sigma,
+                                      st.spin_degeneracy, ptr(occ), ptr(mu), stream()), "addft_occupations")
+        return occ, mu
+
+    def _integer_occupation(self, eig):
+        """occupation.py:52-65 (T = 0 fill)."""
+        st = self.st
+        neff = st.nelec / st.spin_degeneracy
+        flat_e = eig.reshape(-1)
+        flat_w = self._f64(st.kweights)[:, None].expand_as(eig).reshape(-1)
+        order = torch.argsort(flat_e, stable=True)
+        cum = torch.cumsum(flat_w[order], 0)
+        filled = cum <= neff + 1e-12
+        last = int(torch.argmax((cum > neff - 1e-12).to(torch.int32)).item())
+        wl = flat_w[order][last]
+        frac = torch.clamp((neff - (cum[last] - wl)) / wl, 0.0, 1.0)
+        vals = torch.where(filled, 1.0, 0.0).to(torch.float64)
+        if not bool(filled[last]):
+            vals[last] = frac
+        out = torch.zeros_like(flat_w)
+        out[order] = vals
+        return out.reshape(eig.shape)
+
+    def entropy(self, occ):
+        """S~ = -sum w [f ln f + (1-f) ln(1-f)] (occupation.py:68-72)."""
+        f = torch.clamp(occ, 1e-14, 1.0 - 1e-14)
+        s = -(f * torch.log(f) + (1.0 - f) * torch.log(1.0 - f))
+        return float((self._f64(self.st.kweights)[:, None] * s).sum())
+
+    def kerker(self, rho_in, rho_out, alpha, gg0):
+        out = torch.empty_like(rho_in)
+        n1, n2, n3 = self.shape
+        wb = 16 * (self.nfft + n1 + n2 + n3) + 256
+        check(lib().addft_kerker_mix(n1, n2, n3, ptr(self.bmat_d), ptr(rho_in), ptr(rho_out), float(alpha),
+                                     float(gg0), ptr(out), ptr(self.work("kerker", wb)), stream()),
+              "addft_kerker_mix")
+        return out
+
+    # -------------------------------------------------------------- Ewald
+    def ewald(self):
+        """(E_II, dE/dR, dE/deps) with the reference truncation (ewald.py)."""
+        if self.ewald_cache is not None:
+            return self.ewald_cache
+        nmax, eta = 8, self.st.eta_ewald
+        nat = self.nat
+        out = torch.empty(16 * nat + 10, dtype=torch.float64, device=self.dev)
+        wb = int(lib().addft_ewald_work_bytes(nat, nmax))
+        check(lib().addft_ewald(nat, ptr(self.pos_d), ptr(self.z_d), ptr(self.lat_d), ptr(self.bmat_d),
+                                self.omega, eta, nmax, ptr(out), ptr(self.work("ewald", wb)), stream()),
+              "addft_ewald")
+        o = out.cpu().numpy()
+        real = o[:13 * nat].reshape(nat, 13)
+        rec = o[13 * nat:13 * nat + 10]
+        rf = o[13 * nat + 10:].reshape(nat, 3)
+        z = np.asarray(self.st.psp.zval)[self.st.spec_idx]
+        e_self = -(eta / math.sqrt(math.pi)) * float(np.sum(z * z))
+        e_bg = -(math.pi / (2.0 * self.omega * eta ** 2)) * float(np.sum(z)) ** 2
+        e = float(real[:, 0].sum()) + rec[0] + e_self + e_bg
+        dR = real[:, 1:4] + rf
+        de = real[:, 4:13].sum(axis=0).reshape(3, 3) + rec[1:].reshape(3, 3) - np.eye(3) * (rec[0] + e_bg)
+        self.ewald_cache = (e, dR, de)
+        return self.ewald_cache
+
+    # -------------------------------------------------------------- energy
+    def energy_parts(self, psis, occ, rho_pl, rho_psi=None):
+        """Variational energy pieces (energy.py:100-169) with psi and the input
+        density rho; rho_psi is the density rebuilt from psi."""
+        st = self.st
+        if rho_psi is None:
+            rho_psi = self.density(psis, occ)
+        dv = self.omega / self.nfft
+        e_kin = 0.0
+        e_nl = 0.0
+        for ik in range(self.nk):
+            psi = psis[ik].to(torch.complex128)
+            wf = st.kweights[ik] * st.spin_degeneracy * occ[ik]
+            e_kin += float((wf * ((psi.abs() ** 2) @ self.tkin64[ik])).sum())
+            b = self.becp(ik, psi)
+            e_nl += float((wf * (b.conj() * (self.D64 @ b)).real.sum(dim=0)).sum())
+        v_h, hout = self.hartree(rho_pl)
+        vxc, exc = native.xc_lda(rho_pl)
+        e_h = 0.5 * self.omega * float(hout[0])
+        e_xc = dv * float(exc)
+        e_loc_psi = dv * float((self.vloc_pl * rho_psi).sum())
+        e_vh_psi = dv * float((v_h * rho_psi).sum())
+        e_ii = self.ewald()[0]
+        e_var = e_kin + e_nl + e_loc_psi + e_vh_psi - e_h + e_xc + e_ii
+        s = self.entropy(occ)
+        return {"kinetic": e_kin, "nonlocal": e_nl, "local": e_loc_psi, "hartree": e_h, "xc": e_xc,
+                "ewald": e_ii, "E_var": e_var, "entropy": s,
+                "F": e_var - st.sigma * st.spin_degeneracy * s, "rho_psi": rho_psi, "v_h_psi": e_vh_psi}
+
+    def energy_terms(self, psis, occ, rho_pl, active):
+        """Tagged decomposition (energy.py:36-71) on the device."""
+        st = self.st
+        dv = self.omega / self.nfft
+        rho_psi = self.density(psis, occ)
+        out = {}
+        if "kinetic" in active or "nonlocal" in active:
+            e_kin = e_nl = 0.0
+            for ik in range(self.nk):
+                psi = psis[ik].to(torch.complex128)
+                wf = st.kweights[ik] * st.spin_degeneracy * occ[ik]
+                e_kin += float((wf * ((psi.abs() ** 2) @ self.tkin64[ik])).sum())
+                b = self.becp(ik, psi)
+                e_nl += float((wf * (b.conj() * (self.D64 @ b)).real.sum(dim=0)).sum())
+            if "kinetic" in active:
+                out["kinetic"] = e_kin
+            if "nonlocal" in active:
+                out["nonlocal"] = e_nl
+        if "local" in active:
+            out["local"] = dv * float((self.vloc_pl * rho_psi).sum())
+        if "hartree" in active:
+            out["hartree"] = 0.5 * self.omega * float(self.hartree(rho_pl)[1][0])
+        if "exchange" in active or "correlation" in active:
+            n = torch.clamp(rho_pl, min=1e-14)
+            ex = -0.75 * torch.pow(3.0 * n / math.pi, 1.0 / 3.0)
+            _, exc = native.xc_lda(rho_pl)
+            e_x = dv * float((n * ex).sum())
+            if "exchange" in active:
+                out["exchange"] = e_x
+            if "correlation" in active:
+                out["correlation"] = dv * float(exc) - e_x
+        if "ewald" in active:
+            out["ewald"] = self.ewald()[0]
+        return out
+
+    # ------------------------------------------------------------ backward
+    def backward(self, psis, occ, rho_psi):
+        """Envelope dE/dR (nat,3) and dE/deps (3,3) with psi, f frozen and
+        rho = rho(geo, psi) (api.py:103-113).  Returns also per-term parts."""
+        st = self.st
+        L = lib()
+        nat = self.nat
+        parts = {}
+        red = self.work("red", 64 * 10 * 8 + 256)
+        # kinetic: -sum_G dens_G q_m G_n
+        ek = np.zeros((3, 3))
+        out10 = torch.empty(10, dtype=torch.float64, device=self.dev)
+        for ik in range(self.nk):
+            wf = st.kweights[ik] * st.spin_degeneracy * occ[ik]
+            dens = (wf[:, None] * psis[ik].to(torch.complex128).abs() ** 2).sum(dim=0).contiguous()
+            g = self.g0_d
+            check(L.addft_gsum_outer(self.ng, ptr(dens), ptr(self.kq[ik]), ptr(g), ptr(out10), ptr(red), stream()),
+                  "addft_gsum_outer")
+            ek -= out10[1:].cpu().numpy().reshape(3, 3)
+        parts["kinetic"] = (np.zeros((nat, 3)), ek)
+        # local on the rho-sphere
+        rt = self.plan_rho.to_sphere(rho_psi, 1.0 / self.nfft)
+        lout = torch.empty(3 * nat + 10, dtype=torch.float64, device=self.dev)
+        wb = int(L.addft_backward_work_bytes(max(nat, 1)))
+        check(L.addft_local_backward(st.basis.nG_rho, nat, ptr(self.grho_d), ptr(self.qnrho_d), ptr(self.vq_d),
+                                     ptr(self.dvq_d), ptr(rt), ptr(self.pos_d), ptr(lout),
+                                     ptr(self.work("bwd", wb)), stream()), "addft_local_backward")
+        lo = lout.cpu().numpy()
+        e_loc = lo[3 * nat]
+        parts["local"] = (lo[:3 * nat].reshape(nat, 3), -np.eye(3) * e_loc + lo[3 * nat + 1:].reshape(3, 3))
+        # Hartree strain (psi-sphere)
+        _, hout = self.hartree(rho_psi)
+        ho = hout.cpu().numpy()
+        e_h = 0.5 * self.omega * ho[0]
+        parts["hartree"] = (np.zeros((nat, 3)), -np.eye(3) * e_h + 0.5 * self.omega * ho[1:].reshape(3, 3))
+        # XC strain
+        vxc, exc = native.xc_lda(rho_psi)
+        dv = self.omega / self.nfft
+        e_xc = dv * float(exc)
+        parts["xc"] = (np.zeros((nat, 3)), np.eye(3) * (e_xc - dv * float((vxc * rho_psi).sum())))
+        # nonlocal
+        fn = torch.zeros((nat, 3), dtype=torch.float64, device=self.dev)
+        en = np.zeros((3, 3))
+        isq = 1.0 / math.sqrt(self.omega)
+        nout = torch.empty((self.nproj, 12), dtype=torch.float64, device=self.dev)
+        wbn = int(L.addft_backward_work_bytes(self.nproj))
+        for ik in range(self.nk):
+            psi = psis[ik].to(torch.complex128)
+            wf = (st.kweights[ik] * st.spin_degeneracy) * occ[ik]
+            b = self.becp(ik, psi)                          # (nproj, nb)
+            W = (self.D64 @ b).conj() * wf[None, :]
+            Y = torch.matmul(W, psi).contiguous()           # (nproj, nG)
+            kk = np.ascontiguousarray(self.st.kpts[ik], dtype=np.float64)
+            check(L.addft_nonlocal_backward(_lib.C128, self.nproj, self.ng, ptr(self.patom_d), ptr(self.prad_d),
+                                            ptr(self.plm_d), ptr(self.g0_d), ptr(self.kq[ik]), ptr(self.kqn[ik]),
+                                            ptr(self.beta[ik]), ptr(self.dbeta[ik]), ptr(self.pos_d), kk.ctypes.data,
+                                            isq, ptr(Y), ptr(nout), ptr(self.work("bwd_nl", wbn)), stream()),
+                  "addft_nonlocal_backward")
+            fn.index_add_(0, self.patom_l, nout[:, :3])
+            en += nout[:, 3:].sum(dim=0).cpu().numpy().reshape(3, 3)
+        parts["nonlocal"] = (fn.cpu().numpy(), en)
+        e, dR, de = self.ewald()
+        parts["ewald"] = (dR, de)
+        dR_tot = sum(v[0] for v in parts.values())
+        de_tot = sum(v[1] for v in parts.values())
+        return dR_tot, de_tot, parts
+
+    # ---------------------------------------------------------- orders
+    def to_stick(self, psi_ref):
+        """(nb, nG) reference order -> stick order."""
+        return psi_ref[..., self.perm_d.long()].contiguous()
+
+    def to_reference(self, psi_st):
+        return psi_st[..., self.iperm_d].contiguous()
+
+    def to_planar(self, rho_rowmajor):
+        return native.to_planar(rho_rowmajor.contiguous(), self.shape)
+
+    def to_rowmajor(self, rho_pl):
+        return native.from_planar(rho_pl.contiguous(), self.shape)

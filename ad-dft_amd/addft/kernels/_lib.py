@@ -57,6 +57,21 @@ _SIGS = {
     "addft_density_accumulate": ([_VP, _VP, _VP, _I64, _VP, _I32, _VP, _VP], _I32),
     "addft_sphere_to_real": ([_VP, _VP, _D, _VP, _VP, _VP], _I32),
     "addft_real_to_sphere": ([_VP, _VP, _D, _VP, _VP, _VP], _I32),
+    "addft_gvectors": ([_VP, _VP, _I64, _VP, _VP, _VP, _VP, _VP, _VP], _I32),
+    "addft_beta_table": ([_I32, _I32, _VP, _VP, _VP, _I64, _VP, _VP, _VP, _I64, _VP, _VP, _VP], _I32),
+    "addft_vloc_table": ([_I32, _D, _D, _D, _D, _I64, _VP, _VP, _VP, _I64, _VP, _VP, _VP], _I32),
+    "addft_projectors": ([_I32, _I64, _I64, _VP, _VP, _VP, _VP, _VP, _VP, _D, _VP, _VP], _I32),
+    "addft_vloc_g": ([_I64, _I32, _VP, _VP, _VP, _D, _VP, _VP], _I32),
+    "addft_backward_work_bytes": ([_I64], _I64),
+    "addft_local_backward": ([_I64, _I32, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP], _I32),
+    "addft_nonlocal_backward": ([_I32, _I64, _I64, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _D,
+                                 _VP, _VP, _VP, _VP], _I32),
+    "addft_gsum_outer": ([_I64, _VP, _VP, _VP, _VP, _VP, _VP], _I32),
+    "addft_ewald_work_bytes": ([_I32, _I32], _I64),
+    "addft_ewald": ([_I32, _VP, _VP, _VP, _VP, _D, _D, _I32, _VP, _VP, _VP], _I32),
+    "addft_occupations": ([_VP, _VP, _I32, _I32, _D, _D, _D, _VP, _VP, _VP], _I32),
+    "addft_hartree_g": ([_I64, _VP, _VP, _VP, _VP, _VP, _VP], _I32),
+    "addft_kerker_mix": ([_I32, _I32, _I32, _VP, _VP, _VP, _D, _D, _VP, _VP, _VP], _I32),
 }
 
 

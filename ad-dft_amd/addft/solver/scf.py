@@ -1,0 +1,185 @@
+"""Self-consistent field driver on the device.
+
+Mirrors reference `src/addft/solver/scf.py:200-607`: uniform initial density
+nelec/Omega, the physical map n -> H[n] -> lowest nband -> f -> n', the
+residual sum|dn| Omega/N/N_e, mixing that never enters the derivatives, and
+four unmixed "polish" steps at the end.  The eigen-solve is always the
+matrix-free Davidson (the reference switches to it above nG = 3000; below,
+its dense eigh gives the same fixed point).  Mixers:
+  * Kerker (alpha 0.7, gg0 0.5) for more than two atoms (scf.py:574-581);
+  * simplified Broyden for two atoms or fewer (scf.py:285-304, 431-436);
+  * Pulay/DIIS on request (`mixer="diis"`, solver/mixer.py:23-82).
+The converged fixed point does not depend on the mixer.
+
+Precision: "fp64" runs every Hpsi in FP64.  "mixed" runs Hpsi, the
+nonlocal GEMMs and the subspace products in FP32 until the density residual
+falls below `switch_tol`, then finishes (and polishes) in FP64.
+"""
+
+from __future__ import annotations
+
+import math
+import os
+import time
+
+import numpy as np
+import torch
+
+from ..system import SCFState
+from .davidson import davidson
+
+
+class DIIS:
+    """Pulay mixing with history m and restart on growth (mixer.py:48-82)."""
+
+    def __init__(self, m=8, beta=0.5):
+        self.m, self.beta = m, beta
+        self.rins, self.routs = [], []
+
+    def reset(self):
+        self.rins.clear()
+        self.routs.clear()
+
+    def __call__(self, rho_in, rho_out):
+        self.rins.append(rho_in)
+        self.routs.append(rho_out)
+        if len(self.rins) > self.m:
+            self.rins.pop(0)
+            self.routs.pop(0)
+        if len(self.rins) == 1:
+            return rho_in + self.beta * (rho_out - rho_in)
+        n = len(self.rins)
+        rins, routs = torch.stack(self.rins), torch.stack(self.routs)
+        res = routs - rins
+        Bm = torch.zeros((n + 1, n + 1), dtype=torch.float64, device=rho_in.device)
+        Bm[:n, :n] = res @ res.T
+        Bm[:n, n] = -1.0
+        Bm[n, :n] = -1.0
+        Bm += torch.eye(n + 1, dtype=torch.float64, device=rho_in.device) * 1e-12
+        rhs = torch.zeros(n + 1, dtype=torch.float64, device=rho_in.device)
+        rhs[n] = -1.0
+        c = torch.nan_to_num(torch.linalg.solve(Bm, rhs)[:n], nan=0.0, posinf=0.0, neginf=0.0)
+        new = c @ (rins + self.beta * res)
+        if not bool(torch.isfinite(new).all()):
+            self.reset()
+            return rho_in + self.beta * (rho_out - rho_in)
+        return new
+
+
+class Broyden:
+    """Simplified modified Broyden (scf.py:285-304)."""
+
+    def __init__(self, m=8):
+        self.m = m
+        self.hr, self.hf = [], []
+
+    def reset(self):
+        self.hr, self.hf = [], []
+
+    def __call__(self, rho, rho_out, beta):
+        F = rho_out - rho
+        self.hr = (self.hr + [rho])[-self.m:]
+        self.hf = (self.hf + [F])[-self.m:]
+        n = len(self.hr)
+        if n == 1:
+            return rho + beta * F
+        R, H = torch.stack(self.hf), torch.stack(self.hr)
+        dF, dn = R[:-1] - R[1:], H[:-1] - H[1:]
+        a = dF @ dF.T + 1e-12 * torch.eye(n - 1, dtype=torch.float64, device=rho.device)
+        gam = torch.nan_to_num(torch.linalg.solve(a, dF @ F), nan=0.0, posinf=0.0, neginf=0.0)
+        new = rho + beta * F - gam @ (dn + beta * dF)
+        return new if bool(torch.isfinite(new).all()) else rho + beta * F
+
+
+def initial_guess(engine, nb, dtype):
+    """The nb lowest plane waves of the reference order (scf.py:355-360)."""
+    X = torch.zeros((nb, engine.ng), dtype=dtype, device=engine.dev)
+    rows = torch.arange(nb, device=engine.dev)
+    X[rows, engine.iperm_d[:nb]] = 1.0
+    return X
+
+
+def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, beta=0.5,
+              switch_tol=1e-6, verbose=False, rho0=None, polish=4, dav_max_iter=None, log=None):
+    """Run the SCF on engine's current geometry.  Returns (SCFState with
+    stick-order psi list in `psi` and plane-major rho, info dict)."""
+    st = engine.st
+    nb = st.nband
+    dev = engine.dev
+    if mixer == "auto":
+        mixer = "kerker" if engine.nat > 2 else "broyden"
+    fp32 = engine.mixed
+    cdt = torch.complex64 if fp32 else torch.complex128
+    rho = (torch.full((engine.nfft,), st.nelec / engine.omega, dtype=torch.float64, device=dev)
+           if rho0 is None else rho0.clone())
+    X = [initial_guess(engine, nb, cdt) for _ in range(engine.nk)]
+    diis = DIIS(8, beta) if mixer == "diis" else None
+    broy = Broyden(8) if mixer == "broyden" else None
+    prev = 1e9
+    res = 1.0
+    it = 0
+    t0 = time.perf_counter()
+    hist = []
+    eig = occ = mu = None
+    dav_iters = 0
+
+    def one_map(rho, X, fp32, dav_tol, cold):
+        nonlocal dav_iters
+        v, _ = engine.effective_potential(rho)
+        vs = engine.stage(v, mixed_stage=fp32)
+        vmean = float(v.mean())
+        eigs, Xn = [], []
+        for ik in range(engine.nk):
+            diag = engine.tkin64[ik] + vmean
+            lam, Xk, r, nit = davidson(lambda p: engine.hpsi(ik, p, vs, fp32=fp32), X[ik], nb, diag,
+                                       max_iter=(dav_max_iter or (40 if cold else 25)), tol=dav_tol)
+            dav_iters += nit
+            eigs.append(lam)
+            Xn.append(Xk)
+        eig = torch.stack(eigs).to(torch.float64)
+        occ, mu = engine.occupations(eig)
+        rho_out = engine.density(Xn, occ, fp32=fp32)
+        return rho_out, Xn, eig, occ, mu
+
+    for it in range(1, max_iter + 1):
+        floor = 2e-5 if fp32 else 1e-11
+        dav_tol = max(floor, min(1e-3, 0.1 * res))
+        rho_out, X, eig, occ, mu = one_map(rho, X, fp32, dav_tol, it == 1)
+        res = float((rho_out - rho).abs().sum()) * engine.omega / engine.nfft / st.nelec
+        hist.append(res)
+        if verbose or log:
+            msg = f"  scf {it:3d}  drho={res:.3e}  {'fp32' if fp32 else 'fp64'}  t={time.perf_counter() - t0:.2f}s"
+            (log or print)(msg)
+        if fp32 and res < switch_tol:
+            fp32 = False
+            X = [x.to(torch.complex128) for x in X]
+        if mixer == "diis":
+            if res > prev * 1.5:
+                diis.reset()
+            prev = res
+            rho = diis(rho, rho_out)
+        elif mixer == "broyden":
+            if res > prev * 1.5:
+                broy.reset()
+            prev = res
+            rho = broy(rho, rho_out, 0.9 if it <= 8 else 0.5)
+        else:
+            rho = engine.kerker(rho, rho_out, alpha, gg0)
+        if res < tol and not fp32:
+            break
+    if fp32:
+        X = [x.to(torch.complex128) for x in X]
+    # polish: the unmixed map until the fixed point is strict (scf.py:266-272)
+    for _ in range(polish):
+        rho_new, X, eig, occ, mu = one_map(rho, X, False, 1e-11, False)
+        res = float((rho_new - rho).abs().sum()) * engine.omega / engine.nfft / st.nelec
+        rho = rho_new
+        hist.append(res)
+        if res < 1e-14:
+            break
+    # final eigenvectors of H[rho] (scf.py:274-281)
+    _, X, eig, occ, mu = one_map(rho, X, False, 1e-11, False)
+    info = {"iterations": it, "residual_history": hist, "davidson_iterations": dav_iters,
+            "seconds": time.perf_counter() - t0}
+    state = SCFState(psi=X, eig=eig, occ=occ, mu=float(mu), rho_r=rho, n_iter=it, residual=res)
+    return state, info

@@ -141,6 +141,87 @@ int addft_sphere_to_real(addft_sphere_plan plan, const double* d_c, double scale
 int addft_real_to_sphere(addft_sphere_plan plan, const double* d_grid, double scale,
                          double* d_c, void* d_work, void* stream);
 
+/* ------------------------------------------------------------ geometry
+ * G vectors in a chosen order: g[p] = m[perm[p]] @ B (perm NULL: identity),
+ * q = g + k, qn = |q| with the reference safe_norm (basis.py:147-179).
+ * d_bmat: device (3,3) row-major reciprocal basis; h_kpt: host (3) or NULL. */
+int addft_gvectors(const int32_t* d_miller, const int32_t* d_perm, int64_t ng,
+                   const double* d_bmat, const double* h_kpt, double* d_g,
+                   double* d_q, double* d_qn, void* stream);
+/* beta~_ib(q) and d beta~/dq, (nrad, nq) tables (reference geometry/psp.py
+ * :117-138).  kind 0: analytic w q^l exp(-(qb)^2/2) (d_pw, d_pb); kind 1:
+ * Simpson quadrature with d_f[ib][i] = w_i rab_i betar_ib(r_i) r_i.      */
+int addft_beta_table(int kind, int nrad, const int32_t* d_l, const double* d_pw,
+                     const double* d_pb, int64_t mesh, const double* d_r,
+                     const double* d_f, const double* d_qn, int64_t nq,
+                     double* d_beta, double* d_dbeta, void* stream);
+/* v~_loc(q), d v~/dq (reference geometry/psp.py:86-114).  kind 1:
+ * d_aux[i] = w_i rab_i (r_i v_i + Z erf(r_i)) (v in Ha), g0 the q = 0 value. */
+int addft_vloc_table(int kind, double amp, double sig, double z, double g0,
+                     int64_t mesh, const double* d_r, const double* d_aux,
+                     const double* d_qn, int64_t nq, double* d_v, double* d_dv,
+                     void* stream);
+/* Projector matrix P[p][g] = beta~(|q|) Y_lm(q) e^{-iq.R} / sqrt(Omega)
+ * (reference physics/nonlocal_op.py:45-64); rows described by (atom,
+ * radial index, l*l+m).  dtype ADDFT_C64 / ADDFT_C128.                  */
+int addft_projectors(int dtype, int64_t nproj, int64_t ng, const int32_t* d_patom,
+                     const int32_t* d_prad, const int32_t* d_plm, const double* d_q,
+                     const double* d_beta, const double* d_pos, double inv_sqrt_omega,
+                     void* d_P, void* stream);
+/* V_loc(G) = v~(|G|)/Omega sum_I e^{-iG.R_I} (reference physics/local.py
+ * :21-34), complex double out.                                          */
+int addft_vloc_g(int64_t ng, int nat, const double* d_g, const double* d_vq,
+                 const double* d_pos, double inv_omega, double* d_out, void* stream);
+/* Workspace (bytes) for the backward contractions with nrows rows.     */
+int64_t addft_backward_work_bytes(int64_t nrows);
+/* Local term backward with psi frozen: out[3*nat] = dE_loc/dR (Cartesian),
+ * out[3nat] = E_loc, out[3nat+1..] = sum_G v~' (-G_m G_n/|G|) Re(S rho~*).
+ * rho_g: complex double rho~(G) on the same sphere as d_g.              */
+int addft_local_backward(int64_t ng, int nat, const double* d_g, const double* d_qn,
+                         const double* d_vq, const double* d_dvq, const double* d_rho_g,
+                         const double* d_pos, double* d_out, void* d_work, void* stream);
+/* Nonlocal backward: with Y = (w f g_s conj(D becp)) psi (nproj x ng),
+ * out[p][0..2] = 2 Re sum_G conj(d beta_p/dR) Y, out[p][3..11] the same for
+ * d beta_p/d eps_mn (row-major mn).  The backward of the projector
+ * contraction (reference nonlocal_op.py:67-82 differentiated by api.py
+ * :103-113).                                                            */
+int addft_nonlocal_backward(int dtype, int64_t nproj, int64_t ng, const int32_t* d_patom,
+                            const int32_t* d_prad, const int32_t* d_plm, const double* d_g,
+                            const double* d_q, const double* d_qn, const double* d_beta,
+                            const double* d_dbeta, const double* d_pos, const double* h_kpt,
+                            double inv_sqrt_omega, const void* d_Y, double* d_out,
+                            void* d_work, void* stream);
+/* out[0] = sum_G w, out[1+3m+n] = sum_G w a_m b_n (fixed-order sums).   */
+int addft_gsum_outer(int64_t ng, const double* d_w, const double* d_a, const double* d_b,
+                     double* d_out, void* d_work, void* stream);
+
+/* ------------------------------------------------------------ physics
+ * Ewald with the reference truncation (physics/ewald.py:34-77): real space
+ * T in [-nmax, nmax]^3, reciprocal m in [-nmax-2, nmax+2]^3.  Output
+ * layout (16 nat + 10 doubles): per atom 13 real-space values [E_i,
+ * dE/dR_i, dE/deps_i]; 10 reciprocal [E, dE/deps]; reciprocal dE/dR per
+ * atom.  Self and background terms are added by the caller.             */
+int64_t addft_ewald_work_bytes(int nat, int nmax);
+int addft_ewald(int nat, const double* d_pos, const double* d_z, const double* d_lat,
+                const double* d_bmat, double omega, double eta, int nmax, double* d_out,
+                void* d_work, void* stream);
+/* Fermi occupations, fixed 80-step bisection (physics/occupation.py:25-49);
+ * sigma > 0.  eig, occ: (nk, nb); w: (nk).                              */
+int addft_occupations(const double* d_eig, const double* d_w, int nk, int nb,
+                      double nelec, double sigma, double deg, double* d_occ,
+                      double* d_mu, void* stream);
+/* Hartree on a sphere: vh = 4 pi n/G^2 (G^2 > 1e-14), out10[0] =
+ * sum 4 pi |n|^2/G^2, out10[1..9] = sum 4 pi |n|^2 2 G_a G_b / G^4
+ * (physics/hartree.py:21-32).  vh may be NULL.                          */
+int addft_hartree_g(int64_t ng, const double* d_g, const double* d_n_g, double* d_vh_g,
+                    double* d_out10, void* d_work, void* stream);
+/* Kerker mix on a plane-major box (solver/scf.py:215-223):
+ * out = rho_in + alpha Re(N ifftn(fftn(rho_out - rho_in)/N q^2/(q^2+gg0))).
+ * work >= 16 (n1 n2 n3 + n1 + n2 + n3) bytes.                           */
+int addft_kerker_mix(int n1, int n2, int n3, const double* d_bmat, const double* d_rho_in,
+                     const double* d_rho_out, double alpha, double gg0, double* d_out,
+                     void* d_work, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
