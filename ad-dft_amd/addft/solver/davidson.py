@@ -42,12 +42,34 @@ def precondition(R, lam, diag, alpha=1.0):
     return R / p.to(R.real.dtype)
 
 
+DENSE_MAX_NG = 600
+
+
+def dense_lowest(apply, nG, nb, dtype, device):
+    """Lowest nb eigenpairs from the explicit matrix H = [H e_j] (the
+    reference's dense path, solver/eigh.py:30-41, used below nG = 3000 there;
+    here only for spheres too small for a block subspace)."""
+    E = torch.eye(nG, dtype=dtype, device=device)
+    M = apply(E).to(torch.complex128)        # row j = (H e_j)^T
+    H = M.T
+    H = 0.5 * (H + H.conj().T)
+    w, U = torch.linalg.eigh(H)
+    X = U[:, :nb].T.to(dtype).contiguous()
+    R = apply(X) - w[:nb].to(X.real.dtype)[:, None] * X
+    res = float(torch.linalg.vector_norm(R.to(torch.complex128), dim=1).max())
+    return w[:nb].clone(), X, res, 1
+
+
 def davidson(apply, X0, nb, diag, max_sub=None, max_iter=40, tol=1e-9, min_iter=1):
     """Return (eig (nb,) float64, X (nb, nG), max residual norm, iterations).
 
     apply: (m, nG) -> (m, nG) in X0's dtype.  diag: (nG,) float64."""
+    nG = X0.shape[1]
+    if nG <= max(DENSE_MAX_NG, 4 * nb):
+        return dense_lowest(apply, nG, nb, X0.dtype, X0.device)
     if max_sub is None:
         max_sub = max(48, 4 * nb)
+    max_sub = min(max_sub, nG // 2)
     X = _chol_orth(X0.contiguous())
     HX = apply(X)
     Q, HQ = X, HX
