@@ -100,7 +100,7 @@ def initial_guess(engine, nb, dtype):
 
 
 def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, beta=0.5,
-              switch_tol=1e-6, verbose=False, rho0=None, polish=4, dav_max_iter=None, log=None):
+              switch_tol=1e-5, verbose=False, rho0=None, polish=4, dav_max_iter=None, log=None):
     """Run the SCF on engine's current geometry.  Returns (SCFState with
     stick-order psi list in `psi` and plane-major rho, info dict)."""
     st = engine.st
@@ -122,6 +122,7 @@ def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, 
     hist = []
     eig = occ = mu = None
     dav_iters = 0
+    info_switch = None
 
     def one_map(rho, X, fp32, dav_tol, cold):
         nonlocal dav_iters
@@ -150,9 +151,13 @@ def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, 
         if verbose or log:
             msg = f"  scf {it:3d}  drho={res:.3e}  {'fp32' if fp32 else 'fp64'}  t={time.perf_counter() - t0:.2f}s"
             (log or print)(msg)
-        if fp32 and res < switch_tol:
+        # FP32 -> FP64 switch: at switch_tol, or when the FP32 map stops
+        # improving (its noise floor) once the residual is small
+        stalled = len(hist) >= 4 and res < 1e-3 and min(hist[-3:]) > 0.7 * min(hist[:-3])
+        if fp32 and (res < switch_tol or stalled):
             fp32 = False
             X = [x.to(torch.complex128) for x in X]
+            info_switch = it
         if mixer == "diis":
             if res > prev * 1.5:
                 diis.reset()
@@ -180,6 +185,7 @@ def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, 
     # final eigenvectors of H[rho] (scf.py:274-281)
     _, X, eig, occ, mu = one_map(rho, X, False, 1e-11, False)
     info = {"iterations": it, "residual_history": hist, "davidson_iterations": dav_iters,
+            "fp64_from_iteration": info_switch,
             "seconds": time.perf_counter() - t0}
     state = SCFState(psi=X, eig=eig, occ=occ, mu=float(mu), rho_r=rho, n_iter=it, residual=res)
     return state, info
