@@ -44,6 +44,9 @@ class Engine:
     def __init__(self, static, device=None):
         self.st = static
         self.dev = torch.device("cuda") if device is None else torch.device(device)
+        # small dense linear algebra (subspace eigh / Cholesky) through
+        # hipSOLVER: on this stack it is both the robust and the fast choice
+        torch.backends.cuda.preferred_linalg_library("cusolver")
         b = static.basis
         self.shape = tuple(b.fft_shape)
         self.nfft = int(np.prod(self.shape))

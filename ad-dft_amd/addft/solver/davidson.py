@@ -9,29 +9,97 @@ leave the converged eigenpairs unchanged:
   * vectors are rows (nb, nG) in the plan's stick order;
   * the projected matrix is extended block-wise instead of recomputed, and
     the Ritz vectors are formed only for the nb wanted states;
-  * orthonormalisation is Cholesky-QR (twice) with an FP64 Gram matrix, and
-    the Rayleigh-Ritz eigensolve is FP64 (rocSOLVER via torch.linalg.eigh);
-  * iterations stop when every band's residual norm is below `tol`.
+  * orthonormalisation is block Gram-Schmidt + SVQB (twice) with an FP64
+    Gram matrix, and the Rayleigh-Ritz eigensolve is FP64 (rocSOLVER via
+    torch.linalg.eigh);
+  * only unconverged bands add correction vectors, and iterations stop when
+    every band's residual norm is below `tol`.
 """
 
 from __future__ import annotations
 
+import os
+
 import torch
 
+_CHECK = os.environ.get("ADDFT_DEBUG_FINITE", "0") == "1"
 
-def _chol_orth(R):
-    """Rows of R orthonormalised by two passes of Cholesky-QR (FP64 Gram)."""
+
+def _finite(name, t):
+    if _CHECK and not bool(torch.isfinite(t).all()):
+        bad = (~torch.isfinite(t)).nonzero()
+        raise FloatingPointError(f"non-finite values in {name} {tuple(t.shape)}: first at {bad[:4].tolist()}")
+
+
+def eigh(T):
+    """FP64 Hermitian eigensolve.  hipSOLVER's solver can report
+    non-convergence on (nearly) degenerate input; retry once with a 1e-14
+    relative diagonal jitter (which leaves the eigenpairs unchanged at the
+    accuracy used here), then with the MAGMA solver."""
+    def ok(wv):
+        return bool(torch.isfinite(wv[0]).all()) and bool(torch.isfinite(wv[1]).all())
+
+    if not bool(torch.isfinite(T).all()):
+        raise FloatingPointError("non-finite projected matrix")
+    try:
+        wv = torch.linalg.eigh(T)
+        if ok(wv):
+            return wv
+    except torch.linalg.LinAlgError:
+        pass
+    n = T.shape[0]
+    scale = float(T.abs().max()) or 1.0
+    jit = torch.linspace(0.0, 1e-14 * scale, n, dtype=torch.float64, device=T.device)
+    try:
+        wv = torch.linalg.eigh(T + torch.diag(jit).to(T.dtype))
+        if ok(wv):
+            return wv
+    except torch.linalg.LinAlgError:
+        pass
+    # last resort: the MAGMA two-stage solver on the same device
+    prev = torch.backends.cuda.preferred_linalg_library()
+    torch.backends.cuda.preferred_linalg_library("magma")
+    try:
+        return torch.linalg.eigh(T)
+    finally:
+        torch.backends.cuda.preferred_linalg_library(prev)
+
+
+def _svqb(R, drop=1e-10):
+    """Orthonormalise the rows of R (Stathopoulos & Wu, SIAM J. Sci. Comput.
+    23 (2002), SVQB): eigen-decompose the scaled FP64 Gram matrix, drop
+    directions below drop * max, and rescale.  Robust for blocks whose rows
+    differ by orders of magnitude in norm or are nearly dependent."""
+    S = torch.matmul(R, R.conj().T).to(torch.complex128)
+    S = 0.5 * (S + S.conj().T)
+    d = torch.sqrt(torch.clamp(S.diagonal().real, min=1e-300))
+    Ss = S / (d[:, None] * d[None, :])
+    off = Ss - torch.diag(Ss.diagonal())
+    if float(off.abs().max()) < 1e-13:
+        # already orthogonal: only rescale
+        return (R / d.to(R.real.dtype)[:, None]).contiguous()
+    # well conditioned (the usual case): Cholesky of the scaled Gram matrix
+    L, info = torch.linalg.cholesky_ex(Ss)
+    if int(info.item()) == 0 and bool(torch.isfinite(L).all()) and float(L.diagonal().real.min()) > 1e-5:
+        Rs = R / d.to(R.real.dtype)[:, None]
+        return torch.linalg.solve_triangular(L.to(R.dtype), Rs, upper=False).contiguous()
+    w, V = eigh(Ss)
+    keep = w > drop * w.max()
+    Vk = V[:, keep] / torch.sqrt(w[keep])[None, :]
+    M = (Vk.conj().T / d[None, :]).to(R.dtype)      # (k, m)
+    return torch.matmul(M, R).contiguous()
+
+
+def _orth(R, Q=None, drop=1e-10):
+    """Rows of R orthonormal to each other and (if given) to the rows of Q:
+    two rounds of block Gram-Schmidt against Q, each followed by SVQB."""
     for _ in range(2):
-        S = torch.matmul(R, R.conj().T).to(torch.complex128)
-        S = 0.5 * (S + S.conj().T)
-        Lc, info = torch.linalg.cholesky_ex(S)
-        if int(info.item()) != 0:
-            # rank deficient block: fall back to a QR with column pivot-free
-            # Householder on the transposed block
-            Q, _ = torch.linalg.qr(R.to(torch.complex128).T)
-            return Q.T.to(R.dtype).contiguous()
-        R = torch.linalg.solve_triangular(Lc.to(R.dtype), R, upper=False)
-    return R.contiguous()
+        if Q is not None:
+            R = R - torch.matmul(torch.matmul(R, Q.conj().T), Q)
+        R = _svqb(R, drop)
+        if R.shape[0] == 0:
+            break
+    return R
 
 
 def precondition(R, lam, diag, alpha=1.0):
@@ -53,25 +121,32 @@ def dense_lowest(apply, nG, nb, dtype, device):
     M = apply(E).to(torch.complex128)        # row j = (H e_j)^T
     H = M.T
     H = 0.5 * (H + H.conj().T)
-    w, U = torch.linalg.eigh(H)
+    w, U = eigh(H)
     X = U[:, :nb].T.to(dtype).contiguous()
     R = apply(X) - w[:nb].to(X.real.dtype)[:, None] * X
     res = float(torch.linalg.vector_norm(R.to(torch.complex128), dim=1).max())
     return w[:nb].clone(), X, res, 1
 
 
-def davidson(apply, X0, nb, diag, max_sub=None, max_iter=40, tol=1e-9, min_iter=1):
+def davidson(apply, X0, nb, diag, max_sub=None, max_iter=40, tol=1e-9, min_iter=1, nconv=None):
     """Return (eig (nb,) float64, X (nb, nG), max residual norm, iterations).
 
-    apply: (m, nG) -> (m, nG) in X0's dtype.  diag: (nG,) float64."""
+    apply: (m, nG) -> (m, nG) in X0's dtype.  diag: (nG,) float64.
+    nconv: number of leading bands that must meet `tol` (default nb); the
+    remaining nb - nconv are buffer bands that only accelerate convergence."""
+    nconv = nb if nconv is None else int(nconv)
     nG = X0.shape[1]
     if nG <= max(DENSE_MAX_NG, 4 * nb):
         return dense_lowest(apply, nG, nb, X0.dtype, X0.device)
     if max_sub is None:
         max_sub = max(48, 4 * nb)
     max_sub = min(max_sub, nG // 2)
-    X = _chol_orth(X0.contiguous())
+    X = _orth(X0.contiguous())
+    if X.shape[0] < nb:
+        raise ValueError("initial block is rank deficient")
     HX = apply(X)
+    _finite("X0", X)
+    _finite("HX0", HX)
     Q, HQ = X, HX
     T = torch.matmul(Q.conj(), HQ.T).to(torch.complex128)
     T = 0.5 * (T + T.conj().T)
@@ -79,28 +154,32 @@ def davidson(apply, X0, nb, diag, max_sub=None, max_iter=40, tol=1e-9, min_iter=
     lam = None
     it = 0
     for it in range(1, max_iter + 1):
-        w, U = torch.linalg.eigh(T)
+        w, U = eigh(T)
         lam = w[:nb]
         Ub = U[:, :nb].to(Q.dtype)
         X = torch.matmul(Ub.T, Q)
         HX = torch.matmul(Ub.T, HQ)
         R = HX - lam.to(X.real.dtype)[:, None] * X
         rn = torch.linalg.vector_norm(R.to(torch.complex128), dim=1)
-        res = float(rn.max())
+        res = float(rn[:nconv].max())
         if res < tol and it >= min_iter:
             break
         if it == max_iter:
             break
-        R = precondition(R, lam, diag)
-        for _ in range(2):
-            R = R - torch.matmul(torch.matmul(R, Q.conj().T), Q)
-        R = _chol_orth(R)
+        # expand only with the residuals of unconverged bands
+        active = rn > tol
+        R = precondition(R[active], lam[active], diag)
+        R = R / torch.linalg.vector_norm(R.to(torch.complex128), dim=1).to(R.real.dtype)[:, None]
         if Q.shape[0] + R.shape[0] > max_sub:
-            # restart on the current Ritz vectors (davidson.py:66-69); R is
-            # already orthogonal to them since they lie in span(Q)
+            # restart on the current Ritz vectors (davidson.py:66-69)
             Q, HQ = X, HX
             T = torch.diag(lam.to(torch.complex128))
+        R = _orth(R, Q)
+        if R.shape[0] == 0:
+            break
+        _finite("R", R)
         HR = apply(R)
+        _finite("HR", HR)
         B = torch.matmul(Q.conj(), HR.T).to(torch.complex128)
         C = torch.matmul(R.conj(), HR.T).to(torch.complex128)
         C = 0.5 * (C + C.conj().T)

@@ -6,7 +6,9 @@ residual sum|dn| Omega/N/N_e, mixing that never enters the derivatives, and
 four unmixed "polish" steps at the end.  The eigen-solve is always the
 matrix-free Davidson (the reference switches to it above nG = 3000; below,
 its dense eigh gives the same fixed point).  Mixers:
-  * Kerker (alpha 0.7, gg0 0.5) for more than two atoms (scf.py:574-581);
+  * Kerker-preconditioned Pulay (alpha 0.7, gg0 0.5) for more than two
+    atoms -- the reference uses the plain Kerker step there (scf.py:574-581),
+    which drifts on the metallic 64/216-atom cells; `mixer="kerker"` keeps it;
   * simplified Broyden for two atoms or fewer (scf.py:285-304, 431-436);
   * Pulay/DIIS on request (`mixer="diis"`, solver/mixer.py:23-82).
 The converged fixed point does not depend on the mixer.
@@ -66,6 +68,43 @@ class DIIS:
         return new
 
 
+class PulayKerker:
+    """Pulay (DIIS) extrapolation of the input densities with a Kerker-
+    preconditioned step:  rho' = rho_bar + alpha K R_bar, where rho_bar and
+    R_bar are the DIIS combinations minimising |sum c_i R_i| (sum c_i = 1).
+    Restarts when the residual grows by 1.5x, like the reference's DIIS."""
+
+    def __init__(self, engine, m=8, alpha=0.7, gg0=0.5):
+        self.engine, self.m, self.alpha, self.gg0 = engine, m, alpha, gg0
+        self.rins, self.res = [], []
+
+    def reset(self):
+        self.rins, self.res = [], []
+
+    def __call__(self, rho_in, rho_out):
+        self.rins = (self.rins + [rho_in])[-self.m:]
+        self.res = (self.res + [rho_out - rho_in])[-self.m:]
+        n = len(self.rins)
+        if n == 1:
+            return self.engine.kerker(rho_in, rho_out, self.alpha, self.gg0)
+        R = torch.stack(self.res)
+        Bm = torch.zeros((n + 1, n + 1), dtype=torch.float64, device=rho_in.device)
+        Bm[:n, :n] = R @ R.T
+        Bm[:n, n] = -1.0
+        Bm[n, :n] = -1.0
+        Bm[:n, :n] += torch.eye(n, dtype=torch.float64, device=rho_in.device) * 1e-12 * float(Bm[:n, :n].diagonal().max())
+        rhs = torch.zeros(n + 1, dtype=torch.float64, device=rho_in.device)
+        rhs[n] = -1.0
+        c = torch.nan_to_num(torch.linalg.solve(Bm, rhs)[:n], nan=0.0, posinf=0.0, neginf=0.0)
+        rbar = c @ torch.stack(self.rins)
+        Rbar = c @ R
+        new = self.engine.kerker(rbar, rbar + Rbar, self.alpha, self.gg0)
+        if not bool(torch.isfinite(new).all()):
+            self.reset()
+            return self.engine.kerker(rho_in, rho_out, self.alpha, self.gg0)
+        return new
+
+
 class Broyden:
     """Simplified modified Broyden (scf.py:285-304)."""
 
@@ -91,6 +130,13 @@ class Broyden:
         return new if bool(torch.isfinite(new).all()) else rho + beta * F
 
 
+def buffer_bands(nb):
+    """Extra eigenvectors carried by the eigensolver above the nband that are
+    occupied (the reference occupies exactly nband states; the buffer only
+    speeds convergence of the highest of them)."""
+    return 0 if nb < 8 else max(4, nb // 10)
+
+
 def initial_guess(engine, nb, dtype):
     """The nb lowest plane waves of the reference order (scf.py:355-360)."""
     X = torch.zeros((nb, engine.ng), dtype=dtype, device=engine.dev)
@@ -100,21 +146,25 @@ def initial_guess(engine, nb, dtype):
 
 
 def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, beta=0.5,
-              switch_tol=1e-5, verbose=False, rho0=None, polish=4, dav_max_iter=None, log=None):
+              switch_tol=1e-5, verbose=False, rho0=None, polish=4, dav_max_iter=None, log=None,
+              nbuffer=None):
     """Run the SCF on engine's current geometry.  Returns (SCFState with
     stick-order psi list in `psi` and plane-major rho, info dict)."""
     st = engine.st
     nb = st.nband
     dev = engine.dev
     if mixer == "auto":
-        mixer = "kerker" if engine.nat > 2 else "broyden"
+        mixer = "pulay" if engine.nat > 2 else "broyden"
     fp32 = engine.mixed
     cdt = torch.complex64 if fp32 else torch.complex128
     rho = (torch.full((engine.nfft,), st.nelec / engine.omega, dtype=torch.float64, device=dev)
            if rho0 is None else rho0.clone())
-    X = [initial_guess(engine, nb, cdt) for _ in range(engine.nk)]
+    nbuf = buffer_bands(nb) if nbuffer is None else int(nbuffer)
+    nbt = min(nb + nbuf, engine.ng)
+    X = [initial_guess(engine, nbt, cdt) for _ in range(engine.nk)]
     diis = DIIS(8, beta) if mixer == "diis" else None
     broy = Broyden(8) if mixer == "broyden" else None
+    pulay = PulayKerker(engine, 8, alpha, gg0) if mixer == "pulay" else None
     prev = 1e9
     res = 1.0
     it = 0
@@ -124,6 +174,8 @@ def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, 
     dav_iters = 0
     info_switch = None
 
+    last = {}
+
     def one_map(rho, X, fp32, dav_tol, cold):
         nonlocal dav_iters
         v, _ = engine.effective_potential(rho)
@@ -132,24 +184,33 @@ def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, 
         eigs, Xn = [], []
         for ik in range(engine.nk):
             diag = engine.tkin64[ik] + vmean
-            lam, Xk, r, nit = davidson(lambda p: engine.hpsi(ik, p, vs, fp32=fp32), X[ik], nb, diag,
-                                       max_iter=(dav_max_iter or (40 if cold else 25)), tol=dav_tol)
+            lam, Xk, r, nit = davidson(lambda p: engine.hpsi(ik, p, vs, fp32=fp32), X[ik], nbt, diag,
+                                       max_iter=(dav_max_iter or (40 if cold else 25)), tol=dav_tol,
+                                       nconv=nb)
             dav_iters += nit
+            last["dav_res"], last["dav_it"] = r, nit
+            if nbt > nb:
+                last["gap"] = float(lam[nb] - lam[nb - 1])
             eigs.append(lam)
             Xn.append(Xk)
-        eig = torch.stack(eigs).to(torch.float64)
+        eig_all = torch.stack(eigs).to(torch.float64)
+        eig = eig_all[:, :nb].contiguous()
         occ, mu = engine.occupations(eig)
-        rho_out = engine.density(Xn, occ, fp32=fp32)
+        rho_out = engine.density([x[:nb] for x in Xn], occ, fp32=fp32)
         return rho_out, Xn, eig, occ, mu
 
     for it in range(1, max_iter + 1):
         floor = 2e-5 if fp32 else 1e-11
-        dav_tol = max(floor, min(1e-3, 0.1 * res))
+        dav_tol = max(floor, min(1e-4, 0.01 * res))
         rho_out, X, eig, occ, mu = one_map(rho, X, fp32, dav_tol, it == 1)
         res = float((rho_out - rho).abs().sum()) * engine.omega / engine.nfft / st.nelec
         hist.append(res)
         if verbose or log:
-            msg = f"  scf {it:3d}  drho={res:.3e}  {'fp32' if fp32 else 'fp64'}  t={time.perf_counter() - t0:.2f}s"
+            q_out = float(rho_out.sum()) * engine.omega / engine.nfft
+            msg = (f"  scf {it:3d}  drho={res:.3e}  {'fp32' if fp32 else 'fp64'}  t={time.perf_counter() - t0:.2f}s"
+                   f"  N_out={q_out:.6f}  dav_res={last['dav_res']:.2e} dav_it={last['dav_it']}"
+                   f"  eig=[{float(eig.min()):.4f},{float(eig.max()):.4f}] mu={float(mu):.4f}"
+                   f" gap_nb={last.get('gap', float('nan')):.2e} f_top={float(occ[:, -1].max()):.3f}")
             (log or print)(msg)
         # FP32 -> FP64 switch: at switch_tol, or when the FP32 map stops
         # improving (its noise floor) once the residual is small
@@ -168,6 +229,11 @@ def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, 
                 broy.reset()
             prev = res
             rho = broy(rho, rho_out, 0.9 if it <= 8 else 0.5)
+        elif mixer == "pulay":
+            if res > prev * 1.5:
+                pulay.reset()
+            prev = res
+            rho = pulay(rho, rho_out)
         else:
             rho = engine.kerker(rho, rho_out, alpha, gg0)
         if res < tol and not fp32:
@@ -184,6 +250,7 @@ def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, 
             break
     # final eigenvectors of H[rho] (scf.py:274-281)
     _, X, eig, occ, mu = one_map(rho, X, False, 1e-11, False)
+    X = [x[:nb].contiguous() for x in X]
     info = {"iterations": it, "residual_history": hist, "davidson_iterations": dav_iters,
             "fp64_from_iteration": info_switch,
             "seconds": time.perf_counter() - t0}
