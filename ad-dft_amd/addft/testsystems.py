@@ -4,10 +4,18 @@
       nband 2, one l = 0 projector (config 1 of the benchmark set);
   c2: 8-atom diamond conventional cell a = 10.26, seeded +-0.05 Bohr, ecut 10,
       nband 20, projectors l = 0, 0, 1;
-  c3: 2x2x2 supercell (64 atoms), +-0.03 Bohr, ecut 15, nband 144;
+  c3: 2x2x2 supercell (64 atoms), +-0.03 Bohr, ecut 15, nband 144,
+      sigma 0.002 (see below);
   c4: c2 under a seeded 1% symmetric strain, 4x4x4 MP mesh, nband 24,
       sigma 0.01;
-  c5: 3x3x3 supercell (216 atoms), +-0.03 Bohr, ecut 15, nband 480.
+  c5: 3x3x3 supercell (216 atoms), +-0.03 Bohr, ecut 15, nband 480,
+      sigma 0.002.
+The synthetic pseudopotential makes these cells metallic.  With the default
+smearing 0.05 Ha the highest of the nband states that are occupied (the
+reference occupies exactly nband states) keeps ~0.3-0.6 occupation and is
+nearly degenerate with the next state, so the SCF map is discontinuous and
+has no fixed point to converge to; sigma = 0.002 Ha leaves < 1e-7 occupation
+at the cut for c3 and c5.
 Displacements are uniform in [-amp, amp] per Cartesian component from
 numpy.random.default_rng(seed).
 """
@@ -52,7 +60,7 @@ def config(name: str, precision: str = "fp64", seed: int = 0):
         return dict(lattice=lat, frac=frac, psp=psp3, ecut=10.0, nband=20, precision=precision)
     if name == "c3":
         lat, frac = supercell(2, amp=0.03, seed=seed)
-        return dict(lattice=lat, frac=frac, psp=psp3, ecut=15.0, nband=144, precision=precision)
+        return dict(lattice=lat, frac=frac, psp=psp3, ecut=15.0, nband=144, sigma=0.002, precision=precision)
     if name == "c4":
         lat, frac = supercell(1, amp=0.05, seed=seed)
         rng = np.random.default_rng(seed + 1)
@@ -62,5 +70,5 @@ def config(name: str, precision: str = "fp64", seed: int = 0):
                     kmesh=monkhorst_pack((4, 4, 4)), precision=precision)
     if name == "c5":
         lat, frac = supercell(3, amp=0.03, seed=seed)
-        return dict(lattice=lat, frac=frac, psp=psp3, ecut=15.0, nband=480, precision=precision)
+        return dict(lattice=lat, frac=frac, psp=psp3, ecut=15.0, nband=480, sigma=0.002, precision=precision)
     raise ValueError(name)

@@ -112,12 +112,22 @@ __device__ __forceinline__ void stockham_stage(cpx<R>* buf, int n, int count, in
   const int nbf = n / RADIX;
   const int total = count * nbf;
   const int nthr = blockDim.x;
+  // consecutive threads take consecutive elements: along the transform when
+  // it is contiguous (es == 1), across transforms otherwise (LDS banks)
+  const bool ffast = es != 1;
   cpx<R> v[NB][RADIX];
 #pragma unroll
   for (int k = 0; k < NB; ++k) {
     int b = threadIdx.x + k * nthr;
     if (b < total) {
-      int f = b / nbf, t = b - f * nbf;
+      int f, t;
+      if (ffast) {
+        t = b / count;
+        f = b - t * count;
+      } else {
+        f = b / nbf;
+        t = b - f * nbf;
+      }
       const cpx<R>* src = buf + f * fs;
 #pragma unroll
       for (int q = 0; q < RADIX; ++q) v[k][q] = src[(t + q * nbf) * es];
@@ -129,7 +139,14 @@ __device__ __forceinline__ void stockham_stage(cpx<R>* buf, int n, int count, in
   for (int k = 0; k < NB; ++k) {
     int b = threadIdx.x + k * nthr;
     if (b < total) {
-      int f = b / nbf, t = b - f * nbf;
+      int f, t;
+      if (ffast) {
+        t = b / count;
+        f = b - t * count;
+      } else {
+        f = b / nbf;
+        t = b - f * nbf;
+      }
       int j = t % Ns;
       if (Ns > 1) {
 #pragma unroll
@@ -166,6 +183,12 @@ __device__ void lds_fft(cpx<R>* buf, const FftFactors& fac, int count, int fs, i
     }
     Ns *= r;
   }
+}
+
+// Copy a twiddle table into LDS (caller synchronises before use).
+template <typename R>
+__device__ __forceinline__ void stage_twiddles(cpx<R>* dst, const cpx<R>* __restrict__ src, int n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
 }
 
 // Host: factor a 2*3*5-smooth n into radices (8s first, then 4/2, 3, 5).

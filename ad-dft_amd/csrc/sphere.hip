@@ -68,10 +68,12 @@ template <typename R, typename RIN>
 __global__ void __launch_bounds__(kZThreads)
 k_z_inverse(const cpx<RIN>* __restrict__ psi, int64_t ng, const int32_t* __restrict__ soff,
             const int32_t* __restrict__ sid, const int32_t* __restrict__ iz, int64_t nst,
-            int spb, FftFactors f3, const cpx<R>* __restrict__ tw3, cpx<R>* __restrict__ T) {
+            int spb, FftFactors f3, const cpx<R>* __restrict__ gtw3, cpx<R>* __restrict__ T) {
   extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   cpx<R>* L = reinterpret_cast<cpx<R>*>(smem_raw);
   const int n3 = f3.n;
+  cpx<R>* tw3 = L + spb * n3;
+  addft::stage_twiddles(tw3, gtw3, n3);
   const int64_t s0 = (int64_t)blockIdx.x * spb;
   const int cnt = (int)min((int64_t)spb, nst - s0);
   const int64_t b = blockIdx.y;
@@ -98,11 +100,13 @@ template <typename R, typename ROUT>
 __global__ void __launch_bounds__(kZThreads)
 k_z_forward(const cpx<R>* __restrict__ T, int64_t ng, const int32_t* __restrict__ soff,
             const int32_t* __restrict__ sid, const int32_t* __restrict__ iz, int64_t nst,
-            int spb, FftFactors f3, const cpx<R>* __restrict__ tw3, R scale,
+            int spb, FftFactors f3, const cpx<R>* __restrict__ gtw3, R scale,
             const R* __restrict__ tkin, const cpx<R>* __restrict__ psi, cpx<ROUT>* __restrict__ out) {
   extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   cpx<R>* L = reinterpret_cast<cpx<R>*>(smem_raw);
   const int n3 = f3.n;
+  cpx<R>* tw3 = L + spb * n3;
+  addft::stage_twiddles(tw3, gtw3, n3);
   const int64_t s0 = (int64_t)blockIdx.x * spb;
   const int cnt = (int)min((int64_t)spb, nst - s0);
   const int64_t b = blockIdx.y;
@@ -177,12 +181,16 @@ template <typename R, int EPT>
 __global__ void __launch_bounds__(kPThreads)
 k_plane_v(cpx<R>* __restrict__ T, int64_t nst, int nra, const int32_t* __restrict__ srow,
           const int32_t* __restrict__ scol, const int32_t* __restrict__ rowval, FftFactors f1,
-          FftFactors f2, const cpx<R>* __restrict__ tw1, const cpx<R>* __restrict__ tw2,
+          FftFactors f2, const cpx<R>* __restrict__ gtw1, const cpx<R>* __restrict__ gtw2,
           const R* __restrict__ v, int cchunk) {
   extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   const int n1 = f1.n, n2 = f2.n, n3 = gridDim.x;
   cpx<R>* P = reinterpret_cast<cpx<R>*>(smem_raw);
   cpx<R>* W = P + nra * n2;
+  cpx<R>* tw1 = W + cchunk * n1;
+  cpx<R>* tw2 = tw1 + n1;
+  addft::stage_twiddles(tw1, gtw1, n1);
+  addft::stage_twiddles(tw2, gtw2, n2);
   const int z = blockIdx.x;
   const int64_t b = blockIdx.y;
   cpx<R>* Tp = T + (b * n3 + z) * nst;
@@ -211,13 +219,17 @@ template <typename R, int EPT>
 __global__ void __launch_bounds__(kPThreads)
 k_plane_density(const cpx<R>* __restrict__ T, int64_t nst, int nra, const int32_t* __restrict__ srow,
                 const int32_t* __restrict__ scol, const int32_t* __restrict__ rowval, FftFactors f1,
-                FftFactors f2, const cpx<R>* __restrict__ tw1, const cpx<R>* __restrict__ tw2,
+                FftFactors f2, const cpx<R>* __restrict__ gtw1, const cpx<R>* __restrict__ gtw2,
                 const double* __restrict__ w, int64_t nb, int bpg, int cchunk,
                 double* __restrict__ part) {
   extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   const int n1 = f1.n, n2 = f2.n, n3 = gridDim.x;
   cpx<R>* P = reinterpret_cast<cpx<R>*>(smem_raw);
   cpx<R>* W = P + nra * n2;
+  cpx<R>* tw1 = W + cchunk * n1;
+  cpx<R>* tw2 = tw1 + n1;
+  addft::stage_twiddles(tw1, gtw1, n1);
+  addft::stage_twiddles(tw2, gtw2, n2);
   const int z = blockIdx.x;
   const int64_t g = blockIdx.y;
   const int64_t nfft = (int64_t)n1 * n2 * n3;
@@ -263,12 +275,16 @@ template <typename R, int EPT>
 __global__ void __launch_bounds__(kPThreads)
 k_plane_to_real(const cpx<R>* __restrict__ T, int64_t nst, int nra, const int32_t* __restrict__ srow,
                 const int32_t* __restrict__ scol, const int32_t* __restrict__ rowval, FftFactors f1,
-                FftFactors f2, const cpx<R>* __restrict__ tw1, const cpx<R>* __restrict__ tw2,
+                FftFactors f2, const cpx<R>* __restrict__ gtw1, const cpx<R>* __restrict__ gtw2,
                 double scale, int cchunk, double* __restrict__ grid) {
   extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   const int n1 = f1.n, n2 = f2.n;
   cpx<R>* P = reinterpret_cast<cpx<R>*>(smem_raw);
   cpx<R>* W = P + nra * n2;
+  cpx<R>* tw1 = W + cchunk * n1;
+  cpx<R>* tw2 = tw1 + n1;
+  addft::stage_twiddles(tw1, gtw1, n1);
+  addft::stage_twiddles(tw2, gtw2, n2);
   const int z = blockIdx.x;
   load_plane(P, T + (int64_t)z * nst, nra, n2, nst, srow, scol);
   addft::lds_fft<R, EPT, true>(P, f2, nra, n2, 1, tw2);
@@ -291,12 +307,16 @@ template <typename R, int EPT>
 __global__ void __launch_bounds__(kPThreads)
 k_real_to_plane(const double* __restrict__ grid, int64_t nst, int nra, const int32_t* __restrict__ srow,
                 const int32_t* __restrict__ scol, const int32_t* __restrict__ rowval, FftFactors f1,
-                FftFactors f2, const cpx<R>* __restrict__ tw1, const cpx<R>* __restrict__ tw2,
+                FftFactors f2, const cpx<R>* __restrict__ gtw1, const cpx<R>* __restrict__ gtw2,
                 int cchunk, cpx<R>* __restrict__ T) {
   extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   const int n1 = f1.n, n2 = f2.n;
   cpx<R>* P = reinterpret_cast<cpx<R>*>(smem_raw);
   cpx<R>* W = P + nra * n2;
+  cpx<R>* tw1 = W + cchunk * n1;
+  cpx<R>* tw2 = tw1 + n1;
+  addft::stage_twiddles(tw1, gtw1, n1);
+  addft::stage_twiddles(tw2, gtw2, n2);
   const int z = blockIdx.x;
   const double* in = grid + (int64_t)z * n1 * n2;
   for (int c0 = 0; c0 < n2; c0 += cchunk) {
@@ -412,7 +432,7 @@ template <typename R>
 int launch_hpsi(addft_sphere_plan_s* p, const void* psi, const void* tkin, const void* v, int64_t nb,
                 void* out, void* work, hipStream_t st) {
   cpx<R>* T = reinterpret_cast<cpx<R>*>(work);
-  const size_t zlds = (size_t)p->spb * p->n3 * sizeof(cpx<R>);
+  const size_t zlds = (size_t)(p->spb + 1) * p->n3 * sizeof(cpx<R>);
   dim3 zg((unsigned)((p->nst + p->spb - 1) / p->spb), (unsigned)nb);
   hipLaunchKernelGGL((k_z_inverse<R, R>), zg, dim3(kZThreads), zlds, st,
                      reinterpret_cast<const cpx<R>*>(psi), p->ng, p->d_soff, p->d_sid, p->d_iz, p->nst,
@@ -446,7 +466,7 @@ int launch_density(addft_sphere_plan_s* p, const void* psi, const double* w, int
                    int accumulate, void* work, hipStream_t st) {
   cpx<R>* T = reinterpret_cast<cpx<R>*>(work);
   double* part = reinterpret_cast<double*>(reinterpret_cast<char*>(work) + t_bytes(p, nb));
-  const size_t zlds = (size_t)p->spb * p->n3 * sizeof(cpx<R>);
+  const size_t zlds = (size_t)(p->spb + 1) * p->n3 * sizeof(cpx<R>);
   dim3 zg((unsigned)((p->nst + p->spb - 1) / p->spb), (unsigned)nb);
   hipLaunchKernelGGL((k_z_inverse<R, R>), zg, dim3(kZThreads), zlds, st,
                      reinterpret_cast<const cpx<R>*>(psi), p->ng, p->d_soff, p->d_sid, p->d_iz, p->nst,
@@ -558,7 +578,7 @@ int addft_sphere_plan_create(int n1, int n2, int n3, int64_t ng, const int32_t* 
     return ADDFT_EUNSUPPORTED;
   }
   int cch = std::min(n2, (ept * kPThreads) / n1);
-  size_t base = (size_t)plane_elems * cs;
+  size_t base = (size_t)(plane_elems + n1 + n2) * cs;  // plane + twiddles
   while (cch > 1 && base + (size_t)cch * n1 * cs > kLdsMax) --cch;
   if (base + (size_t)cch * n1 * cs > kLdsMax) {
     delete p;
@@ -692,7 +712,7 @@ int addft_sphere_to_real(addft_sphere_plan p, const double* c, double scale, dou
   ADDFT_CHECK_ARG(p->precision == ADDFT_C128, "sphere_to_real needs a C128 plan");
   hipStream_t st = addft::as_stream(stream);
   cpx<double>* T = reinterpret_cast<cpx<double>*>(work);
-  const size_t zlds = (size_t)p->spb * p->n3 * sizeof(cpx<double>);
+  const size_t zlds = (size_t)(p->spb + 1) * p->n3 * sizeof(cpx<double>);
   dim3 zg((unsigned)((p->nst + p->spb - 1) / p->spb), 1u);
   hipLaunchKernelGGL((k_z_inverse<double, double>), zg, dim3(kZThreads), zlds, st,
                      reinterpret_cast<const cpx<double>*>(c), p->ng, p->d_soff, p->d_sid, p->d_iz, p->nst,
@@ -735,7 +755,7 @@ int addft_real_to_sphere(addft_sphere_plan p, const double* grid, double scale, 
   }
 #undef ADDFT_RP
   ADDFT_LAUNCH_CHECK();
-  const size_t zlds = (size_t)p->spb * p->n3 * sizeof(cpx<double>);
+  const size_t zlds = (size_t)(p->spb + 1) * p->n3 * sizeof(cpx<double>);
   dim3 zg((unsigned)((p->nst + p->spb - 1) / p->spb), 1u);
   hipLaunchKernelGGL((k_z_forward<double, double>), zg, dim3(kZThreads), zlds, st, T, p->ng, p->d_soff,
                      p->d_sid, p->d_iz, p->nst, p->spb, p->f3, reinterpret_cast<const cpx<double>*>(p->d_tw3),
