@@ -57,7 +57,7 @@ class Engine:
         with torch.cuda.device(self.dev):
             self.plan64 = native.SpherePlan(self.shape, b.miller, torch.complex128, self.dev)
             self.plan32 = native.SpherePlan(self.shape, b.miller, torch.complex64, self.dev) if self.mixed else None
-            self.plan_rho = native.SpherePlan(self.shape, b.miller_rho, torch.complex128, self.dev)
+            self.plan_rho = native.sphere_plan_or_box(self.shape, b.miller_rho, torch.complex128, self.dev)
         self.perm = self.plan64.perm
         self.iperm = self.plan64.iperm
         self.perm_d = torch.as_tensor(self.perm, device=self.dev)

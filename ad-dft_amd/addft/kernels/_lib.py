@@ -71,6 +71,7 @@ _SIGS = {
     "addft_ewald": ([_I32, _VP, _VP, _VP, _VP, _D, _D, _I32, _VP, _VP, _VP], _I32),
     "addft_occupations": ([_VP, _VP, _I32, _I32, _D, _D, _D, _VP, _VP, _VP], _I32),
     "addft_hartree_g": ([_I64, _VP, _VP, _VP, _VP, _VP, _VP], _I32),
+    "addft_box_fft": ([_I32, _I32, _I32, _VP, _I32, _VP, _VP], _I32),
     "addft_kerker_mix": ([_I32, _I32, _I32, _VP, _VP, _VP, _D, _D, _VP, _VP, _VP], _I32),
 }
 

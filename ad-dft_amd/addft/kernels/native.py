@@ -213,3 +213,56 @@ class SpherePlan:
         check(lib().addft_real_to_sphere(self._h, ptr(grid_planar), float(scale), ptr(out), ptr(self.work(1)),
                                          stream()), "addft_real_to_sphere")
         return out
+
+
+class BoxSphere:
+    """Sphere <-> real grid through a full-box FFT, for spheres whose
+    compressed z-plane does not fit in LDS (e.g. the density sphere of the
+    216-atom cell in FP64).  Same stick order and the same to_real /
+    to_sphere contract as SpherePlan (C128 only)."""
+
+    def __init__(self, fft_shape, miller, device=None):
+        self.shape = tuple(int(x) for x in fft_shape)
+        n1, n2, n3 = self.shape
+        self.nfft = n1 * n2 * n3
+        m = np.asarray(miller, dtype=np.int64).reshape(-1, 3)
+        self.ng = int(m.shape[0])
+        self.cdtype, self.rdtype = torch.complex128, torch.float64
+        self.device = torch.device("cuda") if device is None else torch.device(device)
+        i1, i2, i3 = np.mod(m[:, 0], n1), np.mod(m[:, 1], n2), np.mod(m[:, 2], n3)
+        flat = (i1 * n2 + i2) * n3 + i3
+        self.perm = np.argsort(flat, kind="stable").astype(np.int32)
+        self.iperm = np.empty_like(self.perm)
+        self.iperm[self.perm] = np.arange(self.ng, dtype=np.int32)
+        pidx = ((i3 * n1 + i1) * n2 + i2)[self.perm]
+        self.pidx = torch.as_tensor(pidx.astype(np.int32), device=self.device)
+        self._tw = torch.empty(16 * (n1 + n2 + n3) + 256, dtype=torch.uint8, device=self.device)
+
+    def _fft(self, box, inverse):
+        n1, n2, n3 = self.shape
+        check(lib().addft_box_fft(n1, n2, n3, ptr(box), int(bool(inverse)), ptr(self._tw), stream()),
+              "addft_box_fft")
+
+    def to_real(self, c: torch.Tensor, scale: float = 1.0) -> torch.Tensor:
+        _require(c, torch.complex128)
+        box = pack_scatter(c, self.pidx, self.nfft)
+        self._fft(box, True)
+        return (box.real * scale).contiguous()
+
+    def to_sphere(self, grid_planar: torch.Tensor, scale: float = 1.0) -> torch.Tensor:
+        _require(grid_planar, torch.float64)
+        box = grid_planar.to(torch.complex128).contiguous()
+        self._fft(box, False)
+        return (pack_gather(box, self.pidx) * scale).contiguous()
+
+
+def sphere_plan_or_box(fft_shape, miller, precision, device=None):
+    """A SpherePlan, or a BoxSphere when the plane does not fit in LDS."""
+    try:
+        return SpherePlan(fft_shape, miller, precision, device)
+    except _lib.AddftError as e:
+        if "does not fit" not in str(e) and "too large" not in str(e):
+            raise
+        if precision != torch.complex128:
+            raise
+        return BoxSphere(fft_shape, miller, device)

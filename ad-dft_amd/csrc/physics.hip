@@ -329,6 +329,54 @@ __global__ void k_twiddles(cpx<double>* __restrict__ tw, int n) {
 
 }  // namespace
 
+namespace {
+
+// In-place 3-D transform of a plane-major (n3, n1, n2) complex double box
+// (unnormalised; inverse = e^{+}).  twork: >= n1 + n2 + n3 complex doubles.
+int box_fft3(int n1, int n2, int n3, cpx<double>* d, bool inverse, cpx<double>* tw, hipStream_t st) {
+  int dims[3] = {n3, n1, n2};
+  FftFactors fac[3];
+  cpx<double>* twp[3];
+  int64_t off = 0;
+  for (int i = 0; i < 3; ++i) {
+    ADDFT_CHECK_ARG(addft::factorize(dims[i], &fac[i]), "box dims must be 2*3*5-smooth");
+    ADDFT_CHECK_ARG(dims[i] <= 2048, "axis too long");
+    twp[i] = tw + off;
+    off += dims[i];
+    hipLaunchKernelGGL(k_twiddles, dim3(nblk(dims[i])), dim3(kT), 0, st, twp[i], dims[i]);
+    ADDFT_LAUNCH_CHECK();
+  }
+  for (int ax = 0; ax < 3; ++ax) {
+    const int N = dims[ax];
+    int64_t A, C;
+    if (ax == 0) {
+      A = 1;
+      C = (int64_t)n1 * n2;
+    } else if (ax == 1) {
+      A = n3;
+      C = n2;
+    } else {
+      A = (int64_t)n3 * n1;
+      C = 1;
+    }
+    int cw = std::max(1, 2048 / N);
+    if (C > 1) {
+      while (cw > 1 && C % cw) --cw;
+    }
+    const int64_t nlines = A * C;
+    const size_t lds = (size_t)cw * N * sizeof(cpx<double>);
+    dim3 grid((unsigned)((nlines + cw - 1) / cw));
+    if (inverse)
+      hipLaunchKernelGGL(k_axis_fft<true>, grid, dim3(kT), lds, st, d, A, C, fac[ax], twp[ax], cw);
+    else
+      hipLaunchKernelGGL(k_axis_fft<false>, grid, dim3(kT), lds, st, d, A, C, fac[ax], twp[ax], cw);
+    ADDFT_LAUNCH_CHECK();
+  }
+  return ADDFT_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 int64_t addft_ewald_work_bytes(int nat, int nmax) {
@@ -401,61 +449,25 @@ int addft_kerker_mix(int n1, int n2, int n3, const double* bmat, const double* r
   const int64_t n = (int64_t)n1 * n2 * n3;
   cpx<double>* d = reinterpret_cast<cpx<double>*>(work);
   cpx<double>* tw = d + n;
-  int dims[3] = {n3, n1, n2};
-  FftFactors fac[3];
-  cpx<double>* twp[3];
-  int64_t off = 0;
-  for (int i = 0; i < 3; ++i) {
-    ADDFT_CHECK_ARG(addft::factorize(dims[i], &fac[i]), "box dims must be 2*3*5-smooth");
-    ADDFT_CHECK_ARG(dims[i] <= 2048, "axis too long");
-    twp[i] = tw + off;
-    off += dims[i];
-  }
-  for (int i = 0; i < 3; ++i) {
-    hipLaunchKernelGGL(k_twiddles, dim3(nblk(dims[i])), dim3(kT), 0, st, twp[i], dims[i]);
-    ADDFT_LAUNCH_CHECK();
-  }
   hipLaunchKernelGGL(k_diff_to_complex, dim3(nblk(n)), dim3(kT), 0, st, rho_in, rho_out, n, d);
   ADDFT_LAUNCH_CHECK();
-  // axis sizes: plane-major array (n3, n1, n2)
-  auto run_axis = [&](int ax, bool inv) -> int {
-    const int N = dims[ax];
-    int64_t A, C;
-    if (ax == 0) {
-      A = 1;
-      C = (int64_t)n1 * n2;
-    } else if (ax == 1) {
-      A = n3;
-      C = n2;
-    } else {
-      A = (int64_t)n3 * n1;
-      C = 1;
-    }
-    int cw = std::max(1, 2048 / N);
-    if (C > 1) {
-      while (cw > 1 && C % cw) --cw;
-    }
-    const int64_t nlines = A * C;
-    const size_t lds = (size_t)cw * N * sizeof(cpx<double>);
-    dim3 grid((unsigned)((nlines + cw - 1) / cw));
-    if (inv)
-      hipLaunchKernelGGL(k_axis_fft<true>, grid, dim3(kT), lds, st, d, A, C, fac[ax], twp[ax], cw);
-    else
-      hipLaunchKernelGGL(k_axis_fft<false>, grid, dim3(kT), lds, st, d, A, C, fac[ax], twp[ax], cw);
-    ADDFT_LAUNCH_CHECK();
-    return ADDFT_OK;
-  };
   int rc;
-  for (int ax = 0; ax < 3; ++ax)
-    if ((rc = run_axis(ax, false))) return rc;
+  if ((rc = box_fft3(n1, n2, n3, d, false, tw, st))) return rc;
   hipLaunchKernelGGL(k_kerker_filter, dim3(nblk(n)), dim3(kT), 0, st, d, n1, n2, n3, bmat, gg0, 1.0 / (double)n);
   ADDFT_LAUNCH_CHECK();
-  for (int ax = 0; ax < 3; ++ax)
-    if ((rc = run_axis(ax, true))) return rc;
+  if ((rc = box_fft3(n1, n2, n3, d, true, tw, st))) return rc;
   // N ifftn = unnormalised inverse: scale 1
   hipLaunchKernelGGL(k_kerker_finish, dim3(nblk(n)), dim3(kT), 0, st, rho_in, d, n, alpha, 1.0, out);
   ADDFT_LAUNCH_CHECK();
   return ADDFT_OK;
+}
+
+// In-place unnormalised 3-D FFT of a plane-major complex double box.
+// work: >= 16 (n1 + n2 + n3) bytes (twiddles).
+int addft_box_fft(int n1, int n2, int n3, double* d_data, int inverse, void* d_work, void* stream) {
+  ADDFT_CHECK_ARG(n1 > 0 && n2 > 0 && n3 > 0 && d_data && d_work, "bad argument");
+  return box_fft3(n1, n2, n3, reinterpret_cast<cpx<double>*>(d_data), inverse != 0,
+                  reinterpret_cast<cpx<double>*>(d_work), addft::as_stream(stream));
 }
 
 }  // extern "C"

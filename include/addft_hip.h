@@ -218,6 +218,11 @@ int addft_hartree_g(int64_t ng, const double* d_g, const double* d_n_g, double* 
 /* Kerker mix on a plane-major box (solver/scf.py:215-223):
  * out = rho_in + alpha Re(N ifftn(fftn(rho_out - rho_in)/N q^2/(q^2+gg0))).
  * work >= 16 (n1 n2 n3 + n1 + n2 + n3) bytes.                           */
+/* In-place unnormalised 3-D FFT of a plane-major (n3, n1, n2) complex
+ * double box: three batched axis passes staged in LDS.  Used for spheres
+ * whose compressed plane does not fit in LDS.  work >= 16 (n1+n2+n3) bytes. */
+int addft_box_fft(int n1, int n2, int n3, double* d_data, int inverse, void* d_work,
+                  void* stream);
 int addft_kerker_mix(int n1, int n2, int n3, const double* d_bmat, const double* d_rho_in,
                      const double* d_rho_out, double alpha, double gg0, double* d_out,
                      void* d_work, void* stream);

@@ -220,3 +220,25 @@ def test_hpsi_large_box_properties():
         herm = (m - m.conj().T).abs().max().item()
         print(f"{cdt}: const-V err {(out - exp).abs().max().item():.2e}, hermiticity {herm:.2e}", flush=True)
         assert herm < tol * 100
+
+
+def test_box_sphere_matches_plan():
+    """The full-box fallback (spheres whose plane exceeds LDS) agrees with
+    the stick plan and with the oracle conventions."""
+    torch, native = _torch(), _native()
+    lat, ecut, df = CASES[2]
+    geo, st = _system(lat, ecut, df)
+    b = st.basis
+    plan = native.SpherePlan(b.fft_shape, b.miller_rho, torch.complex128)
+    box = native.BoxSphere(b.fft_shape, b.miller_rho)
+    assert np.array_equal(plan.perm, box.perm)
+    rng = np.random.default_rng(31)
+    vg = rng.normal(size=b.nG_rho) + 1j * rng.normal(size=b.nG_rho)
+    c = torch.as_tensor(vg[plan.perm]).to(_dev())
+    r1, r2 = plan.to_real(c), box.to_real(c)
+    assert (r1 - r2).abs().max().item() < 1e-11
+    ref = OB.real_from_g(b, vg, b.miller_rho)
+    assert np.max(np.abs(native.from_planar(r2, b.fft_shape).cpu().numpy() - ref)) < 1e-10
+    g = torch.as_tensor(0.1 + rng.random(b.nfft)).to(_dev())
+    s1, s2 = plan.to_sphere(g, 1.0 / b.nfft), box.to_sphere(g, 1.0 / b.nfft)
+    assert (s1 - s2).abs().max().item() < 1e-13
