@@ -81,8 +81,11 @@ def _svqb(R, drop=1e-10):
     # well conditioned (the usual case): Cholesky of the scaled Gram matrix
     L, info = torch.linalg.cholesky_ex(Ss)
     if int(info.item()) == 0 and bool(torch.isfinite(L).all()) and float(L.diagonal().real.min()) > 1e-5:
-        Rs = R / d.to(R.real.dtype)[:, None]
-        return torch.linalg.solve_triangular(L.to(R.dtype), Rs, upper=False).contiguous()
+        # R <- L^{-1} D^{-1} R as one GEMM (a large-RHS trsm is slower and
+        # fails to allocate its workspace at the 216-atom size)
+        eye = torch.eye(L.shape[0], dtype=L.dtype, device=L.device)
+        Linv = torch.linalg.solve_triangular(L, eye, upper=False) / d[None, :]
+        return torch.matmul(Linv.to(R.dtype), R).contiguous()
     w, V = eigh(Ss)
     keep = w > drop * w.max()
     Vk = V[:, keep] / torch.sqrt(w[keep])[None, :]
