@@ -28,7 +28,8 @@ import numpy as np
 import torch
 
 from ..system import SCFState
-from .davidson import davidson
+from .chebyshev import chfsi_step, upper_bound
+from .davidson import DENSE_MAX_NG, davidson
 
 
 class DIIS:
@@ -147,7 +148,7 @@ def initial_guess(engine, nb, dtype):
 
 def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, beta=0.5,
               switch_tol=1e-5, verbose=False, rho0=None, polish=4, dav_max_iter=None, log=None,
-              nbuffer=None):
+              nbuffer=None, eigensolver="auto", cheb_degree=8):
     """Run the SCF on engine's current geometry.  Returns (SCFState with
     stick-order psi list in `psi` and plane-major rho, info dict)."""
     st = engine.st
@@ -175,19 +176,41 @@ def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, 
     info_switch = None
 
     last = {}
+    use_cheb = eigensolver == "chebyshev" or (eigensolver == "auto" and nbt >= 32
+                                               and engine.ng > max(DENSE_MAX_NG, 4 * nbt))
+    ritz = [None] * engine.nk
+    bounds = {}
+
+    def solve_k(ik, Xk, vs, fp32, vmean, tol_k, cold):
+        nonlocal dav_iters
+        apply = lambda p: engine.hpsi(ik, p, vs, fp32=fp32)  # noqa: E731
+        if use_cheb and not cold and ritz[ik] is not None:
+            key = (ik, fp32)
+            if key not in bounds:
+                bounds[key] = 1.1 * upper_bound(apply, engine.ng, Xk.dtype, engine.dev)
+            lam = ritz[ik]
+            nit, r = 0, float("inf")
+            for nit in range(1, (41 if tol_k <= 1e-9 else 5)):
+                lam, Xk, _, rn = chfsi_step(apply, Xk, cheb_degree, bounds[key], lam)
+                r = float(rn[:nb].max())
+                if r < tol_k:
+                    break
+            dav_iters += nit
+            return lam, Xk, r, nit
+        diag = engine.tkin64[ik] + vmean
+        lam, Xk, r, nit = davidson(apply, Xk, nbt, diag, max_iter=(dav_max_iter or (40 if cold else 25)),
+                                   tol=(max(tol_k, 1e-3) if (cold and use_cheb) else tol_k), nconv=nb)
+        dav_iters += nit
+        return lam, Xk, r, nit
 
     def one_map(rho, X, fp32, dav_tol, cold):
-        nonlocal dav_iters
         v, _ = engine.effective_potential(rho)
         vs = engine.stage(v, mixed_stage=fp32)
         vmean = float(v.mean())
         eigs, Xn = [], []
         for ik in range(engine.nk):
-            diag = engine.tkin64[ik] + vmean
-            lam, Xk, r, nit = davidson(lambda p: engine.hpsi(ik, p, vs, fp32=fp32), X[ik], nbt, diag,
-                                       max_iter=(dav_max_iter or (40 if cold else 25)), tol=dav_tol,
-                                       nconv=nb)
-            dav_iters += nit
+            lam, Xk, r, nit = solve_k(ik, X[ik], vs, fp32, vmean, dav_tol, cold)
+            ritz[ik] = lam
             last["dav_res"], last["dav_it"] = r, nit
             if nbt > nb:
                 last["gap"] = float(lam[nb] - lam[nb - 1])
@@ -218,6 +241,7 @@ def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, 
         if fp32 and (res < switch_tol or stalled):
             fp32 = False
             X = [x.to(torch.complex128) for x in X]
+            bounds.clear()
             info_switch = it
         if mixer == "diis":
             if res > prev * 1.5:
