@@ -30,6 +30,7 @@ from .geometry.psp import lm_table
 from .kernels import _lib
 from .kernels import native
 from .kernels._lib import check, lib, ptr, stream
+from .solver.linalg import inner
 
 
 def _simpson_weights(n):
@@ -207,7 +208,7 @@ class Engine:
         vh_g = torch.empty_like(n_g)
         out = torch.empty(10, dtype=torch.float64, device=self.dev)
         check(lib().addft_hartree_g(self.ng, ptr(self.g0_d), ptr(n_g), ptr(vh_g), ptr(out),
-                                    ptr(self.work("red", 64 * 10 * 8 + 256)), stream()), "addft_hartree_g")
+                                    ptr(self.work("red", int(lib().addft_reduce_work_bytes()))), stream()), "addft_hartree_g")
         v_h = self.plan64.to_real(vh_g, 1.0)
         return v_h, out
 
@@ -229,13 +230,13 @@ class Engine:
         P = self.P32[ik] if fp32 else self.P64[ik]
         D = self.D32 if fp32 else self.D64
         out = plan.hpsi_local(psi, tk, vs)
-        becp_t = torch.matmul(psi, P.conj().T)          # (nb, nproj) = <beta|psi>^T
+        becp_t = inner(psi, P)                          # (nb, nproj) = <beta|psi>^T
         out.add_(torch.matmul(torch.matmul(becp_t, D), P))
         return out
 
     def becp(self, ik, psi):
         """(nproj, nb) <beta|psi> in FP64 (nonlocal_op.py:67-69)."""
-        return torch.matmul(self.P64[ik].conj(), psi.to(torch.complex128).T)
+        return inner(self.P64[ik], psi.to(torch.complex128), conj_a=True)
 
     def density(self, psis, occ, fp32=False):
         """rho = sum_k w_k sum_n f g_s |psi(r)|^2, plane-major (density.py:18-23)."""
@@ -390,7 +391,7 @@ class Engine:
         L = lib()
         nat = self.nat
         parts = {}
-        red = self.work("red", 64 * 10 * 8 + 256)
+        red = self.work("red", int(L.addft_reduce_work_bytes()))
         # kinetic: -sum_G dens_G q_m G_n
         ek = np.zeros((3, 3))
         out10 = torch.empty(10, dtype=torch.float64, device=self.dev)

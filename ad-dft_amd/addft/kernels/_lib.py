@@ -63,6 +63,7 @@ _SIGS = {
     "addft_projectors": ([_I32, _I64, _I64, _VP, _VP, _VP, _VP, _VP, _VP, _D, _VP, _VP], _I32),
     "addft_vloc_g": ([_I64, _I32, _VP, _VP, _VP, _D, _VP, _VP], _I32),
     "addft_backward_work_bytes": ([_I64], _I64),
+    "addft_reduce_work_bytes": ([], _I64),
     "addft_local_backward": ([_I64, _I32, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP], _I32),
     "addft_nonlocal_backward": ([_I32, _I64, _I64, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _D,
                                  _VP, _VP, _VP, _VP], _I32),

@@ -18,6 +18,7 @@ from __future__ import annotations
 import torch
 
 from .davidson import eigh
+from .linalg import inner
 
 
 def upper_bound(apply, nG, dtype, device, k=10, seed=0):
@@ -72,7 +73,7 @@ def orthonormalise(Y):
     shifted first pass when Y is too ill-conditioned for plain Cholesky
     (Fukaya et al., shifted CholeskyQR, SIAM J. Sci. Comput. 42 (2020))."""
     for it in range(3):
-        S = torch.matmul(Y, Y.conj().T).to(torch.complex128)
+        S = inner(Y, Y).to(torch.complex128)
         S = 0.5 * (S + S.conj().T)
         d = torch.sqrt(torch.clamp(S.diagonal().real, min=1e-300))
         Ss = S / (d[:, None] * d[None, :])
@@ -101,7 +102,7 @@ def chfsi_step(apply, X, degree, b_upper, lam, HX=None):
     Y = filter_block(apply, X, HX, degree, a, b_upper, a0)
     Y = orthonormalise(Y)
     HY = apply(Y)
-    T = torch.matmul(Y.conj(), HY.T).to(torch.complex128)
+    T = inner(Y, HY, conj_a=True).to(torch.complex128)
     T = 0.5 * (T + T.conj().T)
     w, U = eigh(T)
     Ut = U.T.to(Y.dtype)

@@ -22,6 +22,8 @@ import os
 
 import torch
 
+from .linalg import inner
+
 _CHECK = os.environ.get("ADDFT_DEBUG_FINITE", "0") == "1"
 
 
@@ -70,7 +72,7 @@ def _svqb(R, drop=1e-10):
     23 (2002), SVQB): eigen-decompose the scaled FP64 Gram matrix, drop
     directions below drop * max, and rescale.  Robust for blocks whose rows
     differ by orders of magnitude in norm or are nearly dependent."""
-    S = torch.matmul(R, R.conj().T).to(torch.complex128)
+    S = inner(R, R).to(torch.complex128)
     S = 0.5 * (S + S.conj().T)
     d = torch.sqrt(torch.clamp(S.diagonal().real, min=1e-300))
     Ss = S / (d[:, None] * d[None, :])
@@ -98,7 +100,7 @@ def _orth(R, Q=None, drop=1e-10):
     two rounds of block Gram-Schmidt against Q, each followed by SVQB."""
     for _ in range(2):
         if Q is not None:
-            R = R - torch.matmul(torch.matmul(R, Q.conj().T), Q)
+            R = R - torch.matmul(inner(R, Q), Q)
         R = _svqb(R, drop)
         if R.shape[0] == 0:
             break
@@ -151,7 +153,7 @@ def davidson(apply, X0, nb, diag, max_sub=None, max_iter=40, tol=1e-9, min_iter=
     _finite("X0", X)
     _finite("HX0", HX)
     Q, HQ = X, HX
-    T = torch.matmul(Q.conj(), HQ.T).to(torch.complex128)
+    T = inner(Q, HQ, conj_a=True).to(torch.complex128)
     T = 0.5 * (T + T.conj().T)
     res = float("inf")
     lam = None
@@ -183,8 +185,8 @@ def davidson(apply, X0, nb, diag, max_sub=None, max_iter=40, tol=1e-9, min_iter=
         _finite("R", R)
         HR = apply(R)
         _finite("HR", HR)
-        B = torch.matmul(Q.conj(), HR.T).to(torch.complex128)
-        C = torch.matmul(R.conj(), HR.T).to(torch.complex128)
+        B = inner(Q, HR, conj_a=True).to(torch.complex128)
+        C = inner(R, HR, conj_a=True).to(torch.complex128)
         C = 0.5 * (C + C.conj().T)
         T = torch.cat([torch.cat([T, B], dim=1), torch.cat([B.conj().T, C], dim=1)], dim=0)
         Q = torch.cat([Q, R], dim=0)

@@ -3,7 +3,7 @@
 Mirrors reference `src/addft/solver/scf.py:200-607`: uniform initial density
 nelec/Omega, the physical map n -> H[n] -> lowest nband -> f -> n', the
 residual sum|dn| Omega/N/N_e, mixing that never enters the derivatives, and
-four unmixed "polish" steps at the end.  The eigen-solve is always the
+up to four unmixed "polish" steps at the end.  The eigen-solve is always the
 matrix-free Davidson (the reference switches to it above nG = 3000; below,
 its dense eigh gives the same fixed point).  Mixers:
   * Kerker-preconditioned Pulay (alpha 0.7, gg0 0.5) for more than two
@@ -264,16 +264,26 @@ def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, 
             break
     if fp32:
         X = [x.to(torch.complex128) for x in X]
-    # polish: the unmixed map until the fixed point is strict (scf.py:266-272)
-    for _ in range(polish):
-        rho_new, X, eig, occ, mu = one_map(rho, X, False, 1e-11, False)
-        res = float((rho_new - rho).abs().sum()) * engine.omega / engine.nfft / st.nelec
-        rho = rho_new
-        hist.append(res)
-        if res < 1e-14:
+    # polish: unmixed map steps (scf.py:495-504).  The reference takes four
+    # and then the eigenvectors of H[rho] (scf.py:506-508) -- five map
+    # evaluations.  The unmixed map need not be a contraction: in the large
+    # (charge-sloshing) cells one step multiplies the residual by ~20, so the
+    # steps are taken only while the residual keeps falling, and the state
+    # kept is the evaluation with the smallest residual.  Where the map
+    # contracts this is exactly the reference's sequence.
+    best = None
+    for j in range(polish + 1):
+        rho_new, Xn, eig_j, occ_j, mu_j = one_map(rho, X, False, 1e-11, False)
+        res_j = float((rho_new - rho).abs().sum()) * engine.omega / engine.nfft / st.nelec
+        hist.append(res_j)
+        if best is not None and res_j > best[5]:
             break
-    # final eigenvectors of H[rho] (scf.py:274-281)
-    _, X, eig, occ, mu = one_map(rho, X, False, 1e-11, False)
+        best = (rho, Xn, eig_j, occ_j, mu_j, res_j)
+        X = Xn
+        if j == polish or res_j < 1e-14:
+            break
+        rho = rho_new
+    rho, X, eig, occ, mu, res = best
     X = [x[:nb].contiguous() for x in X]
     info = {"iterations": it, "residual_history": hist, "davidson_iterations": dav_iters,
             "fp64_from_iteration": info_switch,

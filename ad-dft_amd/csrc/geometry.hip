@@ -494,6 +494,8 @@ int addft_nonlocal_backward(int dtype, int64_t nproj, int64_t ng, const int32_t*
   return ADDFT_OK;
 }
 
+int64_t addft_reduce_work_bytes(void) { return (int64_t)sizeof(double) * 128 * 10 + 256; }
+
 int addft_gsum_outer(int64_t ng, const double* w, const double* a, const double* b, double* out, void* work,
                      void* stream) {
   ADDFT_CHECK_ARG(w && a && b && out && work && ng > 0, "bad argument");

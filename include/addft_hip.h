@@ -191,7 +191,11 @@ int addft_nonlocal_backward(int dtype, int64_t nproj, int64_t ng, const int32_t*
                             const double* d_dbeta, const double* d_pos, const double* h_kpt,
                             double inv_sqrt_omega, const void* d_Y, double* d_out,
                             void* d_work, void* stream);
-/* out[0] = sum_G w, out[1+3m+n] = sum_G w a_m b_n (fixed-order sums).   */
+/* Workspace (bytes) for the fixed-order block sums of addft_gsum_outer
+ * and addft_hartree_g.                                                  */
+int64_t addft_reduce_work_bytes(void);
+/* out[0] = sum_G w, out[1+3m+n] = sum_G w a_m b_n (fixed-order sums).
+ * d_work: addft_reduce_work_bytes() bytes.                              */
 int addft_gsum_outer(int64_t ng, const double* d_w, const double* d_a, const double* d_b,
                      double* d_out, void* d_work, void* stream);
 
@@ -212,7 +216,8 @@ int addft_occupations(const double* d_eig, const double* d_w, int nk, int nb,
                       double* d_mu, void* stream);
 /* Hartree on a sphere: vh = 4 pi n/G^2 (G^2 > 1e-14), out10[0] =
  * sum 4 pi |n|^2/G^2, out10[1..9] = sum 4 pi |n|^2 2 G_a G_b / G^4
- * (physics/hartree.py:21-32).  vh may be NULL.                          */
+ * (physics/hartree.py:21-32).  vh may be NULL.  d_work:
+ * addft_reduce_work_bytes() bytes.                                      */
 int addft_hartree_g(int64_t ng, const double* d_g, const double* d_n_g, double* d_vh_g,
                     double* d_out10, void* d_work, void* stream);
 /* Kerker mix on a plane-major box (solver/scf.py:215-223):

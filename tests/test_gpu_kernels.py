@@ -242,3 +242,39 @@ def test_box_sphere_matches_plan():
     g = torch.as_tensor(0.1 + rng.random(b.nfft)).to(_dev())
     s1, s2 = plan.to_sphere(g, 1.0 / b.nfft), box.to_sphere(g, 1.0 / b.nfft)
     assert (s1 - s2).abs().max().item() < 1e-13
+
+
+# ------------------------------------------------------- reduction workspace
+@pytest.mark.parametrize("entry", ["gsum_outer", "hartree_g"])
+def test_reduction_workspace_stays_in_bounds(entry):
+    """The fixed-order block sums write only the documented workspace
+    (addft_reduce_work_bytes()): a canary placed right after it survives,
+    and the sums match NumPy."""
+    torch = _torch()
+    from addft.kernels._lib import check, lib, ptr, stream
+    L = lib()
+    wb = int(L.addft_reduce_work_bytes())
+    assert wb % 8 == 0
+    canary = 4096
+    work = torch.full((wb // 8 + canary,), 7.25, dtype=torch.float64, device=_dev())
+    rng = np.random.default_rng(5)
+    ng = 70001
+    g = rng.normal(size=(ng, 3))
+    out = torch.zeros(10, dtype=torch.float64, device=_dev())
+    if entry == "gsum_outer":
+        w, a = rng.random(ng), rng.normal(size=(ng, 3))
+        dw, da, dg = (torch.as_tensor(np.ascontiguousarray(x)).to(_dev()) for x in (w, a, g))
+        check(L.addft_gsum_outer(ng, ptr(dw), ptr(da), ptr(dg), ptr(out), ptr(work), stream()), "gsum_outer")
+        ref = np.concatenate([[w.sum()], np.einsum("g,gm,gn->mn", w, a, g).ravel()])
+    else:
+        n_g = rng.normal(size=ng) + 1j * rng.normal(size=ng)
+        dg = torch.as_tensor(g).to(_dev())
+        dn = torch.as_tensor(n_g).to(_dev())
+        vh = torch.empty_like(dn)
+        check(L.addft_hartree_g(ng, ptr(dg), ptr(dn), ptr(vh), ptr(out), ptr(work), stream()), "hartree_g")
+        g2 = (g ** 2).sum(1)
+        e = 4 * np.pi * np.abs(n_g) ** 2 / g2
+        ref = np.concatenate([[e.sum()], np.einsum("g,ga,gb->ab", 2 * e / g2, g, g).ravel()])
+    torch.cuda.synchronize()
+    assert bool((work[wb // 8:] == 7.25).all())
+    assert np.allclose(out.cpu().numpy(), ref, rtol=1e-11, atol=0)
