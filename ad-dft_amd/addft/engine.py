@@ -69,6 +69,8 @@ class Engine:
         self._prep_psp()
         self._work = {}
         self.geo = None
+        # band-applications of H by precision (a work counter for reports)
+        self.hpsi_bands = {"fp32": 0, "fp64": 0}
 
     # ------------------------------------------------------------- helpers
     def work(self, key, nbytes):
@@ -229,6 +231,7 @@ class Engine:
         tk = self.tkin32[ik] if fp32 else self.tkin64[ik]
         P = self.P32[ik] if fp32 else self.P64[ik]
         D = self.D32 if fp32 else self.D64
+        self.hpsi_bands["fp32" if fp32 else "fp64"] += int(psi.shape[0])
         out = plan.hpsi_local(psi, tk, vs)
         becp_t = inner(psi, P)                          # (nb, nproj) = <beta|psi>^T
         out.add_(torch.matmul(torch.matmul(becp_t, D), P))

@@ -133,12 +133,15 @@ def dense_lowest(apply, nG, nb, dtype, device):
     return w[:nb].clone(), X, res, 1
 
 
-def davidson(apply, X0, nb, diag, max_sub=None, max_iter=40, tol=1e-9, min_iter=1, nconv=None):
+def davidson(apply, X0, nb, diag, max_sub=None, max_iter=40, tol=1e-9, min_iter=1, nconv=None, weight=None):
     """Return (eig (nb,) float64, X (nb, nG), max residual norm, iterations).
 
     apply: (m, nG) -> (m, nG) in X0's dtype.  diag: (nG,) float64.
     nconv: number of leading bands that must meet `tol` (default nb); the
-    remaining nb - nconv are buffer bands that only accelerate convergence."""
+    remaining nb - nconv are buffer bands that only accelerate convergence.
+    weight: optional (nb,) factors in (0, 1] on the residual norms used for
+    the convergence test (occupation weights: a band's error enters the
+    density in proportion to its occupation)."""
     nconv = nb if nconv is None else int(nconv)
     nG = X0.shape[1]
     if nG <= max(DENSE_MAX_NG, 4 * nb):
@@ -166,13 +169,14 @@ def davidson(apply, X0, nb, diag, max_sub=None, max_iter=40, tol=1e-9, min_iter=
         HX = torch.matmul(Ub.T, HQ)
         R = HX - lam.to(X.real.dtype)[:, None] * X
         rn = torch.linalg.vector_norm(R.to(torch.complex128), dim=1)
-        res = float(rn[:nconv].max())
+        rw = rn if weight is None else rn * weight.to(rn.dtype)
+        res = float(rw[:nconv].max())
         if res < tol and it >= min_iter:
             break
         if it == max_iter:
             break
         # expand only with the residuals of unconverged bands
-        active = rn > tol
+        active = rw > tol
         R = precondition(R[active], lam[active], diag)
         R = R / torch.linalg.vector_norm(R.to(torch.complex128), dim=1).to(R.real.dtype)[:, None]
         if Q.shape[0] + R.shape[0] > max_sub:

@@ -1,12 +1,22 @@
 """Benchmark of the Kohn-Sham Hpsi / density / adjoint path on one MI355X.
 
-Default workload (one "step"): one local Hpsi application over all 480 bands
-of the 216-atom Si cell (3x3x3 conventional, a = 30.78 Bohr, ecut = 15 Ha,
-box 120^3, nG = 80 725), FP32 (the mixed-precision form) with the FP64 form
-timed beside it.  Inputs are synthetic and already resident in HBM.
+Headline (one "step"): SCF + forces + stress for one geometry of the
+216-atom Si cell (3x3x3 conventional, a = 30.78 Bohr, seeded +-0.03 Bohr
+displacements, ecut 15 Ha, nband 480, Gamma, sigma 0.002 Ha), in the
+mixed-precision form (FP32 Hpsi / projector / subspace products, FP64
+Rayleigh-Ritz eigensolves, energies and final fixed-point iterations).  The
+step includes the per-geometry tables (projectors, V_loc, Ewald), the SCF
+from the uniform density, and the forces and stress of the variational
+envelope.  The FP64 form (every product in FP64, the reference's
+tolerance 1e-10) runs once beside it for the speed-up and the error.
 
-Prints one JSON line (rank 0).  --gpus N > 1 exits: this build measures one
-GPU.
+Also measured: one local Hpsi over the 480 bands (the HBM-bound kernel
+family), timed with HIP events on the stream it runs on, as the roofline
+figure; and the oracle's NumPy Hpsi on a bounded band sample on the host,
+extrapolated to a CPU seconds-per-geometry by the band-applications of H the
+GPU mixed run made.
+
+Prints one JSON line.  --gpus N > 1 exits: this build measures one GPU.
 """
 
 from __future__ import annotations
@@ -14,6 +24,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import re
 import sys
 import time
 from pathlib import Path
@@ -24,7 +35,11 @@ for _p in (ROOT, ROOT / "ad-dft_amd"):
         sys.path.insert(0, str(_p))
 
 HBM_PEAK_GBS = 8000.0
-A_SI216 = 30.78
+TRAFFIC_FILE = ROOT / "profiles" / "hpsi_local_si216_traffic.md"
+
+
+def say(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def hpsi_bytes(nb, ng, nst, n3, nfft, cbytes, rbytes):
@@ -52,55 +67,10 @@ def timed(fn, steps, warmup, torch):
     return start.elapsed_time(end) / steps  # ms
 
 
-def cpu_baseline_hpsi(nb_sample=48):
-    """The oracle's matrix-free FP64 local Hpsi (NumPy FFT) on a bounded
-    sample of bands of the same 216-atom system, scaled to GB/s with the FP64
-    algorithmic byte count."""
+def hpsi_measure(basis, lat, nb, steps, warmup, torch, native):
+    """Local Hpsi and density over nb bands, FP32 and FP64 forms."""
     import numpy as np
-
-    from oracle import basis as OB
-    lat = A_SI216 * np.eye(3)
-    b = OB.make_basis(lat, 15.0)
-    rng = np.random.default_rng(0)
-    psi = rng.normal(size=(1, nb_sample, b.nG)) + 1j * rng.normal(size=(1, nb_sample, b.nG))
-    v = 0.1 * rng.random(b.nfft)
-    t0 = time.perf_counter()
-    pr = OB.psi_to_real(b, lat, psi)
-    out = OB.real_to_psi(b, lat, pr * v[None, None, :])
-    dt = time.perf_counter() - t0
-    flat = OB.flat_index(b.fft_shape, b.miller)
-    nst = len(np.unique(flat // b.fft_shape[2]))
-    by = hpsi_bytes(nb_sample, b.nG, nst, b.fft_shape[2], b.nfft, 16, 8)
-    del out
-    return {"value": by / dt / 1e9, "unit": "GB/s", "cores": 1, "kind": "port",
-            "sample": f"oracle NumPy FP64 local Hpsi on {nb_sample} bands of the 216-atom cell "
-                      f"({dt:.2f} s), same algorithmic byte count"}
-
-
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--nband", type=int, default=480)
-    ap.add_argument("--no-cpu", action="store_true")
-    args = ap.parse_args()
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(see DESIGN.md for how the path would shard).", file=sys.stderr)
-        sys.exit(2)
-
-    import numpy as np
-    import torch
-
-    from addft.geometry.basis import make_basis
-    from addft.kernels import native
-
     dev = torch.device("cuda:0")
-    torch.cuda.set_device(dev)
-    lat = A_SI216 * np.eye(3)
-    basis = make_basis(lat, 15.0)
-    nb = args.nband
     g = torch.Generator(device="cpu").manual_seed(0)
     res = {}
     for name, cdt, rdt, cb, rb in (("fp32", torch.complex64, torch.float32, 8, 4),
@@ -114,46 +84,188 @@ def main():
         vs = plan.stage(native.to_planar(v, basis.fft_shape), 1.0 / basis.nfft)
         out = torch.empty_like(psi)
         plan.work(nb)
-        ms = timed(lambda: plan.hpsi_local(psi, tk, vs, out), args.steps, args.warmup, torch)
+        ms = timed(lambda: plan.hpsi_local(psi, tk, vs, out), steps, warmup, torch)
         by = hpsi_bytes(nb, basis.nG, plan.n_sticks, basis.fft_shape[2], basis.nfft, cb, rb)
         w = torch.full((nb,), 2.0 / float(np.linalg.det(lat)), dtype=torch.float64, device=dev)
         rho = torch.empty(basis.nfft, dtype=torch.float64, device=dev)
-        msd = timed(lambda: plan.density(psi, w, out=rho), max(2, args.steps // 4), 1, torch)
+        msd = timed(lambda: plan.density(psi, w, out=rho), max(2, steps // 4), 1, torch)
         dby = density_bytes(nb, basis.nG, plan.n_sticks, basis.fft_shape[2], basis.nfft, cb)
         res[name] = {"hpsi_ms": ms, "hpsi_GBps": by / (ms * 1e-3) / 1e9, "hpsi_bytes": by,
                      "density_ms": msd, "density_GBps": dby / (msd * 1e-3) / 1e9,
                      "n_sticks": plan.n_sticks}
         del psi, out, plan
         torch.cuda.empty_cache()
+    return res
 
-    f32 = res["fp32"]
+
+def measured_traffic():
+    """HBM bytes per local-Hpsi call from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated), if present."""
+    if not TRAFFIC_FILE.exists():
+        return None
+    m = re.search(r"traffic_bytes_per_call:\s*([0-9.eE+]+)", TRAFFIC_FILE.read_text())
+    return float(m.group(1)) if m else None
+
+
+def one_geometry(addft, system, tol, torch):
+    """Per-geometry tables + SCF + forces + stress, timed on the host clock
+    with the device synchronised on both sides."""
+    eng = addft.api._engine(system.static, system.geo)
+    eng.geo = None                    # rebuild the per-geometry tables inside the step
+    for k in eng.hpsi_bands:
+        eng.hpsi_bands[k] = 0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    eng.set_geometry(system.geo)
+    F, forces, sig, info = addft.energy_forces_stress(system, tol=tol)
+    torch.cuda.synchronize()
+    return F, forces, sig, time.perf_counter() - t0, info, dict(eng.hpsi_bands)
+
+
+def cpu_baseline(basis, lat, nproj, band_apps, nb_sample=32, nl_cols=8192):
+    """The oracle's FP64 Hpsi on the host: local part (NumPy FFTs) on a
+    sample of bands of the same cell, nonlocal part as the two projector
+    GEMMs on a slice of the G-sphere; per-band cost times the band
+    applications of H counted in the GPU mixed run.  A lower bound of a
+    CPU SCF+forces+stress (subspace algebra and the rest excluded)."""
+    import numpy as np
+    from oracle import basis as OB
+    from threadpoolctl import threadpool_limits
+    with threadpool_limits(1):
+        return _cpu_baseline(OB, np, lat, nproj, band_apps, nb_sample, nl_cols)
+
+
+def _cpu_baseline(OB, np, lat, nproj, band_apps, nb_sample, nl_cols):
+    b = OB.make_basis(lat, 15.0)
+    rng = np.random.default_rng(0)
+    psi = rng.normal(size=(1, nb_sample, b.nG)) + 1j * rng.normal(size=(1, nb_sample, b.nG))
+    v = 0.1 * rng.random(b.nfft)
+    t0 = time.perf_counter()
+    pr = OB.psi_to_real(b, lat, psi)
+    OB.real_to_psi(b, lat, pr * v[None, None, :])
+    t_loc = (time.perf_counter() - t0) / nb_sample
+    P = rng.normal(size=(nproj, nl_cols)) + 1j * rng.normal(size=(nproj, nl_cols))
+    ps = psi[0, :, :nl_cols]
+    D = np.eye(nproj)
+    t0 = time.perf_counter()
+    bec = P.conj() @ ps.T
+    _ = (D @ bec).T @ P
+    t_nl = (time.perf_counter() - t0) / nb_sample * (b.nG / nl_cols)
+    per_band = t_loc + t_nl
+    total = per_band * band_apps
+    return {"value": round(total, 2), "unit": "s/geometry", "cores": 1, "kind": "port",
+            "sample": (f"oracle FP64 Hpsi on {nb_sample} bands of the 216-atom cell (local: NumPy FFT "
+                       f"{t_loc * 1e3:.1f} ms/band; nonlocal: projector GEMMs on {nl_cols} of {b.nG} G, "
+                       f"scaled, {t_nl * 1e3:.1f} ms/band) x {band_apps} band-applications of H counted in "
+                       f"the GPU mixed run; lower bound (subspace algebra excluded), extrapolated")}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=0)
+    ap.add_argument("--workload", default="c5", help="benchmark system (c2, c3, c5)")
+    ap.add_argument("--mixed-tol", type=float, default=1e-7)
+    ap.add_argument("--fp64-tol", type=float, default=1e-10)
+    ap.add_argument("--hpsi-steps", type=int, default=20)
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(DESIGN.md describes how the path would shard).", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import addft
+    from addft import testsystems
+    from addft.kernels import native
+
+    torch.cuda.set_device(0)
+    kw = testsystems.config(args.workload, precision="mixed")
+    lat = np.asarray(kw["lattice"])
+    nat = len(kw["frac"])
+
+    say("library warm-up (8-atom cell, untimed)")
+    addft.energy_forces_stress(addft.setup(**testsystems.config("c2", precision="mixed")), tol=1e-8)
+
+    sys_m = addft.setup(**kw)
+    basis = sys_m.static.basis
+    nb = sys_m.static.nband
+    say(f"local Hpsi roofline: {nb} bands, nG {basis.nG}, box {tuple(basis.fft_shape)}")
+    hp = hpsi_measure(basis, lat, nb, args.hpsi_steps, 3, torch, native)
+    say(f"  fp32 {hp['fp32']['hpsi_ms']:.3f} ms ({hp['fp32']['hpsi_GBps']:.0f} GB/s), "
+        f"fp64 {hp['fp64']['hpsi_ms']:.3f} ms ({hp['fp64']['hpsi_GBps']:.0f} GB/s)")
+
+    ref = None
+    if not args.no_fp64:
+        say(f"FP64 form, tol {args.fp64_tol:g}")
+        sys_d = addft.setup(**testsystems.config(args.workload, precision="fp64"))
+        ref = one_geometry(addft, sys_d, args.fp64_tol, torch)
+        say(f"  F={ref[0]:.10f}  {ref[3]:.2f} s  {ref[4]['iterations']} SCF iterations")
+        addft.api._ENGINES.clear()
+        del sys_d
+        torch.cuda.empty_cache()
+
+    for i in range(args.warmup):
+        say(f"mixed warm-up step {i + 1}/{args.warmup}")
+        one_geometry(addft, sys_m, args.mixed_tol, torch)
+    times, last = [], None
+    torch.cuda.synchronize()
+    for i in range(args.steps):
+        last = one_geometry(addft, sys_m, args.mixed_tol, torch)
+        times.append(last[3])
+        say(f"mixed step {i + 1}/{args.steps}: F={last[0]:.10f}  {last[3]:.2f} s  "
+            f"{last[4]['iterations']} SCF iterations (FP64 from {last[4].get('fp64_from_iteration')})")
+    torch.cuda.synchronize()
+    sec = float(np.mean(times))
+    F, forces, sig, _, info, bands = last
+    nproj = int(addft.api._engine(sys_m.static, sys_m.geo).nproj)
+
+    f32 = hp["fp32"]
+    traffic = measured_traffic()
     line = {
         "metric": "SCF+forces s/geometry, mixed vs FP64 GPU at <=1e-5 Ha/atom; Hpsi HBM GB/s",
-        "value": round(f32["hpsi_GBps"], 2),
-        "unit": "GB/s",
+        "value": round(sec, 3),
+        "unit": "s/geometry",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(f32["hpsi_ms"], 4),
-        "higher_is_better": True,
+        "ms_per_step": round(sec * 1e3, 1),
+        "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "c64",
+        "dtype": "c64/f32 Hpsi and subspace products; f64 eigensolves, energies, final iterations",
         "data": "synthetic",
-        "config": {"workload": "local Hpsi, 216-atom Si cell (3x3x3 conventional, a=30.78 Bohr), "
-                               "ecut 15 Ha, 480 bands, box 120^3, nG 80725, Gamma",
-                   "nband": nb, "nG": basis.nG, "fft_box": list(basis.fft_shape),
-                   "n_sticks": f32["n_sticks"]},
+        "config": {"workload": f"SCF+forces+stress per geometry, {nat}-atom Si Gamma "
+                               f"({args.workload}: a={lat[0, 0]:.2f} Bohr, ecut {kw['ecut']} Ha, "
+                               f"nband {nb}, sigma {kw.get('sigma', 0.05)} Ha, synthetic 3-projector UPF)",
+                   "nG": basis.nG, "fft_box": list(basis.fft_shape), "nproj": nproj,
+                   "mixed_tol": args.mixed_tol, "fp64_tol": args.fp64_tol},
+        "mixed": {"s_per_geometry": round(sec, 3), "scf_iterations": info["iterations"],
+                  "fp64_from_iteration": info.get("fp64_from_iteration"),
+                  "hpsi_band_applications": bands, "F": F},
         "roofline": {"bound": "hbm", "achieved": round(f32["hpsi_GBps"], 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(f32["hpsi_GBps"] / HBM_PEAK_GBS, 4),
-                     "traffic": None,
-                     "kernel": "hpsi_local (k_z_inverse + k_plane_v + k_z_forward)",
-                     "algorithmic_bytes": f32["hpsi_bytes"]},
-        "fp64": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in res["fp64"].items()},
-        "fp32": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in f32.items()},
+                     "traffic": traffic,
+                     "kernel": "addft_hpsi_local FP32 (k_z_inverse + k_plane_v + k_z_forward), "
+                               f"{nb} bands", "algorithmic_bytes": f32["hpsi_bytes"],
+                     "ms_per_call": round(f32["hpsi_ms"], 4)},
+        "hpsi": {p: {k: (round(v, 4) if isinstance(v, float) else v) for k, v in hp[p].items()} for p in hp},
     }
+    if ref is not None:
+        F64, f64, s64, t64, info64, _ = ref
+        line["fp64"] = {"s_per_geometry": round(t64, 3), "scf_iterations": info64["iterations"], "F": F64}
+        line["speedup_mixed_vs_fp64"] = round(t64 / sec, 3)
+        line["error_mixed_vs_fp64"] = {"energy_Ha_per_atom": float(abs(F - F64) / nat),
+                                       "force_Ha_per_Bohr": float(np.abs(forces - f64).max()),
+                                       "stress_Ha_per_Bohr3": float(np.abs(sig - s64).max())}
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline_hpsi()
+        say("CPU baseline (oracle, bounded sample)")
+        line["cpu_baseline"] = cpu_baseline(basis, lat, nproj, bands["fp32"] + bands["fp64"])
     print(json.dumps(line), flush=True)
 
 
