@@ -19,6 +19,7 @@ _LIB = None
 _PATH = None
 
 C64, C128 = 0, 1
+PLANE_LDS, PLANE_LANE = 0, 1
 
 
 class NativeUnavailable(RuntimeError):
@@ -50,6 +51,8 @@ _SIGS = {
     "addft_sphere_plan_destroy": ([_VP], _I32),
     "addft_sphere_plan_info": ([_VP, ctypes.POINTER(_I64), ctypes.POINTER(_I64), _VP], _I32),
     "addft_sphere_work_bytes": ([_VP, _I64], _I64),
+    "addft_sphere_plan_set_kernel": ([_VP, _I32], _I32),
+    "addft_sphere_plan_get_kernel": ([_VP, ctypes.POINTER(_I32), ctypes.POINTER(_I32)], _I32),
     "addft_sphere_permute": ([_VP, _I32, _I32, _VP, _I64, _VP, _VP], _I32),
     "addft_grid_transpose": ([_I32, _I32, _I32, _I32, _VP, _VP, _VP], _I32),
     "addft_grid_stage": ([_VP, _VP, _D, _VP, _VP], _I32),

@@ -136,6 +136,25 @@ class SpherePlan:
     def handle(self):
         return self._h
 
+    @property
+    def plane_kernel(self) -> str:
+        """"lane" (lane-group register FFTs) or "lds" (workgroup Stockham)."""
+        k, avail = _lib.ctypes.c_int(), _lib.ctypes.c_int()
+        check(lib().addft_sphere_plan_get_kernel(self._h, _lib.ctypes.byref(k), _lib.ctypes.byref(avail)),
+              "addft_sphere_plan_get_kernel")
+        return "lane" if k.value == _lib.PLANE_LANE else "lds"
+
+    @property
+    def lane_available(self) -> bool:
+        k, avail = _lib.ctypes.c_int(), _lib.ctypes.c_int()
+        check(lib().addft_sphere_plan_get_kernel(self._h, _lib.ctypes.byref(k), _lib.ctypes.byref(avail)),
+              "addft_sphere_plan_get_kernel")
+        return bool(avail.value)
+
+    def set_plane_kernel(self, kind: str):
+        check(lib().addft_sphere_plan_set_kernel(self._h, _lib.PLANE_LANE if kind == "lane" else _lib.PLANE_LDS),
+              "addft_sphere_plan_set_kernel")
+
     def work(self, nb: int) -> torch.Tensor:
         need = int(lib().addft_sphere_work_bytes(self._h, max(1, nb)))
         if self._work is None or self._work.numel() < need:

@@ -49,23 +49,23 @@ __host__ __device__ __forceinline__ cpx<R> mk(R a, R b) {
   return c;
 }
 template <typename R>
-__device__ __forceinline__ cpx<R> operator+(cpx<R> a, cpx<R> b) { return mk(a.x + b.x, a.y + b.y); }
+__host__ __device__ __forceinline__ cpx<R> operator+(cpx<R> a, cpx<R> b) { return mk(a.x + b.x, a.y + b.y); }
 template <typename R>
-__device__ __forceinline__ cpx<R> operator-(cpx<R> a, cpx<R> b) { return mk(a.x - b.x, a.y - b.y); }
+__host__ __device__ __forceinline__ cpx<R> operator-(cpx<R> a, cpx<R> b) { return mk(a.x - b.x, a.y - b.y); }
 template <typename R>
-__device__ __forceinline__ cpx<R> operator*(cpx<R> a, R s) { return mk(a.x * s, a.y * s); }
+__host__ __device__ __forceinline__ cpx<R> operator*(cpx<R> a, R s) { return mk(a.x * s, a.y * s); }
 template <typename R>
-__device__ __forceinline__ cpx<R> cmul(cpx<R> a, cpx<R> b) {
+__host__ __device__ __forceinline__ cpx<R> cmul(cpx<R> a, cpx<R> b) {
   return mk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 // a * conj(b)
 template <typename R>
-__device__ __forceinline__ cpx<R> cmulc(cpx<R> a, cpx<R> b) {
+__host__ __device__ __forceinline__ cpx<R> cmulc(cpx<R> a, cpx<R> b) {
   return mk(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
 }
 // multiply by -i (forward) or +i (inverse)
 template <bool INV, typename R>
-__device__ __forceinline__ cpx<R> mul_mi(cpx<R> a) {
+__host__ __device__ __forceinline__ cpx<R> mul_mi(cpx<R> a) {
   return INV ? mk(-a.y, a.x) : mk(a.y, -a.x);
 }
 

@@ -21,6 +21,7 @@
 #include <vector>
 
 #include "fft_device.h"
+#include "lane_fft.h"
 
 using addft::cpx;
 using addft::FftFactors;
@@ -48,6 +49,12 @@ struct addft_sphere_plan_s {
   int cchunk;  // columns per plane-pass chunk
   int ept_p;   // elements per thread of the plane pass
   size_t lds_plane;
+  // lane-group plane pass (cubic-in-plane boxes n1 == n2 with a codelet)
+  int lane_ok;        // the sphere's rows and sticks are cyclic intervals
+  int lane_kernel;    // 1: lane-group plane pass, 0: LDS Stockham plane pass
+  int lane_waves;     // wavefronts per workgroup of the lane-group pass
+  int a1;             // first i1 of the cyclic interval of rows
+  int32_t* d_rowinfo; // per row: first stick, first i2, sticks in the row, 0
 };
 
 namespace {
@@ -211,6 +218,147 @@ k_plane_v(cpx<R>* __restrict__ T, int64_t nst, int nra, const int32_t* __restric
   }
   addft::lds_fft<R, EPT, false>(P, f2, nra, n2, 1, tw2);
   store_plane(P, Tp, n2, nst, srow, scol);
+}
+
+// ------------------------------------------------- lane-group plane pass
+// The same fused plane operation as k_plane_v for boxes with n1 == n2 == N:
+// every length-N transform is carried by L lanes (lane::fwd / lane::bwd),
+// so the LDS plane P (rows x N) is shared through workgroup barriers only
+// twice per (plane, band).  Rows and sticks are cyclic intervals of i1 / i2,
+// read and written in place with lane::cyc_pos (no zero fill).
+//   phase 1: rows      sticks -> inverse along i2 -> P[r][y]
+//   phase 2: columns   P[.][y] -> inverse along i1, x V, forward -> P[.][y]
+//   phase 3: rows      P[r][.] -> forward along i2 -> sticks
+// Grid: (band, plane); band fastest so that neighbouring workgroups share
+// the V(r) plane in L2.  rowinfo[r] = (first stick, first i2, count, 0).
+template <typename R, int N, int L, int WAVES>
+__global__ void __launch_bounds__(64 * WAVES)
+k_plane_v_lane(cpx<R>* __restrict__ T, int64_t nst, int nra, int a1, const int4* __restrict__ rowinfo,
+               const cpx<R>* __restrict__ gtw, const R* __restrict__ v, int n3) {
+  using S = addft::lane::Shape<N, L>;
+  constexpr int M = S::M, Q = S::Q, G = S::G;
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+  cpx<R>* tw = reinterpret_cast<cpx<R>*>(smem_raw);
+  cpx<R>* P = tw + N;
+  cpx<R>* scr_all = P + (size_t)nra * N;
+  const int wave = threadIdx.x >> 6, lid = threadIdx.x & 63;
+  const int g = lid / L, t = lid - g * L;
+  const bool act = g < G;
+  // G + 1 tiles per wave: lanes beyond the last full group use the spare one
+  cpx<R>* scr = scr_all + (size_t)(wave * (G + 1) + g) * S::SCR;
+  const int64_t b = blockIdx.x;
+  const int z = blockIdx.y;
+  for (int i = threadIdx.x; i < N; i += blockDim.x) tw[i] = gtw[i];
+  __syncthreads();
+  cpx<R>* Tp = T + (b * n3 + z) * nst;
+
+  // phase 1
+  for (int base = wave * G; base < nra; base += WAVES * G) {
+    const int r = base + g;
+    const bool ok = act && r < nra;
+    const int4 ri = ok ? rowinfo[r] : make_int4(0, 0, 0, 0);
+    cpx<R> x[M];
+#pragma unroll
+    for (int j = 0; j < M; ++j) {
+      const int pos = ok ? addft::lane::cyc_pos(t + L * j, ri.y, ri.z, N) : -1;
+      x[j] = pos >= 0 ? Tp[ri.x + pos] : addft::mk<R>(0, 0);
+    }
+    cpx<R> o[Q][L];
+    addft::lane::fwd<R, N, L, true>(x, o, t, scr, tw);
+    if (ok) {
+      cpx<R>* Pr = P + (size_t)r * N;
+#pragma unroll
+      for (int q = 0; q < Q; ++q)
+        if (q * L + t < M) {
+#pragma unroll
+          for (int k1 = 0; k1 < L; ++k1) Pr[q * L + t + M * k1] = o[q][k1];
+        }
+    }
+  }
+  __syncthreads();
+
+  // phase 2
+  const R* vp = v + (int64_t)z * N * N;
+  for (int base = wave * G; base < N; base += WAVES * G) {
+    const int y = base + g;
+    const bool ok = act && y < N;
+    cpx<R> x[M];
+#pragma unroll
+    for (int j = 0; j < M; ++j) {
+      const int rr = ok ? addft::lane::cyc_pos(t + L * j, a1, nra, N) : -1;
+      x[j] = rr >= 0 ? P[(size_t)rr * N + y] : addft::mk<R>(0, 0);
+    }
+    cpx<R> o[Q][L];
+    addft::lane::fwd<R, N, L, true>(x, o, t, scr, tw);
+    if (ok) {
+#pragma unroll
+      for (int q = 0; q < Q; ++q)
+        if (q * L + t < M) {
+#pragma unroll
+          for (int k1 = 0; k1 < L; ++k1) o[q][k1] = o[q][k1] * vp[(q * L + t + M * k1) * N + y];
+        }
+    }
+    addft::lane::bwd<R, N, L, false>(o, x, t, scr, tw);
+    if (ok) {
+#pragma unroll
+      for (int j = 0; j < M; ++j) {
+        const int rr = addft::lane::cyc_pos(t + L * j, a1, nra, N);
+        if (rr >= 0) P[(size_t)rr * N + y] = x[j];
+      }
+    }
+  }
+  __syncthreads();
+
+  // phase 3
+  for (int base = wave * G; base < nra; base += WAVES * G) {
+    const int r = base + g;
+    const bool ok = act && r < nra;
+    cpx<R> o[Q][L];
+#pragma unroll
+    for (int q = 0; q < Q; ++q)
+#pragma unroll
+      for (int k1 = 0; k1 < L; ++k1)
+        o[q][k1] = (ok && q * L + t < M) ? P[(size_t)r * N + q * L + t + M * k1] : addft::mk<R>(0, 0);
+    cpx<R> x[M];
+    addft::lane::bwd<R, N, L, false>(o, x, t, scr, tw);
+    if (ok) {
+      const int4 ri = rowinfo[r];
+#pragma unroll
+      for (int j = 0; j < M; ++j) {
+        const int pos = addft::lane::cyc_pos(t + L * j, ri.y, ri.z, N);
+        if (pos >= 0) Tp[ri.x + pos] = x[j];
+      }
+    }
+  }
+}
+
+// lanes per transform for the box sizes with a lane-group kernel (0: none)
+constexpr int lane_width(int n) {
+  switch (n) {
+    case 9: return 3;
+    case 12: return 4;
+    case 32: return 4;
+    case 48: return 4;
+    case 50: return 5;
+    case 60: return 4;
+    case 64: return 8;
+    case 72: return 8;
+    case 75: return 5;
+    case 80: return 8;
+    case 96: return 8;
+    case 100: return 4;
+    case 120: return 8;
+    case 128: return 8;
+    default: return 0;
+  }
+}
+
+template <typename R>
+size_t lane_lds_bytes(int n, int nra, int waves) {
+  const int L = lane_width(n);
+  if (L == 0) return 0;
+  const int G = 64 / L;
+  return sizeof(cpx<R>) * ((size_t)n + (size_t)nra * n + (size_t)waves * (G + 1) * L * (L + 1));
 }
 
 // Density: for plane z and band group g (blockIdx.y), accumulate
@@ -419,6 +567,57 @@ int set_plane_attrs_any(int ept, size_t lds) {
   }
 }
 
+constexpr int kLaneWaves = 4;
+
+#define ADDFT_LANE_SIZES(X) \
+  X(9) X(12) X(32) X(48) X(50) X(60) X(64) X(72) X(75) X(80) X(96) X(100) X(120) X(128)
+
+// Launch (or, with nb == 0, only configure the LDS limit of) the lane-group
+// plane pass for the plan's box size.
+template <typename R>
+int launch_plane_lane(const addft_sphere_plan_s* p, cpx<R>* T, const R* v, int64_t nb, hipStream_t st) {
+  const size_t lds = lane_lds_bytes<R>(p->n1, p->nra, kLaneWaves);
+  const dim3 grid((unsigned)std::max<int64_t>(nb, 1), (unsigned)p->n3);
+  auto tw = reinterpret_cast<const cpx<R>*>(p->d_tw1);
+  auto ri = reinterpret_cast<const int4*>(p->d_rowinfo);
+#define ADDFT_LANE_CASE(NN)                                                                         \
+  case NN:                                                                                          \
+    if (nb == 0) {                                                                                  \
+      ADDFT_HIP(hipFuncSetAttribute((const void*)k_plane_v_lane<R, NN, lane_width(NN), kLaneWaves>, \
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));         \
+      return ADDFT_OK;                                                                              \
+    }                                                                                               \
+    hipLaunchKernelGGL((k_plane_v_lane<R, NN, lane_width(NN), kLaneWaves>), grid, dim3(64 * kLaneWaves), \
+                       lds, st, T, p->nst, p->nra, p->a1, ri, tw, v, p->n3);                       \
+    break;
+  switch (p->n1) {
+    ADDFT_LANE_SIZES(ADDFT_LANE_CASE)
+    default:
+      addft::set_error("no lane-group plane kernel for this box");
+      return ADDFT_EINTERNAL;
+  }
+#undef ADDFT_LANE_CASE
+  ADDFT_LAUNCH_CHECK();
+  return ADDFT_OK;
+}
+
+// Sorted list -> start a of the cyclic interval of Z_n it enumerates (in
+// ascending order, wrapped part first); false if it is not one.
+bool cyclic_interval(const std::vector<int>& lst, int n, int* a) {
+  const int len = (int)lst.size();
+  if (len == 0 || len > n) return false;
+  int start = lst[0];
+  for (int i = 0; i + 1 < len; ++i)
+    if (lst[i + 1] != lst[i] + 1) {
+      start = lst[i + 1];
+      break;
+    }
+  for (int i = 0; i < len; ++i)
+    if (addft::lane::cyc_pos(lst[i], start, len, n) != i) return false;
+  *a = start;
+  return true;
+}
+
 int64_t t_bytes(const addft_sphere_plan_s* p, int64_t nb) {
   size_t cs = p->precision == ADDFT_C64 ? 8 : 16;
   return nb * (int64_t)p->n3 * p->nst * (int64_t)cs;
@@ -442,6 +641,10 @@ int launch_hpsi(addft_sphere_plan_s* p, const void* psi, const void* tkin, const
   auto tw1 = reinterpret_cast<const cpx<R>*>(p->d_tw1);
   auto tw2 = reinterpret_cast<const cpx<R>*>(p->d_tw2);
   auto vv = reinterpret_cast<const R*>(v);
+  if (p->lane_kernel) {
+    int rc = launch_plane_lane<R>(p, T, vv, nb, st);
+    if (rc != ADDFT_OK) return rc;
+  } else {
 #define ADDFT_PV(E)                                                                              \
   hipLaunchKernelGGL((k_plane_v<R, E>), pg, dim3(kPThreads), p->lds_plane, st, T, p->nst, p->nra, \
                      p->d_srow, p->d_scol, p->d_rowval, p->f1, p->f2, tw1, tw2, vv, p->cchunk)
@@ -453,6 +656,7 @@ int launch_hpsi(addft_sphere_plan_s* p, const void* psi, const void* tkin, const
   }
 #undef ADDFT_PV
   ADDFT_LAUNCH_CHECK();
+  }
   hipLaunchKernelGGL((k_z_forward<R, R>), zg, dim3(kZThreads), zlds, st, T, p->ng, p->d_soff, p->d_sid,
                      p->d_iz, p->nst, p->spb, p->f3, reinterpret_cast<const cpx<R>*>(p->d_tw3), (R)1,
                      reinterpret_cast<const R*>(tkin), reinterpret_cast<const cpx<R>*>(psi),
@@ -588,22 +792,57 @@ int addft_sphere_plan_create(int n1, int n2, int n3, int64_t ng, const int32_t* 
   p->ept_p = ept;
   p->cchunk = cch;
   p->lds_plane = base + (size_t)cch * n1 * cs;
+  // lane-group plane pass: n1 == n2 with a codelet, rows and the sticks of
+  // every row cyclic intervals, and the plane + exchange tiles within LDS
+  p->lane_ok = 0;
+  p->lane_kernel = 0;
+  p->lane_waves = kLaneWaves;
+  p->a1 = 0;
+  p->d_rowinfo = nullptr;
+  std::vector<int32_t> rowinfo;
+  if (n1 == n2 && lane_width(n1) > 0) {
+    bool ok = true;
+    std::vector<int> rows(rowval.begin(), rowval.end());
+    ok = cyclic_interval(rows, n1, &p->a1);
+    rowinfo.assign((size_t)4 * p->nra, 0);
+    int64_t s = 0;
+    for (int r = 0; ok && r < p->nra; ++r) {
+      const int64_t s0 = s;
+      std::vector<int> cols;
+      while (s < p->nst && srow[s] == r) cols.push_back(scol[s++]);
+      int a2 = 0;
+      ok = cyclic_interval(cols, n2, &a2);
+      rowinfo[4 * r] = (int32_t)s0;
+      rowinfo[4 * r + 1] = a2;
+      rowinfo[4 * r + 2] = (int32_t)cols.size();
+    }
+    ok = ok && s == p->nst;
+    const size_t llds = precision == ADDFT_C64 ? lane_lds_bytes<float>(n1, p->nra, kLaneWaves)
+                                               : lane_lds_bytes<double>(n1, p->nra, kLaneWaves);
+    if (ok && llds <= kLdsMax) {
+      p->lane_ok = 1;
+      p->lane_kernel = 1;
+    }
+  }
   int rc = ADDFT_OK;
   if ((rc = upload_i32(perm, &p->d_perm)) || (rc = upload_i32(iperm, &p->d_iperm)) ||
       (rc = upload_i32(sid, &p->d_sid)) || (rc = upload_i32(iz, &p->d_iz)) ||
       (rc = upload_i32(soff, &p->d_soff)) || (rc = upload_i32(srow, &p->d_srow)) ||
-      (rc = upload_i32(scol, &p->d_scol)) || (rc = upload_i32(rowval, &p->d_rowval))) {
+      (rc = upload_i32(scol, &p->d_scol)) || (rc = upload_i32(rowval, &p->d_rowval)) ||
+      (p->lane_ok && (rc = upload_i32(rowinfo, &p->d_rowinfo)))) {
     delete p;
     return rc;
   }
   if (precision == ADDFT_C64) {
     if ((rc = upload_twiddles<float>(n1, &p->d_tw1)) || (rc = upload_twiddles<float>(n2, &p->d_tw2)) ||
-        (rc = upload_twiddles<float>(n3, &p->d_tw3)) || (rc = set_plane_attrs_any<float>(ept, p->lds_plane)))
+        (rc = upload_twiddles<float>(n3, &p->d_tw3)) || (rc = set_plane_attrs_any<float>(ept, p->lds_plane)) ||
+        (p->lane_ok && (rc = launch_plane_lane<float>(p, nullptr, nullptr, 0, nullptr))))
       return rc;
   } else {
     if ((rc = upload_twiddles<double>(n1, &p->d_tw1)) || (rc = upload_twiddles<double>(n2, &p->d_tw2)) ||
         (rc = upload_twiddles<double>(n3, &p->d_tw3)) ||
-        (rc = set_plane_attrs_any<double>(ept, p->lds_plane)))
+        (rc = set_plane_attrs_any<double>(ept, p->lds_plane)) ||
+        (p->lane_ok && (rc = launch_plane_lane<double>(p, nullptr, nullptr, 0, nullptr))))
       return rc;
   }
   *out = p;
@@ -613,7 +852,7 @@ int addft_sphere_plan_create(int n1, int n2, int n3, int64_t ng, const int32_t* 
 int addft_sphere_plan_destroy(addft_sphere_plan p) {
   if (!p) return ADDFT_OK;
   void* ptrs[] = {p->d_perm, p->d_iperm, p->d_sid, p->d_iz, p->d_soff, p->d_srow,
-                  p->d_scol, p->d_rowval, p->d_tw1, p->d_tw2, p->d_tw3};
+                  p->d_scol, p->d_rowval, p->d_tw1, p->d_tw2, p->d_tw3, p->d_rowinfo};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   delete p;
@@ -625,6 +864,24 @@ int addft_sphere_plan_info(addft_sphere_plan p, int64_t* n_sticks, int64_t* n_ro
   if (n_sticks) *n_sticks = p->nst;
   if (n_rows) *n_rows = p->nra;
   if (h_perm) std::memcpy(h_perm, p->h_perm.data(), sizeof(int32_t) * p->ng);
+  return ADDFT_OK;
+}
+
+int addft_sphere_plan_set_kernel(addft_sphere_plan p, int kernel) {
+  ADDFT_CHECK_ARG(p != nullptr, "null plan");
+  ADDFT_CHECK_ARG(kernel == ADDFT_PLANE_LDS || kernel == ADDFT_PLANE_LANE, "bad kernel id");
+  if (kernel == ADDFT_PLANE_LANE && !p->lane_ok) {
+    addft::set_error("no lane-group plane kernel for this sphere and box");
+    return ADDFT_EUNSUPPORTED;
+  }
+  p->lane_kernel = kernel == ADDFT_PLANE_LANE;
+  return ADDFT_OK;
+}
+
+int addft_sphere_plan_get_kernel(addft_sphere_plan p, int* kernel, int* lane_available) {
+  ADDFT_CHECK_ARG(p != nullptr, "null plan");
+  if (kernel) *kernel = p->lane_kernel ? ADDFT_PLANE_LANE : ADDFT_PLANE_LDS;
+  if (lane_available) *lane_available = p->lane_ok;
   return ADDFT_OK;
 }
 

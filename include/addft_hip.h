@@ -92,6 +92,18 @@ int addft_sphere_plan_destroy(addft_sphere_plan plan);
  * (h_perm[p] = reference index of stick-order position p), may be NULL. */
 int addft_sphere_plan_info(addft_sphere_plan plan, int64_t* n_sticks,
                            int64_t* n_rows, int32_t* h_perm);
+/* Plane-pass kernel of addft_hpsi_local.  ADDFT_PLANE_LDS: workgroup-wide
+ * Stockham transforms in LDS (any box).  ADDFT_PLANE_LANE: each transform
+ * on a few lanes of a wavefront with register codelets, for n1 == n2 in
+ * the supported sizes when the sphere's rows and row sticks are cyclic
+ * intervals; the default where available.  Both compute the reference's
+ * potential_apply (ops/hpsi.py:29-32) on the same stick layout.
+ * set_kernel returns ADDFT_EUNSUPPORTED for a lane request the plan
+ * cannot serve; get_kernel reports the current choice and availability. */
+#define ADDFT_PLANE_LDS 0
+#define ADDFT_PLANE_LANE 1
+int addft_sphere_plan_set_kernel(addft_sphere_plan plan, int kernel);
+int addft_sphere_plan_get_kernel(addft_sphere_plan plan, int* kernel, int* lane_available);
 /* Bytes of workspace for Hpsi / density / transforms on nb bands.   */
 int64_t addft_sphere_work_bytes(addft_sphere_plan plan, int64_t nb);
 

@@ -278,3 +278,46 @@ def test_reduction_workspace_stays_in_bounds(entry):
     torch.cuda.synchronize()
     assert bool((work[wb // 8:] == 7.25).all())
     assert np.allclose(out.cpu().numpy(), ref, rtol=1e-11, atol=0)
+
+
+# ------------------------------------------------ plane-pass kernel choice
+@pytest.mark.parametrize("a,ecut", [(10.26, 10.0), (20.52, 15.0), (30.78, 15.0)])
+@pytest.mark.parametrize("cdt,tol", [("complex128", 1e-12), ("complex64", 3e-6)])
+def test_plane_kernels_agree(a, ecut, cdt, tol):
+    """The lane-group plane pass and the workgroup Stockham pass give the
+    same local Hpsi (boxes 32^3, 75^3, 120^3)."""
+    torch, native = _torch(), _native()
+    basis = OB.make_basis(a * np.eye(3), ecut)
+    plan = _plan(basis, cdt)
+    assert plan.lane_available and plan.plane_kernel == "lane"
+    cd = getattr(torch, cdt)
+    rd = torch.float32 if cdt == "complex64" else torch.float64
+    g = torch.Generator(device="cpu").manual_seed(11)
+    nb = 3
+    psi = torch.complex(torch.randn(nb, basis.nG, generator=g, dtype=torch.float64),
+                        torch.randn(nb, basis.nG, generator=g, dtype=torch.float64))
+    psi = (psi / psi.norm(dim=1, keepdim=True)).to(cd).to(_dev())
+    tk = torch.as_tensor(OS.kinetic_diag(basis, a * np.eye(3), np.zeros(3))[plan.perm]).to(rd).to(_dev())
+    v = torch.as_tensor(0.3 * np.cos(np.arange(basis.nfft) * 0.013) + 0.1).to(_dev())
+    vs = plan.stage(v, 1.0 / basis.nfft)
+    h_lane = plan.hpsi_local(psi, tk, vs)
+    plan.set_plane_kernel("lds")
+    assert plan.plane_kernel == "lds"
+    h_lds = plan.hpsi_local(psi, tk, vs)
+    plan.set_plane_kernel("lane")
+    err = (h_lane - h_lds).abs().max().item() / h_lds.abs().max().item()
+    print(f"box {basis.fft_shape} {cdt}: lane vs lds rel err {err:.2e}", flush=True)
+    assert err < tol
+
+
+def test_lane_kernel_unavailable_is_an_error():
+    """A non-cubic plane has no lane-group kernel: asking for it fails
+    with ADDFT_EUNSUPPORTED and leaves the plan on the LDS pass."""
+    torch, native = _torch(), _native()
+    from addft.kernels._lib import AddftError
+    geo, st = _system(*CASES[2])
+    plan = _plan(st.basis, "complex128")
+    assert not plan.lane_available and plan.plane_kernel == "lds"
+    with pytest.raises(AddftError):
+        plan.set_plane_kernel("lane")
+    assert plan.plane_kernel == "lds"
