@@ -20,6 +20,7 @@ subspace products, small dense linear algebra, and elementwise glue.
 from __future__ import annotations
 
 import math
+import os
 
 import numpy as np
 import torch
@@ -31,6 +32,7 @@ from .kernels import _lib
 from .kernels import native
 from .kernels._lib import check, lib, ptr, stream
 from .solver.linalg import inner
+from .kernels.nonlocal_bf16 import ProjectorBF16x3
 
 
 def _simpson_weights(n):
@@ -55,6 +57,8 @@ class Engine:
         self.nk = int(static.kpts.shape[0])
         self.nat = int(static.spec_idx.shape[0])
         self.mixed = static.precision == "mixed"
+        # FP32-phase projector products as BF16x3 split GEMMs (ADDFT_NL_BF16X3=0: complex64 GEMMs)
+        self.nl_bf16 = self.mixed and os.environ.get("ADDFT_NL_BF16X3", "1") != "0"
         with torch.cuda.device(self.dev):
             self.plan64 = native.SpherePlan(self.shape, b.miller, torch.complex128, self.dev)
             self.plan32 = native.SpherePlan(self.shape, b.miller, torch.complex64, self.dev) if self.mixed else None
@@ -162,7 +166,7 @@ class Engine:
         check(L.addft_gvectors(ptr(self.miller_d), ptr(self.perm_d), ng, ptr(self.bmat_d), None, ptr(self.g0_d),
                                None, ptr(qn0), stream()), "addft_gvectors")
         self.kq, self.kqn, self.tkin64, self.tkin32, self.beta, self.dbeta = [], [], [], [], [], []
-        self.P64, self.P32 = [], []
+        self.P64, self.P32, self.PB = [], [], []
         isq = 1.0 / math.sqrt(self.omega)
         for k in self.st.kpts:
             kk = np.ascontiguousarray(k, dtype=np.float64)
@@ -185,6 +189,11 @@ class Engine:
             self.dbeta.append(dbeta)
             self.P64.append(P)
             self.P32.append(P.to(torch.complex64) if self.mixed else None)
+            if self.nl_bf16:
+                nsl = max(1, min(32, (6 * ng) // 15000))
+                self.PB.append(ProjectorBF16x3(P, nslice=nsl))
+            else:
+                self.PB.append(None)
         # local potential on the density sphere
         ngr = self.st.basis.nG_rho
         self.grho_d = torch.empty((ngr, 3), dtype=torch.float64, device=self.dev)
@@ -233,6 +242,10 @@ class Engine:
         D = self.D32 if fp32 else self.D64
         self.hpsi_bands["fp32" if fp32 else "fp64"] += int(psi.shape[0])
         out = plan.hpsi_local(psi, tk, vs)
+        if fp32 and self.PB[ik] is not None:
+            pb = self.PB[ik]
+            out.add_(pb.apply(torch.matmul(pb.becp_t(psi), D)))
+            return out
         becp_t = inner(psi, P)                          # (nb, nproj) = <beta|psi>^T
         out.add_(torch.matmul(torch.matmul(becp_t, D), P))
         return out

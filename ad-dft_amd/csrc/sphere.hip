@@ -55,6 +55,8 @@ struct addft_sphere_plan_s {
   int lane_waves;     // wavefronts per workgroup of the lane-group pass
   int a1;             // first i1 of the cyclic interval of rows
   int32_t* d_rowinfo; // per row: first stick, first i2, sticks in the row, 0
+  int lane_z;         // lane-group z pass (n3 has a codelet, sticks are cyclic intervals)
+  int32_t* d_a3;      // first i3 of each stick's cyclic interval
 };
 
 namespace {
@@ -332,6 +334,213 @@ k_plane_v_lane(cpx<R>* __restrict__ T, int64_t nst, int nra, int a1, const int4*
   }
 }
 
+// ------------------------------------------------ lane-group z pass
+// One stick per group of L lanes; the n3 outputs of the workgroup's
+// SPB = WAVES * G sticks are staged in LDS ([z][stick], padded) so that the
+// stick buffer T[b][z][s] is written (read) in rows of SPB consecutive
+// sticks.  a3[s]: first i3 of the stick's coefficients (a cyclic interval,
+// listed in ascending i3 from soff[s]).
+template <typename R, int N, int L, int WAVES>
+__global__ void __launch_bounds__(64 * WAVES)
+k_z_inverse_lane(const cpx<R>* __restrict__ psi, int64_t ng, const int32_t* __restrict__ soff,
+                 const int32_t* __restrict__ a3, int64_t nst, const cpx<R>* __restrict__ gtw,
+                 cpx<R>* __restrict__ T) {
+  using S = addft::lane::Shape<N, L>;
+  constexpr int M = S::M, Q = S::Q, G = S::G, SPB = WAVES * G, SST = SPB + 1;
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+  cpx<R>* tw = reinterpret_cast<cpx<R>*>(smem_raw);
+  cpx<R>* stage = tw + N;
+  cpx<R>* scr_all = stage + (size_t)N * SST;
+  const int wave = threadIdx.x >> 6, lid = threadIdx.x & 63;
+  const int g = lid / L, t = lid - g * L;
+  const bool act = g < G;
+  cpx<R>* scr = scr_all + (size_t)(wave * (G + 1) + g) * S::SCR;
+  const int64_t s0 = (int64_t)blockIdx.x * SPB;
+  const int cnt = (int)min((int64_t)SPB, nst - s0);
+  const int64_t b = blockIdx.y;
+  for (int i = threadIdx.x; i < N; i += blockDim.x) tw[i] = gtw[i];
+  __syncthreads();
+  const int sl = wave * G + g;
+  const bool ok = act && sl < cnt;
+  int p0 = 0, len = 0, a = 0;
+  if (ok) {
+    p0 = soff[s0 + sl];
+    len = soff[s0 + sl + 1] - p0;
+    a = a3[s0 + sl];
+  }
+  const cpx<R>* src = psi + b * ng + p0;
+  cpx<R> x[M];
+#pragma unroll
+  for (int j = 0; j < M; ++j) {
+    const int pos = ok ? addft::lane::cyc_pos(t + L * j, a, len, N) : -1;
+    x[j] = pos >= 0 ? src[pos] : addft::mk<R>(0, 0);
+  }
+  cpx<R> o[Q][L];
+  addft::lane::fwd<R, N, L, true>(x, o, t, scr, tw);
+  if (ok) {
+#pragma unroll
+    for (int q = 0; q < Q; ++q)
+      if (q * L + t < M) {
+#pragma unroll
+        for (int k1 = 0; k1 < L; ++k1) stage[(q * L + t + M * k1) * SST + sl] = o[q][k1];
+      }
+  }
+  __syncthreads();
+  cpx<R>* dst = T + (b * N) * nst + s0;
+  for (int i = threadIdx.x; i < N * cnt; i += blockDim.x) {
+    const int z = i / cnt, j = i - z * cnt;
+    dst[(int64_t)z * nst + j] = stage[z * SST + j];
+  }
+}
+
+template <typename R, typename ROUT, int N, int L, int WAVES>
+__global__ void __launch_bounds__(64 * WAVES)
+k_z_forward_lane(const cpx<R>* __restrict__ T, int64_t ng, const int32_t* __restrict__ soff,
+                 const int32_t* __restrict__ a3, int64_t nst, const cpx<R>* __restrict__ gtw, R scale,
+                 const R* __restrict__ tkin, const cpx<R>* __restrict__ psi, cpx<ROUT>* __restrict__ out) {
+  using S = addft::lane::Shape<N, L>;
+  constexpr int M = S::M, Q = S::Q, G = S::G, SPB = WAVES * G, SST = SPB + 1;
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+  cpx<R>* tw = reinterpret_cast<cpx<R>*>(smem_raw);
+  cpx<R>* stage = tw + N;
+  cpx<R>* scr_all = stage + (size_t)N * SST;
+  const int wave = threadIdx.x >> 6, lid = threadIdx.x & 63;
+  const int g = lid / L, t = lid - g * L;
+  const bool act = g < G;
+  cpx<R>* scr = scr_all + (size_t)(wave * (G + 1) + g) * S::SCR;
+  const int64_t s0 = (int64_t)blockIdx.x * SPB;
+  const int cnt = (int)min((int64_t)SPB, nst - s0);
+  const int64_t b = blockIdx.y;
+  for (int i = threadIdx.x; i < N; i += blockDim.x) tw[i] = gtw[i];
+  const cpx<R>* src = T + (b * N) * nst + s0;
+  for (int i = threadIdx.x; i < N * cnt; i += blockDim.x) {
+    const int z = i / cnt, j = i - z * cnt;
+    stage[z * SST + j] = src[(int64_t)z * nst + j];
+  }
+  __syncthreads();
+  const int sl = wave * G + g;
+  const bool ok = act && sl < cnt;
+  cpx<R> o[Q][L];
+#pragma unroll
+  for (int q = 0; q < Q; ++q)
+#pragma unroll
+    for (int k1 = 0; k1 < L; ++k1)
+      o[q][k1] = (ok && q * L + t < M) ? stage[(q * L + t + M * k1) * SST + sl] : addft::mk<R>(0, 0);
+  cpx<R> x[M];
+  addft::lane::bwd<R, N, L, false>(o, x, t, scr, tw);
+  if (ok) {
+    const int p0 = soff[s0 + sl], len = soff[s0 + sl + 1] - p0, a = a3[s0 + sl];
+    cpx<ROUT>* dst = out + b * ng + p0;
+#pragma unroll
+    for (int j = 0; j < M; ++j) {
+      const int pos = addft::lane::cyc_pos(t + L * j, a, len, N);
+      if (pos >= 0) {
+        cpx<R> val = x[j] * scale;
+        if (tkin != nullptr) {
+          const cpx<R> c = psi[b * ng + p0 + pos];
+          const R tk = tkin[p0 + pos];
+          val.x += tk * c.x;
+          val.y += tk * c.y;
+        }
+        dst[pos] = addft::mk<ROUT>((ROUT)val.x, (ROUT)val.y);
+      }
+    }
+  }
+}
+
+// ------------------------------------------------ lane-group density pass
+// part[grp][z][i1][i2] = sum_{b in group} w_b |IFFT_plane(T_b)|^2 for plane
+// z = blockIdx.x and band group grp = blockIdx.y.  Each lane group keeps
+// the |psi|^2 sums of its CT columns in registers across the group's bands
+// and writes them once.
+template <typename R, int N, int L, int WAVES>
+__global__ void __launch_bounds__(64 * WAVES)
+k_plane_density_lane(const cpx<R>* __restrict__ T, int64_t nst, int nra, int a1,
+                     const int4* __restrict__ rowinfo, const cpx<R>* __restrict__ gtw,
+                     const double* __restrict__ w, int64_t nb, int bpg, double* __restrict__ part) {
+  using S = addft::lane::Shape<N, L>;
+  constexpr int M = S::M, Q = S::Q, G = S::G;
+  constexpr int CT = (N + WAVES * G - 1) / (WAVES * G);   // column tasks per group
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+  cpx<R>* tw = reinterpret_cast<cpx<R>*>(smem_raw);
+  cpx<R>* P = tw + N;
+  cpx<R>* scr_all = P + (size_t)nra * N;
+  const int wave = threadIdx.x >> 6, lid = threadIdx.x & 63;
+  const int g = lid / L, t = lid - g * L;
+  const bool act = g < G;
+  cpx<R>* scr = scr_all + (size_t)(wave * (G + 1) + g) * S::SCR;
+  const int z = blockIdx.x, n3 = gridDim.x;
+  const int64_t grp = blockIdx.y;
+  for (int i = threadIdx.x; i < N; i += blockDim.x) tw[i] = gtw[i];
+  double acc[CT][Q][L];
+#pragma unroll
+  for (int c = 0; c < CT; ++c)
+#pragma unroll
+    for (int q = 0; q < Q; ++q)
+#pragma unroll
+      for (int k1 = 0; k1 < L; ++k1) acc[c][q][k1] = 0.0;
+  const int64_t b0 = grp * bpg, b1 = min(nb, b0 + bpg);
+  for (int64_t b = b0; b < b1; ++b) {
+    __syncthreads();   // twiddles staged / previous band's columns done with P
+    const cpx<R>* Tp = T + (b * n3 + z) * nst;
+    for (int base = wave * G; base < nra; base += WAVES * G) {
+      const int r = base + g;
+      const bool ok = act && r < nra;
+      const int4 ri = ok ? rowinfo[r] : make_int4(0, 0, 0, 0);
+      cpx<R> x[M];
+#pragma unroll
+      for (int j = 0; j < M; ++j) {
+        const int pos = ok ? addft::lane::cyc_pos(t + L * j, ri.y, ri.z, N) : -1;
+        x[j] = pos >= 0 ? Tp[ri.x + pos] : addft::mk<R>(0, 0);
+      }
+      cpx<R> o[Q][L];
+      addft::lane::fwd<R, N, L, true>(x, o, t, scr, tw);
+      if (ok) {
+        cpx<R>* Pr = P + (size_t)r * N;
+#pragma unroll
+        for (int q = 0; q < Q; ++q)
+          if (q * L + t < M) {
+#pragma unroll
+            for (int k1 = 0; k1 < L; ++k1) Pr[q * L + t + M * k1] = o[q][k1];
+          }
+      }
+    }
+    __syncthreads();
+    const double wb = w[b];
+#pragma unroll
+    for (int c = 0; c < CT; ++c) {
+      const int y = c * WAVES * G + wave * G + g;
+      const bool ok = act && y < N;
+      cpx<R> x[M];
+#pragma unroll
+      for (int j = 0; j < M; ++j) {
+        const int rr = ok ? addft::lane::cyc_pos(t + L * j, a1, nra, N) : -1;
+        x[j] = rr >= 0 ? P[(size_t)rr * N + y] : addft::mk<R>(0, 0);
+      }
+      cpx<R> o[Q][L];
+      addft::lane::fwd<R, N, L, true>(x, o, t, scr, tw);
+#pragma unroll
+      for (int q = 0; q < Q; ++q)
+#pragma unroll
+        for (int k1 = 0; k1 < L; ++k1)
+          acc[c][q][k1] += wb * ((double)o[q][k1].x * (double)o[q][k1].x + (double)o[q][k1].y * (double)o[q][k1].y);
+    }
+  }
+  double* out = part + (grp * n3 + z) * (int64_t)N * N;
+#pragma unroll
+  for (int c = 0; c < CT; ++c) {
+    const int y = c * WAVES * G + wave * G + g;
+    if (act && y < N) {
+#pragma unroll
+      for (int q = 0; q < Q; ++q)
+        if (q * L + t < M) {
+#pragma unroll
+          for (int k1 = 0; k1 < L; ++k1) out[(int64_t)(q * L + t + M * k1) * N + y] = acc[c][q][k1];
+        }
+    }
+  }
+}
+
 // lanes per transform for the box sizes with a lane-group kernel (0: none)
 constexpr int lane_width(int n) {
   switch (n) {
@@ -601,6 +810,128 @@ int launch_plane_lane(const addft_sphere_plan_s* p, cpx<R>* T, const R* v, int64
   return ADDFT_OK;
 }
 
+constexpr int kZLaneWaves = 4;
+constexpr int kLaneBandsPerGroup = 32;
+template <typename R>
+constexpr int dens_lane_waves() { return sizeof(R) == 4 ? 8 : 4; }
+
+template <typename R>
+size_t zlane_lds_bytes(int n) {
+  const int L = lane_width(n);
+  if (L == 0) return 0;
+  const int G = 64 / L, spb = kZLaneWaves * G;
+  return sizeof(cpx<R>) * ((size_t)n + (size_t)n * (spb + 1) + (size_t)kZLaneWaves * (G + 1) * L * (L + 1));
+}
+
+int zlane_spb(int n) { return kZLaneWaves * (64 / lane_width(n)); }
+
+// Sphere (stick order) -> stick buffer T, inverse transform along i3.
+// nb == 0 on a lane plan only configures the kernels' LDS limits.
+template <typename R>
+int launch_z_inverse(const addft_sphere_plan_s* p, const cpx<R>* psi, cpx<R>* T, int64_t nb, hipStream_t st) {
+  auto tw3 = reinterpret_cast<const cpx<R>*>(p->d_tw3);
+  if (!p->lane_z) {
+    const size_t zlds = (size_t)(p->spb + 1) * p->n3 * sizeof(cpx<R>);
+    dim3 zg((unsigned)((p->nst + p->spb - 1) / p->spb), (unsigned)nb);
+    hipLaunchKernelGGL((k_z_inverse<R, R>), zg, dim3(kZThreads), zlds, st, psi, p->ng, p->d_soff, p->d_sid,
+                       p->d_iz, p->nst, p->spb, p->f3, tw3, T);
+    ADDFT_LAUNCH_CHECK();
+    return ADDFT_OK;
+  }
+  const size_t lds = zlane_lds_bytes<R>(p->n3);
+  const int spb = zlane_spb(p->n3);
+  const dim3 grid((unsigned)((p->nst + spb - 1) / spb), (unsigned)std::max<int64_t>(nb, 1));
+#define ADDFT_ZI_CASE(NN)                                                                              \
+  case NN:                                                                                             \
+    if (nb == 0) {                                                                                     \
+      ADDFT_HIP(hipFuncSetAttribute((const void*)k_z_inverse_lane<R, NN, lane_width(NN), kZLaneWaves>, \
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));            \
+      return ADDFT_OK;                                                                                 \
+    }                                                                                                  \
+    hipLaunchKernelGGL((k_z_inverse_lane<R, NN, lane_width(NN), kZLaneWaves>), grid,                   \
+                       dim3(64 * kZLaneWaves), lds, st, psi, p->ng, p->d_soff, p->d_a3, p->nst, tw3, T);  \
+    break;
+  switch (p->n3) {
+    ADDFT_LANE_SIZES(ADDFT_ZI_CASE)
+    default:
+      addft::set_error("no lane-group z kernel for this box");
+      return ADDFT_EINTERNAL;
+  }
+#undef ADDFT_ZI_CASE
+  ADDFT_LAUNCH_CHECK();
+  return ADDFT_OK;
+}
+
+// Stick buffer -> sphere: forward along i3, gather, scale, optional kinetic
+// epilogue out = scale*F + tkin*psi.
+template <typename R, typename ROUT>
+int launch_z_forward(const addft_sphere_plan_s* p, const cpx<R>* T, R scale, const R* tkin, const cpx<R>* psi,
+                     cpx<ROUT>* out, int64_t nb, hipStream_t st) {
+  auto tw3 = reinterpret_cast<const cpx<R>*>(p->d_tw3);
+  if (!p->lane_z) {
+    const size_t zlds = (size_t)(p->spb + 1) * p->n3 * sizeof(cpx<R>);
+    dim3 zg((unsigned)((p->nst + p->spb - 1) / p->spb), (unsigned)nb);
+    hipLaunchKernelGGL((k_z_forward<R, ROUT>), zg, dim3(kZThreads), zlds, st, T, p->ng, p->d_soff, p->d_sid,
+                       p->d_iz, p->nst, p->spb, p->f3, tw3, scale, tkin, psi, out);
+    ADDFT_LAUNCH_CHECK();
+    return ADDFT_OK;
+  }
+  const size_t lds = zlane_lds_bytes<R>(p->n3);
+  const int spb = zlane_spb(p->n3);
+  const dim3 grid((unsigned)((p->nst + spb - 1) / spb), (unsigned)std::max<int64_t>(nb, 1));
+#define ADDFT_ZF_CASE(NN)                                                                                     \
+  case NN:                                                                                                    \
+    if (nb == 0) {                                                                                            \
+      ADDFT_HIP(hipFuncSetAttribute((const void*)k_z_forward_lane<R, ROUT, NN, lane_width(NN), kZLaneWaves>, \
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));                   \
+      return ADDFT_OK;                                                                                        \
+    }                                                                                                         \
+    hipLaunchKernelGGL((k_z_forward_lane<R, ROUT, NN, lane_width(NN), kZLaneWaves>), grid,                    \
+                       dim3(64 * kZLaneWaves), lds, st, T, p->ng, p->d_soff, p->d_a3, p->nst, tw3, scale, tkin, \
+                       psi, out);                                                                             \
+    break;
+  switch (p->n3) {
+    ADDFT_LANE_SIZES(ADDFT_ZF_CASE)
+    default:
+      addft::set_error("no lane-group z kernel for this box");
+      return ADDFT_EINTERNAL;
+  }
+#undef ADDFT_ZF_CASE
+  ADDFT_LAUNCH_CHECK();
+  return ADDFT_OK;
+}
+
+// Lane-group density plane pass into part[group] (nb == 0: configure only).
+template <typename R>
+int launch_density_lane(const addft_sphere_plan_s* p, const cpx<R>* T, const double* w, int64_t nb, double* part,
+                        hipStream_t st) {
+  constexpr int W = dens_lane_waves<R>();
+  const size_t lds = lane_lds_bytes<R>(p->n1, p->nra, W);
+  const int64_t ngr = (nb + kLaneBandsPerGroup - 1) / kLaneBandsPerGroup;
+  const dim3 grid((unsigned)p->n3, (unsigned)std::max<int64_t>(ngr, 1));
+  auto tw = reinterpret_cast<const cpx<R>*>(p->d_tw1);
+  auto ri = reinterpret_cast<const int4*>(p->d_rowinfo);
+#define ADDFT_DL_CASE(NN)                                                                          \
+  case NN:                                                                                         \
+    if (nb == 0) {                                                                                 \
+      ADDFT_HIP(hipFuncSetAttribute((const void*)k_plane_density_lane<R, NN, lane_width(NN), W>,   \
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));        \
+      return ADDFT_OK;                                                                             \
+    }                                                                                              \
+    hipLaunchKernelGGL((k_plane_density_lane<R, NN, lane_width(NN), W>), grid, dim3(64 * W), lds, st, T, \
+                       p->nst, p->nra, p->a1, ri, tw, w, nb, kLaneBandsPerGroup, part);            \
+    break;
+  switch (p->n1) {
+    ADDFT_LANE_SIZES(ADDFT_DL_CASE)
+    default:
+      addft::set_error("no lane-group density kernel for this box");
+      return ADDFT_EINTERNAL;
+  }
+#undef ADDFT_DL_CASE
+  ADDFT_LAUNCH_CHECK();
+  return ADDFT_OK;
+}
+
 // Sorted list -> start a of the cyclic interval of Z_n it enumerates (in
 // ascending order, wrapped part first); false if it is not one.
 bool cyclic_interval(const std::vector<int>& lst, int n, int* a) {
@@ -631,38 +962,30 @@ template <typename R>
 int launch_hpsi(addft_sphere_plan_s* p, const void* psi, const void* tkin, const void* v, int64_t nb,
                 void* out, void* work, hipStream_t st) {
   cpx<R>* T = reinterpret_cast<cpx<R>*>(work);
-  const size_t zlds = (size_t)(p->spb + 1) * p->n3 * sizeof(cpx<R>);
-  dim3 zg((unsigned)((p->nst + p->spb - 1) / p->spb), (unsigned)nb);
-  hipLaunchKernelGGL((k_z_inverse<R, R>), zg, dim3(kZThreads), zlds, st,
-                     reinterpret_cast<const cpx<R>*>(psi), p->ng, p->d_soff, p->d_sid, p->d_iz, p->nst,
-                     p->spb, p->f3, reinterpret_cast<const cpx<R>*>(p->d_tw3), T);
-  ADDFT_LAUNCH_CHECK();
-  dim3 pg((unsigned)p->n3, (unsigned)nb);
-  auto tw1 = reinterpret_cast<const cpx<R>*>(p->d_tw1);
-  auto tw2 = reinterpret_cast<const cpx<R>*>(p->d_tw2);
+  int rc = launch_z_inverse<R>(p, reinterpret_cast<const cpx<R>*>(psi), T, nb, st);
+  if (rc != ADDFT_OK) return rc;
   auto vv = reinterpret_cast<const R*>(v);
   if (p->lane_kernel) {
-    int rc = launch_plane_lane<R>(p, T, vv, nb, st);
+    rc = launch_plane_lane<R>(p, T, vv, nb, st);
     if (rc != ADDFT_OK) return rc;
   } else {
+    dim3 pg((unsigned)p->n3, (unsigned)nb);
+    auto tw1 = reinterpret_cast<const cpx<R>*>(p->d_tw1);
+    auto tw2 = reinterpret_cast<const cpx<R>*>(p->d_tw2);
 #define ADDFT_PV(E)                                                                              \
   hipLaunchKernelGGL((k_plane_v<R, E>), pg, dim3(kPThreads), p->lds_plane, st, T, p->nst, p->nra, \
                      p->d_srow, p->d_scol, p->d_rowval, p->f1, p->f2, tw1, tw2, vv, p->cchunk)
-  switch (p->ept_p) {
-    case 4: ADDFT_PV(4); break;
-    case 8: ADDFT_PV(8); break;
-    case 16: ADDFT_PV(16); break;
-    default: ADDFT_PV(32); break;
-  }
+    switch (p->ept_p) {
+      case 4: ADDFT_PV(4); break;
+      case 8: ADDFT_PV(8); break;
+      case 16: ADDFT_PV(16); break;
+      default: ADDFT_PV(32); break;
+    }
 #undef ADDFT_PV
-  ADDFT_LAUNCH_CHECK();
+    ADDFT_LAUNCH_CHECK();
   }
-  hipLaunchKernelGGL((k_z_forward<R, R>), zg, dim3(kZThreads), zlds, st, T, p->ng, p->d_soff, p->d_sid,
-                     p->d_iz, p->nst, p->spb, p->f3, reinterpret_cast<const cpx<R>*>(p->d_tw3), (R)1,
-                     reinterpret_cast<const R*>(tkin), reinterpret_cast<const cpx<R>*>(psi),
-                     reinterpret_cast<cpx<R>*>(out));
-  ADDFT_LAUNCH_CHECK();
-  return ADDFT_OK;
+  return launch_z_forward<R, R>(p, T, (R)1, reinterpret_cast<const R*>(tkin), reinterpret_cast<const cpx<R>*>(psi),
+                                reinterpret_cast<cpx<R>*>(out), nb, st);
 }
 
 template <typename R>
@@ -670,28 +993,31 @@ int launch_density(addft_sphere_plan_s* p, const void* psi, const double* w, int
                    int accumulate, void* work, hipStream_t st) {
   cpx<R>* T = reinterpret_cast<cpx<R>*>(work);
   double* part = reinterpret_cast<double*>(reinterpret_cast<char*>(work) + t_bytes(p, nb));
-  const size_t zlds = (size_t)(p->spb + 1) * p->n3 * sizeof(cpx<R>);
-  dim3 zg((unsigned)((p->nst + p->spb - 1) / p->spb), (unsigned)nb);
-  hipLaunchKernelGGL((k_z_inverse<R, R>), zg, dim3(kZThreads), zlds, st,
-                     reinterpret_cast<const cpx<R>*>(psi), p->ng, p->d_soff, p->d_sid, p->d_iz, p->nst,
-                     p->spb, p->f3, reinterpret_cast<const cpx<R>*>(p->d_tw3), T);
-  ADDFT_LAUNCH_CHECK();
-  const int64_t ngr = density_groups(nb);
-  dim3 pg((unsigned)p->n3, (unsigned)ngr);
-  auto tw1 = reinterpret_cast<const cpx<R>*>(p->d_tw1);
-  auto tw2 = reinterpret_cast<const cpx<R>*>(p->d_tw2);
+  int rc = launch_z_inverse<R>(p, reinterpret_cast<const cpx<R>*>(psi), T, nb, st);
+  if (rc != ADDFT_OK) return rc;
+  int64_t ngr;
+  if (p->lane_kernel) {
+    ngr = (nb + kLaneBandsPerGroup - 1) / kLaneBandsPerGroup;
+    rc = launch_density_lane<R>(p, T, w, nb, part, st);
+    if (rc != ADDFT_OK) return rc;
+  } else {
+    ngr = density_groups(nb);
+    dim3 pg((unsigned)p->n3, (unsigned)ngr);
+    auto tw1 = reinterpret_cast<const cpx<R>*>(p->d_tw1);
+    auto tw2 = reinterpret_cast<const cpx<R>*>(p->d_tw2);
 #define ADDFT_PD(E)                                                                                     \
   hipLaunchKernelGGL((k_plane_density<R, E>), pg, dim3(kPThreads), p->lds_plane, st, T, p->nst, p->nra, \
                      p->d_srow, p->d_scol, p->d_rowval, p->f1, p->f2, tw1, tw2, w, nb, kBandsPerGroup,  \
                      p->cchunk, part)
-  switch (p->ept_p) {
-    case 4: ADDFT_PD(4); break;
-    case 8: ADDFT_PD(8); break;
-    case 16: ADDFT_PD(16); break;
-    default: ADDFT_PD(32); break;
-  }
+    switch (p->ept_p) {
+      case 4: ADDFT_PD(4); break;
+      case 8: ADDFT_PD(8); break;
+      case 16: ADDFT_PD(16); break;
+      default: ADDFT_PD(32); break;
+    }
 #undef ADDFT_PD
-  ADDFT_LAUNCH_CHECK();
+    ADDFT_LAUNCH_CHECK();
+  }
   const int64_t nfft = (int64_t)p->n1 * p->n2 * p->n3;
   hipLaunchKernelGGL(k_reduce_parts, dim3(grid_blocks(nfft)), dim3(256), 0, st, part, ngr, nfft, rho,
                      accumulate);
@@ -817,32 +1143,59 @@ int addft_sphere_plan_create(int n1, int n2, int n3, int64_t ng, const int32_t* 
       rowinfo[4 * r + 2] = (int32_t)cols.size();
     }
     ok = ok && s == p->nst;
-    const size_t llds = precision == ADDFT_C64 ? lane_lds_bytes<float>(n1, p->nra, kLaneWaves)
-                                               : lane_lds_bytes<double>(n1, p->nra, kLaneWaves);
+    const size_t llds =
+        precision == ADDFT_C64
+            ? std::max(lane_lds_bytes<float>(n1, p->nra, kLaneWaves),
+                       lane_lds_bytes<float>(n1, p->nra, dens_lane_waves<float>()))
+            : std::max(lane_lds_bytes<double>(n1, p->nra, kLaneWaves),
+                       lane_lds_bytes<double>(n1, p->nra, dens_lane_waves<double>()));
     if (ok && llds <= kLdsMax) {
       p->lane_ok = 1;
       p->lane_kernel = 1;
     }
+  }
+  // lane-group z pass: n3 with a codelet and every stick a cyclic interval of i3
+  p->lane_z = 0;
+  p->d_a3 = nullptr;
+  std::vector<int32_t> a3;
+  if (lane_width(n3) > 0) {
+    bool ok = true;
+    a3.assign((size_t)p->nst, 0);
+    for (int64_t sk = 0; ok && sk < p->nst; ++sk) {
+      std::vector<int> zs(iz.begin() + soff[sk], iz.begin() + soff[sk + 1]);
+      int a = 0;
+      ok = cyclic_interval(zs, n3, &a);
+      a3[sk] = a;
+    }
+    const size_t zl = precision == ADDFT_C64 ? zlane_lds_bytes<float>(n3) : zlane_lds_bytes<double>(n3);
+    p->lane_z = ok && zl <= kLdsMax;
   }
   int rc = ADDFT_OK;
   if ((rc = upload_i32(perm, &p->d_perm)) || (rc = upload_i32(iperm, &p->d_iperm)) ||
       (rc = upload_i32(sid, &p->d_sid)) || (rc = upload_i32(iz, &p->d_iz)) ||
       (rc = upload_i32(soff, &p->d_soff)) || (rc = upload_i32(srow, &p->d_srow)) ||
       (rc = upload_i32(scol, &p->d_scol)) || (rc = upload_i32(rowval, &p->d_rowval)) ||
-      (p->lane_ok && (rc = upload_i32(rowinfo, &p->d_rowinfo)))) {
+      (p->lane_ok && (rc = upload_i32(rowinfo, &p->d_rowinfo))) ||
+      (p->lane_z && (rc = upload_i32(a3, &p->d_a3)))) {
     delete p;
     return rc;
   }
   if (precision == ADDFT_C64) {
     if ((rc = upload_twiddles<float>(n1, &p->d_tw1)) || (rc = upload_twiddles<float>(n2, &p->d_tw2)) ||
         (rc = upload_twiddles<float>(n3, &p->d_tw3)) || (rc = set_plane_attrs_any<float>(ept, p->lds_plane)) ||
-        (p->lane_ok && (rc = launch_plane_lane<float>(p, nullptr, nullptr, 0, nullptr))))
+        (p->lane_ok && (rc = launch_plane_lane<float>(p, nullptr, nullptr, 0, nullptr))) ||
+        (p->lane_ok && (rc = launch_density_lane<float>(p, nullptr, nullptr, 0, nullptr, nullptr))) ||
+        (p->lane_z && (rc = launch_z_inverse<float>(p, nullptr, nullptr, 0, nullptr))) ||
+        (p->lane_z && (rc = launch_z_forward<float, float>(p, nullptr, 1.0f, nullptr, nullptr, nullptr, 0, nullptr))))
       return rc;
   } else {
     if ((rc = upload_twiddles<double>(n1, &p->d_tw1)) || (rc = upload_twiddles<double>(n2, &p->d_tw2)) ||
         (rc = upload_twiddles<double>(n3, &p->d_tw3)) ||
         (rc = set_plane_attrs_any<double>(ept, p->lds_plane)) ||
-        (p->lane_ok && (rc = launch_plane_lane<double>(p, nullptr, nullptr, 0, nullptr))))
+        (p->lane_ok && (rc = launch_plane_lane<double>(p, nullptr, nullptr, 0, nullptr))) ||
+        (p->lane_ok && (rc = launch_density_lane<double>(p, nullptr, nullptr, 0, nullptr, nullptr))) ||
+        (p->lane_z && (rc = launch_z_inverse<double>(p, nullptr, nullptr, 0, nullptr))) ||
+        (p->lane_z && (rc = launch_z_forward<double, double>(p, nullptr, 1.0, nullptr, nullptr, nullptr, 0, nullptr))))
       return rc;
   }
   *out = p;
@@ -852,7 +1205,7 @@ int addft_sphere_plan_create(int n1, int n2, int n3, int64_t ng, const int32_t* 
 int addft_sphere_plan_destroy(addft_sphere_plan p) {
   if (!p) return ADDFT_OK;
   void* ptrs[] = {p->d_perm, p->d_iperm, p->d_sid, p->d_iz, p->d_soff, p->d_srow,
-                  p->d_scol, p->d_rowval, p->d_tw1, p->d_tw2, p->d_tw3, p->d_rowinfo};
+                  p->d_scol, p->d_rowval, p->d_tw1, p->d_tw2, p->d_tw3, p->d_rowinfo, p->d_a3};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   delete p;
@@ -969,12 +1322,8 @@ int addft_sphere_to_real(addft_sphere_plan p, const double* c, double scale, dou
   ADDFT_CHECK_ARG(p->precision == ADDFT_C128, "sphere_to_real needs a C128 plan");
   hipStream_t st = addft::as_stream(stream);
   cpx<double>* T = reinterpret_cast<cpx<double>*>(work);
-  const size_t zlds = (size_t)(p->spb + 1) * p->n3 * sizeof(cpx<double>);
-  dim3 zg((unsigned)((p->nst + p->spb - 1) / p->spb), 1u);
-  hipLaunchKernelGGL((k_z_inverse<double, double>), zg, dim3(kZThreads), zlds, st,
-                     reinterpret_cast<const cpx<double>*>(c), p->ng, p->d_soff, p->d_sid, p->d_iz, p->nst,
-                     p->spb, p->f3, reinterpret_cast<const cpx<double>*>(p->d_tw3), T);
-  ADDFT_LAUNCH_CHECK();
+  int rc = launch_z_inverse<double>(p, reinterpret_cast<const cpx<double>*>(c), T, 1, st);
+  if (rc != ADDFT_OK) return rc;
   auto tw1 = reinterpret_cast<const cpx<double>*>(p->d_tw1);
   auto tw2 = reinterpret_cast<const cpx<double>*>(p->d_tw2);
 #define ADDFT_PR(E)                                                                                  \
@@ -1012,14 +1361,8 @@ int addft_real_to_sphere(addft_sphere_plan p, const double* grid, double scale, 
   }
 #undef ADDFT_RP
   ADDFT_LAUNCH_CHECK();
-  const size_t zlds = (size_t)(p->spb + 1) * p->n3 * sizeof(cpx<double>);
-  dim3 zg((unsigned)((p->nst + p->spb - 1) / p->spb), 1u);
-  hipLaunchKernelGGL((k_z_forward<double, double>), zg, dim3(kZThreads), zlds, st, T, p->ng, p->d_soff,
-                     p->d_sid, p->d_iz, p->nst, p->spb, p->f3, reinterpret_cast<const cpx<double>*>(p->d_tw3),
-                     scale, (const double*)nullptr, (const cpx<double>*)nullptr,
-                     reinterpret_cast<cpx<double>*>(c));
-  ADDFT_LAUNCH_CHECK();
-  return ADDFT_OK;
+  return launch_z_forward<double, double>(p, T, scale, (const double*)nullptr, (const cpx<double>*)nullptr,
+                                          reinterpret_cast<cpx<double>*>(c), 1, st);
 }
 
 }  // extern "C"

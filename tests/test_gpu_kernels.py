@@ -300,14 +300,18 @@ def test_plane_kernels_agree(a, ecut, cdt, tol):
     tk = torch.as_tensor(OS.kinetic_diag(basis, a * np.eye(3), np.zeros(3))[plan.perm]).to(rd).to(_dev())
     v = torch.as_tensor(0.3 * np.cos(np.arange(basis.nfft) * 0.013) + 0.1).to(_dev())
     vs = plan.stage(v, 1.0 / basis.nfft)
+    w = torch.tensor([0.7, 1.3, 0.2], dtype=torch.float64, device=_dev())
     h_lane = plan.hpsi_local(psi, tk, vs)
+    d_lane = plan.density(psi, w)
     plan.set_plane_kernel("lds")
     assert plan.plane_kernel == "lds"
     h_lds = plan.hpsi_local(psi, tk, vs)
+    d_lds = plan.density(psi, w)
     plan.set_plane_kernel("lane")
     err = (h_lane - h_lds).abs().max().item() / h_lds.abs().max().item()
-    print(f"box {basis.fft_shape} {cdt}: lane vs lds rel err {err:.2e}", flush=True)
-    assert err < tol
+    derr = (d_lane - d_lds).abs().max().item() / d_lds.abs().max().item()
+    print(f"box {basis.fft_shape} {cdt}: lane vs lds rel err Hpsi {err:.2e} density {derr:.2e}", flush=True)
+    assert err < tol and derr < tol
 
 
 def test_lane_kernel_unavailable_is_an_error():

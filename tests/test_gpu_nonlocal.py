@@ -1,0 +1,36 @@
+"""BF16x3 projector products (FP32 phase of the mixed form) against the
+complex128 products they replace (reference physics/nonlocal_op.py:67-76).
+Bar: relative error below 1e-4 of the largest element (the split product
+keeps ~16 mantissa bits; FP32 keeps 24)."""
+
+import pytest
+
+from conftest import gpu_available
+
+pytestmark = [pytest.mark.gpu, pytest.mark.skipif(not gpu_available(), reason="needs a GPU")]
+
+
+@pytest.mark.parametrize("npj,ng,nb,nslice", [(40, 2049, 24, 1), (320, 23847, 48, 9), (64, 60001, 16, 32)])
+def test_bf16x3_products_match_complex128(npj, ng, nb, nslice):
+    import torch
+    from addft.kernels.nonlocal_bf16 import ProjectorBF16x3
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device="cpu").manual_seed(npj + ng)
+    P = torch.complex(torch.randn(npj, ng, generator=g, dtype=torch.float64),
+                      torch.randn(npj, ng, generator=g, dtype=torch.float64)).to(dev) / ng ** 0.5
+    psi = torch.complex(torch.randn(nb, ng, generator=g, dtype=torch.float64),
+                        torch.randn(nb, ng, generator=g, dtype=torch.float64)).to(dev) / ng ** 0.5
+    S = torch.complex(torch.randn(nb, npj, generator=g, dtype=torch.float64),
+                      torch.randn(nb, npj, generator=g, dtype=torch.float64)).to(dev)
+    pb = ProjectorBF16x3(P, nslice=nslice)
+    b = pb.becp_t(psi.to(torch.complex64)).to(torch.complex128)
+    bref = psi @ P.conj().T
+    eb = ((b - bref).abs().max() / bref.abs().max()).item()
+    o = pb.apply(S.to(torch.complex64)).to(torch.complex128)
+    oref = S @ P
+    eo = ((o - oref).abs().max() / oref.abs().max()).item()
+    print(f"npj {npj} ng {ng}: becp rel err {eb:.2e}, apply rel err {eo:.2e}", flush=True)
+    assert eb < 1e-4 and eo < 1e-4
+    # and clearly better than a single bf16 product
+    b1 = (psi.to(torch.complex64).to(torch.complex128) @ P.conj().T)
+    assert eb < 1e-3
