@@ -1,0 +1,64 @@
+"""Pins of the oracle's SCF-map response and DeltaQ adjoint (CPU).
+
+* J dn against central differences of the map in the density;
+* dDeltaQ/dR and dDeltaQ/deps by the implicit adjoint against central
+  differences of DeltaQ with the SCF re-converged at each displaced
+  geometry (the reference's check, tests/test_forces.py:187-217, on its
+  system: a = 6, ecut 1.5, density factor 2, nband 2, sigma 0.05, Gaussian
+  weight at atom 2, width 0.9), with Richardson extrapolation of the
+  differences (error h^4), and on a partially occupied variant (nband 6)
+  where f' and dmu enter.  That variant has four states within 1.5 mHa of
+  one another at the Fermi level, so DeltaQ(eps) is strongly
+  non-quadratic: its differences need steps of 1e-5.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import basis as B
+from oracle import psp as PS
+from oracle import response as RS
+from oracle import solver as S
+
+
+def _system(nband):
+    geo, st = S.setup(6.0 * np.eye(3), np.array([[0, 0, 0], [0.53, 0.47, 0.49]]), PS.synthetic(), 1.5, nband,
+                      sigma=0.05, density_factor=2.0)
+    state = S.solve_scf_dense(geo, st, tol=1e-14, mixer="kerker", max_iter=600)
+    w = RS.make_gaussian_weight(st.basis, B.positions(geo.lattice, geo.frac)[1], 0.9, geo.lattice)
+    return geo, st, state, w
+
+
+def test_response_matches_map_differences():
+    geo, st, state, _ = _system(6)
+    apply = RS.make_response(geo, st, state)
+    d = np.random.default_rng(0).normal(size=st.basis.nfft) * 1e-3
+    h = 1e-3
+    fd = (S.scf_map(geo, st, state.rho_r + h * d) - S.scf_map(geo, st, state.rho_r - h * d)) / (2 * h)
+    assert np.abs(apply(d) - fd).max() < 1e-6 * np.abs(fd).max()
+    # the reference's sign of the occupation term (response.py:100-102) is
+    # off for partially occupied bands ...
+    ref = RS.make_response(geo, st, state, occ_sign="reference")(d)
+    assert np.abs(ref - fd).max() > 0.05 * np.abs(fd).max()
+    # ... and identical where f' = 0 (its own test system, nband 2)
+    geo2, st2, state2, _ = _system(2)
+    a = RS.make_response(geo2, st2, state2)(d)
+    b = RS.make_response(geo2, st2, state2, occ_sign="reference")(d)
+    assert np.abs(a - b).max() < 1e-14
+
+
+def _richardson(geo, st, w, what, idx, h):
+    f1 = RS.delta_q_fd(geo, st, w, what, idx, h)
+    f2 = RS.delta_q_fd(geo, st, w, what, idx, h / 2)
+    return (4 * f2 - f1) / 3
+
+
+@pytest.mark.parametrize("nband,h", [(2, 2e-4), (6, 1e-5)])
+def test_delta_q_adjoint_matches_rescf_differences(nband, h):
+    geo, st, state, w = _system(nband)
+    dR, de, info = RS.delta_q_grad(geo, st, state, w)
+    assert info["adjoint_residual"] < 1e-10
+    for what, idx, got in [("R", (0, 0), dR[0, 0]), ("R", (1, 2), dR[1, 2]),
+                           ("eps", (0, 0), de[0, 0]), ("eps", (0, 1), de[0, 1])]:
+        fd = _richardson(geo, st, w, what, idx, h)
+        assert abs(got - fd) < 2e-8 + 1e-6 * abs(fd), (what, idx, got, fd)
