@@ -188,3 +188,40 @@ def energy_forces_stress(system: System, **scf_kwargs):
     F, g, parts = _gradient(s, system.geo)
     sig = stress_from_grad(volume(system.geo.lattice), 0.5 * (g.eps + g.eps.T))
     return F, forces_from_grad(system.geo, g), sig, s.info
+
+
+# ------------------------------------------------------------------ DeltaQ
+def make_gaussian_weight(system: System, center, sigma: float = 0.8) -> np.ndarray:
+    """w = exp(-|r - c|^2 / 2 sigma^2) at the box points r = (i / n) A (no
+    minimum image), scaled to sum nfft; row-major grid (constraint.py
+    :49-55)."""
+    basis = system.static.basis
+    shape = tuple(basis.fft_shape)
+    idx = np.stack(np.meshgrid(*(np.arange(n) for n in shape), indexing="ij"), axis=-1).reshape(-1, 3)
+    r = (idx / np.asarray(shape, dtype=np.float64)) @ np.asarray(system.geo.lattice, dtype=np.float64)
+    w = np.exp(-0.5 * np.sum((r - np.asarray(center, dtype=np.float64)) ** 2, axis=-1) / sigma ** 2)
+    return w / np.sum(w) * basis.nfft
+
+
+def delta_q(system: System, state: SCFState, weight) -> float:
+    """C = (Omega/N) sum_j w_j n_j on the row-major grid (constraint.py:26-34)."""
+    rho = torch.as_tensor(state.rho_r).to(torch.float64)
+    w = torch.as_tensor(np.asarray(weight, dtype=np.float64)).to(rho.device)
+    return volume(system.geo.lattice) / system.static.basis.nfft * float(torch.dot(w, rho))
+
+
+def delta_q_grad(system: System, state: SCFState | None = None, weight=None, rtol: float = 1e-10,
+                 occ_sign: str = "exact", **scf_kwargs):
+    """dDeltaQ/d(positions) by the implicit adjoint (constraint.py:37-46,
+    implicit.py:32-92): returns (DeltaQGradient(dR, frac, value, diagnostics)).
+    occ_sign="reference" reproduces the reference's sign of the occupation
+    term of the response (physics/response.py:100-102), which differs from
+    the derivative of the map where bands are partially occupied."""
+    from .response import DeltaQGradient, delta_q_grad as _dq
+    s = _solved(system, state, scf_kwargs)
+    eng = s.engine
+    w_pl = eng.to_planar(torch.as_tensor(np.asarray(weight, dtype=np.float64)).to(eng.dev))
+    dR, diag = _dq(eng, s.psi, s.eig, s.occ, s.rho, w_pl, rtol=rtol, occ_sign=occ_sign)
+    A = np.asarray(system.geo.lattice, dtype=np.float64)
+    value = eng.omega / eng.nfft * float(torch.dot(w_pl, s.rho))
+    return DeltaQGradient(dR=dR, frac=dR @ A.T, value=value, diagnostics=diag)

@@ -420,15 +420,7 @@ class Engine:
             ek -= out10[1:].cpu().numpy().reshape(3, 3)
         parts["kinetic"] = (np.zeros((nat, 3)), ek)
         # local on the rho-sphere
-        rt = self.plan_rho.to_sphere(rho_psi, 1.0 / self.nfft)
-        lout = torch.empty(3 * nat + 10, dtype=torch.float64, device=self.dev)
-        wb = int(L.addft_backward_work_bytes(max(nat, 1)))
-        check(L.addft_local_backward(st.basis.nG_rho, nat, ptr(self.grho_d), ptr(self.qnrho_d), ptr(self.vq_d),
-                                     ptr(self.dvq_d), ptr(rt), ptr(self.pos_d), ptr(lout),
-                                     ptr(self.work("bwd", wb)), stream()), "addft_local_backward")
-        lo = lout.cpu().numpy()
-        e_loc = lo[3 * nat]
-        parts["local"] = (lo[:3 * nat].reshape(nat, 3), -np.eye(3) * e_loc + lo[3 * nat + 1:].reshape(3, 3))
+        parts["local"] = self.local_gradient(rho_psi)
         # Hartree strain (psi-sphere)
         _, hout = self.hartree(rho_psi)
         ho = hout.cpu().numpy()
@@ -440,31 +432,57 @@ class Engine:
         e_xc = dv * float(exc)
         parts["xc"] = (np.zeros((nat, 3)), np.eye(3) * (e_xc - dv * float((vxc * rho_psi).sum())))
         # nonlocal
-        fn = torch.zeros((nat, 3), dtype=torch.float64, device=self.dev)
+        fn = np.zeros((nat, 3))
         en = np.zeros((3, 3))
-        isq = 1.0 / math.sqrt(self.omega)
-        nout = torch.empty((self.nproj, 12), dtype=torch.float64, device=self.dev)
-        wbn = int(L.addft_backward_work_bytes(self.nproj))
         for ik in range(self.nk):
             psi = psis[ik].to(torch.complex128)
             wf = (st.kweights[ik] * st.spin_degeneracy) * occ[ik]
             b = self.becp(ik, psi)                          # (nproj, nb)
             W = (self.D64 @ b).conj() * wf[None, :]
             Y = torch.matmul(W, psi).contiguous()           # (nproj, nG)
-            kk = np.ascontiguousarray(self.st.kpts[ik], dtype=np.float64)
-            check(L.addft_nonlocal_backward(_lib.C128, self.nproj, self.ng, ptr(self.patom_d), ptr(self.prad_d),
-                                            ptr(self.plm_d), ptr(self.g0_d), ptr(self.kq[ik]), ptr(self.kqn[ik]),
-                                            ptr(self.beta[ik]), ptr(self.dbeta[ik]), ptr(self.pos_d), kk.ctypes.data,
-                                            isq, ptr(Y), ptr(nout), ptr(self.work("bwd_nl", wbn)), stream()),
-                  "addft_nonlocal_backward")
-            fn.index_add_(0, self.patom_l, nout[:, :3])
-            en += nout[:, 3:].sum(dim=0).cpu().numpy().reshape(3, 3)
-        parts["nonlocal"] = (fn.cpu().numpy(), en)
+            f_k, e_k = self.nonlocal_gradient(ik, Y)
+            fn += f_k
+            en += e_k
+        parts["nonlocal"] = (fn, en)
         e, dR, de = self.ewald()
         parts["ewald"] = (dR, de)
         dR_tot = sum(v[0] for v in parts.values())
         de_tot = sum(v[1] for v in parts.values())
         return dR_tot, de_tot, parts
+
+    def local_gradient(self, rho_pl):
+        """(dE/dR (nat, 3), dE/deps (3, 3)) of E_loc = (Omega/N) sum V_loc rho
+        at fixed rho (plane-major), i.e. Tr[dV_loc/dtheta rho]."""
+        L = lib()
+        nat = self.nat
+        rt = self.plan_rho.to_sphere(rho_pl, 1.0 / self.nfft)
+        lout = torch.empty(3 * nat + 10, dtype=torch.float64, device=self.dev)
+        wb = int(L.addft_backward_work_bytes(max(nat, 1)))
+        check(L.addft_local_backward(self.st.basis.nG_rho, nat, ptr(self.grho_d), ptr(self.qnrho_d), ptr(self.vq_d),
+                                     ptr(self.dvq_d), ptr(rt), ptr(self.pos_d), ptr(lout),
+                                     ptr(self.work("bwd", wb)), stream()), "addft_local_backward")
+        lo = lout.cpu().numpy()
+        e_loc = lo[3 * nat]
+        return lo[:3 * nat].reshape(nat, 3), -np.eye(3) * e_loc + lo[3 * nat + 1:].reshape(3, 3)
+
+    def nonlocal_gradient(self, ik, Y):
+        """2 Re sum_G conj(d beta_p/dtheta) Y_p summed per atom: (nat, 3) and
+        (3, 3) for Y (nproj, nG) = Gamma (beta D) with Gamma the (weighted)
+        density matrix -- Tr[dV_NL/dtheta Gamma]."""
+        L = lib()
+        isq = 1.0 / math.sqrt(self.omega)
+        nout = torch.empty((self.nproj, 12), dtype=torch.float64, device=self.dev)
+        wbn = int(L.addft_backward_work_bytes(self.nproj))
+        kk = np.ascontiguousarray(self.st.kpts[ik], dtype=np.float64)
+        Y = Y.to(torch.complex128).contiguous()
+        check(L.addft_nonlocal_backward(_lib.C128, self.nproj, self.ng, ptr(self.patom_d), ptr(self.prad_d),
+                                        ptr(self.plm_d), ptr(self.g0_d), ptr(self.kq[ik]), ptr(self.kqn[ik]),
+                                        ptr(self.beta[ik]), ptr(self.dbeta[ik]), ptr(self.pos_d), kk.ctypes.data,
+                                        isq, ptr(Y), ptr(nout), ptr(self.work("bwd_nl", wbn)), stream()),
+              "addft_nonlocal_backward")
+        fn = torch.zeros((self.nat, 3), dtype=torch.float64, device=self.dev)
+        fn.index_add_(0, self.patom_l, nout[:, :3])
+        return fn.cpu().numpy(), nout[:, 3:].sum(dim=0).cpu().numpy().reshape(3, 3)
 
     # ---------------------------------------------------------- orders
     def to_stick(self, psi_ref):

@@ -1,0 +1,261 @@
+"""Density response of the SCF map and the DeltaQ implicit adjoint, on the
+device and matrix-free.
+
+Reference: physics/response.py:33-128 (make_response), diff/implicit.py
+:32-92 (implicit_grad), observables/constraint.py:26-55 (delta_q).  The
+reference builds chi0 from the FULL dense eigensystem of H[n*] (feasible only
+for the smallest cells).  Here the same response is split exactly into
+
+  * the pairs (m, n) of the nband computed states (the state's psi, which are
+    eigenvectors of H[n*]): coefficient <m|dV|n> / (e_n - e_m), cut at
+    |e_n - e_m| <= 1e-6 as in the reference (response.py:90-92);
+  * the complement of that set, for every band with f_n > 1e-12, by the
+    Sternheimer equation  Q (H - e_n) Q x_n = -Q dV psi_n  (Q = 1 - sum over
+    the computed states), solved by block preconditioned conjugate
+    gradients with the FP64 Hpsi -- the sum over all higher eigenstates of
+    the reference without forming them;
+  * the occupation change df_n = f'_n (dmu - de_n), dmu per k-point as in the
+    reference (response.py:96-103), with the sign fixed (see occ_sign).
+
+Then, with K the Hartree (psi-sphere) + f_xc kernel, J = chi0 K and
+J^T = K chi0 (both operators are symmetric on the grid), and the adjoint of
+DeltaQ = (Omega/N) sum w n solves (I - J)^T lam = (Omega/N) w by BiCGSTAB
+(implicit.py:55-69).  The map term lam . dF/dR|_n* = Tr[dH/dR dP_lam] is the
+backward of the local and nonlocal operators with the density matrix
+response dP_lam to the potential lam (dF/dR = chi0 dH/dR and chi0 is
+symmetric); it reuses the envelope-force backward kernels, scaled by
+N/Omega from their trace normalisation to the plain grid dot product.
+"""
+
+from __future__ import annotations
+
+import math
+from typing import NamedTuple
+
+import numpy as np
+import torch
+
+from .kernels import native
+from .solver.davidson import precondition
+from .solver.linalg import inner
+
+
+class DeltaQGradient(NamedTuple):
+    dR: np.ndarray        # dC/dR, Cartesian (nat, 3)
+    frac: np.ndarray      # dC/dfrac
+    value: float          # C
+    diagnostics: dict
+
+
+def bicgstab(apply, b, rtol=1e-10, maxiter=200):
+    """van der Vorst BiCGSTAB for apply(x) = b (real vectors), as
+    jax.scipy.sparse.linalg.bicgstab (used by implicit.py:59).  Returns
+    (x, iterations)."""
+    x = torch.zeros_like(b)
+    r = b.clone()
+    rhat = r.clone()
+    bnorm = float(torch.linalg.vector_norm(b)) or 1.0
+    rho = alpha = omega = 1.0
+    v = torch.zeros_like(b)
+    p = torch.zeros_like(b)
+    it = 0
+    for it in range(1, maxiter + 1):
+        rho_new = float(torch.dot(rhat, r))
+        if rho_new == 0.0:
+            break
+        beta = (rho_new / rho) * (alpha / omega)
+        p = r + beta * (p - omega * v)
+        v = apply(p)
+        alpha = rho_new / float(torch.dot(rhat, v))
+        s = r - alpha * v
+        if float(torch.linalg.vector_norm(s)) < rtol * bnorm:
+            x = x + alpha * p
+            break
+        t = apply(s)
+        omega = float(torch.dot(t, s)) / float(torch.dot(t, t))
+        x = x + alpha * p + omega * s
+        r = s - omega * t
+        if float(torch.linalg.vector_norm(r)) < rtol * bnorm:
+            break
+        rho = rho_new
+    return x, it
+
+
+class Response:
+    """chi0, K and J at a converged state (stick-order psi per k, complex128)."""
+
+    def __init__(self, engine, psis, eig, occ, rho_pl, deg_tol=1e-6, occ_sign="exact",
+                 cg_tol=1e-11, cg_maxiter=500, fthr=1e-12):
+        self.e = engine
+        st = engine.st
+        self.psi = [p.to(torch.complex128).contiguous() for p in psis]
+        self.eig = eig.to(torch.float64)
+        self.occ = occ.to(torch.float64)
+        self.deg_tol = float(deg_tol)
+        self.sgn = 1.0 if occ_sign == "exact" else -1.0
+        self.cg_tol = float(cg_tol)
+        self.cg_maxiter = int(cg_maxiter)
+        self.fthr = float(fthr)
+        v, _ = engine.effective_potential(rho_pl)
+        self.vs = engine.stage(v)
+        self.vmean = float(v.mean())
+        self.kxc = native.xc_fxc(rho_pl)
+        self.zero_t = torch.zeros(engine.ng, dtype=torch.float64, device=engine.dev)
+        self.kw = [float(st.kweights[ik]) * st.spin_degeneracy for ik in range(engine.nk)]
+        self.stats = {"chi0": 0, "cg_iterations": 0, "hpsi_bands": 0}
+        self.kept = None
+
+    # --------------------------------------------------------------- K
+    def K(self, dn):
+        """Hartree (psi-sphere) + f_xc kernel on a plane-major density change."""
+        vh, _ = self.e.hartree(dn)
+        return vh + self.kxc * dn
+
+    # ------------------------------------------------------- Sternheimer
+    def _project(self, ik, V):
+        psi = self.psi[ik]
+        return V - torch.matmul(inner(V, psi), psi)
+
+    def sternheimer(self, ik, Vpsi, eps):
+        """x_n (rows) with Q (H - e_n) Q x_n = -Q Vpsi_n, x_n in range(Q)."""
+        e = self.e
+        b = -self._project(ik, Vpsi)
+        bn = torch.linalg.vector_norm(b, dim=1)
+        diag = e.tkin64[ik] + self.vmean
+        x = torch.zeros_like(b)
+        r = b.clone()
+        z = self._project(ik, precondition(r, eps, diag))
+        p = z.clone()
+        rz = torch.sum((r.conj() * z).real, dim=1)
+        active = torch.arange(b.shape[0], device=b.device)
+        it = 0
+        for it in range(1, self.cg_maxiter + 1):
+            pa = p[active]
+            Ap = self._project(ik, e.hpsi(ik, pa, self.vs) - eps[active, None].to(pa.real.dtype) * pa)
+            self.stats["hpsi_bands"] += int(active.numel())
+            pAp = torch.sum((pa.conj() * Ap).real, dim=1)
+            alpha = rz[active] / pAp
+            x[active] += alpha[:, None] * pa
+            r[active] -= alpha[:, None] * Ap
+            rn = torch.linalg.vector_norm(r[active], dim=1)
+            done = rn <= self.cg_tol * torch.clamp(bn[active], min=1e-300)
+            keep = active[~done]
+            if keep.numel() == 0:
+                break
+            zk = self._project(ik, precondition(r[keep], eps[keep], diag))
+            rz_new = torch.sum((r[keep].conj() * zk).real, dim=1)
+            beta = rz_new / rz[keep]
+            p[keep] = zk + beta[:, None] * p[keep]
+            rz[keep] = rz_new
+            active = keep
+        self.stats["cg_iterations"] += it
+        return x
+
+    # -------------------------------------------------------------- chi0
+    def chi0(self, dv, keep=False):
+        """Density response (plane-major) to the local potential dv."""
+        e = self.e
+        st = e.st
+        self.stats["chi0"] += 1
+        vs = e.stage(dv)
+        out = torch.zeros(e.nfft, dtype=torch.float64, device=e.dev)
+        kept = []
+        for ik in range(e.nk):
+            psi = self.psi[ik]
+            eps = self.eig[ik]
+            f = self.occ[ik]
+            Vpsi = e.plan64.hpsi_local(psi, self.zero_t, vs)
+            Lam = inner(psi, Vpsi, conj_a=True)           # Lam[m, n] = <m|dV|n>
+            Lam = 0.5 * (Lam + Lam.conj().T)
+            de = Lam.diagonal().real
+            d = eps[None, :] - eps[:, None]               # d[m, n] = e_n - e_m
+            off = d.abs() > self.deg_tol
+            coef = torch.where(off, Lam / torch.where(off, d, torch.ones_like(d)).to(Lam.dtype),
+                               torch.zeros_like(Lam))
+            dpsi = torch.matmul(coef.T, psi)              # dpsi_n = sum_m coef[m, n] psi_m
+            occ_idx = torch.nonzero(f > self.fthr).flatten()
+            if occ_idx.numel() > 0:
+                dpsi[occ_idx] += self.sternheimer(ik, Vpsi[occ_idx], eps[occ_idx])
+            if st.sigma > 1e-6:
+                fp = f * (1.0 - f) / st.sigma
+                dmu = float(torch.sum(fp * de)) / (float(torch.sum(fp)) + 1e-30)
+                df = self.sgn * fp * (dmu - de)
+            else:
+                df = torch.zeros_like(f)
+            c = self.kw[ik] / e.omega
+            out += self._cross_density(psi, dpsi, c * f)
+            out += e.plan64.density(psi, (c * df).contiguous())
+            if keep:
+                kept.append((dpsi, df))
+        if keep:
+            self.kept = kept
+        return out
+
+    def _cross_density(self, psi, phi, w):
+        """sum_n w_n 2 Re(conj(psi_n(r)) phi_n(r)) by polarisation:
+        (|psi + t phi|^2 - |psi - t phi|^2) / (2t), t_n = |psi_n| / |phi_n|."""
+        e = self.e
+        pn = torch.linalg.vector_norm(psi, dim=1)
+        fn = torch.linalg.vector_norm(phi, dim=1)
+        t = torch.where(fn > 0, pn / torch.clamp(fn, min=1e-300), torch.ones_like(pn)).clamp(max=1e8)
+        a = psi + t[:, None].to(psi.dtype) * phi
+        b = psi - t[:, None].to(psi.dtype) * phi
+        wt = (w / (2.0 * t)).contiguous()
+        return e.plan64.density(a, wt) - e.plan64.density(b, wt)
+
+    # ------------------------------------------------------------ J, J^T
+    def J(self, dn):
+        return self.chi0(self.K(dn))
+
+    def JT(self, y):
+        return self.K(self.chi0(y))
+
+    # --------------------------------------------------- map term dF/dR
+    def map_term_positions(self, lam):
+        """lam . dF/dR|_n* (plain sum over the grid points, as the
+        reference's jnp.sum(lam * scf_map(...)), implicit.py:78-80) =
+        (N/Omega) Tr[dH/dR dP_lam]: the backward kernels return traces with
+        the (Omega/N) quadrature weight of <psi|V|psi>.  (nat, 3), Cartesian."""
+        e = self.e
+        rho_lam = self.chi0(lam, keep=True)
+        dR, _ = e.local_gradient(rho_lam)
+        for ik in range(e.nk):
+            psi = self.psi[ik]
+            dpsi, df = self.kept[ik]
+            f = self.occ[ik]
+            c = self.kw[ik] * f
+            dc = self.kw[ik] * df
+            B = e.becp(ik, psi)                           # (nproj, nb) <beta|psi>
+            dB = e.becp(ik, dpsi)                         # <beta|dpsi>
+            BD = torch.matmul(B.conj().T, e.D64)          # (nb, nproj) <psi_n|(beta D)_p>
+            dBD = torch.matmul(dB.conj().T, e.D64)
+            Y = (torch.matmul((c[:, None] * BD).T.to(torch.complex128), dpsi)
+                 + torch.matmul((c[:, None] * dBD + dc[:, None] * BD).T.to(torch.complex128), psi))
+            f_k, _ = e.nonlocal_gradient(ik, Y)
+            dR = dR + f_k
+        return dR * (e.nfft / e.omega)
+
+
+def delta_q_value(engine, rho_pl, weight_pl):
+    """C = (Omega/N) sum_j w_j n_j (constraint.py:26-34)."""
+    return engine.omega / engine.nfft * float(torch.dot(weight_pl, rho_pl))
+
+
+def delta_q_grad(engine, psis, eig, occ, rho_pl, weight_pl, rtol=1e-10, maxiter=200, **resp_kw):
+    """dC/dR for C = (Omega/N) sum w n at the SCF fixed point n* (implicit.py
+    :32-92): (I - J)^T lam = (Omega/N) w, then dC/dR = lam . dF/dR|_n*
+    (the state term vanishes for positions: w and n* are grid arrays)."""
+    resp = Response(engine, psis, eig, occ, rho_pl, **resp_kw)
+    g = (engine.omega / engine.nfft) * weight_pl
+
+    def aT(y):
+        return y - resp.JT(y)
+
+    lam, nit = bicgstab(aT, g, rtol=rtol, maxiter=maxiter)
+    res = float(torch.linalg.vector_norm(aT(lam) - g) / (torch.linalg.vector_norm(g) + 1e-300))
+    if res > 1e-6:
+        raise RuntimeError(f"adjoint BiCGSTAB did not converge: residual {res:.3e}")
+    dR = resp.map_term_positions(lam)
+    diag = {"adjoint_residual": res, "bicgstab_iterations": nit, "lam_norm": float(torch.linalg.vector_norm(lam)),
+            "g_norm": float(torch.linalg.vector_norm(g)), **resp.stats}
+    return dR, diag

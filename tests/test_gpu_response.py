@@ -1,0 +1,99 @@
+"""GPU parity of the matrix-free SCF-map response and the DeltaQ adjoint.
+
+* J dn (in-set pairs + Sternheimer complement + occupation change) against
+  the oracle's dense-eigensystem response (reference physics/response.py
+  :33-114 with the occupation sign fixed) -- bar 1e-8 relative;
+* dDeltaQ/dR against the oracle's implicit adjoint on the reference's own
+  DeltaQ system (tests/test_forces.py:187-217) and on its partially occupied
+  variant -- bar 1e-7;
+* the 8-atom cell: dDeltaQ/dR against central differences of DeltaQ with
+  the SCF re-converged on the GPU (Richardson, steps 1e-4 / 5e-5) -- bar 1e-5
+  relative.
+"""
+
+import numpy as np
+import pytest
+
+from conftest import gpu_available
+from oracle import basis as OB
+from oracle import psp as OPS
+from oracle import response as ORS
+from oracle import solver as OS
+
+pytestmark = [pytest.mark.gpu, pytest.mark.skipif(not gpu_available(), reason="needs a GPU")]
+
+LAT = 6.0 * np.eye(3)
+FRAC = np.array([[0.0, 0.0, 0.0], [0.53, 0.47, 0.49]])
+
+
+def _oracle(nband):
+    geo, st = OS.setup(LAT, FRAC, OPS.synthetic(), 1.5, nband, sigma=0.05, density_factor=2.0)
+    state = OS.solve_scf_dense(geo, st, tol=1e-14, mixer="kerker", max_iter=600)
+    w = ORS.make_gaussian_weight(st.basis, OB.positions(LAT, FRAC)[1], 0.9, LAT)
+    return geo, st, state, w
+
+
+def _device(nband):
+    import addft
+    s = addft.setup(lattice=LAT, frac=FRAC, ecut=1.5, nband=nband, density_factor=2.0, sigma=0.05)
+    st = addft.scf(s, tol=1e-13)
+    return addft, s, st
+
+
+@pytest.mark.parametrize("nband", [2, 6])
+def test_response_matches_oracle(nband):
+    import torch
+    from addft import api
+    from addft.response import Response
+    geo, ost, ostate, w = _oracle(nband)
+    addft, s, st = _device(nband)
+    so = api._from_state(s, st)
+    resp = Response(so.engine, so.psi, so.eig, so.occ, so.rho)
+    dn = np.random.default_rng(1).normal(size=ost.basis.nfft) * 1e-3
+    ref = ORS.make_response(geo, ost, ostate)(dn)
+    eng = so.engine
+    got = eng.to_rowmajor(resp.J(eng.to_planar(torch.as_tensor(dn).to(eng.dev)))).cpu().numpy()
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    print(f"nband {nband}: J rel err {err:.2e}", flush=True)
+    assert err < 1e-8
+
+
+@pytest.mark.parametrize("nband", [2, 6])
+def test_delta_q_grad_matches_oracle(nband):
+    geo, ost, ostate, w = _oracle(nband)
+    dR_ref, _, info = ORS.delta_q_grad(geo, ost, ostate, w)
+    addft, s, st = _device(nband)
+    wd = addft.make_gaussian_weight(s, OB.positions(LAT, FRAC)[1], 0.9)
+    assert np.abs(wd - w).max() < 1e-12
+    g = addft.delta_q_grad(s, st, wd)
+    err = np.abs(g.dR - dR_ref).max()
+    print(f"nband {nband}: dC/dR err {err:.2e} (|dC/dR| {np.abs(dR_ref).max():.3e}); {g.diagnostics}", flush=True)
+    assert g.diagnostics["adjoint_residual"] < 1e-8
+    assert err < 1e-7 * max(1.0, np.abs(dR_ref).max())
+    assert abs(g.value - ORS.delta_q(ost.basis, LAT, ostate.rho_r, w)) < 1e-10
+
+
+def test_delta_q_grad_si8_matches_rescf_differences():
+    import addft
+    from addft import testsystems
+    kw = testsystems.config("c2", precision="fp64")
+    s = addft.setup(**kw)
+    st = addft.scf(s, tol=1e-12)
+    R = np.asarray(kw["frac"]) @ np.asarray(kw["lattice"])
+    w = addft.make_gaussian_weight(s, R[3], 1.2)
+    g = addft.delta_q_grad(s, st, w)
+    print(f"si8 adjoint: {g.diagnostics}", flush=True)
+
+    def C(a, c, h):
+        Rp = R.copy()
+        Rp[a, c] += h
+        kw2 = dict(kw, frac=Rp @ np.linalg.inv(kw["lattice"]))
+        s2 = addft.setup(**kw2)
+        return addft.delta_q(s2, addft.scf(s2, tol=1e-12), w)
+
+    for a, c in [(3, 0), (0, 2)]:
+        d1 = (C(a, c, 1e-4) - C(a, c, -1e-4)) / 2e-4
+        d2 = (C(a, c, 5e-5) - C(a, c, -5e-5)) / 1e-4
+        fd = (4 * d2 - d1) / 3
+        print(f"si8 dC/dR[{a},{c}]: adjoint {g.dR[a, c]:.10e} differences {fd:.10e}", flush=True)
+        assert abs(g.dR[a, c] - fd) < 1e-5 * max(abs(fd), 1e-3)
