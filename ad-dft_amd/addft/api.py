@@ -212,8 +212,9 @@ def delta_q(system: System, state: SCFState, weight) -> float:
 
 def delta_q_grad(system: System, state: SCFState | None = None, weight=None, rtol: float = 1e-10,
                  occ_sign: str = "exact", **scf_kwargs):
-    """dDeltaQ/d(positions) by the implicit adjoint (constraint.py:37-46,
-    implicit.py:32-92): returns (DeltaQGradient(dR, frac, value, diagnostics)).
+    """dDeltaQ/d(positions, strain) by the implicit adjoint (constraint.py
+    :37-46, implicit.py:32-92): returns DeltaQGradient(dR, frac, eps, value,
+    diagnostics).
     occ_sign="reference" reproduces the reference's sign of the occupation
     term of the response (physics/response.py:100-102), which differs from
     the derivative of the map where bands are partially occupied."""
@@ -221,7 +222,7 @@ def delta_q_grad(system: System, state: SCFState | None = None, weight=None, rto
     s = _solved(system, state, scf_kwargs)
     eng = s.engine
     w_pl = eng.to_planar(torch.as_tensor(np.asarray(weight, dtype=np.float64)).to(eng.dev))
-    dR, diag = _dq(eng, s.psi, s.eig, s.occ, s.rho, w_pl, rtol=rtol, occ_sign=occ_sign)
+    dR, de, diag = _dq(eng, s.psi, s.eig, s.occ, s.rho, w_pl, rtol=rtol, occ_sign=occ_sign)
     A = np.asarray(system.geo.lattice, dtype=np.float64)
     value = eng.omega / eng.nfft * float(torch.dot(w_pl, s.rho))
-    return DeltaQGradient(dR=dR, frac=dR @ A.T, value=value, diagnostics=diag)
+    return DeltaQGradient(dR=dR, frac=dR @ A.T, eps=de, value=value, diagnostics=diag)

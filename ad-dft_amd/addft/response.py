@@ -35,6 +35,7 @@ from typing import NamedTuple
 import numpy as np
 import torch
 
+from .kernels import _lib as _lib_mod
 from .kernels import native
 from .solver.davidson import precondition
 from .solver.linalg import inner
@@ -43,6 +44,7 @@ from .solver.linalg import inner
 class DeltaQGradient(NamedTuple):
     dR: np.ndarray        # dC/dR, Cartesian (nat, 3)
     frac: np.ndarray      # dC/dfrac
+    eps: np.ndarray       # dC/deps_ab, lattice A (I + eps) at fixed frac (3, 3)
     value: float          # C
     diagnostics: dict
 
@@ -218,6 +220,7 @@ class Response:
         the (Omega/N) quadrature weight of <psi|V|psi>.  (nat, 3), Cartesian."""
         e = self.e
         rho_lam = self.chi0(lam, keep=True)
+        self._last_rho = rho_lam
         dR, _ = e.local_gradient(rho_lam)
         for ik in range(e.nk):
             psi = self.psi[ik]
@@ -236,15 +239,59 @@ class Response:
         return dR * (e.nfft / e.omega)
 
 
+    def map_term_strain(self, lam, rho_lam, rho_star):
+        """lam . dF/deps|_n* (3, 3), after chi0(lam, keep=True).  With the
+        coefficients normalised in G-space, F = rho^/Omega and the grid phases
+        G.r_j do not depend on the strain, so
+          lam . dF/deps = -(lam . n*) I + (N/Omega) Tr[dH^/deps dP^_lam],
+        where the trace runs over the G-space matrices: kinetic -q_a G_b on the
+        diagonal, V_loc and V_NL through their backward kernels, and V_H(q) =
+        4 pi n*(q)/q^2 with n*(q) fixed, d/deps_ab = 4 pi n*(q) 2 q_a q_b / q^4
+        (a cross term with rho_lam, by polarisation of the Hartree strain sums).
+        v_xc(n*) is a fixed grid array and does not enter."""
+        e = self.e
+        L = _lib_mod.lib()
+        _, de = e.local_gradient(rho_lam)
+        out10 = torch.empty(10, dtype=torch.float64, device=e.dev)
+        red = e.work("red", int(L.addft_reduce_work_bytes()))
+        for ik in range(e.nk):
+            psi = self.psi[ik]
+            dpsi, df = self.kept[ik]
+            c = self.kw[ik] * self.occ[ik]
+            dc = self.kw[ik] * df
+            dens = ((c[:, None] * 2.0 * (psi.conj() * dpsi).real) + dc[:, None] * psi.abs() ** 2).sum(dim=0)
+            dens = dens.contiguous()
+            _lib_mod.check(L.addft_gsum_outer(e.ng, _lib_mod.ptr(dens), _lib_mod.ptr(e.kq[ik]), _lib_mod.ptr(e.g0_d),
+                                              _lib_mod.ptr(out10), _lib_mod.ptr(red), _lib_mod.stream()),
+                           "addft_gsum_outer")
+            de = de - out10[1:].cpu().numpy().reshape(3, 3)
+            B = e.becp(ik, psi)
+            dB = e.becp(ik, dpsi)
+            BD = torch.matmul(B.conj().T, e.D64)
+            dBD = torch.matmul(dB.conj().T, e.D64)
+            Y = (torch.matmul((c[:, None] * BD).T.to(torch.complex128), dpsi)
+                 + torch.matmul((c[:, None] * dBD + dc[:, None] * BD).T.to(torch.complex128), psi))
+            _, e_k = e.nonlocal_gradient(ik, Y)
+            de = de + e_k
+        # Hartree: Omega sum_q 4 pi 2 q_a q_b / q^4 Re(n*(q) conj(rho_lam(q)))
+        t = float(torch.linalg.vector_norm(rho_star)) / max(float(torch.linalg.vector_norm(rho_lam)), 1e-300)
+        _, sp = e.hartree(rho_star + t * rho_lam)
+        _, sm = e.hartree(rho_star - t * rho_lam)
+        de = de + e.omega * ((sp[1:] - sm[1:]) / (4.0 * t)).cpu().numpy().reshape(3, 3)
+        return de * (e.nfft / e.omega) - float(torch.dot(lam, rho_star)) * np.eye(3)
+
+
 def delta_q_value(engine, rho_pl, weight_pl):
     """C = (Omega/N) sum_j w_j n_j (constraint.py:26-34)."""
     return engine.omega / engine.nfft * float(torch.dot(weight_pl, rho_pl))
 
 
 def delta_q_grad(engine, psis, eig, occ, rho_pl, weight_pl, rtol=1e-10, maxiter=200, **resp_kw):
-    """dC/dR for C = (Omega/N) sum w n at the SCF fixed point n* (implicit.py
-    :32-92): (I - J)^T lam = (Omega/N) w, then dC/dR = lam . dF/dR|_n*
-    (the state term vanishes for positions: w and n* are grid arrays)."""
+    """dC/dR and dC/deps for C = (Omega/N) sum w n at the SCF fixed point n*
+    (implicit.py:32-92): (I - J)^T lam = (Omega/N) w, then
+    dC/dtheta = dC/dtheta|_n* + lam . dF/dtheta|_n*; the state term is 0 for
+    positions and C I for the strain (w and n* are grid arrays, Omega
+    scales)."""
     resp = Response(engine, psis, eig, occ, rho_pl, **resp_kw)
     g = (engine.omega / engine.nfft) * weight_pl
 
@@ -256,6 +303,9 @@ def delta_q_grad(engine, psis, eig, occ, rho_pl, weight_pl, rtol=1e-10, maxiter=
     if res > 1e-6:
         raise RuntimeError(f"adjoint BiCGSTAB did not converge: residual {res:.3e}")
     dR = resp.map_term_positions(lam)
+    rho_lam = resp._last_rho
+    C = delta_q_value(engine, rho_pl, weight_pl)
+    de = resp.map_term_strain(lam, rho_lam, rho_pl) + C * np.eye(3)
     diag = {"adjoint_residual": res, "bicgstab_iterations": nit, "lam_norm": float(torch.linalg.vector_norm(lam)),
             "g_norm": float(torch.linalg.vector_norm(g)), **resp.stats}
-    return dR, diag
+    return dR, de, diag

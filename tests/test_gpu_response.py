@@ -3,9 +3,9 @@
 * J dn (in-set pairs + Sternheimer complement + occupation change) against
   the oracle's dense-eigensystem response (reference physics/response.py
   :33-114 with the occupation sign fixed) -- bar 1e-8 relative;
-* dDeltaQ/dR against the oracle's implicit adjoint on the reference's own
-  DeltaQ system (tests/test_forces.py:187-217) and on its partially occupied
-  variant -- bar 1e-7;
+* dDeltaQ/dR and dDeltaQ/deps against the oracle's implicit adjoint on the
+  reference's own DeltaQ system (tests/test_forces.py:187-217) and on its
+  partially occupied variant -- bar 1e-7;
 * the 8-atom cell: dDeltaQ/dR against central differences of DeltaQ with
   the SCF re-converged on the GPU (Richardson, steps 1e-4 / 5e-5) -- bar 1e-5
   relative.
@@ -61,7 +61,7 @@ def test_response_matches_oracle(nband):
 @pytest.mark.parametrize("nband", [2, 6])
 def test_delta_q_grad_matches_oracle(nband):
     geo, ost, ostate, w = _oracle(nband)
-    dR_ref, _, info = ORS.delta_q_grad(geo, ost, ostate, w)
+    dR_ref, de_ref, info = ORS.delta_q_grad(geo, ost, ostate, w)
     addft, s, st = _device(nband)
     wd = addft.make_gaussian_weight(s, OB.positions(LAT, FRAC)[1], 0.9)
     assert np.abs(wd - w).max() < 1e-12
@@ -71,6 +71,9 @@ def test_delta_q_grad_matches_oracle(nband):
     assert g.diagnostics["adjoint_residual"] < 1e-8
     assert err < 1e-7 * max(1.0, np.abs(dR_ref).max())
     assert abs(g.value - ORS.delta_q(ost.basis, LAT, ostate.rho_r, w)) < 1e-10
+    erre = np.abs(g.eps - de_ref).max()
+    print(f"nband {nband}: dC/deps err {erre:.2e} (|dC/deps| {np.abs(de_ref).max():.3e})", flush=True)
+    assert erre < 1e-7 * max(1.0, np.abs(de_ref).max())
 
 
 def test_delta_q_grad_si8_matches_rescf_differences():
