@@ -234,15 +234,17 @@ class Engine:
         return plan.stage(v_pl, 1.0 / self.nfft)
 
     # ---------------------------------------------------------- operators
-    def hpsi(self, ik, psi, vs, fp32=False):
-        """T + V_loc,eff + V_NL on (nb, nG) stick-order psi."""
+    def hpsi(self, ik, psi, vs, fp32=False, nl_bf16=True):
+        """T + V_loc,eff + V_NL on (nb, nG) stick-order psi.  In FP32 the
+        projector products are BF16x3 split GEMMs unless nl_bf16 is False
+        (complex64 GEMMs)."""
         plan = self.plan32 if fp32 else self.plan64
         tk = self.tkin32[ik] if fp32 else self.tkin64[ik]
         P = self.P32[ik] if fp32 else self.P64[ik]
         D = self.D32 if fp32 else self.D64
         self.hpsi_bands["fp32" if fp32 else "fp64"] += int(psi.shape[0])
         out = plan.hpsi_local(psi, tk, vs)
-        if fp32 and self.PB[ik] is not None:
+        if fp32 and nl_bf16 and self.PB[ik] is not None:
             pb = self.PB[ik]
             out.add_(pb.apply(torch.matmul(pb.becp_t(psi), D)))
             return out

@@ -133,7 +133,8 @@ def dense_lowest(apply, nG, nb, dtype, device):
     return w[:nb].clone(), X, res, 1
 
 
-def davidson(apply, X0, nb, diag, max_sub=None, max_iter=40, tol=1e-9, min_iter=1, nconv=None, weight=None):
+def davidson(apply, X0, nb, diag, max_sub=None, max_iter=40, tol=1e-9, min_iter=1, nconv=None, weight=None,
+             apply_lo=None):
     """Return (eig (nb,) float64, X (nb, nG), max residual norm, iterations).
 
     apply: (m, nG) -> (m, nG) in X0's dtype.  diag: (nG,) float64.
@@ -141,7 +142,13 @@ def davidson(apply, X0, nb, diag, max_sub=None, max_iter=40, tol=1e-9, min_iter=
     remaining nb - nconv are buffer bands that only accelerate convergence.
     weight: optional (nb,) factors in (0, 1] on the residual norms used for
     the convergence test (occupation weights: a band's error enters the
-    density in proportion to its occupation)."""
+    density in proportion to its occupation).
+    apply_lo: optional lower-precision H for the correction vectors only.
+    H X0 is applied with `apply` (exact); the coupling block <Q|H|R> is
+    formed from the exact H Q as (H Q)^H R, so the FP32 error of H R enters
+    only the small <R|H|R> block and the R-components of the new H X (both
+    proportional to the current residual): the residuals, and so the
+    converged eigenpairs, keep the accuracy of `apply`."""
     nconv = nb if nconv is None else int(nconv)
     nG = X0.shape[1]
     if nG <= max(DENSE_MAX_NG, 4 * nb):
@@ -187,9 +194,13 @@ def davidson(apply, X0, nb, diag, max_sub=None, max_iter=40, tol=1e-9, min_iter=
         if R.shape[0] == 0:
             break
         _finite("R", R)
-        HR = apply(R)
+        if apply_lo is None:
+            HR = apply(R)
+            B = inner(Q, HR, conj_a=True).to(torch.complex128)
+        else:
+            HR = apply_lo(R)
+            B = inner(HQ, R, conj_a=True).to(torch.complex128)
         _finite("HR", HR)
-        B = inner(Q, HR, conj_a=True).to(torch.complex128)
         C = inner(R, HR, conj_a=True).to(torch.complex128)
         C = 0.5 * (C + C.conj().T)
         T = torch.cat([torch.cat([T, B], dim=1), torch.cat([B.conj().T, C], dim=1)], dim=0)

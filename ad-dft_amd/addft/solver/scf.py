@@ -13,9 +13,13 @@ its dense eigh gives the same fixed point).  Mixers:
   * Pulay/DIIS on request (`mixer="diis"`, solver/mixer.py:23-82).
 The converged fixed point does not depend on the mixer.
 
-Precision: "fp64" runs every Hpsi in FP64.  "mixed" runs Hpsi, the
-nonlocal GEMMs and the subspace products in FP32 until the density residual
-falls below `switch_tol`, then finishes (and polishes) in FP64.
+Precision: "fp64" runs every Hpsi in FP64.  "mixed" runs Hpsi (BF16x3
+projector products), the density and the subspace products in FP32 until the
+density residual falls below `switch_tol` (or stalls), then finishes (and
+polishes) in FP64 -- with `refine` (default) the FP64 maps apply the FP64 H
+once to the current block and converge it by Davidson steps whose correction
+vectors are multiplied by H in FP32 (their coupling to the block taken from
+the exact H X), so the residuals keep FP64 accuracy.
 """
 
 from __future__ import annotations
@@ -152,7 +156,8 @@ def initial_guess(engine, nb, dtype):
 
 def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, beta=0.5,
               switch_tol=1e-5, verbose=False, rho0=None, polish=4, dav_max_iter=None, log=None,
-              nbuffer=None, eigensolver="auto", cheb_degree=8):
+              nbuffer=None, eigensolver="auto", cheb_degree=8, refine=True, reset_on_switch=True,
+              refine_max_sub=None, cheb_passes=4):
     """Run the SCF on engine's current geometry.  Returns (SCFState with
     stick-order psi list in `psi` and plane-major rho, info dict)."""
     st = engine.st
@@ -185,9 +190,22 @@ def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, 
     ritz = [None] * engine.nk
     bounds = {}
 
-    def solve_k(ik, Xk, vs, fp32, vmean, tol_k, cold):
+    def solve_k(ik, Xk, vs, fp32, vmean, tol_k, cold, vs_lo=None):
         nonlocal dav_iters
         apply = lambda p: engine.hpsi(ik, p, vs, fp32=fp32)  # noqa: E731
+        if vs_lo is not None and not cold:
+            # FP64 phase of the mixed form: exact H X once per map, FP32
+            # corrections (complex64 projector GEMMs) in a Davidson whose
+            # Rayleigh-Ritz couples them through the exact H X
+            def apply_lo(p):
+                return engine.hpsi(ik, p.to(torch.complex64), vs_lo, fp32=True,
+                                   nl_bf16=False).to(torch.complex128)
+            diag = engine.tkin64[ik] + vmean
+            lam, Xk, r, nit = davidson(apply, Xk, nbt, diag, max_iter=(dav_max_iter or 25), tol=tol_k,
+                                       nconv=nb, weight=weights(ik), apply_lo=apply_lo,
+                                       max_sub=(refine_max_sub or 2 * nbt))
+            dav_iters += nit
+            return lam, Xk, r, nit
         if use_cheb and not cold and ritz[ik] is not None:
             key = (ik, fp32)
             if key not in bounds:
@@ -195,7 +213,7 @@ def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, 
             lam = ritz[ik]
             wk = weights(ik)
             nit, r = 0, float("inf")
-            for nit in range(1, (41 if tol_k <= 1e-9 else 5)):
+            for nit in range(1, (41 if tol_k <= 1e-9 else cheb_passes + 1)):
                 lam, Xk, _, rn = chfsi_step(apply, Xk, cheb_degree, bounds[key], lam)
                 r = float((rn[:nb] * wk[:nb]).max())
                 if r < tol_k:
@@ -231,10 +249,11 @@ def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, 
     def one_map(rho, X, fp32, dav_tol, cold):
         v, _ = engine.effective_potential(rho)
         vs = engine.stage(v, mixed_stage=fp32)
+        vs_lo = engine.stage(v, mixed_stage=True) if (engine.mixed and not fp32 and refine) else None
         vmean = float(v.mean())
         eigs, Xn = [], []
         for ik in range(engine.nk):
-            lam, Xk, r, nit = solve_k(ik, X[ik], vs, fp32, vmean, dav_tol, cold)
+            lam, Xk, r, nit = solve_k(ik, X[ik], vs, fp32, vmean, dav_tol, cold, vs_lo)
             ritz[ik] = lam
             last["dav_res"], last["dav_it"] = r, nit
             if nbt > nb:
@@ -270,6 +289,12 @@ def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, 
             X = [x.to(torch.complex128) for x in X]
             bounds.clear()
             info_switch = it
+            if reset_on_switch:
+                # the history holds FP32-noisy residuals; start it afresh
+                for m in (diis, broy, pulay):
+                    if m is not None:
+                        m.reset()
+                prev = 1e9
         if mixer == "diis":
             if res > prev * 1.5:
                 diis.reset()
