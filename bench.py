@@ -171,6 +171,9 @@ def main():
     ap.add_argument("--hpsi-steps", type=int, default=20)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--hpsi-only", action="store_true",
+                    help="only the local-Hpsi roofline measurement (and a 1 GiB device copy that calibrates "
+                         "the FETCH_SIZE/WRITE_SIZE counters); for profiling runs")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -188,6 +191,28 @@ def main():
     kw = testsystems.config(args.workload, precision="mixed")
     lat = np.asarray(kw["lattice"])
     nat = len(kw["frac"])
+
+    if args.hpsi_only:
+        sys_m = addft.setup(**kw)
+        basis = sys_m.static.basis
+        nb = sys_m.static.nband
+        hp = hpsi_measure(basis, lat, nb, args.hpsi_steps, 3, torch, native)
+        a = torch.ones(2 ** 28, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)                       # counter calibration: 1 GiB read + 1 GiB write
+        torch.cuda.synchronize()
+        f32 = hp["fp32"]
+        print(json.dumps({"metric": "local Hpsi HBM GB/s (algorithmic bytes / HIP-event time)",
+                          "value": round(f32["hpsi_GBps"], 2), "unit": "GB/s", "n_gpus": 1,
+                          "steps": args.hpsi_steps, "higher_is_better": True, "dtype": "c64",
+                          "config": {"workload": f"local Hpsi, {nat}-atom Si cell, {nb} bands, box "
+                                                 f"{tuple(basis.fft_shape)}"},
+                          "roofline": {"bound": "hbm", "achieved": round(f32["hpsi_GBps"], 2),
+                                       "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                       "frac": round(f32["hpsi_GBps"] / HBM_PEAK_GBS, 4),
+                                       "traffic": measured_traffic()},
+                          "hpsi": hp}), flush=True)
+        return
 
     say("library warm-up (8-atom cell, untimed)")
     addft.energy_forces_stress(addft.setup(**testsystems.config("c2", precision="mixed")), tol=1e-8)
