@@ -194,6 +194,14 @@ class Engine:
                 self.PB.append(ProjectorBF16x3(P, nslice=nsl))
             else:
                 self.PB.append(None)
+        # k-stacked tables for the batched Hpsi (several k-points on one sphere)
+        if self.nk > 1:
+            self.tk64_all = torch.stack(self.tkin64).contiguous()
+            self.tk32_all = torch.stack(self.tkin32).contiguous() if self.mixed else None
+            self.P64_all = torch.stack(self.P64).contiguous()
+            self.PH64_all = self.P64_all.conj().transpose(1, 2).contiguous()
+            self.P32_all = self.P64_all.to(torch.complex64) if self.mixed else None
+            self.PH32_all = self.PH64_all.to(torch.complex64) if self.mixed else None
         # local potential on the density sphere
         ngr = self.st.basis.nG_rho
         self.grho_d = torch.empty((ngr, 3), dtype=torch.float64, device=self.dev)
@@ -251,6 +259,32 @@ class Engine:
         becp_t = inner(psi, P)                          # (nb, nproj) = <beta|psi>^T
         out.add_(torch.matmul(torch.matmul(becp_t, D), P))
         return out
+
+    def hpsi_batch(self, psi3, vs, fp32=False):
+        """T_k + V + V_NL,k on psi3 (nk, m, nG) for all k-points at once: one
+        local launch over nk * m bands (per-k kinetic rows) and batched
+        projector GEMMs (reference hpsi.py:35-50, looped over k there)."""
+        nk, m, ng = psi3.shape
+        plan = self.plan32 if fp32 else self.plan64
+        tk = self.tk32_all if fp32 else self.tk64_all
+        P = self.P32_all if fp32 else self.P64_all
+        PH = self.PH32_all if fp32 else self.PH64_all
+        D = self.D32 if fp32 else self.D64
+        self.hpsi_bands["fp32" if fp32 else "fp64"] += int(nk * m)
+        out = plan.hpsi_local_kbatch(psi3.reshape(nk * m, ng), tk, m, vs).view(nk, m, ng)
+        becp_t = torch.bmm(psi3, PH)                    # (nk, m, nproj)
+        out.add_(torch.bmm(torch.matmul(becp_t, D), P))
+        return out
+
+    def density_batch(self, psi3, occ, fp32=False):
+        """rho for psi3 (nk, nb, nG) and occ (nk, nb) in one launch."""
+        plan = self.plan32 if fp32 else self.plan64
+        nk, nb, ng = psi3.shape
+        kw = torch.as_tensor(np.asarray(self.st.kweights, dtype=np.float64), device=self.dev)
+        w = ((kw[:, None] * self.st.spin_degeneracy / self.omega) * occ.to(torch.float64)).reshape(-1).contiguous()
+        psi = psi3.reshape(nk * nb, ng)
+        psi = psi.to(plan.cdtype) if psi.dtype != plan.cdtype else psi
+        return plan.density(psi.contiguous(), w)
 
     def becp(self, ik, psi):
         """(nproj, nb) <beta|psi> in FP64 (nonlocal_op.py:67-69)."""

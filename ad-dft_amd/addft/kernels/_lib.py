@@ -57,6 +57,7 @@ _SIGS = {
     "addft_grid_transpose": ([_I32, _I32, _I32, _I32, _VP, _VP, _VP], _I32),
     "addft_grid_stage": ([_VP, _VP, _D, _VP, _VP], _I32),
     "addft_hpsi_local": ([_VP, _VP, _VP, _VP, _I64, _VP, _VP, _VP], _I32),
+    "addft_hpsi_local_kbatch": ([_VP, _VP, _VP, _I64, _VP, _I64, _VP, _VP, _VP], _I32),
     "addft_density_accumulate": ([_VP, _VP, _VP, _I64, _VP, _I32, _VP, _VP], _I32),
     "addft_sphere_to_real": ([_VP, _VP, _D, _VP, _VP, _VP], _I32),
     "addft_real_to_sphere": ([_VP, _VP, _D, _VP, _VP, _VP], _I32),

@@ -202,6 +202,22 @@ class SpherePlan:
                                      ptr(self.work(nb)), stream()), "addft_hpsi_local")
         return out
 
+    def hpsi_local_kbatch(self, psi: torch.Tensor, tkin: torch.Tensor, bands_per_k: int,
+                          v_staged: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
+        """T_k psi + V psi for (nk * bands_per_k, nG) psi, tkin (nk, nG)."""
+        _require(psi, self.cdtype)
+        _require(tkin, self.rdtype)
+        _require(v_staged, self.rdtype)
+        nb = psi.shape[0]
+        if (psi.shape[-1] != self.ng or tkin.shape[-1] != self.ng or nb % bands_per_k
+                or tkin.numel() != (nb // bands_per_k) * self.ng or v_staged.numel() != self.nfft):
+            raise ValueError("shape mismatch in hpsi_local_kbatch")
+        if out is None:
+            out = torch.empty_like(psi)
+        check(lib().addft_hpsi_local_kbatch(self._h, ptr(psi), ptr(tkin), int(bands_per_k), ptr(v_staged), nb,
+                                            ptr(out), ptr(self.work(nb)), stream()), "addft_hpsi_local_kbatch")
+        return out
+
     def density(self, psi: torch.Tensor, w: torch.Tensor, out: torch.Tensor | None = None,
                 accumulate: bool = False) -> torch.Tensor:
         """sum_b w_b |IFFT(psi_b)|^2, plane-major float64."""

@@ -32,6 +32,7 @@ import numpy as np
 import torch
 
 from ..system import SCFState
+from . import batched as kbatch
 from .chebyshev import chfsi_step, upper_bound
 from .davidson import DENSE_MAX_NG, davidson
 
@@ -157,7 +158,7 @@ def initial_guess(engine, nb, dtype):
 def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, beta=0.5,
               switch_tol=1e-5, verbose=False, rho0=None, polish=4, dav_max_iter=None, log=None,
               nbuffer=None, eigensolver="auto", cheb_degree=8, refine=True, reset_on_switch=True,
-              refine_max_sub=None, cheb_passes=4):
+              refine_max_sub=None, cheb_passes=4, batch_k=True):
     """Run the SCF on engine's current geometry.  Returns (SCFState with
     stick-order psi list in `psi` and plane-major rho, info dict)."""
     st = engine.st
@@ -189,6 +190,39 @@ def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, 
                                                and engine.ng > max(DENSE_MAX_NG, 4 * nbt))
     ritz = [None] * engine.nk
     bounds = {}
+    # all k-points in one batch (same sphere, too large for the dense path)
+    batched = batch_k and engine.nk > 1 and engine.ng > max(DENSE_MAX_NG, 4 * nbt)
+    ritz_b = [None]
+
+    def solve_batched(Xb, vs, fp32, vmean, tol_k, cold, vs_lo=None):
+        nonlocal dav_iters
+        apply = lambda p: engine.hpsi_batch(p, vs, fp32=fp32)  # noqa: E731
+        W = torch.stack([weights(ik) for ik in range(engine.nk)])
+        diag = engine.tk64_all + vmean
+        if vs_lo is not None and not cold:
+            def apply_lo(p):
+                return engine.hpsi_batch(p.to(torch.complex64), vs_lo, fp32=True).to(torch.complex128)
+            lam, Xb, r, nit = kbatch.davidson(apply, Xb, nbt, diag, max_iter=(dav_max_iter or 25), tol=tol_k,
+                                              nconv=nb, weight=W, apply_lo=apply_lo,
+                                              max_sub=(refine_max_sub or 2 * nbt))
+        elif use_cheb and not cold and ritz_b[0] is not None:
+            key = ("batch", fp32)
+            if key not in bounds:
+                bounds[key] = 1.1 * kbatch.upper_bound(apply, engine.nk, engine.ng, Xb.dtype, engine.dev)
+            lam = ritz_b[0]
+            nit, r = 0, float("inf")
+            for nit in range(1, (41 if tol_k <= 1e-9 else cheb_passes + 1)):
+                lam, Xb, _, rn = kbatch.chfsi_step(apply, Xb, cheb_degree, bounds[key], lam)
+                r = float((rn[:, :nb] * W[:, :nb]).max())
+                if r < tol_k:
+                    break
+        else:
+            lam, Xb, r, nit = kbatch.davidson(apply, Xb, nbt, diag, max_iter=(dav_max_iter or (40 if cold else 25)),
+                                              tol=(max(tol_k, 1e-3) if (cold and use_cheb) else tol_k), nconv=nb,
+                                              weight=W)
+        dav_iters += nit
+        ritz_b[0] = lam
+        return lam, Xb, r, nit
 
     def solve_k(ik, Xk, vs, fp32, vmean, tol_k, cold, vs_lo=None):
         nonlocal dav_iters
@@ -251,6 +285,18 @@ def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, 
         vs = engine.stage(v, mixed_stage=fp32)
         vs_lo = engine.stage(v, mixed_stage=True) if (engine.mixed and not fp32 and refine) else None
         vmean = float(v.mean())
+        if batched:
+            lam, Xb, r, nit = solve_batched(torch.stack(list(X)), vs, fp32, vmean, dav_tol, cold, vs_lo)
+            last["dav_res"], last["dav_it"] = r, nit
+            if nbt > nb:
+                last["gap"] = float((lam[:, nb] - lam[:, nb - 1]).min())
+            eig = lam[:, :nb].to(torch.float64).contiguous()
+            occ, mu = engine.occupations(eig)
+            for ik in range(engine.nk):
+                occ_prev[ik] = occ[ik].to(torch.float64)
+                ritz[ik] = lam[ik]
+            rho_out = engine.density_batch(Xb[:, :nb], occ, fp32=fp32)
+            return rho_out, list(Xb.unbind(0)), eig, occ, mu
         eigs, Xn = [], []
         for ik in range(engine.nk):
             lam, Xk, r, nit = solve_k(ik, X[ik], vs, fp32, vmean, dav_tol, cold, vs_lo)

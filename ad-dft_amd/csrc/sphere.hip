@@ -110,7 +110,8 @@ __global__ void __launch_bounds__(kZThreads)
 k_z_forward(const cpx<R>* __restrict__ T, int64_t ng, const int32_t* __restrict__ soff,
             const int32_t* __restrict__ sid, const int32_t* __restrict__ iz, int64_t nst,
             int spb, FftFactors f3, const cpx<R>* __restrict__ gtw3, R scale,
-            const R* __restrict__ tkin, const cpx<R>* __restrict__ psi, cpx<ROUT>* __restrict__ out) {
+            const R* __restrict__ tkin, int64_t tk_bpk, const cpx<R>* __restrict__ psi,
+            cpx<ROUT>* __restrict__ out) {
   extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   cpx<R>* L = reinterpret_cast<cpx<R>*>(smem_raw);
   const int n3 = f3.n;
@@ -132,7 +133,7 @@ k_z_forward(const cpx<R>* __restrict__ T, int64_t ng, const int32_t* __restrict_
     cpx<R> v = L[(sid[p] - s0) * n3 + iz[p]] * scale;
     if (tkin != nullptr) {
       cpx<R> x = psi[b * ng + p];
-      R t = tkin[p];
+      R t = tkin[(tk_bpk > 0 ? (b / tk_bpk) * ng : 0) + p];
       v.x += t * x.x;
       v.y += t * x.y;
     }
@@ -397,7 +398,8 @@ template <typename R, typename ROUT, int N, int L, int WAVES>
 __global__ void __launch_bounds__(64 * WAVES)
 k_z_forward_lane(const cpx<R>* __restrict__ T, int64_t ng, const int32_t* __restrict__ soff,
                  const int32_t* __restrict__ a3, int64_t nst, const cpx<R>* __restrict__ gtw, R scale,
-                 const R* __restrict__ tkin, const cpx<R>* __restrict__ psi, cpx<ROUT>* __restrict__ out) {
+                 const R* __restrict__ tkin, int64_t tk_bpk, const cpx<R>* __restrict__ psi,
+                 cpx<ROUT>* __restrict__ out) {
   using S = addft::lane::Shape<N, L>;
   constexpr int M = S::M, Q = S::Q, G = S::G, SPB = WAVES * G, SST = SPB + 1;
   extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
@@ -438,7 +440,7 @@ k_z_forward_lane(const cpx<R>* __restrict__ T, int64_t ng, const int32_t* __rest
         cpx<R> val = x[j] * scale;
         if (tkin != nullptr) {
           const cpx<R> c = psi[b * ng + p0 + pos];
-          const R tk = tkin[p0 + pos];
+          const R tk = tkin[(tk_bpk > 0 ? (b / tk_bpk) * ng : 0) + p0 + pos];
           val.x += tk * c.x;
           val.y += tk * c.y;
         }
@@ -866,13 +868,13 @@ int launch_z_inverse(const addft_sphere_plan_s* p, const cpx<R>* psi, cpx<R>* T,
 // epilogue out = scale*F + tkin*psi.
 template <typename R, typename ROUT>
 int launch_z_forward(const addft_sphere_plan_s* p, const cpx<R>* T, R scale, const R* tkin, const cpx<R>* psi,
-                     cpx<ROUT>* out, int64_t nb, hipStream_t st) {
+                     cpx<ROUT>* out, int64_t nb, hipStream_t st, int64_t tk_bpk = 0) {
   auto tw3 = reinterpret_cast<const cpx<R>*>(p->d_tw3);
   if (!p->lane_z) {
     const size_t zlds = (size_t)(p->spb + 1) * p->n3 * sizeof(cpx<R>);
     dim3 zg((unsigned)((p->nst + p->spb - 1) / p->spb), (unsigned)nb);
     hipLaunchKernelGGL((k_z_forward<R, ROUT>), zg, dim3(kZThreads), zlds, st, T, p->ng, p->d_soff, p->d_sid,
-                       p->d_iz, p->nst, p->spb, p->f3, tw3, scale, tkin, psi, out);
+                       p->d_iz, p->nst, p->spb, p->f3, tw3, scale, tkin, tk_bpk, psi, out);
     ADDFT_LAUNCH_CHECK();
     return ADDFT_OK;
   }
@@ -888,7 +890,7 @@ int launch_z_forward(const addft_sphere_plan_s* p, const cpx<R>* T, R scale, con
     }                                                                                                         \
     hipLaunchKernelGGL((k_z_forward_lane<R, ROUT, NN, lane_width(NN), kZLaneWaves>), grid,                    \
                        dim3(64 * kZLaneWaves), lds, st, T, p->ng, p->d_soff, p->d_a3, p->nst, tw3, scale, tkin, \
-                       psi, out);                                                                             \
+                       tk_bpk, psi, out);                                                                     \
     break;
   switch (p->n3) {
     ADDFT_LANE_SIZES(ADDFT_ZF_CASE)
@@ -960,7 +962,7 @@ int64_t density_groups(int64_t nb) { return (nb + kBandsPerGroup - 1) / kBandsPe
 
 template <typename R>
 int launch_hpsi(addft_sphere_plan_s* p, const void* psi, const void* tkin, const void* v, int64_t nb,
-                void* out, void* work, hipStream_t st) {
+                void* out, void* work, hipStream_t st, int64_t tk_bpk = 0) {
   cpx<R>* T = reinterpret_cast<cpx<R>*>(work);
   int rc = launch_z_inverse<R>(p, reinterpret_cast<const cpx<R>*>(psi), T, nb, st);
   if (rc != ADDFT_OK) return rc;
@@ -985,7 +987,7 @@ int launch_hpsi(addft_sphere_plan_s* p, const void* psi, const void* tkin, const
     ADDFT_LAUNCH_CHECK();
   }
   return launch_z_forward<R, R>(p, T, (R)1, reinterpret_cast<const R*>(tkin), reinterpret_cast<const cpx<R>*>(psi),
-                                reinterpret_cast<cpx<R>*>(out), nb, st);
+                                reinterpret_cast<cpx<R>*>(out), nb, st, tk_bpk);
 }
 
 template <typename R>
@@ -1301,6 +1303,18 @@ int addft_hpsi_local(addft_sphere_plan p, const void* psi, const void* tkin, con
   hipStream_t st = addft::as_stream(stream);
   if (p->precision == ADDFT_C64) return launch_hpsi<float>(p, psi, tkin, v, nb, out, work, st);
   return launch_hpsi<double>(p, psi, tkin, v, nb, out, work, st);
+}
+
+int addft_hpsi_local_kbatch(addft_sphere_plan p, const void* psi, const void* tkin, int64_t bands_per_k,
+                            const void* v, int64_t nb, void* out, void* work, void* stream) {
+  ADDFT_CHECK_ARG(p && psi && tkin && v && out && work, "null argument");
+  ADDFT_CHECK_ARG(out != psi, "out may not alias psi");
+  ADDFT_CHECK_ARG(nb >= 0 && nb < 65536, "nb out of range");
+  ADDFT_CHECK_ARG(bands_per_k > 0 && nb % bands_per_k == 0, "nb must be a multiple of bands_per_k");
+  if (nb == 0) return ADDFT_OK;
+  hipStream_t st = addft::as_stream(stream);
+  if (p->precision == ADDFT_C64) return launch_hpsi<float>(p, psi, tkin, v, nb, out, work, st, bands_per_k);
+  return launch_hpsi<double>(p, psi, tkin, v, nb, out, work, st, bands_per_k);
 }
 
 int addft_density_accumulate(addft_sphere_plan p, const void* psi, const double* w, int64_t nb, double* rho,

@@ -134,6 +134,13 @@ int addft_grid_stage(addft_sphere_plan plan, const double* d_in, double scale,
 int addft_hpsi_local(addft_sphere_plan plan, const void* d_psi, const void* d_tkin,
                      const void* d_vstaged, int64_t nb, void* d_out, void* d_work,
                      void* stream);
+/* The same for nb = nk * bands_per_k bands of several k-points sharing the
+ * sphere (the reference's union shell, basis.py:89-121): band b uses the
+ * kinetic row b / bands_per_k of h_tkin (nk x ng, stick order).  One launch
+ * per pass for all k-points (batched form of hpsi.py:35-50 over k).      */
+int addft_hpsi_local_kbatch(addft_sphere_plan plan, const void* d_psi, const void* d_tkin,
+                            int64_t bands_per_k, const void* d_v, int64_t nb,
+                            void* d_out, void* d_work, void* stream);
 
 /* Density accumulation rho(r) += sum_b w_b |IFFT(psi_b)(r)|^2 into a
  * plane-major double grid (reference physics/density.py:18-23 with
