@@ -269,14 +269,24 @@ struct Shape {
 // The steps are separate functions so that a host emulation can run each
 // one for all lanes of a group before the next (the wave_sync points).
 
+// The lane's twiddles w_N^{t k2}, k2 = 0..M-1, from the length-N table
+// (forward sign); kept in registers for every transform of the kernel.
+template <typename R, int N, int L>
+__host__ __device__ __forceinline__ void lane_twiddles(int t, const cpx<R>* tw, cpx<R>* twr) {
+  constexpr int M = Shape<N, L>::M;
+#pragma unroll
+  for (int k2 = 0; k2 < M; ++k2) twr[k2] = tw[(t * k2) % N];
+}
+
 // natural -> (in-register DFT_M, twiddle): v[k2] = w_N^{s t k2} sum_j x[t+Lj] w_M^{s j k2}
+// twr: the lane's twiddles (lane_twiddles)
 template <typename R, int N, int L, bool INV>
-__host__ __device__ __forceinline__ void fwd_a(cpx<R>* v, int t, const cpx<R>* tw) {
+__host__ __device__ __forceinline__ void fwd_a(cpx<R>* v, int t, const cpx<R>* twr) {
   constexpr int M = Shape<N, L>::M;
   rdft<M, INV>(v);
 #pragma unroll
   for (int k2 = 1; k2 < M; ++k2) {
-    const cpx<R> w = tw[(t * k2) % N];
+    const cpx<R> w = twr[k2];
     v[k2] = INV ? cmulc(v[k2], w) : cmul(v[k2], w);
   }
 }
@@ -325,13 +335,13 @@ __host__ __device__ __forceinline__ void bwd_read(cpx<R>* v, int q, int t, const
     if (k2 < M) v[k2] = scr[i * SLD + t];
   }
 }
-// (twiddle, in-register DFT_M) -> natural
+// (twiddle, in-register DFT_M) -> natural; twr: the lane's twiddles
 template <typename R, int N, int L, bool INV>
-__host__ __device__ __forceinline__ void bwd_b(cpx<R>* v, int t, const cpx<R>* tw) {
+__host__ __device__ __forceinline__ void bwd_b(cpx<R>* v, int t, const cpx<R>* twr) {
   constexpr int M = Shape<N, L>::M;
 #pragma unroll
   for (int k2 = 1; k2 < M; ++k2) {
-    const cpx<R> w = tw[(t * k2) % N];
+    const cpx<R> w = twr[k2];
     v[k2] = INV ? cmulc(v[k2], w) : cmul(v[k2], w);
   }
   rdft<M, INV>(v);

@@ -242,8 +242,9 @@ k_plane_v_lane(cpx<R>* __restrict__ T, int64_t nst, int nra, int a1, const int4*
   constexpr int M = S::M, Q = S::Q, G = S::G;
   extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   cpx<R>* tw = reinterpret_cast<cpx<R>*>(smem_raw);
-  cpx<R>* P = tw + N;
-  cpx<R>* scr_all = P + (size_t)nra * N;
+  cpx<R>* P = tw + N;                        // nra rows, then a zero row and a scratch row
+  cpx<R>* scr_all = P + (size_t)(nra + 2) * N;
+  const int zoff = nra * N, toff = (nra + 1) * N;
   const int wave = threadIdx.x >> 6, lid = threadIdx.x & 63;
   const int g = lid / L, t = lid - g * L;
   const bool act = g < G;
@@ -251,11 +252,25 @@ k_plane_v_lane(cpx<R>* __restrict__ T, int64_t nst, int nra, int a1, const int4*
   cpx<R>* scr = scr_all + (size_t)(wave * (G + 1) + g) * S::SCR;
   const int64_t b = blockIdx.x;
   const int z = blockIdx.y;
-  for (int i = threadIdx.x; i < N; i += blockDim.x) tw[i] = gtw[i];
+  for (int i = threadIdx.x; i < N; i += blockDim.x) {
+    tw[i] = gtw[i];
+    P[zoff + i] = addft::mk<R>(0, 0);
+  }
   __syncthreads();
+  cpx<R> twr[M];
+  addft::lane::lane_twiddles<R, N, L>(t, tw, twr);
+  // column pass: where the lane's elements i1 = t + L j sit in P (zero row /
+  // scratch row where i1 carries no stick row)
+  int cld[M], cst[M];
+#pragma unroll
+  for (int j = 0; j < M; ++j) {
+    const int rr = addft::lane::cyc_pos(t + L * j, a1, nra, N);
+    cld[j] = rr >= 0 ? rr * N : zoff;
+    cst[j] = rr >= 0 ? rr * N : toff;
+  }
   cpx<R>* Tp = T + (b * n3 + z) * nst;
 
-  // phase 1
+  // phase 1: rows, sticks -> inverse along i2 -> P
   for (int base = wave * G; base < nra; base += WAVES * G) {
     const int r = base + g;
     const bool ok = act && r < nra;
@@ -264,66 +279,67 @@ k_plane_v_lane(cpx<R>* __restrict__ T, int64_t nst, int nra, int a1, const int4*
 #pragma unroll
     for (int j = 0; j < M; ++j) {
       const int pos = ok ? addft::lane::cyc_pos(t + L * j, ri.y, ri.z, N) : -1;
-      x[j] = pos >= 0 ? Tp[ri.x + pos] : addft::mk<R>(0, 0);
+      const cpx<R> val = Tp[ri.x + (pos < 0 ? 0 : pos)];
+      x[j] = pos >= 0 ? val : addft::mk<R>(0, 0);
     }
     cpx<R> o[Q][L];
-    addft::lane::fwd<R, N, L, true>(x, o, t, scr, tw);
-    if (ok) {
-      cpx<R>* Pr = P + (size_t)r * N;
-#pragma unroll
-      for (int q = 0; q < Q; ++q)
-        if (q * L + t < M) {
-#pragma unroll
-          for (int k1 = 0; k1 < L; ++k1) Pr[q * L + t + M * k1] = o[q][k1];
-        }
-    }
-  }
-  __syncthreads();
-
-  // phase 2
-  const R* vp = v + (int64_t)z * N * N;
-  for (int base = wave * G; base < N; base += WAVES * G) {
-    const int y = base + g;
-    const bool ok = act && y < N;
-    cpx<R> x[M];
-#pragma unroll
-    for (int j = 0; j < M; ++j) {
-      const int rr = ok ? addft::lane::cyc_pos(t + L * j, a1, nra, N) : -1;
-      x[j] = rr >= 0 ? P[(size_t)rr * N + y] : addft::mk<R>(0, 0);
-    }
-    cpx<R> o[Q][L];
-    addft::lane::fwd<R, N, L, true>(x, o, t, scr, tw);
+    addft::lane::fwd<R, N, L, true>(x, o, t, scr, twr);
     if (ok) {
 #pragma unroll
-      for (int q = 0; q < Q; ++q)
-        if (q * L + t < M) {
+      for (int q = 0; q < Q; ++q) {
+        const bool kv = q * L + t < M;
 #pragma unroll
-          for (int k1 = 0; k1 < L; ++k1) o[q][k1] = o[q][k1] * vp[(q * L + t + M * k1) * N + y];
-        }
-    }
-    addft::lane::bwd<R, N, L, false>(o, x, t, scr, tw);
-    if (ok) {
-#pragma unroll
-      for (int j = 0; j < M; ++j) {
-        const int rr = addft::lane::cyc_pos(t + L * j, a1, nra, N);
-        if (rr >= 0) P[(size_t)rr * N + y] = x[j];
+        for (int k1 = 0; k1 < L; ++k1) P[kv ? r * N + q * L + t + M * k1 : toff + k1] = o[q][k1];
       }
     }
   }
   __syncthreads();
 
-  // phase 3
-  for (int base = wave * G; base < nra; base += WAVES * G) {
-    const int r = base + g;
-    const bool ok = act && r < nra;
+  // phase 2: columns, inverse along i1, x V, forward along i1
+  const R* vp = v + (int64_t)z * N * N;
+  for (int base = wave * G; base < N; base += WAVES * G) {
+    const int y = base + g;
+    const bool ok = act && y < N;
+    const int yy = ok ? y : 0;
+    // V(i1, y) for the lane's split elements, fetched ahead of the transform
+    R vv[Q][L];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int k2 = min(q * L + t, M - 1);
+#pragma unroll
+      for (int k1 = 0; k1 < L; ++k1) vv[q][k1] = vp[(k2 + M * k1) * N + yy];
+    }
+    cpx<R> x[M];
+#pragma unroll
+    for (int j = 0; j < M; ++j) x[j] = P[cld[j] + yy];
     cpx<R> o[Q][L];
+    addft::lane::fwd<R, N, L, true>(x, o, t, scr, twr);
 #pragma unroll
     for (int q = 0; q < Q; ++q)
 #pragma unroll
-      for (int k1 = 0; k1 < L; ++k1)
-        o[q][k1] = (ok && q * L + t < M) ? P[(size_t)r * N + q * L + t + M * k1] : addft::mk<R>(0, 0);
+      for (int k1 = 0; k1 < L; ++k1) o[q][k1] = o[q][k1] * vv[q][k1];
+    addft::lane::bwd<R, N, L, false>(o, x, t, scr, twr);
+    if (ok) {
+#pragma unroll
+      for (int j = 0; j < M; ++j) P[cst[j] + y] = x[j];
+    }
+  }
+  __syncthreads();
+
+  // phase 3: rows, P -> forward along i2 -> sticks
+  for (int base = wave * G; base < nra; base += WAVES * G) {
+    const int r = base + g;
+    const bool ok = act && r < nra;
+    const int rb = ok ? r * N : zoff;
+    cpx<R> o[Q][L];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const bool kv = q * L + t < M;
+#pragma unroll
+      for (int k1 = 0; k1 < L; ++k1) o[q][k1] = P[kv ? rb + q * L + t + M * k1 : zoff + k1];
+    }
     cpx<R> x[M];
-    addft::lane::bwd<R, N, L, false>(o, x, t, scr, tw);
+    addft::lane::bwd<R, N, L, false>(o, x, t, scr, twr);
     if (ok) {
       const int4 ri = rowinfo[r];
 #pragma unroll
@@ -361,6 +377,8 @@ k_z_inverse_lane(const cpx<R>* __restrict__ psi, int64_t ng, const int32_t* __re
   const int64_t b = blockIdx.y;
   for (int i = threadIdx.x; i < N; i += blockDim.x) tw[i] = gtw[i];
   __syncthreads();
+  cpx<R> twr[M];
+  addft::lane::lane_twiddles<R, N, L>(t, tw, twr);
   const int sl = wave * G + g;
   const bool ok = act && sl < cnt;
   int p0 = 0, len = 0, a = 0;
@@ -374,10 +392,11 @@ k_z_inverse_lane(const cpx<R>* __restrict__ psi, int64_t ng, const int32_t* __re
 #pragma unroll
   for (int j = 0; j < M; ++j) {
     const int pos = ok ? addft::lane::cyc_pos(t + L * j, a, len, N) : -1;
-    x[j] = pos >= 0 ? src[pos] : addft::mk<R>(0, 0);
+    const cpx<R> val = src[pos < 0 ? 0 : pos];
+    x[j] = pos >= 0 ? val : addft::mk<R>(0, 0);
   }
   cpx<R> o[Q][L];
-  addft::lane::fwd<R, N, L, true>(x, o, t, scr, tw);
+  addft::lane::fwd<R, N, L, true>(x, o, t, scr, twr);
   if (ok) {
 #pragma unroll
     for (int q = 0; q < Q; ++q)
@@ -420,6 +439,8 @@ k_z_forward_lane(const cpx<R>* __restrict__ T, int64_t ng, const int32_t* __rest
     stage[z * SST + j] = src[(int64_t)z * nst + j];
   }
   __syncthreads();
+  cpx<R> twr[M];
+  addft::lane::lane_twiddles<R, N, L>(t, tw, twr);
   const int sl = wave * G + g;
   const bool ok = act && sl < cnt;
   cpx<R> o[Q][L];
@@ -429,7 +450,7 @@ k_z_forward_lane(const cpx<R>* __restrict__ T, int64_t ng, const int32_t* __rest
     for (int k1 = 0; k1 < L; ++k1)
       o[q][k1] = (ok && q * L + t < M) ? stage[(q * L + t + M * k1) * SST + sl] : addft::mk<R>(0, 0);
   cpx<R> x[M];
-  addft::lane::bwd<R, N, L, false>(o, x, t, scr, tw);
+  addft::lane::bwd<R, N, L, false>(o, x, t, scr, twr);
   if (ok) {
     const int p0 = soff[s0 + sl], len = soff[s0 + sl + 1] - p0, a = a3[s0 + sl];
     cpx<ROUT>* dst = out + b * ng + p0;
@@ -466,14 +487,27 @@ k_plane_density_lane(const cpx<R>* __restrict__ T, int64_t nst, int nra, int a1,
   extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   cpx<R>* tw = reinterpret_cast<cpx<R>*>(smem_raw);
   cpx<R>* P = tw + N;
-  cpx<R>* scr_all = P + (size_t)nra * N;
+  cpx<R>* scr_all = P + (size_t)(nra + 2) * N;   // nra rows, a zero row, one spare
   const int wave = threadIdx.x >> 6, lid = threadIdx.x & 63;
   const int g = lid / L, t = lid - g * L;
   const bool act = g < G;
   cpx<R>* scr = scr_all + (size_t)(wave * (G + 1) + g) * S::SCR;
   const int z = blockIdx.x, n3 = gridDim.x;
   const int64_t grp = blockIdx.y;
-  for (int i = threadIdx.x; i < N; i += blockDim.x) tw[i] = gtw[i];
+  const int zoff = nra * N;
+  for (int i = threadIdx.x; i < N; i += blockDim.x) {
+    tw[i] = gtw[i];
+    P[zoff + i] = addft::mk<R>(0, 0);
+  }
+  __syncthreads();
+  cpx<R> twr[M];
+  addft::lane::lane_twiddles<R, N, L>(t, tw, twr);
+  int cld[M];
+#pragma unroll
+  for (int j = 0; j < M; ++j) {
+    const int rr = addft::lane::cyc_pos(t + L * j, a1, nra, N);
+    cld[j] = rr >= 0 ? rr * N : zoff;
+  }
   double acc[CT][Q][L];
 #pragma unroll
   for (int c = 0; c < CT; ++c)
@@ -493,18 +527,18 @@ k_plane_density_lane(const cpx<R>* __restrict__ T, int64_t nst, int nra, int a1,
 #pragma unroll
       for (int j = 0; j < M; ++j) {
         const int pos = ok ? addft::lane::cyc_pos(t + L * j, ri.y, ri.z, N) : -1;
-        x[j] = pos >= 0 ? Tp[ri.x + pos] : addft::mk<R>(0, 0);
+        const cpx<R> val = Tp[ri.x + (pos < 0 ? 0 : pos)];
+        x[j] = pos >= 0 ? val : addft::mk<R>(0, 0);
       }
       cpx<R> o[Q][L];
-      addft::lane::fwd<R, N, L, true>(x, o, t, scr, tw);
+      addft::lane::fwd<R, N, L, true>(x, o, t, scr, twr);
       if (ok) {
-        cpx<R>* Pr = P + (size_t)r * N;
 #pragma unroll
-        for (int q = 0; q < Q; ++q)
-          if (q * L + t < M) {
+        for (int q = 0; q < Q; ++q) {
+          const bool kv = q * L + t < M;
 #pragma unroll
-            for (int k1 = 0; k1 < L; ++k1) Pr[q * L + t + M * k1] = o[q][k1];
-          }
+          for (int k1 = 0; k1 < L; ++k1) P[kv ? r * N + q * L + t + M * k1 : zoff + N + k1] = o[q][k1];
+        }
       }
     }
     __syncthreads();
@@ -512,15 +546,12 @@ k_plane_density_lane(const cpx<R>* __restrict__ T, int64_t nst, int nra, int a1,
 #pragma unroll
     for (int c = 0; c < CT; ++c) {
       const int y = c * WAVES * G + wave * G + g;
-      const bool ok = act && y < N;
+      const int yy = (act && y < N) ? y : 0;
       cpx<R> x[M];
 #pragma unroll
-      for (int j = 0; j < M; ++j) {
-        const int rr = ok ? addft::lane::cyc_pos(t + L * j, a1, nra, N) : -1;
-        x[j] = rr >= 0 ? P[(size_t)rr * N + y] : addft::mk<R>(0, 0);
-      }
+      for (int j = 0; j < M; ++j) x[j] = P[cld[j] + yy];
       cpx<R> o[Q][L];
-      addft::lane::fwd<R, N, L, true>(x, o, t, scr, tw);
+      addft::lane::fwd<R, N, L, true>(x, o, t, scr, twr);
 #pragma unroll
       for (int q = 0; q < Q; ++q)
 #pragma unroll
@@ -569,7 +600,7 @@ size_t lane_lds_bytes(int n, int nra, int waves) {
   const int L = lane_width(n);
   if (L == 0) return 0;
   const int G = 64 / L;
-  return sizeof(cpx<R>) * ((size_t)n + (size_t)nra * n + (size_t)waves * (G + 1) * L * (L + 1));
+  return sizeof(cpx<R>) * ((size_t)n + (size_t)(nra + 2) * n + (size_t)waves * (G + 1) * L * (L + 1));
 }
 
 // Density: for plane z and band group g (blockIdx.y), accumulate
