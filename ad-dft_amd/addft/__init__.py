@@ -27,4 +27,5 @@ def __getattr__(name):
 
 __all__ = ["System", "setup", "scf", "energy", "energy_explicit", "energy_and_forces", "stress",
            "energy_decomposition", "make_energy", "energy_forces_stress", "Cell", "strained", "volume",
-           "gamma", "monkhorst_pack", "Pseudo", "synthetic", "from_upf", "units"]
+           "gamma", "monkhorst_pack", "Pseudo", "synthetic", "from_upf", "units", "delta_q", "delta_q_grad",
+           "make_gaussian_weight"]

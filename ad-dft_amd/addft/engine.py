@@ -63,6 +63,10 @@ class Engine:
             self.plan64 = native.SpherePlan(self.shape, b.miller, torch.complex128, self.dev)
             self.plan32 = native.SpherePlan(self.shape, b.miller, torch.complex64, self.dev) if self.mixed else None
             self.plan_rho = native.sphere_plan_or_box(self.shape, b.miller_rho, torch.complex128, self.dev)
+        from . import tuning
+        for plan in (self.plan64, self.plan32):
+            if plan is not None:
+                tuning.tune_plan(plan, nb=max(1, min(static.nband, 64)))
         self.perm = self.plan64.perm
         self.iperm = self.plan64.iperm
         self.perm_d = torch.as_tensor(self.perm, device=self.dev)

@@ -23,18 +23,11 @@ def slices_for(K):
     return min(_MAX_SLICES, K // _SLICE)
 
 
-def inner(A, B, conj_a=False):
-    """A @ B^H for row blocks A (m, K), B (n, K) of the same dtype.
-    conj_a: return conj(A) @ B^T instead (= (A^H-style) <A|B>^T layout used
-    by the projected matrices)."""
+def inner_sliced(A, B, nc):
+    """A @ B^H with the reduction split into nc slices (nc <= 1: one GEMM)."""
     m, K = A.shape
     n = B.shape[0]
-    nc = slices_for(K)
-    if conj_a:
-        A, B = A.conj(), B.conj()
-    # complex128 products with >= 1000 output columns (the projector block)
-    # already fill the chip; splitting only adds the partial-sum traffic
-    if nc < 2 or (A.dtype == torch.complex128 and n >= 1000):
+    if nc <= 1:
         return torch.matmul(A, B.conj().T)
     kc = -(-K // nc)
     pad = kc * nc - K
@@ -44,3 +37,38 @@ def inner(A, B, conj_a=False):
     Ab = A.reshape(m, nc, kc).transpose(0, 1).contiguous()
     Bb = B.reshape(n, nc, kc).permute(1, 2, 0).conj().contiguous()
     return torch.bmm(Ab, Bb).sum(0)
+
+
+def _default_slices(dtype, n, K):
+    nc = slices_for(K)
+    # complex128 products with >= 1000 output columns (the projector block)
+    # already fill the chip; splitting only adds the partial-sum traffic
+    if nc < 2 or (dtype == torch.complex128 and n >= 1000):
+        return 1
+    return nc
+
+
+_TUNE_MIN_K = 20000
+
+
+def _choose_slices(A, B):
+    K, n = A.shape[1], B.shape[0]
+    default = _default_slices(A.dtype, n, K)
+    if K < _TUNE_MIN_K or not A.is_cuda or torch.cuda.is_current_stream_capturing():
+        return default
+    from .. import tuning
+    if not tuning.enabled():
+        return default
+    key = f"slices:{A.dtype}:{K}:{'wide' if n >= 1000 else 'narrow'}"
+    cands = {str(c): (lambda c=c: inner_sliced(A, B, c)) for c in (1, 8, 16, 32) if c == 1 or K // c >= 1024}
+    return int(tuning.decide(key, cands))
+
+
+def inner(A, B, conj_a=False):
+    """A @ B^H for row blocks A (m, K), B (n, K) of the same dtype.
+    conj_a: return conj(A) @ B^T instead (= (A^H-style) <A|B>^T layout used
+    by the projected matrices).  The number of reduction slices is tuned
+    once per (dtype, K, output width) on the device (addft.tuning)."""
+    if conj_a:
+        A, B = A.conj(), B.conj()
+    return inner_sliced(A, B, _choose_slices(A, B))

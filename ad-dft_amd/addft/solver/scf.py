@@ -352,7 +352,11 @@ def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, 
             prev = res
             rho = broy(rho, rho_out, 0.9 if it <= 8 else 0.5)
         elif mixer == "pulay":
-            if res > prev * 1.5:
+            # restart on growth only once the history can extrapolate: a
+            # plain Kerker step (history of one) can amplify the long-
+            # wavelength error of these metallic cells, and restarting at
+            # every growing step would leave only such steps
+            if res > prev * 1.5 and len(pulay.rins) >= 3:
                 pulay.reset()
             prev = res
             rho = pulay(rho, rho_out)

@@ -1,0 +1,108 @@
+"""Autotuning of the launch choices that depend on the box, the sphere and
+the band count: the plane-pass kernel of a sphere plan (lane-group register
+FFTs or workgroup Stockham, `addft_sphere_plan_set_kernel`) and the number
+of reduction slices of the G-sphere inner products (solver/linalg.py).
+
+Each decision is made once per key by timing the candidates on the device
+(HIP events on the current stream, after one warm-up call) and is kept for
+the process; `ADDFT_TUNE_CACHE=<path>` also stores the decisions as JSON so
+later processes reuse them.  `ADDFT_AUTOTUNE=0` keeps the built-in choices.
+"""
+
+from __future__ import annotations
+
+import json
+import os
+from pathlib import Path
+
+import torch
+
+_DECISIONS: dict = {}
+_LOADED = False
+
+
+def enabled() -> bool:
+    return os.environ.get("ADDFT_AUTOTUNE", "1") != "0"
+
+
+def _cache_path():
+    p = os.environ.get("ADDFT_TUNE_CACHE")
+    return Path(p) if p else None
+
+
+def _load():
+    global _LOADED
+    if _LOADED:
+        return
+    _LOADED = True
+    p = _cache_path()
+    if p is not None and p.exists():
+        try:
+            _DECISIONS.update(json.loads(p.read_text()))
+        except (OSError, ValueError):
+            pass
+
+
+def _store():
+    p = _cache_path()
+    if p is not None:
+        try:
+            p.write_text(json.dumps(_DECISIONS, indent=1, sort_keys=True))
+        except OSError:
+            pass
+
+
+def time_ms(fn, reps=3):
+    """Average milliseconds of fn() on the current stream."""
+    fn()
+    start = torch.cuda.Event(enable_timing=True)
+    end = torch.cuda.Event(enable_timing=True)
+    start.record()
+    for _ in range(reps):
+        fn()
+    end.record()
+    end.synchronize()
+    return start.elapsed_time(end) / reps
+
+
+def decide(key: str, candidates: dict):
+    """Pick the fastest of {name: callable} for `key` (cached)."""
+    _load()
+    if key in _DECISIONS and _DECISIONS[key]["choice"] in candidates:
+        return _DECISIONS[key]["choice"]
+    times = {name: time_ms(fn) for name, fn in candidates.items()}
+    choice = min(times, key=times.get)
+    _DECISIONS[key] = {"choice": choice, "ms": times}
+    _store()
+    return choice
+
+
+def decisions() -> dict:
+    _load()
+    return dict(_DECISIONS)
+
+
+# ------------------------------------------------------------ sphere plans
+def tune_plan(plan, nb: int = 32, seed: int = 0):
+    """Choose the plane-pass kernel of `plan` (kernels.native.SpherePlan) by
+    timing one local Hpsi on nb random bands with each available kernel."""
+    if not enabled() or not plan.lane_available:
+        return plan.plane_kernel
+    key = f"plane:{plan.shape}:{plan.ng}:{plan.cdtype}:{nb}"
+    g = torch.Generator(device="cpu").manual_seed(seed)
+    psi = torch.complex(torch.randn(nb, plan.ng, generator=g), torch.randn(nb, plan.ng, generator=g))
+    psi = psi.to(plan.cdtype).to(plan.device).contiguous()
+    tk = torch.zeros(plan.ng, dtype=plan.rdtype, device=plan.device)
+    vs = torch.full((plan.nfft,), 1e-3, dtype=plan.rdtype, device=plan.device)
+    out = torch.empty_like(psi)
+    current = plan.plane_kernel
+
+    def run(kind):
+        def fn():
+            plan.set_plane_kernel(kind)
+            plan.hpsi_local(psi, tk, vs, out)
+        return fn
+
+    choice = decide(key, {"lane": run("lane"), "lds": run("lds")})
+    plan.set_plane_kernel(choice if choice else current)
+    return plan.plane_kernel

@@ -184,7 +184,7 @@ def main():
     import torch
 
     import addft
-    from addft import testsystems
+    from addft import testsystems, tuning
     from addft.kernels import native
 
     torch.cuda.set_device(0)
@@ -280,6 +280,7 @@ def main():
                                f"{nb} bands", "algorithmic_bytes": f32["hpsi_bytes"],
                      "ms_per_call": round(f32["hpsi_ms"], 4)},
         "hpsi": {p: {k: (round(v, 4) if isinstance(v, float) else v) for k, v in hp[p].items()} for p in hp},
+        "tuning": {k: v["choice"] for k, v in tuning.decisions().items()},
     }
     if ref is not None:
         F64, f64, s64, t64, info64, _ = ref
