@@ -158,7 +158,7 @@ def initial_guess(engine, nb, dtype):
 def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, beta=0.5,
               switch_tol=1e-5, verbose=False, rho0=None, polish=4, dav_max_iter=None, log=None,
               nbuffer=None, eigensolver="auto", cheb_degree=8, refine=True, reset_on_switch=True,
-              refine_max_sub=None, cheb_passes=4, batch_k=True):
+              refine_max_sub=None, cheb_passes=3, batch_k=True, cold_max_sub=None):
     """Run the SCF on engine's current geometry.  Returns (SCFState with
     stick-order psi list in `psi` and plane-major rho, info dict)."""
     st = engine.st
@@ -257,7 +257,7 @@ def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, 
         diag = engine.tkin64[ik] + vmean
         lam, Xk, r, nit = davidson(apply, Xk, nbt, diag, max_iter=(dav_max_iter or (40 if cold else 25)),
                                    tol=(max(tol_k, 1e-3) if (cold and use_cheb) else tol_k), nconv=nb,
-                                   weight=weights(ik))
+                                   weight=weights(ik), max_sub=((cold_max_sub or 3 * nbt) if cold else None))
         dav_iters += nit
         return lam, Xk, r, nit
 
