@@ -4,8 +4,9 @@ With several k-points on one G-sphere (the reference's union shell,
 geometry/basis.py:89-121) every k-point's block has the same shape, so the
 blocks are stacked as (nk, nbt, nG) and each step of the solver is one
 launch for all of them: the Hpsi (Engine.hpsi_batch), the Gram and projected
-matrices (batched GEMM); the small Cholesky / eigendecompositions of the
-whole batch go to host LAPACK in one call.  The algorithms are those of solver/davidson.py (block Davidson,
+matrices (batched GEMM) and the Cholesky factorisations (batched device
+solver); the small eigendecompositions of the whole batch go to host LAPACK
+on a thread pool.  The algorithms are those of solver/davidson.py (block Davidson,
 reference solver/davidson.py:32-77) and solver/chebyshev.py (Chebyshev-
 filtered subspace iteration) without band locking -- fixed block sizes keep
 the batch rectangular -- and give the same converged eigenpairs.
@@ -13,33 +14,56 @@ the batch rectangular -- and give the same converged eigenpairs.
 
 from __future__ import annotations
 
+import os
+from concurrent.futures import ThreadPoolExecutor
+
+import numpy as np
 import torch
+from threadpoolctl import threadpool_limits
 
 
-# Batches of small Hermitian problems (nk matrices of a few hundred rows)
-# are solved with host LAPACK: the device solvers run them one matrix at a
-# time in many tiny kernels (measured: ~13 s of 19 s of kernel time at 64
-# k-points), while the host solves the whole batch in milliseconds.
+# Batches of small Hermitian eigenproblems (nk matrices of a few hundred rows)
+# are solved with host LAPACK: the device eigensolver runs them one matrix at
+# a time in many tiny kernels (measured: ~13 s of 19 s of kernel time at 64
+# k-points), while a host thread pool solves the whole batch in milliseconds.
 HOST_MAX = 256
+_POOL = None
+
+
+def _pool():
+    global _POOL
+    if _POOL is None:
+        _POOL = ThreadPoolExecutor(max_workers=max(1, min(16, len(os.sched_getaffinity(0)))))
+    return _POOL
 
 
 def eigh(T):
-    if T.shape[-1] <= HOST_MAX:
-        w, U = torch.linalg.eigh(T.cpu())
-        return w.to(T.device), U.to(T.device)
-    return torch.linalg.eigh(T)
+    """Batched Hermitian eigendecomposition.  Small matrices: one LAPACK call
+    per matrix on a host thread pool (single-threaded BLAS inside each), which
+    runs the batch in parallel; measured 3-5x faster than a sequential host
+    batch and far faster than the device solver at these sizes."""
+    if T.shape[-1] > HOST_MAX:
+        return torch.linalg.eigh(T)
+    A = T.cpu().numpy()
+    if A.shape[0] == 1 or A.shape[-1] <= 32:
+        w, U = np.linalg.eigh(A)
+    else:
+        with threadpool_limits(1):
+            parts = list(_pool().map(np.linalg.eigh, A))
+        w = np.stack([p[0] for p in parts])
+        U = np.stack([p[1] for p in parts])
+    return torch.from_numpy(w).to(T.device), torch.from_numpy(U).to(T.device)
 
 
 def chol_inverse(S):
     """L^-1 of the Cholesky factors of the Hermitian batch S, or None when a
-    factorisation fails (host LAPACK for small matrices)."""
-    dev = S.device
-    A = S.cpu() if S.shape[-1] <= HOST_MAX else S
-    L, info = torch.linalg.cholesky_ex(A)
-    if bool((info != 0).any()) or not bool(torch.isfinite(L).all()):
+    factorisation fails (device solver: batched factorisation and triangular
+    solves are fast there, unlike the batched eigensolver)."""
+    L, info = torch.linalg.cholesky_ex(S)
+    if bool((info != 0).any()):
         return None
     eye = torch.eye(L.shape[-1], dtype=L.dtype, device=L.device).expand_as(L)
-    return torch.linalg.solve_triangular(L, eye, upper=False).to(dev)
+    return torch.linalg.solve_triangular(L, eye, upper=False)
 
 
 def _gram(A, B):
@@ -180,9 +204,9 @@ def davidson(apply, X0, nbt, diag, max_iter=40, tol=1e-9, nconv=None, weight=Non
         if Q.shape[1] + nbt > max_sub:
             Q, HQ = X, HX
             T = torch.diag_embed(lam.to(torch.complex128))
-        for _ in range(2):
-            R = R - torch.bmm(_gram(R, Q), Q)
-            R = orth(R, passes=1)
+        R = R - torch.bmm(_gram(R, Q), Q)
+        R = R - torch.bmm(_gram(R, Q), Q)
+        R = orth(R, passes=2)
         if apply_lo is None:
             HR = apply(R)
             B = _proj(Q, HR).to(torch.complex128)
