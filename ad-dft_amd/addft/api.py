@@ -211,18 +211,21 @@ def delta_q(system: System, state: SCFState, weight) -> float:
 
 
 def delta_q_grad(system: System, state: SCFState | None = None, weight=None, rtol: float = 1e-10,
-                 occ_sign: str = "exact", **scf_kwargs):
+                 occ_sign: str = "exact", precondition: bool = True, **scf_kwargs):
     """dDeltaQ/d(positions, strain) by the implicit adjoint (constraint.py
     :37-46, implicit.py:32-92): returns DeltaQGradient(dR, frac, eps, value,
     diagnostics).
     occ_sign="reference" reproduces the reference's sign of the occupation
     term of the response (physics/response.py:100-102), which differs from
-    the derivative of the map where bands are partially occupied."""
+    the derivative of the map where bands are partially occupied.
+    precondition: right-precondition the adjoint BiCGSTAB with a Kerker
+    filter (same solution, fewer response products)."""
     from .response import DeltaQGradient, delta_q_grad as _dq
     s = _solved(system, state, scf_kwargs)
     eng = s.engine
     w_pl = eng.to_planar(torch.as_tensor(np.asarray(weight, dtype=np.float64)).to(eng.dev))
-    dR, de, diag = _dq(eng, s.psi, s.eig, s.occ, s.rho, w_pl, rtol=rtol, occ_sign=occ_sign)
+    dR, de, diag = _dq(eng, s.psi, s.eig, s.occ, s.rho, w_pl, rtol=rtol, occ_sign=occ_sign,
+                     precondition=precondition)
     A = np.asarray(system.geo.lattice, dtype=np.float64)
     value = eng.omega / eng.nfft * float(torch.dot(w_pl, s.rho))
     return DeltaQGradient(dR=dR, frac=dR @ A.T, eps=de, value=value, diagnostics=diag)

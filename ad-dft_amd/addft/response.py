@@ -49,10 +49,14 @@ class DeltaQGradient(NamedTuple):
     diagnostics: dict
 
 
-def bicgstab(apply, b, rtol=1e-10, maxiter=200):
+def bicgstab(apply, b, rtol=1e-10, maxiter=200, precond=None):
     """van der Vorst BiCGSTAB for apply(x) = b (real vectors), as
-    jax.scipy.sparse.linalg.bicgstab (used by implicit.py:59).  Returns
+    jax.scipy.sparse.linalg.bicgstab (used by implicit.py:59), optionally
+    right-preconditioned: apply(precond(y)) = b, x = precond(y).  Returns
     (x, iterations)."""
+    if precond is not None:
+        y, it = bicgstab(lambda v: apply(precond(v)), b, rtol=rtol, maxiter=maxiter)
+        return precond(y), it
     x = torch.zeros_like(b)
     r = b.clone()
     rhat = r.clone()
@@ -83,11 +87,67 @@ def bicgstab(apply, b, rtol=1e-10, maxiter=200):
     return x, it
 
 
+def gmres(apply, b, rtol=1e-10, maxiter=200, restart=60, precond=None):
+    """Restarted GMRES (modified Gram-Schmidt Arnoldi, Givens rotations on the
+    Hessenberg matrix) for apply(x) = b, right-preconditioned when precond is
+    given: apply(precond(y)) = b, x = precond(y).  One product per iteration,
+    where BiCGSTAB needs two.  Returns (x, products)."""
+    bnorm = float(torch.linalg.vector_norm(b)) or 1.0
+    x = torch.zeros_like(b)
+    M = precond if precond is not None else (lambda v: v)
+    total = 0
+    r = b.clone()
+    while total < maxiter:
+        beta = float(torch.linalg.vector_norm(r))
+        if beta < rtol * bnorm:
+            break
+        m = min(restart, maxiter - total)
+        V = [r / beta]
+        H = np.zeros((m + 1, m))
+        cs = np.zeros(m)
+        sn = np.zeros(m)
+        gvec = np.zeros(m + 1)
+        gvec[0] = beta
+        j = 0
+        for j in range(m):
+            w = apply(M(V[j]))
+            total += 1
+            for i in range(j + 1):
+                H[i, j] = float(torch.dot(V[i], w))
+                w = w - H[i, j] * V[i]
+            H[j + 1, j] = float(torch.linalg.vector_norm(w))
+            for i in range(j):
+                hi, hi1 = H[i, j], H[i + 1, j]
+                H[i, j] = cs[i] * hi + sn[i] * hi1
+                H[i + 1, j] = -sn[i] * hi + cs[i] * hi1
+            den = np.hypot(H[j, j], H[j + 1, j])
+            cs[j], sn[j] = (1.0, 0.0) if den == 0.0 else (H[j, j] / den, H[j + 1, j] / den)
+            hnext = H[j + 1, j]
+            H[j, j] = cs[j] * H[j, j] + sn[j] * hnext
+            H[j + 1, j] = 0.0
+            gvec[j + 1] = -sn[j] * gvec[j]
+            gvec[j] = cs[j] * gvec[j]
+            if abs(gvec[j + 1]) < rtol * bnorm or hnext == 0.0:
+                j += 1
+                break
+            V.append(w / hnext)
+        else:
+            j = m
+        y = np.linalg.solve(np.triu(H[:j, :j]), gvec[:j]) if j > 0 else np.zeros(0)
+        z = torch.zeros_like(b)
+        for i in range(j):
+            z = z + float(y[i]) * V[i]
+        x = x + M(z)
+        r = b - apply(x)
+        total += 1
+    return x, total
+
+
 class Response:
     """chi0, K and J at a converged state (stick-order psi per k, complex128)."""
 
     def __init__(self, engine, psis, eig, occ, rho_pl, deg_tol=1e-6, occ_sign="exact",
-                 cg_tol=1e-11, cg_maxiter=500, fthr=1e-12):
+                 cg_tol=1e-11, cg_maxiter=500, fthr=1e-12, weighted=True):
         self.e = engine
         st = engine.st
         self.psi = [p.to(torch.complex128).contiguous() for p in psis]
@@ -98,6 +158,7 @@ class Response:
         self.cg_tol = float(cg_tol)
         self.cg_maxiter = int(cg_maxiter)
         self.fthr = float(fthr)
+        self.weighted = bool(weighted)
         v, _ = engine.effective_potential(rho_pl)
         self.vs = engine.stage(v)
         self.vmean = float(v.mean())
@@ -118,11 +179,16 @@ class Response:
         psi = self.psi[ik]
         return V - torch.matmul(inner(V, psi), psi)
 
-    def sternheimer(self, ik, Vpsi, eps):
-        """x_n (rows) with Q (H - e_n) Q x_n = -Q Vpsi_n, x_n in range(Q)."""
+    def sternheimer(self, ik, Vpsi, eps, weight=None):
+        """x_n (rows) with Q (H - e_n) Q x_n = -Q Vpsi_n, x_n in range(Q).
+        weight (<= 1 per band, e.g. f_n / max f): band n stops at
+        weight_n |r_n| <= cg_tol |b_n| -- its error enters the density
+        response scaled by its occupation."""
         e = self.e
         b = -self._project(ik, Vpsi)
         bn = torch.linalg.vector_norm(b, dim=1)
+        if weight is not None:
+            bn = bn / torch.clamp(weight.to(bn.dtype), min=1e-300)
         diag = e.tkin64[ik] + self.vmean
         x = torch.zeros_like(b)
         r = b.clone()
@@ -177,7 +243,9 @@ class Response:
             dpsi = torch.matmul(coef.T, psi)              # dpsi_n = sum_m coef[m, n] psi_m
             occ_idx = torch.nonzero(f > self.fthr).flatten()
             if occ_idx.numel() > 0:
-                dpsi[occ_idx] += self.sternheimer(ik, Vpsi[occ_idx], eps[occ_idx])
+                fo = f[occ_idx]
+                dpsi[occ_idx] += self.sternheimer(ik, Vpsi[occ_idx], eps[occ_idx],
+                                                  weight=fo / fo.max() if self.weighted else None)
             if st.sigma > 1e-6:
                 fp = f * (1.0 - f) / st.sigma
                 dmu = float(torch.sum(fp * de)) / (float(torch.sum(fp)) + 1e-30)
@@ -286,7 +354,20 @@ def delta_q_value(engine, rho_pl, weight_pl):
     return engine.omega / engine.nfft * float(torch.dot(weight_pl, rho_pl))
 
 
-def delta_q_grad(engine, psis, eig, occ, rho_pl, weight_pl, rtol=1e-10, maxiter=200, **resp_kw):
+def dielectric_preconditioner(engine, q0sq=2.0):
+    """Approximate inverse of I - K chi0 (a metal-like dielectric,
+    1 + q0^2/G^2): the Kerker filter G^2 / (G^2 + q0^2) on G != 0, identity on
+    the mean (G = 0)."""
+    zero = torch.zeros(engine.nfft, dtype=torch.float64, device=engine.dev)
+
+    def apply(y):
+        return engine.kerker(zero, y, 1.0, q0sq) + y.mean()
+
+    return apply
+
+
+def delta_q_grad(engine, psis, eig, occ, rho_pl, weight_pl, rtol=1e-10, maxiter=200, precondition=True,
+                 method="gmres", **resp_kw):
     """dC/dR and dC/deps for C = (Omega/N) sum w n at the SCF fixed point n*
     (implicit.py:32-92): (I - J)^T lam = (Omega/N) w, then
     dC/dtheta = dC/dtheta|_n* + lam . dF/dtheta|_n*; the state term is 0 for
@@ -298,7 +379,9 @@ def delta_q_grad(engine, psis, eig, occ, rho_pl, weight_pl, rtol=1e-10, maxiter=
     def aT(y):
         return y - resp.JT(y)
 
-    lam, nit = bicgstab(aT, g, rtol=rtol, maxiter=maxiter)
+    solver = gmres if method == "gmres" else bicgstab
+    lam, nit = solver(aT, g, rtol=rtol, maxiter=maxiter,
+                      precond=dielectric_preconditioner(engine) if precondition else None)
     res = float(torch.linalg.vector_norm(aT(lam) - g) / (torch.linalg.vector_norm(g) + 1e-300))
     if res > 1e-6:
         raise RuntimeError(f"adjoint BiCGSTAB did not converge: residual {res:.3e}")
@@ -306,6 +389,6 @@ def delta_q_grad(engine, psis, eig, occ, rho_pl, weight_pl, rtol=1e-10, maxiter=
     rho_lam = resp._last_rho
     C = delta_q_value(engine, rho_pl, weight_pl)
     de = resp.map_term_strain(lam, rho_lam, rho_pl) + C * np.eye(3)
-    diag = {"adjoint_residual": res, "bicgstab_iterations": nit, "lam_norm": float(torch.linalg.vector_norm(lam)),
+    diag = {"adjoint_residual": res, "krylov_method": method, "krylov_iterations": nit, "lam_norm": float(torch.linalg.vector_norm(lam)),
             "g_norm": float(torch.linalg.vector_norm(g)), **resp.stats}
     return dR, de, diag

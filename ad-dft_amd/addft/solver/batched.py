@@ -148,9 +148,10 @@ def upper_bound(apply, nk, nG, dtype, device, steps=10, seed=0):
     return (theta + torch.linalg.vector_norm(f, dim=2)[:, 0].double().cpu()).to(device)
 
 
-def chfsi_step(apply, X, degree, b_up, lam):
+def chfsi_step(apply, X, degree, b_up, lam, HX=None):
     """One scaled Chebyshev filter (Zhou-Saad) + Rayleigh-Ritz per k.
-    b_up (nk,), lam (nk, nbt) ascending.  Returns (lam, X, HX, residual norms)."""
+    b_up (nk,), lam (nk, nbt) ascending; HX = H X when known.  Returns
+    (lam, X, HX, residual norms)."""
     rd = X.real.dtype
     a = lam[:, -1].to(rd)[:, None, None]
     a0 = lam[:, 0].to(rd)[:, None, None]
@@ -159,7 +160,7 @@ def chfsi_step(apply, X, degree, b_up, lam):
     c = 0.5 * (b + a)
     sigma = e / (a0 - c)
     tau = 2.0 / sigma
-    Y = (apply(X) - c * X) * (sigma / e)
+    Y = ((apply(X) if HX is None else HX) - c * X) * (sigma / e)
     Xm = X
     for _ in range(2, degree + 1):
         sigma_new = 1.0 / (tau - sigma)

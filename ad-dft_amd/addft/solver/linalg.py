@@ -24,19 +24,34 @@ def slices_for(K):
 
 
 def inner_sliced(A, B, nc):
-    """A @ B^H with the reduction split into nc slices (nc <= 1: one GEMM)."""
+    """A @ B^H with the reduction split into nc slices (nc <= 1: one GEMM).
+    The slices are strided views of the row-major blocks (batch stride kc,
+    leading dimension K) and B^H enters as a conjugate-transposed operand, so
+    nothing is copied; the K mod nc tail is one more small GEMM."""
     m, K = A.shape
     n = B.shape[0]
-    if nc <= 1:
+    if nc <= 1 or K // nc < 1:
         return torch.matmul(A, B.conj().T)
-    kc = -(-K // nc)
-    pad = kc * nc - K
-    if pad:
-        A = torch.nn.functional.pad(A, (0, pad))
-        B = torch.nn.functional.pad(B, (0, pad))
-    Ab = A.reshape(m, nc, kc).transpose(0, 1).contiguous()
-    Bb = B.reshape(n, nc, kc).permute(1, 2, 0).conj().contiguous()
-    return torch.bmm(Ab, Bb).sum(0)
+    if not (A.stride(1) == 1 and B.stride(1) == 1):
+        A, B = A.contiguous(), B.contiguous()
+    kc = K // nc
+    Km = kc * nc
+    Ab = A[:, :Km].reshape(m, nc, kc).transpose(0, 1)
+    Bb = B[:, :Km].reshape(n, nc, kc).transpose(0, 1)
+    out = torch.bmm(Ab, Bb.transpose(1, 2).conj()).sum(0)
+    if Km < K:
+        out += torch.matmul(A[:, Km:], B[:, Km:].conj().T)
+    return out
+
+
+def gram_tall(A, B=None):
+    """A @ B^H for a few very long rows (histories of grid-sized vectors in
+    the density mixers: n <= 10 rows of 10^5-10^7 points).  One GEMM of that
+    shape has a single output tile and streams the whole history through one
+    compute unit; ~K/2048 reduction slices spread it over the chip."""
+    B = A if B is None else B
+    K = A.shape[1]
+    return inner_sliced(A, B, max(1, min(1024, K // 2048)))
 
 
 def _default_slices(dtype, n, K):

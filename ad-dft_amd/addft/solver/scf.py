@@ -35,6 +35,7 @@ from ..system import SCFState
 from . import batched as kbatch
 from .chebyshev import chfsi_step, upper_bound
 from .davidson import DENSE_MAX_NG, davidson
+from .linalg import gram_tall
 
 
 class DIIS:
@@ -60,7 +61,7 @@ class DIIS:
         rins, routs = torch.stack(self.rins), torch.stack(self.routs)
         res = routs - rins
         Bm = torch.zeros((n + 1, n + 1), dtype=torch.float64, device=rho_in.device)
-        Bm[:n, :n] = res @ res.T
+        Bm[:n, :n] = gram_tall(res)
         Bm[:n, n] = -1.0
         Bm[n, :n] = -1.0
         Bm += torch.eye(n + 1, dtype=torch.float64, device=rho_in.device) * 1e-12
@@ -95,7 +96,7 @@ class PulayKerker:
             return self.engine.kerker(rho_in, rho_out, self.alpha, self.gg0)
         R = torch.stack(self.res)
         Bm = torch.zeros((n + 1, n + 1), dtype=torch.float64, device=rho_in.device)
-        Bm[:n, :n] = R @ R.T
+        Bm[:n, :n] = gram_tall(R)
         Bm[:n, n] = -1.0
         Bm[n, :n] = -1.0
         Bm[:n, :n] += torch.eye(n, dtype=torch.float64, device=rho_in.device) * 1e-12 * float(Bm[:n, :n].diagonal().max())
@@ -130,8 +131,8 @@ class Broyden:
             return rho + beta * F
         R, H = torch.stack(self.hf), torch.stack(self.hr)
         dF, dn = R[:-1] - R[1:], H[:-1] - H[1:]
-        a = dF @ dF.T + 1e-12 * torch.eye(n - 1, dtype=torch.float64, device=rho.device)
-        gam = torch.nan_to_num(torch.linalg.solve(a, dF @ F), nan=0.0, posinf=0.0, neginf=0.0)
+        a = gram_tall(dF) + 1e-12 * torch.eye(n - 1, dtype=torch.float64, device=rho.device)
+        gam = torch.nan_to_num(torch.linalg.solve(a, gram_tall(dF, F[None])[:, 0]), nan=0.0, posinf=0.0, neginf=0.0)
         new = rho + beta * F - gam @ (dn + beta * dF)
         return new if bool(torch.isfinite(new).all()) else rho + beta * F
 
@@ -158,9 +159,15 @@ def initial_guess(engine, nb, dtype):
 def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, beta=0.5,
               switch_tol=1e-5, verbose=False, rho0=None, polish=4, dav_max_iter=None, log=None,
               nbuffer=None, eigensolver="auto", cheb_degree=8, refine=True, reset_on_switch=True,
-              refine_max_sub=None, cheb_passes=3, batch_k=True, cold_max_sub=None):
+              refine_max_sub=None, cheb_passes=3, batch_k=True, cold_max_sub=None, dav_factor=0.1,
+              refine_dav_factor=None, cold_max_iter=8, polish_gain_max=1.0):
     """Run the SCF on engine's current geometry.  Returns (SCFState with
-    stick-order psi list in `psi` and plane-major rho, info dict)."""
+    stick-order psi list in `psi` and plane-major rho, info dict).
+    The eigensolve of each map stops at dav_factor x the density residual
+    (refine_dav_factor in the FP64 phase of the mixed form).  The first map
+    starts from plane waves with at most cold_max_iter Davidson iterations in
+    a subspace of cold_max_sub (default 2 nbt) vectors: the eigenvectors of
+    the trial density need only be rough, the later maps refine them."""
     st = engine.st
     nb = st.nband
     dev = engine.dev
@@ -211,13 +218,15 @@ def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, 
                 bounds[key] = 1.1 * kbatch.upper_bound(apply, engine.nk, engine.ng, Xb.dtype, engine.dev)
             lam = ritz_b[0]
             nit, r = 0, float("inf")
+            HXb = None
             for nit in range(1, (41 if tol_k <= 1e-9 else cheb_passes + 1)):
-                lam, Xb, _, rn = kbatch.chfsi_step(apply, Xb, cheb_degree, bounds[key], lam)
+                lam, Xb, HXb, rn = kbatch.chfsi_step(apply, Xb, cheb_degree, bounds[key], lam, HX=HXb)
                 r = float((rn[:, :nb] * W[:, :nb]).max())
                 if r < tol_k:
                     break
         else:
-            lam, Xb, r, nit = kbatch.davidson(apply, Xb, nbt, diag, max_iter=(dav_max_iter or (40 if cold else 25)),
+            lam, Xb, r, nit = kbatch.davidson(apply, Xb, nbt, diag,
+                                              max_iter=(dav_max_iter or (cold_max_iter if cold else 25)),
                                               tol=(max(tol_k, 1e-3) if (cold and use_cheb) else tol_k), nconv=nb,
                                               weight=W)
         dav_iters += nit
@@ -247,17 +256,18 @@ def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, 
             lam = ritz[ik]
             wk = weights(ik)
             nit, r = 0, float("inf")
+            HXk = None            # H X is known after the first pass of this map
             for nit in range(1, (41 if tol_k <= 1e-9 else cheb_passes + 1)):
-                lam, Xk, _, rn = chfsi_step(apply, Xk, cheb_degree, bounds[key], lam)
+                lam, Xk, HXk, rn = chfsi_step(apply, Xk, cheb_degree, bounds[key], lam, HX=HXk)
                 r = float((rn[:nb] * wk[:nb]).max())
                 if r < tol_k:
                     break
             dav_iters += nit
             return lam, Xk, r, nit
         diag = engine.tkin64[ik] + vmean
-        lam, Xk, r, nit = davidson(apply, Xk, nbt, diag, max_iter=(dav_max_iter or (40 if cold else 25)),
+        lam, Xk, r, nit = davidson(apply, Xk, nbt, diag, max_iter=(dav_max_iter or (cold_max_iter if cold else 25)),
                                    tol=(max(tol_k, 1e-3) if (cold and use_cheb) else tol_k), nconv=nb,
-                                   weight=weights(ik), max_sub=((cold_max_sub or 3 * nbt) if cold else None))
+                                   weight=weights(ik), max_sub=((cold_max_sub or 2 * nbt) if cold else None))
         dav_iters += nit
         return lam, Xk, r, nit
 
@@ -314,12 +324,20 @@ def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, 
         rho_out = engine.density([x[:nb] for x in Xn], occ, fp32=fp32)
         return rho_out, Xn, eig, occ, mu
 
+    tstamps = []
+    pairs = []
+    refine_phase = engine.mixed and refine
+    if refine_dav_factor is None:
+        refine_dav_factor = dav_factor
     for it in range(1, max_iter + 1):
         floor = 2e-5 if fp32 else 1e-11
-        dav_tol = max(floor, min(1e-4, 0.01 * res))
+        fac = dav_factor if (fp32 or not refine_phase) else refine_dav_factor
+        dav_tol = max(floor, min(1e-4, fac * res))
         rho_out, X, eig, occ, mu = one_map(rho, X, fp32, dav_tol, it == 1)
         res = float((rho_out - rho).abs().sum()) * engine.omega / engine.nfft / st.nelec
         hist.append(res)
+        tstamps.append(time.perf_counter() - t0)
+        pairs = (pairs + [(rho, rho_out)])[-2:]
         if verbose or log:
             q_out = float(rho_out.sum()) * engine.omega / engine.nfft
             msg = (f"  scf {it:3d}  drho={res:.3e}  {'fp32' if fp32 else 'fp64'}  t={time.perf_counter() - t0:.2f}s"
@@ -373,23 +391,35 @@ def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, 
     # steps are taken only while the residual keeps falling, and the state
     # kept is the evaluation with the smallest residual.  Where the map
     # contracts this is exactly the reference's sequence.
+    # Gain of the unmixed map along the last mixed step,
+    # |F(n_k) - F(n_k-1)| / |n_k - n_k-1|: above 1 an unmixed step is
+    # expected to grow the residual (and would be discarded by the rule
+    # above), so only the eigenvectors of H[rho] are computed.
+    gain = None
+    if len(pairs) == 2:
+        dn = float(torch.linalg.vector_norm(pairs[1][0] - pairs[0][0]))
+        if dn > 0.0:
+            gain = float(torch.linalg.vector_norm(pairs[1][1] - pairs[0][1])) / dn
+    n_polish = 0 if (gain is not None and gain > polish_gain_max) else polish
     best = None
-    polish_tol = max(1e-11, min(1e-4, 0.01 * tol))
-    for j in range(polish + 1):
+    polish_tol = max(1e-11, min(1e-4, (refine_dav_factor if refine_phase else dav_factor) * tol))
+    for j in range(n_polish + 1):
         rho_new, Xn, eig_j, occ_j, mu_j = one_map(rho, X, False, polish_tol, False)
         res_j = float((rho_new - rho).abs().sum()) * engine.omega / engine.nfft / st.nelec
         hist.append(res_j)
+        tstamps.append(time.perf_counter() - t0)
         if best is not None and res_j > best[5]:
             break
         best = (rho, Xn, eig_j, occ_j, mu_j, res_j)
         X = Xn
-        if j == polish or res_j < 1e-14:
+        if j == n_polish or res_j < 1e-14:
             break
         rho = rho_new
     rho, X, eig, occ, mu, res = best
     X = [x[:nb].contiguous() for x in X]
     info = {"iterations": it, "residual_history": hist, "davidson_iterations": dav_iters,
-            "fp64_from_iteration": info_switch,
+            "fp64_from_iteration": info_switch, "map_end_seconds": tstamps, "map_gain": gain,
+            "polish_steps": n_polish,
             "seconds": time.perf_counter() - t0}
     state = SCFState(psi=X, eig=eig, occ=occ, mu=float(mu), rho_r=rho, n_iter=it, residual=res)
     return state, info
