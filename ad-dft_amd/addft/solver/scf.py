@@ -160,14 +160,17 @@ def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, 
               switch_tol=1e-5, verbose=False, rho0=None, polish=4, dav_max_iter=None, log=None,
               nbuffer=None, eigensolver="auto", cheb_degree=8, refine=True, reset_on_switch=True,
               refine_max_sub=None, cheb_passes=3, batch_k=True, cold_max_sub=None, dav_factor=0.1,
-              refine_dav_factor=None, cold_max_iter=8, polish_gain_max=1.0):
+              refine_dav_factor=None, cold_max_iter=8, polish_gain_max=1.0, psi0=None):
     """Run the SCF on engine's current geometry.  Returns (SCFState with
     stick-order psi list in `psi` and plane-major rho, info dict).
     The eigensolve of each map stops at dav_factor x the density residual
     (refine_dav_factor in the FP64 phase of the mixed form).  The first map
     starts from plane waves with at most cold_max_iter Davidson iterations in
     a subspace of cold_max_sub (default 2 nbt) vectors: the eigenvectors of
-    the trial density need only be rough, the later maps refine them."""
+    the trial density need only be rough, the later maps refine them.
+    psi0 (per k, stick order) with rho0 warm-starts the SCF from a nearby
+    solution (e.g. a displaced geometry): no cold start; rows beyond psi0's
+    are filled with plane waves."""
     st = engine.st
     nb = st.nband
     dev = engine.dev
@@ -175,11 +178,23 @@ def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, 
         mixer = "pulay" if engine.nat > 2 else "broyden"
     fp32 = engine.mixed
     cdt = torch.complex64 if fp32 else torch.complex128
-    rho = (torch.full((engine.nfft,), st.nelec / engine.omega, dtype=torch.float64, device=dev)
-           if rho0 is None else rho0.clone())
+    if rho0 is None:
+        rho = torch.full((engine.nfft,), st.nelec / engine.omega, dtype=torch.float64, device=dev)
+    else:
+        # a density from another cell (a strained geometry) holds a different
+        # electron count; the Kerker-preconditioned mixers never change the
+        # G = 0 component, so the count is set here
+        rho = rho0.to(device=dev, dtype=torch.float64).clone()
+        q = float(rho.sum()) * engine.omega / engine.nfft
+        if q > 0.0:
+            rho *= st.nelec / q
     nbuf = buffer_bands(nb) if nbuffer is None else int(nbuffer)
     nbt = min(nb + nbuf, engine.ng)
     X = [initial_guess(engine, nbt, cdt) for _ in range(engine.nk)]
+    if psi0 is not None:
+        for ik in range(engine.nk):
+            p = torch.as_tensor(psi0[ik]).to(device=dev, dtype=cdt)[:nbt]
+            X[ik] = torch.cat([p, X[ik][p.shape[0]:]], dim=0).contiguous()
     diis = DIIS(8, beta) if mixer == "diis" else None
     broy = Broyden(8) if mixer == "broyden" else None
     pulay = PulayKerker(engine, 8, alpha, gg0) if mixer == "pulay" else None
@@ -333,7 +348,7 @@ def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, 
         floor = 2e-5 if fp32 else 1e-11
         fac = dav_factor if (fp32 or not refine_phase) else refine_dav_factor
         dav_tol = max(floor, min(1e-4, fac * res))
-        rho_out, X, eig, occ, mu = one_map(rho, X, fp32, dav_tol, it == 1)
+        rho_out, X, eig, occ, mu = one_map(rho, X, fp32, dav_tol, it == 1 and psi0 is None)
         res = float((rho_out - rho).abs().sum()) * engine.omega / engine.nfft / st.nelec
         hist.append(res)
         tstamps.append(time.perf_counter() - t0)

@@ -160,6 +160,71 @@ def _cpu_baseline(OB, np, lat, nproj, band_apps, nb_sample, nl_cols):
                        f"the GPU mixed run; lower bound (subspace algebra excluded), extrapolated")}
 
 
+def fd_check(addft, testsystems, workload, precision, tol, np, torch, h=1e-3, seed=0, only="all"):
+    """Central differences of the Mermin free energy, F(x + h) - F(x - h)
+    over 2h, with the SCF re-converged at each displaced geometry (warm
+    start from the undisplaced density and orbitals, same static basis:
+    the frozen G-sphere), on 6 seeded Cartesian force components and the
+    strain derivatives dF/deps of 2 stress components; compared with the
+    analytic forces and -Omega sigma of the envelope backward pass."""
+    from addft.api import System, _gradient, _solve, forces_from_grad, stress_from_grad
+    from addft.geometry.cell import make_cell, volume
+    kw = testsystems.config(workload, precision=precision)
+    s = addft.setup(**kw)
+    lat = np.asarray(kw["lattice"], dtype=np.float64)
+    frac = np.asarray(kw["frac"], dtype=np.float64)
+    t0 = time.perf_counter()
+    sol = _solve(s, tol=tol)
+    F0, g, _ = _gradient(sol, s.geo)
+    forces = forces_from_grad(s.geo, g)
+    omega = volume(lat)
+    sig = stress_from_grad(omega, 0.5 * (g.eps + g.eps.T))
+    say(f"fd-check {workload} {precision}: reference SCF+gradient {time.perf_counter() - t0:.1f} s")
+    rng = np.random.default_rng(seed)
+    atoms = rng.choice(len(frac), size=6, replace=False)
+    dirs = rng.integers(0, 3, size=6)
+    strains = [(0, 0), (0, 1)]
+    if only == "forces":
+        strains = []
+    if only == "strain":
+        atoms, dirs = [], []
+
+    def free_energy(lat2, frac2):
+        s2 = System(geo=make_cell(lat2, frac2), static=s.static)
+        r = _solve(s2, tol=tol, rho0=sol.rho, psi0=sol.psi)
+        return r.engine.energy_parts(r.psi, r.occ, r.rho)["F"], r.info["iterations"]
+
+    rows = []
+    R = frac @ lat
+    for a, c in zip(atoms, dirs):
+        e = []
+        for sg in (1.0, -1.0):
+            R2 = R.copy()
+            R2[a, c] += sg * h
+            e.append(free_energy(lat, R2 @ np.linalg.inv(lat)))
+        fd = -(e[0][0] - e[1][0]) / (2 * h)
+        rows.append({"kind": "force", "atom": int(a), "dir": int(c), "analytic": float(forces[a, c]),
+                     "fd": float(fd), "abs_err": float(abs(fd - forces[a, c])), "scf_iters": [e[0][1], e[1][1]]})
+        say(f"  F[{a},{c}] analytic {forces[a, c]:+.8f}  fd {fd:+.8f}  |d| {abs(fd - forces[a, c]):.2e}")
+    for (a, b) in strains:
+        e = []
+        for sg in (1.0, -1.0):
+            eps = np.zeros((3, 3))
+            eps[a, b] = sg * h
+            e.append(free_energy(lat @ (np.eye(3) + eps), frac))
+        fd = (e[0][0] - e[1][0]) / (2 * h)
+        ana = -omega * sig[a, b]
+        rows.append({"kind": "strain", "comp": [a, b], "analytic_dF_deps": float(ana), "fd": float(fd),
+                     "abs_err": float(abs(fd - ana)), "sigma_err_Ha_per_Bohr3": float(abs(fd - ana) / omega),
+                     "scf_iters": [e[0][1], e[1][1]]})
+        say(f"  dF/deps[{a},{b}] analytic {ana:+.8f}  fd {fd:+.8f}  |d| {abs(fd - ana):.2e}")
+    return {"fd_check": workload, "precision": precision, "tol": tol, "h": h, "F": F0,
+            "max_force_err_Ha_per_Bohr": max([r["abs_err"] for r in rows if r["kind"] == "force"], default=None),
+            "max_stress_err_Ha_per_Bohr3": max([r["sigma_err_Ha_per_Bohr3"] for r in rows if r["kind"] == "strain"],
+                                               default=None),
+            "seconds": time.perf_counter() - t0, "components": rows}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -171,6 +236,10 @@ def main():
     ap.add_argument("--hpsi-steps", type=int, default=20)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--fd-check", choices=["fp64", "mixed"], default=None,
+                    help="central differences (h = 1e-3) of the free energy on 6 seeded force components "
+                         "and 2 stress components of the workload, against the analytic forces and stress")
+    ap.add_argument("--fd-only", choices=["all", "forces", "strain"], default="all")
     ap.add_argument("--hpsi-only", action="store_true",
                     help="only the local-Hpsi roofline measurement (and a 1 GiB device copy that calibrates "
                          "the FETCH_SIZE/WRITE_SIZE counters); for profiling runs")
@@ -191,6 +260,12 @@ def main():
     kw = testsystems.config(args.workload, precision="mixed")
     lat = np.asarray(kw["lattice"])
     nat = len(kw["frac"])
+
+    if args.fd_check:
+        tol = args.fp64_tol if args.fd_check == "fp64" else args.mixed_tol
+        print(json.dumps(fd_check(addft, testsystems, args.workload, args.fd_check, tol, np, torch,
+                                  only=args.fd_only)), flush=True)
+        return
 
     if args.hpsi_only:
         sys_m = addft.setup(**kw)
