@@ -87,11 +87,14 @@ def bicgstab(apply, b, rtol=1e-10, maxiter=200, precond=None):
     return x, it
 
 
-def gmres(apply, b, rtol=1e-10, maxiter=200, restart=60, precond=None):
+def gmres(apply, b, rtol=1e-10, maxiter=200, restart=60, precond=None, on_residual=None):
     """Restarted GMRES (modified Gram-Schmidt Arnoldi, Givens rotations on the
     Hessenberg matrix) for apply(x) = b, right-preconditioned when precond is
     given: apply(precond(y)) = b, x = precond(y).  One product per iteration,
-    where BiCGSTAB needs two.  Returns (x, products)."""
+    where BiCGSTAB needs two.  on_residual(rel) is called before every
+    product with the current relative residual (1.0 for the products that
+    form a true residual), for inexact products.  Returns (x, products)."""
+    note = on_residual if on_residual is not None else (lambda rel: None)
     bnorm = float(torch.linalg.vector_norm(b)) or 1.0
     x = torch.zeros_like(b)
     M = precond if precond is not None else (lambda v: v)
@@ -110,6 +113,7 @@ def gmres(apply, b, rtol=1e-10, maxiter=200, restart=60, precond=None):
         gvec[0] = beta
         j = 0
         for j in range(m):
+            note(abs(gvec[j]) / bnorm)
             w = apply(M(V[j]))
             total += 1
             for i in range(j + 1):
@@ -138,6 +142,7 @@ def gmres(apply, b, rtol=1e-10, maxiter=200, restart=60, precond=None):
         for i in range(j):
             z = z + float(y[i]) * V[i]
         x = x + M(z)
+        note(1.0)
         r = b - apply(x)
         total += 1
     return x, total
@@ -367,7 +372,7 @@ def dielectric_preconditioner(engine, q0sq=2.0):
 
 
 def delta_q_grad(engine, psis, eig, occ, rho_pl, weight_pl, rtol=1e-10, maxiter=200, precondition=True,
-                 method="gmres", **resp_kw):
+                 method="gmres", log=None, relax=0.1, **resp_kw):
     """dC/dR and dC/deps for C = (Omega/N) sum w n at the SCF fixed point n*
     (implicit.py:32-92): (I - J)^T lam = (Omega/N) w, then
     dC/dtheta = dC/dtheta|_n* + lam . dF/dtheta|_n*; the state term is 0 for
@@ -377,11 +382,26 @@ def delta_q_grad(engine, psis, eig, occ, rho_pl, weight_pl, rtol=1e-10, maxiter=
     g = (engine.omega / engine.nfft) * weight_pl
 
     def aT(y):
-        return y - resp.JT(y)
+        out = y - resp.JT(y)
+        if log is not None:
+            log(f"adjoint product {resp.stats['chi0']}: cg iterations {resp.stats['cg_iterations']}")
+        return out
 
-    solver = gmres if method == "gmres" else bicgstab
-    lam, nit = solver(aT, g, rtol=rtol, maxiter=maxiter,
-                      precond=dielectric_preconditioner(engine) if precondition else None)
+    pre = dielectric_preconditioner(engine) if precondition else None
+    base_tol = resp.cg_tol
+    if method == "gmres":
+        # inexact Krylov (Simoncini & Szyld, SIAM J. Sci. Comput. 25 (2003)
+        # 454; Bouras & Fraysse, SIAM J. Matrix Anal. Appl. 26 (2005) 660):
+        # once the residual has fallen to rel, a product error of
+        # rtol / rel (relative) no longer limits the attainable accuracy, so
+        # the Sternheimer solves inside chi0 stop at relax * rtol / rel
+        def on_residual(rel):
+            if relax:
+                resp.cg_tol = min(1e-4, max(base_tol, relax * rtol / max(rel, 1e-300)))
+        lam, nit = gmres(aT, g, rtol=rtol, maxiter=maxiter, precond=pre, on_residual=on_residual)
+    else:
+        lam, nit = bicgstab(aT, g, rtol=rtol, maxiter=maxiter, precond=pre)
+    resp.cg_tol = base_tol
     res = float(torch.linalg.vector_norm(aT(lam) - g) / (torch.linalg.vector_norm(g) + 1e-300))
     if res > 1e-6:
         raise RuntimeError(f"adjoint BiCGSTAB did not converge: residual {res:.3e}")
