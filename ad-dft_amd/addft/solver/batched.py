@@ -148,24 +148,33 @@ def upper_bound(apply, nk, nG, dtype, device, steps=10, seed=0):
     return (theta + torch.linalg.vector_norm(f, dim=2)[:, 0].double().cpu()).to(device)
 
 
-def chfsi_step(apply, X, degree, b_up, lam, HX=None):
+def chfsi_step(apply, X, degree, b_up, lam, HX=None, filt=None):
     """One scaled Chebyshev filter (Zhou-Saad) + Rayleigh-Ritz per k.
-    b_up (nk,), lam (nk, nbt) ascending; HX = H X when known.  Returns
-    (lam, X, HX, residual norms)."""
+    b_up (nk,), lam (nk, nbt) ascending; HX = H X when known.  filt(X, HX,
+    abc) runs the filter instead (a captured graph, solver/graphs.py), with
+    abc[k] = (largest Ritz value, smallest, upper bound).  Returns (lam, X,
+    HX, residual norms)."""
     rd = X.real.dtype
-    a = lam[:, -1].to(rd)[:, None, None]
-    a0 = lam[:, 0].to(rd)[:, None, None]
-    b = b_up.to(rd)[:, None, None]
-    e = 0.5 * (b - a)
-    c = 0.5 * (b + a)
-    sigma = e / (a0 - c)
-    tau = 2.0 / sigma
-    Y = ((apply(X) if HX is None else HX) - c * X) * (sigma / e)
-    Xm = X
-    for _ in range(2, degree + 1):
-        sigma_new = 1.0 / (tau - sigma)
-        Ynew = (apply(Y) - c * Y) * (2.0 * sigma_new / e) - (sigma * sigma_new) * Xm
-        Xm, Y, sigma = Y, Ynew, sigma_new
+    if HX is None:
+        HX = apply(X)
+    if filt is not None:
+        abc = torch.stack([lam[:, -1].to(torch.float64), lam[:, 0].to(torch.float64),
+                           b_up.to(torch.float64)], dim=1)
+        Y = filt(X, HX, abc)
+    else:
+        a = lam[:, -1].to(rd)[:, None, None]
+        a0 = lam[:, 0].to(rd)[:, None, None]
+        b = b_up.to(rd)[:, None, None]
+        e = 0.5 * (b - a)
+        c = 0.5 * (b + a)
+        sigma = e / (a0 - c)
+        tau = 2.0 / sigma
+        Y = (HX - c * X) * (sigma / e)
+        Xm = X
+        for _ in range(2, degree + 1):
+            sigma_new = 1.0 / (tau - sigma)
+            Ynew = (apply(Y) - c * Y) * (2.0 * sigma_new / e) - (sigma * sigma_new) * Xm
+            Xm, Y, sigma = Y, Ynew, sigma_new
     Y = orth(Y)
     w, X, HX = rayleigh_ritz(Y, apply(Y))
     _, rn = residual_norms(HX, X, w)

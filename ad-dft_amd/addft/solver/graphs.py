@@ -1,0 +1,99 @@
+"""hipGraph capture of the Chebyshev filter over all k-points.
+
+The filter p_m(H) X of one ChFSI pass (solver/batched.py::chfsi_step) is a
+fixed sequence of m batched Hpsi calls (Engine.hpsi_batch: one local launch
+over all k-point band blocks plus the batched projector GEMMs) and the
+three-term recurrence -- about a hundred small launches whose host cost
+exceeds their device time on small cells with many k-points.  It is captured
+once per (precision, shape, degree) as a graph (torch.cuda.CUDAGraph, a
+hipGraph on ROCm) and replayed; inputs enter through static buffers that are
+refreshed by device copies before each replay:
+
+  X   (nk, m, nG)  the block to filter
+  HX  (nk, m, nG)  H X (known from the previous pass, or computed eagerly)
+  abc (nk, 3)      per-k filter window: a = largest Ritz value of the block,
+                   a0 = smallest, b = upper bound of the spectrum
+  vs               the staged local potential of the current SCF map
+
+Nothing inside the captured region clears memory with a memset node, allocates
+outside the graph pool or synchronises with the host.
+"""
+
+from __future__ import annotations
+
+import os
+
+import torch
+
+
+def enabled() -> bool:
+    return os.environ.get("ADDFT_GRAPHS", "1") != "0" and torch.cuda.is_available()
+
+
+def _recurrence(apply, X, HX, abc, degree):
+    """Scaled Chebyshev filter (Zhou-Saad Alg. 3.2) with per-k windows."""
+    rd = X.real.dtype
+    a = abc[:, 0, None, None].to(rd)
+    a0 = abc[:, 1, None, None].to(rd)
+    b = abc[:, 2, None, None].to(rd)
+    e = 0.5 * (b - a)
+    c = 0.5 * (b + a)
+    sigma = e / (a0 - c)
+    tau = 2.0 / sigma
+    Y = (HX - c * X) * (sigma / e)
+    Xm = X
+    for _ in range(2, degree + 1):
+        sigma_new = 1.0 / (tau - sigma)
+        Ynew = (apply(Y) - c * Y) * (2.0 * sigma_new / e) - (sigma * sigma_new) * Xm
+        Xm, Y, sigma = Y, Ynew, sigma_new
+    return Y
+
+
+class FilterGraph:
+    """One captured filter.  hpsi(p, vs) -> H p for a (nk, m, nG) block."""
+
+    def __init__(self, hpsi, X, vs, degree):
+        self.degree = int(degree)
+        self.shape, self.dtype = tuple(X.shape), X.dtype
+        self.X = X.detach().clone()
+        self.HX = torch.zeros_like(self.X)
+        self.vs = vs.detach().clone()
+        self.abc = torch.zeros((X.shape[0], 3), dtype=torch.float64, device=X.device)
+        self.abc[:, 0], self.abc[:, 1], self.abc[:, 2] = 1.0, 0.0, 2.0
+
+        def apply(p):
+            return hpsi(p, self.vs)
+
+        self._apply = apply
+        # warm-up on a side stream (library handles and workspaces are created
+        # outside the capture), then capture on it
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            for _ in range(2):
+                _recurrence(apply, self.X, self.HX, self.abc, self.degree)
+        torch.cuda.current_stream().wait_stream(side)
+        torch.cuda.synchronize()
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            self.Y = _recurrence(apply, self.X, self.HX, self.abc, self.degree)
+        self.replays = 0
+
+    def matches(self, X, degree):
+        return tuple(X.shape) == self.shape and X.dtype == self.dtype and int(degree) == self.degree
+
+    def __call__(self, X, HX, abc, vs):
+        """p_m(H) X for the window abc; returns the graph's static output
+        (valid until the next replay)."""
+        self.X.copy_(X)
+        self.HX.copy_(HX)
+        self.abc.copy_(abc)
+        self.vs.copy_(vs)
+        self.graph.replay()
+        self.replays += 1
+        return self.Y
+
+
+def eager_filter(hpsi, X, HX, abc, vs, degree):
+    """The same filter without capture (reference for tests and fallback)."""
+    return _recurrence(lambda p: hpsi(p, vs), X, HX, abc, degree)

@@ -35,6 +35,7 @@ from ..system import SCFState
 from . import batched as kbatch
 from .chebyshev import chfsi_step, upper_bound
 from .davidson import DENSE_MAX_NG, davidson
+from . import graphs as fgraphs
 from .linalg import gram_tall
 
 
@@ -160,7 +161,7 @@ def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, 
               switch_tol=1e-5, verbose=False, rho0=None, polish=4, dav_max_iter=None, log=None,
               nbuffer=None, eigensolver="auto", cheb_degree=8, refine=True, reset_on_switch=True,
               refine_max_sub=None, cheb_passes=3, batch_k=True, cold_max_sub=None, dav_factor=0.1,
-              refine_dav_factor=None, cold_max_iter=8, polish_gain_max=1.0, psi0=None):
+              refine_dav_factor=None, cold_max_iter=8, polish_gain_max=1.0, psi0=None, graphs=True):
     """Run the SCF on engine's current geometry.  Returns (SCFState with
     stick-order psi list in `psi` and plane-major rho, info dict).
     The eigensolve of each map stops at dav_factor x the density residual
@@ -170,7 +171,8 @@ def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, 
     the trial density need only be rough, the later maps refine them.
     psi0 (per k, stick order) with rho0 warm-starts the SCF from a nearby
     solution (e.g. a displaced geometry): no cold start; rows beyond psi0's
-    are filled with plane waves."""
+    are filled with plane waves.  graphs: with several k-points batched,
+    replay the Chebyshev filter as a captured graph (solver/graphs.py)."""
     st = engine.st
     nb = st.nband
     dev = engine.dev
@@ -216,6 +218,26 @@ def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, 
     batched = batch_k and engine.nk > 1 and engine.ng > max(DENSE_MAX_NG, 4 * nbt)
     ritz_b = [None]
 
+    # the batched Chebyshev filter as one captured graph per (precision, shape)
+    use_graphs = batched and graphs and fgraphs.enabled()
+    graph_cache = {}
+
+    def filter_graph(Xb, vs, fp32):
+        tag = "fp32" if fp32 else "fp64"
+        key = (fp32, tuple(Xb.shape), cheb_degree)
+        fg = graph_cache.get(key)
+        if fg is None:
+            before = engine.hpsi_bands[tag]
+            fg = fgraphs.FilterGraph(lambda p, v: engine.hpsi_batch(p, v, fp32=fp32), Xb, vs, cheb_degree)
+            engine.hpsi_bands[tag] = before
+            graph_cache[key] = fg
+
+        def run(X, HX, abc):
+            engine.hpsi_bands[tag] += (cheb_degree - 1) * X.shape[0] * X.shape[1]
+            return fg(X, HX, abc, vs)
+
+        return run
+
     def solve_batched(Xb, vs, fp32, vmean, tol_k, cold, vs_lo=None):
         nonlocal dav_iters
         apply = lambda p: engine.hpsi_batch(p, vs, fp32=fp32)  # noqa: E731
@@ -234,8 +256,9 @@ def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, 
             lam = ritz_b[0]
             nit, r = 0, float("inf")
             HXb = None
+            filt = filter_graph(Xb, vs, fp32) if use_graphs else None
             for nit in range(1, (41 if tol_k <= 1e-9 else cheb_passes + 1)):
-                lam, Xb, HXb, rn = kbatch.chfsi_step(apply, Xb, cheb_degree, bounds[key], lam, HX=HXb)
+                lam, Xb, HXb, rn = kbatch.chfsi_step(apply, Xb, cheb_degree, bounds[key], lam, HX=HXb, filt=filt)
                 r = float((rn[:, :nb] * W[:, :nb]).max())
                 if r < tol_k:
                     break

@@ -31,6 +31,7 @@ struct addft_sphere_plan_s {
   int64_t ng;
   int precision;
   int64_t nst;
+  int64_t ldt;        // row stride of the stick buffer T[b][i3][.]: nst rounded up to 16
   int nra;
   FftFactors f1, f2, f3;
   int32_t* d_perm;    // stick-order position -> reference index
@@ -76,7 +77,7 @@ size_t csize() {
 template <typename R, typename RIN>
 __global__ void __launch_bounds__(kZThreads)
 k_z_inverse(const cpx<RIN>* __restrict__ psi, int64_t ng, const int32_t* __restrict__ soff,
-            const int32_t* __restrict__ sid, const int32_t* __restrict__ iz, int64_t nst,
+            const int32_t* __restrict__ sid, const int32_t* __restrict__ iz, int64_t nst, int64_t ldt,
             int spb, FftFactors f3, const cpx<R>* __restrict__ gtw3, cpx<R>* __restrict__ T) {
   extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   cpx<R>* L = reinterpret_cast<cpx<R>*>(smem_raw);
@@ -96,10 +97,10 @@ k_z_inverse(const cpx<RIN>* __restrict__ psi, int64_t ng, const int32_t* __restr
   }
   __syncthreads();
   addft::lds_fft<R, kZEpt, true>(L, f3, cnt, n3, 1, tw3);
-  cpx<R>* dst = T + (b * n3) * nst + s0;
+  cpx<R>* dst = T + (b * n3) * ldt + s0;
   for (int i = threadIdx.x; i < cnt * n3; i += blockDim.x) {
     int z = i / cnt, j = i - z * cnt;
-    dst[(int64_t)z * nst + j] = L[j * n3 + z];
+    dst[(int64_t)z * ldt + j] = L[j * n3 + z];
   }
 }
 
@@ -108,7 +109,7 @@ k_z_inverse(const cpx<RIN>* __restrict__ psi, int64_t ng, const int32_t* __restr
 template <typename R, typename ROUT>
 __global__ void __launch_bounds__(kZThreads)
 k_z_forward(const cpx<R>* __restrict__ T, int64_t ng, const int32_t* __restrict__ soff,
-            const int32_t* __restrict__ sid, const int32_t* __restrict__ iz, int64_t nst,
+            const int32_t* __restrict__ sid, const int32_t* __restrict__ iz, int64_t nst, int64_t ldt,
             int spb, FftFactors f3, const cpx<R>* __restrict__ gtw3, R scale,
             const R* __restrict__ tkin, int64_t tk_bpk, const cpx<R>* __restrict__ psi,
             cpx<ROUT>* __restrict__ out) {
@@ -120,10 +121,10 @@ k_z_forward(const cpx<R>* __restrict__ T, int64_t ng, const int32_t* __restrict_
   const int64_t s0 = (int64_t)blockIdx.x * spb;
   const int cnt = (int)min((int64_t)spb, nst - s0);
   const int64_t b = blockIdx.y;
-  const cpx<R>* src = T + (b * n3) * nst + s0;
+  const cpx<R>* src = T + (b * n3) * ldt + s0;
   for (int i = threadIdx.x; i < cnt * n3; i += blockDim.x) {
     int z = i / cnt, j = i - z * cnt;
-    L[j * n3 + z] = src[(int64_t)z * nst + j];
+    L[j * n3 + z] = src[(int64_t)z * ldt + j];
   }
   __syncthreads();
   addft::lds_fft<R, kZEpt, false>(L, f3, cnt, n3, 1, tw3);
@@ -189,7 +190,7 @@ __device__ __forceinline__ void compress_columns(cpx<R>* P, const cpx<R>* W, int
 // band b = blockIdx.y.  v is plane-major in precision R, pre-scaled.
 template <typename R, int EPT>
 __global__ void __launch_bounds__(kPThreads)
-k_plane_v(cpx<R>* __restrict__ T, int64_t nst, int nra, const int32_t* __restrict__ srow,
+k_plane_v(cpx<R>* __restrict__ T, int64_t nst, int64_t ldt, int nra, const int32_t* __restrict__ srow,
           const int32_t* __restrict__ scol, const int32_t* __restrict__ rowval, FftFactors f1,
           FftFactors f2, const cpx<R>* __restrict__ gtw1, const cpx<R>* __restrict__ gtw2,
           const R* __restrict__ v, int cchunk) {
@@ -203,7 +204,7 @@ k_plane_v(cpx<R>* __restrict__ T, int64_t nst, int nra, const int32_t* __restric
   addft::stage_twiddles(tw2, gtw2, n2);
   const int z = blockIdx.x;
   const int64_t b = blockIdx.y;
-  cpx<R>* Tp = T + (b * n3 + z) * nst;
+  cpx<R>* Tp = T + (b * n3 + z) * ldt;
   load_plane(P, Tp, nra, n2, nst, srow, scol);
   addft::lds_fft<R, EPT, true>(P, f2, nra, n2, 1, tw2);
   const R* vp = v + (int64_t)z * n1 * n2;
@@ -236,7 +237,7 @@ k_plane_v(cpx<R>* __restrict__ T, int64_t nst, int nra, const int32_t* __restric
 // the V(r) plane in L2.  rowinfo[r] = (first stick, first i2, count, 0).
 template <typename R, int N, int L, int WAVES>
 __global__ void __launch_bounds__(64 * WAVES)
-k_plane_v_lane(cpx<R>* __restrict__ T, int64_t nst, int nra, int a1, const int4* __restrict__ rowinfo,
+k_plane_v_lane(cpx<R>* __restrict__ T, int64_t ldt, int nra, int a1, const int4* __restrict__ rowinfo,
                const cpx<R>* __restrict__ gtw, const R* __restrict__ v, int n3) {
   using S = addft::lane::Shape<N, L>;
   constexpr int M = S::M, Q = S::Q, G = S::G;
@@ -268,7 +269,7 @@ k_plane_v_lane(cpx<R>* __restrict__ T, int64_t nst, int nra, int a1, const int4*
     cld[j] = rr >= 0 ? rr * N : zoff;
     cst[j] = rr >= 0 ? rr * N : toff;
   }
-  cpx<R>* Tp = T + (b * n3 + z) * nst;
+  cpx<R>* Tp = T + (b * n3 + z) * ldt;
 
   // phase 1: rows, sticks -> inverse along i2 -> P
   for (int base = wave * G; base < nra; base += WAVES * G) {
@@ -360,7 +361,7 @@ k_plane_v_lane(cpx<R>* __restrict__ T, int64_t nst, int nra, int a1, const int4*
 template <typename R, int N, int L, int WAVES>
 __global__ void __launch_bounds__(64 * WAVES)
 k_z_inverse_lane(const cpx<R>* __restrict__ psi, int64_t ng, const int32_t* __restrict__ soff,
-                 const int32_t* __restrict__ a3, int64_t nst, const cpx<R>* __restrict__ gtw,
+                 const int32_t* __restrict__ a3, int64_t nst, int64_t ldt, const cpx<R>* __restrict__ gtw,
                  cpx<R>* __restrict__ T) {
   using S = addft::lane::Shape<N, L>;
   constexpr int M = S::M, Q = S::Q, G = S::G, SPB = WAVES * G, SST = SPB + 1;
@@ -406,17 +407,18 @@ k_z_inverse_lane(const cpx<R>* __restrict__ psi, int64_t ng, const int32_t* __re
       }
   }
   __syncthreads();
-  cpx<R>* dst = T + (b * N) * nst + s0;
+  cpx<R>* dst = T + (b * N) * ldt + s0;
   for (int i = threadIdx.x; i < N * cnt; i += blockDim.x) {
     const int z = i / cnt, j = i - z * cnt;
-    dst[(int64_t)z * nst + j] = stage[z * SST + j];
+    dst[(int64_t)z * ldt + j] = stage[z * SST + j];
   }
 }
 
 template <typename R, typename ROUT, int N, int L, int WAVES>
 __global__ void __launch_bounds__(64 * WAVES)
 k_z_forward_lane(const cpx<R>* __restrict__ T, int64_t ng, const int32_t* __restrict__ soff,
-                 const int32_t* __restrict__ a3, int64_t nst, const cpx<R>* __restrict__ gtw, R scale,
+                 const int32_t* __restrict__ a3, int64_t nst, int64_t ldt, const cpx<R>* __restrict__ gtw,
+                 R scale,
                  const R* __restrict__ tkin, int64_t tk_bpk, const cpx<R>* __restrict__ psi,
                  cpx<ROUT>* __restrict__ out) {
   using S = addft::lane::Shape<N, L>;
@@ -433,10 +435,10 @@ k_z_forward_lane(const cpx<R>* __restrict__ T, int64_t ng, const int32_t* __rest
   const int cnt = (int)min((int64_t)SPB, nst - s0);
   const int64_t b = blockIdx.y;
   for (int i = threadIdx.x; i < N; i += blockDim.x) tw[i] = gtw[i];
-  const cpx<R>* src = T + (b * N) * nst + s0;
+  const cpx<R>* src = T + (b * N) * ldt + s0;
   for (int i = threadIdx.x; i < N * cnt; i += blockDim.x) {
     const int z = i / cnt, j = i - z * cnt;
-    stage[z * SST + j] = src[(int64_t)z * nst + j];
+    stage[z * SST + j] = src[(int64_t)z * ldt + j];
   }
   __syncthreads();
   cpx<R> twr[M];
@@ -478,7 +480,7 @@ k_z_forward_lane(const cpx<R>* __restrict__ T, int64_t ng, const int32_t* __rest
 // and writes them once.
 template <typename R, int N, int L, int WAVES>
 __global__ void __launch_bounds__(64 * WAVES)
-k_plane_density_lane(const cpx<R>* __restrict__ T, int64_t nst, int nra, int a1,
+k_plane_density_lane(const cpx<R>* __restrict__ T, int64_t ldt, int nra, int a1,
                      const int4* __restrict__ rowinfo, const cpx<R>* __restrict__ gtw,
                      const double* __restrict__ w, int64_t nb, int bpg, double* __restrict__ part) {
   using S = addft::lane::Shape<N, L>;
@@ -518,7 +520,7 @@ k_plane_density_lane(const cpx<R>* __restrict__ T, int64_t nst, int nra, int a1,
   const int64_t b0 = grp * bpg, b1 = min(nb, b0 + bpg);
   for (int64_t b = b0; b < b1; ++b) {
     __syncthreads();   // twiddles staged / previous band's columns done with P
-    const cpx<R>* Tp = T + (b * n3 + z) * nst;
+    const cpx<R>* Tp = T + (b * n3 + z) * ldt;
     for (int base = wave * G; base < nra; base += WAVES * G) {
       const int r = base + g;
       const bool ok = act && r < nra;
@@ -607,7 +609,7 @@ size_t lane_lds_bytes(int n, int nra, int waves) {
 // part[g][z][x][y] = sum_{b in group} w_b |IFFT_plane(T_b)|^2.
 template <typename R, int EPT>
 __global__ void __launch_bounds__(kPThreads)
-k_plane_density(const cpx<R>* __restrict__ T, int64_t nst, int nra, const int32_t* __restrict__ srow,
+k_plane_density(const cpx<R>* __restrict__ T, int64_t nst, int64_t ldt, int nra, const int32_t* __restrict__ srow,
                 const int32_t* __restrict__ scol, const int32_t* __restrict__ rowval, FftFactors f1,
                 FftFactors f2, const cpx<R>* __restrict__ gtw1, const cpx<R>* __restrict__ gtw2,
                 const double* __restrict__ w, int64_t nb, int bpg, int cchunk,
@@ -628,7 +630,7 @@ k_plane_density(const cpx<R>* __restrict__ T, int64_t nst, int nra, const int32_
   const int64_t b0 = g * bpg, b1 = min(nb, b0 + bpg);
   for (int64_t b = b0; b < b1; ++b) {
     __syncthreads();
-    const cpx<R>* Tp = T + (b * n3 + z) * nst;
+    const cpx<R>* Tp = T + (b * n3 + z) * ldt;
     for (int i = threadIdx.x; i < nra * n2; i += blockDim.x) P[i] = addft::mk<R>(0, 0);
     __syncthreads();
     for (int64_t j = threadIdx.x; j < nst; j += blockDim.x) P[srow[j] * n2 + scol[j]] = Tp[j];
@@ -663,7 +665,7 @@ __global__ void k_reduce_parts(const double* __restrict__ part, int64_t ngroups,
 // Sticks -> full real plane (plane-major, double), real part times scale.
 template <typename R, int EPT>
 __global__ void __launch_bounds__(kPThreads)
-k_plane_to_real(const cpx<R>* __restrict__ T, int64_t nst, int nra, const int32_t* __restrict__ srow,
+k_plane_to_real(const cpx<R>* __restrict__ T, int64_t nst, int64_t ldt, int nra, const int32_t* __restrict__ srow,
                 const int32_t* __restrict__ scol, const int32_t* __restrict__ rowval, FftFactors f1,
                 FftFactors f2, const cpx<R>* __restrict__ gtw1, const cpx<R>* __restrict__ gtw2,
                 double scale, int cchunk, double* __restrict__ grid) {
@@ -676,7 +678,7 @@ k_plane_to_real(const cpx<R>* __restrict__ T, int64_t nst, int nra, const int32_
   addft::stage_twiddles(tw1, gtw1, n1);
   addft::stage_twiddles(tw2, gtw2, n2);
   const int z = blockIdx.x;
-  load_plane(P, T + (int64_t)z * nst, nra, n2, nst, srow, scol);
+  load_plane(P, T + (int64_t)z * ldt, nra, n2, nst, srow, scol);
   addft::lds_fft<R, EPT, true>(P, f2, nra, n2, 1, tw2);
   double* out = grid + (int64_t)z * n1 * n2;
   for (int c0 = 0; c0 < n2; c0 += cchunk) {
@@ -695,7 +697,7 @@ k_plane_to_real(const cpx<R>* __restrict__ T, int64_t nst, int nra, const int32_
 // keep the active rows, forward along i2, keep the stick columns.
 template <typename R, int EPT>
 __global__ void __launch_bounds__(kPThreads)
-k_real_to_plane(const double* __restrict__ grid, int64_t nst, int nra, const int32_t* __restrict__ srow,
+k_real_to_plane(const double* __restrict__ grid, int64_t nst, int64_t ldt, int nra, const int32_t* __restrict__ srow,
                 const int32_t* __restrict__ scol, const int32_t* __restrict__ rowval, FftFactors f1,
                 FftFactors f2, const cpx<R>* __restrict__ gtw1, const cpx<R>* __restrict__ gtw2,
                 int cchunk, cpx<R>* __restrict__ T) {
@@ -720,7 +722,7 @@ k_real_to_plane(const double* __restrict__ grid, int64_t nst, int nra, const int
     compress_columns(P, W, n2, nra, c0, cc, rowval);
   }
   addft::lds_fft<R, EPT, false>(P, f2, nra, n2, 1, tw2);
-  store_plane(P, T + (int64_t)z * nst, n2, nst, srow, scol);
+  store_plane(P, T + (int64_t)z * ldt, n2, nst, srow, scol);
 }
 
 // ------------------------------------------------------------ small kernels
@@ -830,7 +832,7 @@ int launch_plane_lane(const addft_sphere_plan_s* p, cpx<R>* T, const R* v, int64
       return ADDFT_OK;                                                                              \
     }                                                                                               \
     hipLaunchKernelGGL((k_plane_v_lane<R, NN, lane_width(NN), kLaneWaves>), grid, dim3(64 * kLaneWaves), \
-                       lds, st, T, p->nst, p->nra, p->a1, ri, tw, v, p->n3);                       \
+                       lds, st, T, p->ldt, p->nra, p->a1, ri, tw, v, p->n3);                       \
     break;
   switch (p->n1) {
     ADDFT_LANE_SIZES(ADDFT_LANE_CASE)
@@ -867,7 +869,7 @@ int launch_z_inverse(const addft_sphere_plan_s* p, const cpx<R>* psi, cpx<R>* T,
     const size_t zlds = (size_t)(p->spb + 1) * p->n3 * sizeof(cpx<R>);
     dim3 zg((unsigned)((p->nst + p->spb - 1) / p->spb), (unsigned)nb);
     hipLaunchKernelGGL((k_z_inverse<R, R>), zg, dim3(kZThreads), zlds, st, psi, p->ng, p->d_soff, p->d_sid,
-                       p->d_iz, p->nst, p->spb, p->f3, tw3, T);
+                       p->d_iz, p->nst, p->ldt, p->spb, p->f3, tw3, T);
     ADDFT_LAUNCH_CHECK();
     return ADDFT_OK;
   }
@@ -882,7 +884,7 @@ int launch_z_inverse(const addft_sphere_plan_s* p, const cpx<R>* psi, cpx<R>* T,
       return ADDFT_OK;                                                                                 \
     }                                                                                                  \
     hipLaunchKernelGGL((k_z_inverse_lane<R, NN, lane_width(NN), kZLaneWaves>), grid,                   \
-                       dim3(64 * kZLaneWaves), lds, st, psi, p->ng, p->d_soff, p->d_a3, p->nst, tw3, T);  \
+                       dim3(64 * kZLaneWaves), lds, st, psi, p->ng, p->d_soff, p->d_a3, p->nst, p->ldt, tw3, T);  \
     break;
   switch (p->n3) {
     ADDFT_LANE_SIZES(ADDFT_ZI_CASE)
@@ -905,7 +907,7 @@ int launch_z_forward(const addft_sphere_plan_s* p, const cpx<R>* T, R scale, con
     const size_t zlds = (size_t)(p->spb + 1) * p->n3 * sizeof(cpx<R>);
     dim3 zg((unsigned)((p->nst + p->spb - 1) / p->spb), (unsigned)nb);
     hipLaunchKernelGGL((k_z_forward<R, ROUT>), zg, dim3(kZThreads), zlds, st, T, p->ng, p->d_soff, p->d_sid,
-                       p->d_iz, p->nst, p->spb, p->f3, tw3, scale, tkin, tk_bpk, psi, out);
+                       p->d_iz, p->nst, p->ldt, p->spb, p->f3, tw3, scale, tkin, tk_bpk, psi, out);
     ADDFT_LAUNCH_CHECK();
     return ADDFT_OK;
   }
@@ -920,7 +922,7 @@ int launch_z_forward(const addft_sphere_plan_s* p, const cpx<R>* T, R scale, con
       return ADDFT_OK;                                                                                        \
     }                                                                                                         \
     hipLaunchKernelGGL((k_z_forward_lane<R, ROUT, NN, lane_width(NN), kZLaneWaves>), grid,                    \
-                       dim3(64 * kZLaneWaves), lds, st, T, p->ng, p->d_soff, p->d_a3, p->nst, tw3, scale, tkin, \
+                       dim3(64 * kZLaneWaves), lds, st, T, p->ng, p->d_soff, p->d_a3, p->nst, p->ldt, tw3, scale, tkin, \
                        tk_bpk, psi, out);                                                                     \
     break;
   switch (p->n3) {
@@ -952,7 +954,7 @@ int launch_density_lane(const addft_sphere_plan_s* p, const cpx<R>* T, const dou
       return ADDFT_OK;                                                                             \
     }                                                                                              \
     hipLaunchKernelGGL((k_plane_density_lane<R, NN, lane_width(NN), W>), grid, dim3(64 * W), lds, st, T, \
-                       p->nst, p->nra, p->a1, ri, tw, w, nb, kLaneBandsPerGroup, part);            \
+                       p->ldt, p->nra, p->a1, ri, tw, w, nb, kLaneBandsPerGroup, part);            \
     break;
   switch (p->n1) {
     ADDFT_LANE_SIZES(ADDFT_DL_CASE)
@@ -984,7 +986,7 @@ bool cyclic_interval(const std::vector<int>& lst, int n, int* a) {
 
 int64_t t_bytes(const addft_sphere_plan_s* p, int64_t nb) {
   size_t cs = p->precision == ADDFT_C64 ? 8 : 16;
-  return nb * (int64_t)p->n3 * p->nst * (int64_t)cs;
+  return nb * (int64_t)p->n3 * p->ldt * (int64_t)cs;
 }
 
 constexpr int kBandsPerGroup = 8;
@@ -1006,7 +1008,7 @@ int launch_hpsi(addft_sphere_plan_s* p, const void* psi, const void* tkin, const
     auto tw1 = reinterpret_cast<const cpx<R>*>(p->d_tw1);
     auto tw2 = reinterpret_cast<const cpx<R>*>(p->d_tw2);
 #define ADDFT_PV(E)                                                                              \
-  hipLaunchKernelGGL((k_plane_v<R, E>), pg, dim3(kPThreads), p->lds_plane, st, T, p->nst, p->nra, \
+  hipLaunchKernelGGL((k_plane_v<R, E>), pg, dim3(kPThreads), p->lds_plane, st, T, p->nst, p->ldt, p->nra, \
                      p->d_srow, p->d_scol, p->d_rowval, p->f1, p->f2, tw1, tw2, vv, p->cchunk)
     switch (p->ept_p) {
       case 4: ADDFT_PV(4); break;
@@ -1039,7 +1041,7 @@ int launch_density(addft_sphere_plan_s* p, const void* psi, const double* w, int
     auto tw1 = reinterpret_cast<const cpx<R>*>(p->d_tw1);
     auto tw2 = reinterpret_cast<const cpx<R>*>(p->d_tw2);
 #define ADDFT_PD(E)                                                                                     \
-  hipLaunchKernelGGL((k_plane_density<R, E>), pg, dim3(kPThreads), p->lds_plane, st, T, p->nst, p->nra, \
+  hipLaunchKernelGGL((k_plane_density<R, E>), pg, dim3(kPThreads), p->lds_plane, st, T, p->nst, p->ldt, p->nra, \
                      p->d_srow, p->d_scol, p->d_rowval, p->f1, p->f2, tw1, tw2, w, nb, kBandsPerGroup,  \
                      p->cchunk, part)
     switch (p->ept_p) {
@@ -1127,6 +1129,7 @@ int addft_sphere_plan_create(int n1, int n2, int n3, int64_t ng, const int32_t* 
   }
   soff.push_back((int32_t)ng);
   p->nst = (int64_t)srow.size();
+  p->ldt = (p->nst + 15) / 16 * 16;
   p->nra = (int)rowval.size();
   p->h_perm = perm;
   // launch shapes
@@ -1373,7 +1376,7 @@ int addft_sphere_to_real(addft_sphere_plan p, const double* c, double scale, dou
   auto tw2 = reinterpret_cast<const cpx<double>*>(p->d_tw2);
 #define ADDFT_PR(E)                                                                                  \
   hipLaunchKernelGGL((k_plane_to_real<double, E>), dim3(p->n3), dim3(kPThreads), p->lds_plane, st, T, \
-                     p->nst, p->nra, p->d_srow, p->d_scol, p->d_rowval, p->f1, p->f2, tw1, tw2, scale, \
+                     p->nst, p->ldt, p->nra, p->d_srow, p->d_scol, p->d_rowval, p->f1, p->f2, tw1, tw2, scale, \
                      p->cchunk, grid)
   switch (p->ept_p) {
     case 4: ADDFT_PR(4); break;
@@ -1396,7 +1399,7 @@ int addft_real_to_sphere(addft_sphere_plan p, const double* grid, double scale, 
   auto tw2 = reinterpret_cast<const cpx<double>*>(p->d_tw2);
 #define ADDFT_RP(E)                                                                                  \
   hipLaunchKernelGGL((k_real_to_plane<double, E>), dim3(p->n3), dim3(kPThreads), p->lds_plane, st, grid, \
-                     p->nst, p->nra, p->d_srow, p->d_scol, p->d_rowval, p->f1, p->f2, tw1, tw2,       \
+                     p->nst, p->ldt, p->nra, p->d_srow, p->d_scol, p->d_rowval, p->f1, p->f2, tw1, tw2,       \
                      p->cchunk, T)
   switch (p->ept_p) {
     case 4: ADDFT_RP(4); break;
