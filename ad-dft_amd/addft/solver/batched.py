@@ -148,12 +148,13 @@ def upper_bound(apply, nk, nG, dtype, device, steps=10, seed=0):
     return (theta + torch.linalg.vector_norm(f, dim=2)[:, 0].double().cpu()).to(device)
 
 
-def chfsi_step(apply, X, degree, b_up, lam, HX=None, filt=None):
+def chfsi_step(apply, X, degree, b_up, lam, HX=None, filt=None, rr=True):
     """One scaled Chebyshev filter (Zhou-Saad) + Rayleigh-Ritz per k.
     b_up (nk,), lam (nk, nbt) ascending; HX = H X when known.  filt(X, HX,
     abc) runs the filter instead (a captured graph, solver/graphs.py), with
     abc[k] = (largest Ritz value, smallest, upper bound).  Returns (lam, X,
-    HX, residual norms)."""
+    HX, residual norms); rr=False returns (lam, orthonormalised block, None,
+    None) as solver/chebyshev.py::chfsi_step."""
     rd = X.real.dtype
     if HX is None:
         HX = apply(X)
@@ -176,6 +177,8 @@ def chfsi_step(apply, X, degree, b_up, lam, HX=None, filt=None):
             Ynew = (apply(Y) - c * Y) * (2.0 * sigma_new / e) - (sigma * sigma_new) * Xm
             Xm, Y, sigma = Y, Ynew, sigma_new
     Y = orth(Y)
+    if not rr:
+        return lam, Y, None, None
     w, X, HX = rayleigh_ritz(Y, apply(Y))
     _, rn = residual_norms(HX, X, w)
     return w, X, HX, rn

@@ -94,13 +94,19 @@ def orthonormalise(Y):
     return Y
 
 
-def chfsi_step(apply, X, degree, b_upper, lam, HX=None):
+def chfsi_step(apply, X, degree, b_upper, lam, HX=None, rr=True):
     """One filter + Rayleigh-Ritz step.  X: (nbt, nG) current Ritz vectors
-    with Ritz values lam (ascending).  Returns (lam, X, HX, residual norms)."""
+    with Ritz values lam (ascending).  Returns (lam, X, HX, residual norms).
+    rr=False stops after the orthonormalisation and returns (lam, Y, None,
+    None): a rotation within the block leaves its span -- all that the next
+    filter acts on -- unchanged, so the passes between the first (which
+    checks convergence) and the last of one SCF map need no eigensolve."""
     a = float(lam[-1])
     a0 = float(lam[0])
     Y = filter_block(apply, X, HX, degree, a, b_upper, a0)
     Y = orthonormalise(Y)
+    if not rr:
+        return lam, Y, None, None
     HY = apply(Y)
     T = inner(Y, HY, conj_a=True).to(torch.complex128)
     T = 0.5 * (T + T.conj().T)

@@ -161,7 +161,8 @@ def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, 
               switch_tol=1e-5, verbose=False, rho0=None, polish=4, dav_max_iter=None, log=None,
               nbuffer=None, eigensolver="auto", cheb_degree=8, refine=True, reset_on_switch=True,
               refine_max_sub=None, cheb_passes=3, batch_k=True, cold_max_sub=None, dav_factor=0.1,
-              refine_dav_factor=None, cold_max_iter=8, polish_gain_max=1.0, psi0=None, graphs=True):
+              refine_dav_factor=None, cold_max_iter=8, polish_gain_max=1.0, psi0=None, graphs=True,
+              cheb_rr_all=True):
     """Run the SCF on engine's current geometry.  Returns (SCFState with
     stick-order psi list in `psi` and plane-major rho, info dict).
     The eigensolve of each map stops at dav_factor x the density residual
@@ -257,8 +258,13 @@ def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, 
             nit, r = 0, float("inf")
             HXb = None
             filt = filter_graph(Xb, vs, fp32) if use_graphs else None
-            for nit in range(1, (41 if tol_k <= 1e-9 else cheb_passes + 1)):
-                lam, Xb, HXb, rn = kbatch.chfsi_step(apply, Xb, cheb_degree, bounds[key], lam, HX=HXb, filt=filt)
+            npass = 41 if tol_k <= 1e-9 else cheb_passes + 1
+            for nit in range(1, npass):
+                rr = cheb_rr_all or npass > cheb_passes + 1 or nit == 1 or nit == npass - 1
+                lam, Xb, HXb, rn = kbatch.chfsi_step(apply, Xb, cheb_degree, bounds[key], lam, HX=HXb, filt=filt,
+                                                     rr=rr)
+                if not rr:
+                    continue
                 r = float((rn[:, :nb] * W[:, :nb]).max())
                 if r < tol_k:
                     break
@@ -294,9 +300,15 @@ def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, 
             lam = ritz[ik]
             wk = weights(ik)
             nit, r = 0, float("inf")
-            HXk = None            # H X is known after the first pass of this map
-            for nit in range(1, (41 if tol_k <= 1e-9 else cheb_passes + 1)):
-                lam, Xk, HXk, rn = chfsi_step(apply, Xk, cheb_degree, bounds[key], lam, HX=HXk)
+            HXk = None            # H X is known after a Rayleigh-Ritz pass of this map
+            npass = 41 if tol_k <= 1e-9 else cheb_passes + 1
+            for nit in range(1, npass):
+                # Rayleigh-Ritz after the first pass (convergence check) and the
+                # last; every pass in the open-ended tight mode
+                rr = cheb_rr_all or npass > cheb_passes + 1 or nit == 1 or nit == npass - 1
+                lam, Xk, HXk, rn = chfsi_step(apply, Xk, cheb_degree, bounds[key], lam, HX=HXk, rr=rr)
+                if not rr:
+                    continue
                 r = float((rn[:nb] * wk[:nb]).max())
                 if r < tol_k:
                     break
