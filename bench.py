@@ -245,8 +245,8 @@ def main():
                          "the FETCH_SIZE/WRITE_SIZE counters); for profiling runs")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(DESIGN.md describes how the path would shard).", file=sys.stderr)
+        print("bench.py measures one GPU only: there is no multi-GPU path (k-points or band blocks "
+              "would shard across GPUs with one all-reduce of the density per SCF step).", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np

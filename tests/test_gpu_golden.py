@@ -2,7 +2,7 @@
 tests/golden/make_golden.py from the NumPy FP64 restatement).
 
 FP64 bar: 1e-8 Ha/atom, 1e-6 Ha/Bohr; mixed bar: 1e-5 Ha/atom, 1e-4 Ha/Bohr
-Stress: 1e-6 (FP64) / 1e-5 (mixed) Ha/Bohr^3.
+(the accuracy targets of the path).  Stress: 1e-6 (FP64) / 1e-5 (mixed) Ha/Bohr^3.
 """
 
 from __future__ import annotations

@@ -1,6 +1,6 @@
 """GPU SCF, energies, forces and stress against the oracle on small systems.
 
-Bars: FP64 path within 1e-8 Ha/atom in energy
+Accuracy bars: FP64 path within 1e-8 Ha/atom in energy
 and 1e-6 Ha/Bohr in forces; mixed path within 1e-5 Ha/atom and 1e-4 Ha/Bohr.
 Stress is compared at 1e-6 Ha/Bohr^3 (FP64).  The oracle is the NumPy
 restatement of the reference (dense eigh SCF, analytic envelope gradients).
