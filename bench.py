@@ -240,6 +240,7 @@ def main():
                     help="central differences (h = 1e-3) of the free energy on 6 seeded force components "
                          "and 2 stress components of the workload, against the analytic forces and stress")
     ap.add_argument("--fd-only", choices=["all", "forces", "strain"], default="all")
+    ap.add_argument("--fd-h", type=float, default=1e-3, help="step of the --fd-check differences")
     ap.add_argument("--hpsi-only", action="store_true",
                     help="only the local-Hpsi roofline measurement (and a 1 GiB device copy that calibrates "
                          "the FETCH_SIZE/WRITE_SIZE counters); for profiling runs")
@@ -264,7 +265,7 @@ def main():
     if args.fd_check:
         tol = args.fp64_tol if args.fd_check == "fp64" else args.mixed_tol
         print(json.dumps(fd_check(addft, testsystems, args.workload, args.fd_check, tol, np, torch,
-                                  only=args.fd_only)), flush=True)
+                                  h=args.fd_h, only=args.fd_only)), flush=True)
         return
 
     if args.hpsi_only:

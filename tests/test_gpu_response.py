@@ -100,3 +100,29 @@ def test_delta_q_grad_si8_matches_rescf_differences():
         fd = (4 * d2 - d1) / 3
         print(f"si8 dC/dR[{a},{c}]: adjoint {g.dR[a, c]:.10e} differences {fd:.10e}", flush=True)
         assert abs(g.dR[a, c] - fd) < 1e-5 * max(abs(fd), 1e-3)
+
+
+def test_delta_q_mixed_state_matches_fp64():
+    """DeltaQ and its adjoint gradient evaluated at the mixed-precision SCF
+    state (FP32/BF16x3 then FP64 refinement, stopped at 1e-8) against the FP64
+    state at 1e-12 on the 8-atom cell: the adjoint itself runs in FP64 in both,
+    so the difference measures the state's error."""
+    import addft
+    from addft import testsystems
+    out = {}
+    for prec, tol in (("fp64", 1e-12), ("mixed", 1e-8)):
+        kw = testsystems.config("c2", precision=prec)
+        s = addft.setup(**kw)
+        st = addft.scf(s, tol=tol)
+        R = np.asarray(kw["frac"]) @ np.asarray(kw["lattice"])
+        w = addft.make_gaussian_weight(s, R[3], 1.2)
+        out[prec] = (addft.delta_q(s, st, w), addft.delta_q_grad(s, st, w))
+    (c64, g64), (c32, g32) = out["fp64"], out["mixed"]
+    dc = abs(c64 - c32)
+    dr = np.abs(g64.dR - g32.dR).max()
+    de = np.abs(g64.eps - g32.eps).max()
+    print(f"DeltaQ mixed vs fp64: |dC| {dc:.2e} (C {c64:.8f})  |d dC/dR| {dr:.2e} (max {np.abs(g64.dR).max():.3e})"
+          f"  |d dC/deps| {de:.2e}", flush=True)
+    assert dc < 1e-6 * max(1.0, abs(c64))
+    assert dr < 1e-5 * max(1.0, np.abs(g64.dR).max())
+    assert de < 1e-5 * max(1.0, np.abs(g64.eps).max())
