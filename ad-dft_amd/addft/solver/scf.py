@@ -159,14 +159,21 @@ def initial_guess(engine, nb, dtype):
 
 def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, beta=0.5,
               switch_tol=1e-5, verbose=False, rho0=None, polish=4, dav_max_iter=None, log=None,
-              nbuffer=None, eigensolver="auto", cheb_degree=8, refine=True, reset_on_switch=True,
-              refine_max_sub=None, cheb_passes=3, batch_k=True, cold_max_sub=None, dav_factor=0.1,
+              nbuffer=None, eigensolver="auto", cheb_degree=None, refine=True, reset_on_switch=True,
+              refine_max_sub=None, cheb_passes=None, batch_k=True, cold_max_sub=None, dav_factor=0.1,
               refine_dav_factor=None, cold_max_iter=8, polish_gain_max=1.0, psi0=None, graphs=True,
-              cheb_rr_all=True):
+              cheb_rr_all=True, refine_max_iter=2):
     """Run the SCF on engine's current geometry.  Returns (SCFState with
     stick-order psi list in `psi` and plane-major rho, info dict).
     The eigensolve of each map stops at dav_factor x the density residual
-    (refine_dav_factor in the FP64 phase of the mixed form).  The first map
+    (refine_dav_factor in the FP64 phase of the mixed form), after at most
+    cheb_passes filters of degree cheb_degree (default: one degree-14 filter
+    and one Rayleigh-Ritz per map, measured fastest for both precisions at
+    216 atoms; three degree-8 passes when all k-points are batched, measured
+    fastest on the 64-k-point cell)
+    or refine_max_iter refinement-Davidson steps (FP64 phase of the mixed
+    form: the FP32 eigenvectors are corrected a little at every map while
+    the density converges, not to the full tolerance at each).  The first map
     starts from plane waves with at most cold_max_iter Davidson iterations in
     a subspace of cold_max_sub (default 2 nbt) vectors: the eigenvectors of
     the trial density need only be rough, the later maps refine them.
@@ -217,6 +224,10 @@ def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, 
     bounds = {}
     # all k-points in one batch (same sphere, too large for the dense path)
     batched = batch_k and engine.nk > 1 and engine.ng > max(DENSE_MAX_NG, 4 * nbt)
+    if cheb_degree is None:
+        cheb_degree = 8 if batched else 14
+    if cheb_passes is None:
+        cheb_passes = 3 if batched else 1
     ritz_b = [None]
 
     # the batched Chebyshev filter as one captured graph per (precision, shape)
@@ -247,7 +258,8 @@ def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, 
         if vs_lo is not None and not cold:
             def apply_lo(p):
                 return engine.hpsi_batch(p.to(torch.complex64), vs_lo, fp32=True).to(torch.complex128)
-            lam, Xb, r, nit = kbatch.davidson(apply, Xb, nbt, diag, max_iter=(dav_max_iter or 25), tol=tol_k,
+            lam, Xb, r, nit = kbatch.davidson(apply, Xb, nbt, diag, max_iter=(refine_max_iter or dav_max_iter or 25),
+                                              tol=tol_k,
                                               nconv=nb, weight=W, apply_lo=apply_lo,
                                               max_sub=(refine_max_sub or 2 * nbt))
         elif use_cheb and not cold and ritz_b[0] is not None:
@@ -288,8 +300,8 @@ def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, 
                 return engine.hpsi(ik, p.to(torch.complex64), vs_lo, fp32=True,
                                    nl_bf16=False).to(torch.complex128)
             diag = engine.tkin64[ik] + vmean
-            lam, Xk, r, nit = davidson(apply, Xk, nbt, diag, max_iter=(dav_max_iter or 25), tol=tol_k,
-                                       nconv=nb, weight=weights(ik), apply_lo=apply_lo,
+            lam, Xk, r, nit = davidson(apply, Xk, nbt, diag, max_iter=(refine_max_iter or dav_max_iter or 25),
+                                       tol=tol_k, nconv=nb, weight=weights(ik), apply_lo=apply_lo,
                                        max_sub=(refine_max_sub or 2 * nbt))
             dav_iters += nit
             return lam, Xk, r, nit
