@@ -6,7 +6,7 @@ residual sum|dn| Omega/N/N_e, mixing that never enters the derivatives, and
 up to four unmixed "polish" steps at the end.  The eigen-solve is always the
 matrix-free Davidson (the reference switches to it above nG = 3000; below,
 its dense eigh gives the same fixed point).  Mixers:
-  * Kerker-preconditioned Pulay (alpha 0.7, gg0 0.5) for more than two
+  * Kerker-preconditioned Pulay (alpha 0.7, q0^2 = pulay_gg0 = 1.0) for more than two
     atoms -- the reference uses the plain Kerker step there (scf.py:574-581),
     which drifts on the metallic 64/216-atom cells; `mixer="kerker"` keeps it;
   * simplified Broyden for two atoms or fewer (scf.py:285-304, 431-436);
@@ -162,7 +162,7 @@ def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, 
               nbuffer=None, eigensolver="auto", cheb_degree=None, refine=True, reset_on_switch=True,
               refine_max_sub=None, cheb_passes=None, batch_k=True, cold_max_sub=None, dav_factor=0.1,
               refine_dav_factor=None, cold_max_iter=8, polish_gain_max=1.0, psi0=None, graphs=True,
-              cheb_rr_all=True, refine_max_iter=2):
+              cheb_rr_all=True, refine_max_iter=2, mix_history=8, pulay_gg0=1.0):
     """Run the SCF on engine's current geometry.  Returns (SCFState with
     stick-order psi list in `psi` and plane-major rho, info dict).
     The eigensolve of each map stops at dav_factor x the density residual
@@ -207,7 +207,7 @@ def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, 
             X[ik] = torch.cat([p, X[ik][p.shape[0]:]], dim=0).contiguous()
     diis = DIIS(8, beta) if mixer == "diis" else None
     broy = Broyden(8) if mixer == "broyden" else None
-    pulay = PulayKerker(engine, 8, alpha, gg0) if mixer == "pulay" else None
+    pulay = PulayKerker(engine, mix_history, alpha, pulay_gg0) if mixer == "pulay" else None
     prev = 1e9
     res = 1.0
     it = 0
