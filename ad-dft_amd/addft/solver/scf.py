@@ -161,7 +161,7 @@ def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, 
               switch_tol=1e-5, verbose=False, rho0=None, polish=4, dav_max_iter=None, log=None,
               nbuffer=None, eigensolver="auto", cheb_degree=None, refine=True, reset_on_switch=True,
               refine_max_sub=None, cheb_passes=None, batch_k=True, cold_max_sub=None, dav_factor=0.1,
-              refine_dav_factor=None, cold_max_iter=8, polish_gain_max=1.0, psi0=None, graphs=True,
+              refine_dav_factor=None, cold_max_iter=4, polish_gain_max=1.0, psi0=None, graphs=True,
               cheb_rr_all=True, refine_max_iter=2, mix_history=8, pulay_gg0=1.0):
     """Run the SCF on engine's current geometry.  Returns (SCFState with
     stick-order psi list in `psi` and plane-major rho, info dict).
@@ -174,7 +174,7 @@ def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, 
     or refine_max_iter refinement-Davidson steps (FP64 phase of the mixed
     form: the FP32 eigenvectors are corrected a little at every map while
     the density converges, not to the full tolerance at each).  The first map
-    starts from plane waves with at most cold_max_iter Davidson iterations in
+    starts from plane waves with at most cold_max_iter (4) Davidson iterations in
     a subspace of cold_max_sub (default 2 nbt) vectors: the eigenvectors of
     the trial density need only be rough, the later maps refine them.
     psi0 (per k, stick order) with rho0 warm-starts the SCF from a nearby
