@@ -29,6 +29,8 @@ from __future__ import annotations
 
 import torch
 
+from . import _lib
+
 
 def real_matrix(P: torch.Tensor) -> torch.Tensor:
     """The real (2 nG, 2 nproj) FP32 matrix W of P (layout in the module
@@ -47,7 +49,7 @@ def real_matrix(P: torch.Tensor) -> torch.Tensor:
 class ProjectorBF16x3:
     """Split-bf16 images of one k-point's projector matrix P (nproj, nG)."""
 
-    def __init__(self, P: torch.Tensor, nslice: int = 32):
+    def __init__(self, P: torch.Tensor, nslice: int = 64):
         npj, ng = P.shape
         self.np, self.ng, self.nslice = int(npj), int(ng), int(nslice)
         dev = P.device
@@ -58,6 +60,7 @@ class ProjectorBF16x3:
         K = 2 * ng
         self.K = K
         self.kc = -(-3 * K // self.nslice)
+        self.kc += self.kc % 2                     # even row length for the split kernel
         rows = self.kc * self.nslice
         W3 = torch.zeros((rows, 2 * npj), dtype=torch.bfloat16, device=dev)
         W3[:K] = hi
@@ -72,15 +75,16 @@ class ProjectorBF16x3:
         self._buf = None
 
     def _split_rows(self, X):
-        """(nb, K) FP32 -> (nb, nslice * kc) bf16 holding [hi | hi | lo | 0]."""
+        """(nb, K) FP32 -> (nb, nslice * kc) bf16 holding [hi | hi | lo | 0]
+        (one pass of addft_split_bf16x3; the zero padding is written once)."""
         nb, K = X.shape
         n = self.kc * self.nslice
         if self._buf is None or self._buf.shape[0] < nb:
             self._buf = torch.zeros((nb, n), dtype=torch.bfloat16, device=X.device)
         B = self._buf[:nb]
-        B[:, :K].copy_(X)
-        B[:, K:2 * K].copy_(B[:, :K])
-        B[:, 2 * K:3 * K].copy_(X - B[:, :K].to(torch.float32))
+        X = X.contiguous()
+        _lib.check(_lib.lib().addft_split_bf16x3(_lib.ptr(X), nb, K, n, _lib.ptr(B), _lib.stream()),
+                   "addft_split_bf16x3")
         return B
 
     def becp_t(self, psi: torch.Tensor) -> torch.Tensor:
@@ -95,9 +99,10 @@ class ProjectorBF16x3:
 
     def apply(self, S: torch.Tensor) -> torch.Tensor:
         """S (nb, nproj) complex64 -> S P (nb, nG) complex64."""
-        Sr = torch.cat([S.real, S.imag], dim=1).to(torch.float32)
-        hi = Sr.to(torch.bfloat16)
-        lo = (Sr - hi.to(torch.float32)).to(torch.bfloat16)
-        Sc = torch.cat([hi, hi, lo], dim=1)
+        nb = S.shape[0]
+        Sr = torch.cat([S.real, S.imag], dim=1).to(torch.float32).contiguous()
+        Sc = torch.empty((nb, 6 * self.np), dtype=torch.bfloat16, device=S.device)
+        _lib.check(_lib.lib().addft_split_bf16x3(_lib.ptr(Sr), nb, 2 * self.np, 6 * self.np, _lib.ptr(Sc),
+                                                 _lib.stream()), "addft_split_bf16x3")
         out = torch.mm(Sc, self.V3, out_dtype=torch.float32)
         return torch.view_as_complex(out.view(S.shape[0], self.ng, 2))

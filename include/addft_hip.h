@@ -66,6 +66,16 @@ int addft_pack_gather(int dtype, const void* d_grid, const int32_t* d_idx,
                       int64_t batch, int64_t ng, int64_t nfft, void* d_coeffs,
                       void* stream);
 
+/* ------------------------------------------------------------ BF16x3 split
+ * FP32 rows x (rows, k) -> bf16 rows [bf16(x) | bf16(x) | bf16(x - bf16(x))]
+ * (rows, ld), round to nearest even; columns [3k, ld) untouched.  The
+ * operand of the split-precision projector GEMMs that replace the FP32
+ * complex products of reference physics/nonlocal_op.py:67-76 (no native
+ * counterpart there: the reference runs those products in XLA).
+ * errors: k odd or negative, ld < 3k or odd.                            */
+int addft_split_bf16x3(const float* d_x, int64_t rows, int64_t k, int64_t ld,
+                       void* d_out, void* stream);
+
 /* ------------------------------------------------------------ LDA XC
  * Per point n = max(rho, 1e-14): v_xc = d(n eps_xc)/dn (Dirac + PZ81),
  * and the raw sum  sum_i n_i eps_xc(n_i)  written to *d_exc (one double),

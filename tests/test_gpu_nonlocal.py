@@ -34,3 +34,24 @@ def test_bf16x3_products_match_complex128(npj, ng, nb, nslice):
     # and clearly better than a single bf16 product
     b1 = (psi.to(torch.complex64).to(torch.complex128) @ P.conj().T)
     assert eb < 1e-3
+
+
+def test_split_bf16x3_matches_torch_rounding():
+    """addft_split_bf16x3 writes [bf16(x) | bf16(x) | bf16(x - bf16(x))]
+    with torch's round-to-nearest-even, bit for bit, and leaves the padding
+    columns untouched."""
+    import torch
+    from addft.kernels import _lib
+    g = torch.Generator(device="cpu").manual_seed(4)
+    rows, k, ld = 37, 1002, 3 * 1002 + 10
+    x = (torch.randn(rows, k, generator=g) * torch.logspace(-6, 6, k)).to(torch.float32).cuda()
+    x[0, :4] = torch.tensor([0.0, -0.0, 1e-40, -3.0e38])
+    out = torch.full((rows, ld), 7.0, dtype=torch.bfloat16, device="cuda")
+    _lib.check(_lib.lib().addft_split_bf16x3(_lib.ptr(x), rows, k, ld, _lib.ptr(out), _lib.stream()))
+    hi = x.to(torch.bfloat16)
+    lo = (x - hi.to(torch.float32)).to(torch.bfloat16)
+    torch.cuda.synchronize()
+    assert torch.equal(out[:, :k].view(torch.int16), hi.view(torch.int16))
+    assert torch.equal(out[:, k:2 * k].view(torch.int16), hi.view(torch.int16))
+    assert torch.equal(out[:, 2 * k:3 * k].view(torch.int16), lo.view(torch.int16))
+    assert bool((out[:, 3 * k:] == 7.0).all())
