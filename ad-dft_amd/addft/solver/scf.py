@@ -162,7 +162,7 @@ def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, 
               nbuffer=None, eigensolver="auto", cheb_degree=None, refine=True, reset_on_switch=True,
               refine_max_sub=None, cheb_passes=None, batch_k=True, cold_max_sub=None, dav_factor=0.1,
               refine_dav_factor=None, cold_max_iter=4, polish_gain_max=1.0, psi0=None, graphs=True,
-              cheb_rr_all=True, refine_max_iter=2, mix_history=8, pulay_gg0=1.0):
+              cheb_rr_all=True, refine_max_iter=2, mix_history=8, pulay_gg0=1.0, refine_nl_bf16=False):
     """Run the SCF on engine's current geometry.  Returns (SCFState with
     stick-order psi list in `psi` and plane-major rho, info dict).
     The eigensolve of each map stops at dav_factor x the density residual
@@ -298,7 +298,7 @@ def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, 
             # Rayleigh-Ritz couples them through the exact H X
             def apply_lo(p):
                 return engine.hpsi(ik, p.to(torch.complex64), vs_lo, fp32=True,
-                                   nl_bf16=False).to(torch.complex128)
+                                   nl_bf16=refine_nl_bf16).to(torch.complex128)
             diag = engine.tkin64[ik] + vmean
             lam, Xk, r, nit = davidson(apply, Xk, nbt, diag, max_iter=(refine_max_iter or dav_max_iter or 25),
                                        tol=tol_k, nconv=nb, weight=weights(ik), apply_lo=apply_lo,

@@ -106,3 +106,18 @@ def tune_plan(plan, nb: int = 32, seed: int = 0):
     choice = decide(key, {"lane": run("lane"), "lds": run("lds")})
     plan.set_plane_kernel(choice if choice else current)
     return plan.plane_kernel
+
+
+# ------------------------------------------------------- block products
+def pretune_inner(K: int, m: int, widths, dtypes, device):
+    """Make the reduction-slice decisions of solver/linalg.py::inner for
+    blocks of m rows over K points, against each width in `widths` and each
+    complex dtype, ahead of a timed run (decisions depend on dtype, K and
+    whether the width reaches 1000, not on the values)."""
+    if not enabled():
+        return
+    from .solver.linalg import inner
+    for dt in dtypes:
+        A = torch.zeros((m, K), dtype=dt, device=device)
+        for n in widths:
+            inner(A, torch.zeros((n, K), dtype=dt, device=device))

