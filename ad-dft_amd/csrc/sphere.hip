@@ -436,9 +436,34 @@ k_z_forward_lane(const cpx<R>* __restrict__ T, int64_t ng, const int32_t* __rest
   const int64_t b = blockIdx.y;
   for (int i = threadIdx.x; i < N; i += blockDim.x) tw[i] = gtw[i];
   const cpx<R>* src = T + (b * N) * ldt + s0;
-  for (int i = threadIdx.x; i < N * cnt; i += blockDim.x) {
-    const int z = i / cnt, j = i - z * cnt;
-    stage[z * SST + j] = src[(int64_t)z * ldt + j];
+  if constexpr (L * G == 64) {
+    if (cnt == SPB) {
+      // full block: N * SPB = M * blockDim elements, M per thread; all loads
+      // are issued before the first LDS store
+      cpx<R> v[M];
+#pragma unroll
+      for (int k = 0; k < M; ++k) {
+        const int i = threadIdx.x + k * (64 * WAVES);
+        const int z = i / SPB, j = i - z * SPB;
+        v[k] = src[(int64_t)z * ldt + j];
+      }
+#pragma unroll
+      for (int k = 0; k < M; ++k) {
+        const int i = threadIdx.x + k * (64 * WAVES);
+        const int z = i / SPB, j = i - z * SPB;
+        stage[z * SST + j] = v[k];
+      }
+    } else {
+      for (int i = threadIdx.x; i < N * cnt; i += blockDim.x) {
+        const int z = i / cnt, j = i - z * cnt;
+        stage[z * SST + j] = src[(int64_t)z * ldt + j];
+      }
+    }
+  } else {
+    for (int i = threadIdx.x; i < N * cnt; i += blockDim.x) {
+      const int z = i / cnt, j = i - z * cnt;
+      stage[z * SST + j] = src[(int64_t)z * ldt + j];
+    }
   }
   __syncthreads();
   cpx<R> twr[M];
@@ -456,19 +481,38 @@ k_z_forward_lane(const cpx<R>* __restrict__ T, int64_t ng, const int32_t* __rest
   if (ok) {
     const int p0 = soff[s0 + sl], len = soff[s0 + sl + 1] - p0, a = a3[s0 + sl];
     cpx<ROUT>* dst = out + b * ng + p0;
+    int pos[M];
 #pragma unroll
-    for (int j = 0; j < M; ++j) {
-      const int pos = addft::lane::cyc_pos(t + L * j, a, len, N);
-      if (pos >= 0) {
-        cpx<R> val = x[j] * scale;
-        if (tkin != nullptr) {
-          const cpx<R> c = psi[b * ng + p0 + pos];
-          const R tk = tkin[(tk_bpk > 0 ? (b / tk_bpk) * ng : 0) + p0 + pos];
-          val.x += tk * c.x;
-          val.y += tk * c.y;
-        }
-        dst[pos] = addft::mk<ROUT>((ROUT)val.x, (ROUT)val.y);
+    for (int j = 0; j < M; ++j) pos[j] = addft::lane::cyc_pos(t + L * j, a, len, N);
+    if (tkin != nullptr) {
+      // kinetic epilogue: every load issued first, from clamped in-stick
+      // positions (a stick holds at least one coefficient), then the stores
+      const cpx<R>* ps = psi + b * ng + p0;
+      const R* tkp = tkin + (tk_bpk > 0 ? (b / tk_bpk) * ng : 0) + p0;
+      cpx<R> c[M];
+      R tk[M];
+#pragma unroll
+      for (int j = 0; j < M; ++j) {
+        const int q = pos[j] < 0 ? 0 : pos[j];
+        c[j] = ps[q];
+        tk[j] = tkp[q];
       }
+#pragma unroll
+      for (int j = 0; j < M; ++j) {
+        if (pos[j] >= 0) {
+          cpx<R> val = x[j] * scale;
+          val.x += tk[j] * c[j].x;
+          val.y += tk[j] * c[j].y;
+          dst[pos[j]] = addft::mk<ROUT>((ROUT)val.x, (ROUT)val.y);
+        }
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < M; ++j)
+        if (pos[j] >= 0) {
+          const cpx<R> val = x[j] * scale;
+          dst[pos[j]] = addft::mk<ROUT>((ROUT)val.x, (ROUT)val.y);
+        }
     }
   }
 }
