@@ -107,3 +107,36 @@ def test_mixed_precision_matches_fp64_oracle():
     print(f"mixed: dF/atom {abs(F - F_ref) / nat:.2e} df {np.max(np.abs(f - f_ref)):.2e}", flush=True)
     assert abs(F - F_ref) / nat < 1e-5
     assert np.max(np.abs(f - f_ref)) < 1e-4
+
+
+@pytest.mark.parametrize("what", ["displaced", "strained"])
+def test_warm_start_reaches_the_cold_fixed_point(what):
+    """solve_scf(rho0=, psi0=) from the solution of a nearby geometry reaches
+    the same free energy as a cold start there; for a strained cell the given
+    density (other volume, other electron count) is renormalised first."""
+    import addft
+    from addft import testsystems
+    from addft.api import System, _solve
+    from addft.geometry.cell import make_cell
+    kw = testsystems.config("c2", precision="fp64")
+    s = addft.setup(**kw)
+    lat = np.asarray(kw["lattice"], dtype=np.float64)
+    frac = np.asarray(kw["frac"], dtype=np.float64)
+    sol = _solve(s, tol=1e-11)
+    if what == "displaced":
+        R = frac @ lat
+        R[2, 1] += 0.01
+        lat2, frac2 = lat, R @ np.linalg.inv(lat)
+    else:
+        lat2, frac2 = lat @ (np.eye(3) + np.diag([0.01, 0.0, -0.005])), frac
+    s2 = System(geo=make_cell(lat2, frac2), static=s.static)
+    warm = _solve(s2, tol=1e-11, rho0=sol.rho, psi0=sol.psi)
+    Fw = warm.engine.energy_parts(warm.psi, warm.occ, warm.rho)["F"]
+    cold = _solve(s2, tol=1e-11)
+    Fc = cold.engine.energy_parts(cold.psi, cold.occ, cold.rho)["F"]
+    q = float(warm.rho.sum()) * warm.engine.omega / warm.engine.nfft
+    print(f"{what}: warm {Fw:.12f} ({warm.info['iterations']} it)  cold {Fc:.12f} ({cold.info['iterations']} it)  "
+          f"N {q:.10f}", flush=True)
+    assert abs(Fw - Fc) < 1e-9
+    assert abs(q - warm.engine.st.nelec) < 1e-8
+    assert warm.info["iterations"] <= cold.info["iterations"]
