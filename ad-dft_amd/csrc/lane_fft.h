@@ -256,7 +256,14 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-template <int N, int L>
+// a^-1 mod m for gcd(a, m) = 1, m > 1
+constexpr int inv_mod_c(int a, int m) {
+  int x = 1;
+  while ((a * x) % m != 1 && x < m) ++x;
+  return x;
+}
+
+template <int N, int L, bool PF = false>
 struct Shape {
   static constexpr int M = N / L;             // registers in the natural distribution
   static constexpr int Q = (M + L - 1) / L;   // k2 values per lane in the split distribution
@@ -264,30 +271,72 @@ struct Shape {
   static constexpr int SLD = L + 1;           // padded row of the exchange tile
   static constexpr int SCR = L * SLD;         // exchange tile per group (complex elements)
   static_assert(N % L == 0, "L must divide N");
+  // PF and gcd(L, M) = 1: the index maps of Good and Thomas (prime-factor
+  // algorithm) make the two stages independent DFTs with no twiddles:
+  // natural n = (M t + L j) mod N for lane t, register j; split element
+  // (k1, k2) at the k with k = k1 mod L and k = k2 mod M (CRT).  Otherwise
+  // Cooley-Tukey: n = t + L j, k = k2 + M k1, twiddles w_N^{t k2} between.
+  // The prime-factor lanes hold elements M apart (not adjacent), so it pays
+  // only where the data come from LDS, not for coalesced global loads.
+  static constexpr bool PFA = PF && L > 1 && M > 1 && gcd_c(L, M) == 1;
+  static constexpr int CA = PFA ? M * inv_mod_c(M % L, L) % N : 0;   // = 1 mod L, 0 mod M
+  static constexpr int CB = PFA ? L * inv_mod_c(L % M, M) % N : 0;   // = 0 mod L, 1 mod M
+  __host__ __device__ static constexpr int nat(int t, int j) {
+    if constexpr (PFA) {
+      const int v = M * t + L * j;   // < 2N
+      return v >= N ? v - N : v;
+    } else {
+      return t + L * j;
+    }
+  }
+  __host__ __device__ static constexpr int spl(int k1, int k2) {
+    return PFA ? (k1 * CA + k2 * CB) % N : k2 + M * k1;
+  }
 };
+
+// A lane's natural positions np[j] = nat(t, j) and split positions
+// sp[q][k1] = spl(k1, min(qL + t, M - 1)), computed once per kernel.
+template <int N, int L, bool PF = false>
+__host__ __device__ __forceinline__ void positions(int t, int* np, int (*sp)[L]) {
+  using S = Shape<N, L, PF>;
+#pragma unroll
+  for (int j = 0; j < S::M; ++j) np[j] = S::nat(t, j);
+#pragma unroll
+  for (int q = 0; q < S::Q; ++q) {
+    const int k2 = q * L + t < S::M ? q * L + t : S::M - 1;
+#pragma unroll
+    for (int k1 = 0; k1 < L; ++k1) sp[q][k1] = S::spl(k1, k2);
+  }
+}
 
 // The steps are separate functions so that a host emulation can run each
 // one for all lanes of a group before the next (the wave_sync points).
 
 // The lane's twiddles w_N^{t k2}, k2 = 0..M-1, from the length-N table
 // (forward sign); kept in registers for every transform of the kernel.
-template <typename R, int N, int L>
+template <typename R, int N, int L, bool PF = false>
 __host__ __device__ __forceinline__ void lane_twiddles(int t, const cpx<R>* tw, cpx<R>* twr) {
   constexpr int M = Shape<N, L>::M;
+  if constexpr (Shape<N, L, PF>::PFA) {
+    return;   // prime-factor lengths need none
+  } else {
 #pragma unroll
-  for (int k2 = 0; k2 < M; ++k2) twr[k2] = tw[(t * k2) % N];
+    for (int k2 = 0; k2 < M; ++k2) twr[k2] = tw[(t * k2) % N];
+  }
 }
 
 // natural -> (in-register DFT_M, twiddle): v[k2] = w_N^{s t k2} sum_j x[t+Lj] w_M^{s j k2}
 // twr: the lane's twiddles (lane_twiddles)
-template <typename R, int N, int L, bool INV>
+template <typename R, int N, int L, bool INV, bool PF = false>
 __host__ __device__ __forceinline__ void fwd_a(cpx<R>* v, int t, const cpx<R>* twr) {
   constexpr int M = Shape<N, L>::M;
   rdft<M, INV>(v);
+  if constexpr (!Shape<N, L, PF>::PFA) {
 #pragma unroll
-  for (int k2 = 1; k2 < M; ++k2) {
-    const cpx<R> w = twr[k2];
-    v[k2] = INV ? cmulc(v[k2], w) : cmul(v[k2], w);
+    for (int k2 = 1; k2 < M; ++k2) {
+      const cpx<R> w = twr[k2];
+      v[k2] = INV ? cmulc(v[k2], w) : cmul(v[k2], w);
+    }
   }
 }
 // round q: lane t writes its v[k2], k2 in [qL, qL+L), to tile row k2-qL, column t
@@ -336,22 +385,24 @@ __host__ __device__ __forceinline__ void bwd_read(cpx<R>* v, int q, int t, const
   }
 }
 // (twiddle, in-register DFT_M) -> natural; twr: the lane's twiddles
-template <typename R, int N, int L, bool INV>
+template <typename R, int N, int L, bool INV, bool PF = false>
 __host__ __device__ __forceinline__ void bwd_b(cpx<R>* v, int t, const cpx<R>* twr) {
   constexpr int M = Shape<N, L>::M;
+  if constexpr (!Shape<N, L, PF>::PFA) {
 #pragma unroll
-  for (int k2 = 1; k2 < M; ++k2) {
-    const cpx<R> w = twr[k2];
-    v[k2] = INV ? cmulc(v[k2], w) : cmul(v[k2], w);
+    for (int k2 = 1; k2 < M; ++k2) {
+      const cpx<R> w = twr[k2];
+      v[k2] = INV ? cmulc(v[k2], w) : cmul(v[k2], w);
+    }
   }
   rdft<M, INV>(v);
 }
 
-// natural v[M] -> split o[Q][L]
-template <typename R, int N, int L, bool INV>
+// natural v[M] -> split o[Q][L]  (PF: prime-factor maps where N allows)
+template <typename R, int N, int L, bool INV, bool PF = false>
 __device__ __forceinline__ void fwd(cpx<R>* v, cpx<R> (*o)[L], int t, cpx<R>* scr, const cpx<R>* tw) {
   constexpr int Q = Shape<N, L>::Q;
-  fwd_a<R, N, L, INV>(v, t, tw);
+  fwd_a<R, N, L, INV, PF>(v, t, tw);
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
     fwd_write<R, N, L>(v, q, t, scr);
@@ -361,7 +412,7 @@ __device__ __forceinline__ void fwd(cpx<R>* v, cpx<R> (*o)[L], int t, cpx<R>* sc
   }
 }
 // split o[Q][L] -> natural v[M]
-template <typename R, int N, int L, bool INV>
+template <typename R, int N, int L, bool INV, bool PF = false>
 __device__ __forceinline__ void bwd(cpx<R> (*o)[L], cpx<R>* v, int t, cpx<R>* scr, const cpx<R>* tw) {
   constexpr int Q = Shape<N, L>::Q;
   bwd_a<R, N, L, INV>(o, t);
@@ -372,7 +423,7 @@ __device__ __forceinline__ void bwd(cpx<R> (*o)[L], cpx<R>* v, int t, cpx<R>* sc
     bwd_read<R, N, L>(v, q, t, scr);
     wave_sync();
   }
-  bwd_b<R, N, L, INV>(v, t, tw);
+  bwd_b<R, N, L, INV, PF>(v, t, tw);
 }
 
 // Position of index i in a cyclic interval [a, a+len) mod n listed in

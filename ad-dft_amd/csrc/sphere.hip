@@ -260,12 +260,16 @@ k_plane_v_lane(cpx<R>* __restrict__ T, int64_t ldt, int nra, int a1, const int4*
   __syncthreads();
   cpx<R> twr[M];
   addft::lane::lane_twiddles<R, N, L>(t, tw, twr);
-  // column pass: where the lane's elements i1 = t + L j sit in P (zero row /
-  // scratch row where i1 carries no stick row)
+  // column pass (data from LDS only): prime-factor maps where N allows, no
+  // twiddles.  Where the lane's elements i1 = nat(t, j) sit in P (zero row /
+  // scratch row where i1 carries no stick row), and the rows spl(k1, k2) of
+  // its split elements (for V).
+  int cnp[M], csp[Q][L];
+  addft::lane::positions<N, L, true>(t, cnp, csp);
   int cld[M], cst[M];
 #pragma unroll
   for (int j = 0; j < M; ++j) {
-    const int rr = addft::lane::cyc_pos(t + L * j, a1, nra, N);
+    const int rr = addft::lane::cyc_pos(cnp[j], a1, nra, N);
     cld[j] = rr >= 0 ? rr * N : zoff;
     cst[j] = rr >= 0 ? rr * N : toff;
   }
@@ -305,21 +309,19 @@ k_plane_v_lane(cpx<R>* __restrict__ T, int64_t ldt, int nra, int a1, const int4*
     // V(i1, y) for the lane's split elements, fetched ahead of the transform
     R vv[Q][L];
 #pragma unroll
-    for (int q = 0; q < Q; ++q) {
-      const int k2 = min(q * L + t, M - 1);
+    for (int q = 0; q < Q; ++q)
 #pragma unroll
-      for (int k1 = 0; k1 < L; ++k1) vv[q][k1] = vp[(k2 + M * k1) * N + yy];
-    }
+      for (int k1 = 0; k1 < L; ++k1) vv[q][k1] = vp[csp[q][k1] * N + yy];
     cpx<R> x[M];
 #pragma unroll
     for (int j = 0; j < M; ++j) x[j] = P[cld[j] + yy];
     cpx<R> o[Q][L];
-    addft::lane::fwd<R, N, L, true>(x, o, t, scr, twr);
+    addft::lane::fwd<R, N, L, true, true>(x, o, t, scr, twr);
 #pragma unroll
     for (int q = 0; q < Q; ++q)
 #pragma unroll
       for (int k1 = 0; k1 < L; ++k1) o[q][k1] = o[q][k1] * vv[q][k1];
-    addft::lane::bwd<R, N, L, false>(o, x, t, scr, twr);
+    addft::lane::bwd<R, N, L, false, true>(o, x, t, scr, twr);
     if (ok) {
 #pragma unroll
       for (int j = 0; j < M; ++j) P[cst[j] + y] = x[j];
@@ -548,10 +550,13 @@ k_plane_density_lane(const cpx<R>* __restrict__ T, int64_t ldt, int nra, int a1,
   __syncthreads();
   cpx<R> twr[M];
   addft::lane::lane_twiddles<R, N, L>(t, tw, twr);
+  // column transforms read LDS only: prime-factor maps where N allows
+  int cnp[M], csp[Q][L];
+  addft::lane::positions<N, L, true>(t, cnp, csp);
   int cld[M];
 #pragma unroll
   for (int j = 0; j < M; ++j) {
-    const int rr = addft::lane::cyc_pos(t + L * j, a1, nra, N);
+    const int rr = addft::lane::cyc_pos(cnp[j], a1, nra, N);
     cld[j] = rr >= 0 ? rr * N : zoff;
   }
   double acc[CT][Q][L];
@@ -597,7 +602,7 @@ k_plane_density_lane(const cpx<R>* __restrict__ T, int64_t ldt, int nra, int a1,
 #pragma unroll
       for (int j = 0; j < M; ++j) x[j] = P[cld[j] + yy];
       cpx<R> o[Q][L];
-      addft::lane::fwd<R, N, L, true>(x, o, t, scr, twr);
+      addft::lane::fwd<R, N, L, true, true>(x, o, t, scr, twr);
 #pragma unroll
       for (int q = 0; q < Q; ++q)
 #pragma unroll
@@ -614,7 +619,7 @@ k_plane_density_lane(const cpx<R>* __restrict__ T, int64_t ldt, int nra, int a1,
       for (int q = 0; q < Q; ++q)
         if (q * L + t < M) {
 #pragma unroll
-          for (int k1 = 0; k1 < L; ++k1) out[(int64_t)(q * L + t + M * k1) * N + y] = acc[c][q][k1];
+          for (int k1 = 0; k1 < L; ++k1) out[(int64_t)csp[q][k1] * N + y] = acc[c][q][k1];
         }
     }
   }
