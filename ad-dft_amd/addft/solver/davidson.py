@@ -134,7 +134,7 @@ def dense_lowest(apply, nG, nb, dtype, device):
 
 
 def davidson(apply, X0, nb, diag, max_sub=None, max_iter=40, tol=1e-9, min_iter=1, nconv=None, weight=None,
-             apply_lo=None):
+             apply_lo=None, first_rr=True):
     """Return (eig (nb,) float64, X (nb, nG), max residual norm, iterations).
 
     apply: (m, nG) -> (m, nG) in X0's dtype.  diag: (nG,) float64.
@@ -148,7 +148,11 @@ def davidson(apply, X0, nb, diag, max_sub=None, max_iter=40, tol=1e-9, min_iter=
     formed from the exact H Q as (H Q)^H R, so the FP32 error of H R enters
     only the small <R|H|R> block and the R-components of the new H X (both
     proportional to the current residual): the residuals, and so the
-    converged eigenpairs, keep the accuracy of `apply`."""
+    converged eigenpairs, keep the accuracy of `apply`.
+    first_rr=False: the first residuals use the Rayleigh quotients of X0
+    itself (no eigensolve of X0^H H X0): for X0 already close to
+    eigenvectors (the previous SCF map's) the first Rayleigh-Ritz then runs
+    on the expanded block, which also rotates X0."""
     nconv = nb if nconv is None else int(nconv)
     nG = X0.shape[1]
     if nG <= max(DENSE_MAX_NG, 4 * nb):
@@ -169,11 +173,15 @@ def davidson(apply, X0, nb, diag, max_sub=None, max_iter=40, tol=1e-9, min_iter=
     lam = None
     it = 0
     for it in range(1, max_iter + 1):
-        w, U = eigh(T)
-        lam = w[:nb]
-        Ub = U[:, :nb].to(Q.dtype)
-        X = torch.matmul(Ub.T, Q)
-        HX = torch.matmul(Ub.T, HQ)
+        if it == 1 and not first_rr and Q.shape[0] == nb:
+            lam = T.diagonal().real.clone()
+            X, HX = Q, HQ
+        else:
+            w, U = eigh(T)
+            lam = w[:nb]
+            Ub = U[:, :nb].to(Q.dtype)
+            X = torch.matmul(Ub.T, Q)
+            HX = torch.matmul(Ub.T, HQ)
         R = HX - lam.to(X.real.dtype)[:, None] * X
         rn = torch.linalg.vector_norm(R.to(torch.complex128), dim=1)
         rw = rn if weight is None else rn * weight.to(rn.dtype)

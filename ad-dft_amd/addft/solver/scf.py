@@ -162,7 +162,8 @@ def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, 
               nbuffer=None, eigensolver="auto", cheb_degree=None, refine=True, reset_on_switch=True,
               refine_max_sub=None, cheb_passes=None, batch_k=True, cold_max_sub=None, dav_factor=0.1,
               refine_dav_factor=None, cold_max_iter=4, polish_gain_max=1.0, psi0=None, graphs=True,
-              cheb_rr_all=True, refine_max_iter=2, mix_history=8, pulay_gg0=1.0, refine_nl_bf16=False):
+              cheb_rr_all=True, refine_max_iter=2, mix_history=8, pulay_gg0=1.0, refine_nl_bf16=False,
+              refine_first_rr=False):
     """Run the SCF on engine's current geometry.  Returns (SCFState with
     stick-order psi list in `psi` and plane-major rho, info dict).
     The eigensolve of each map stops at dav_factor x the density residual
@@ -302,7 +303,7 @@ def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, 
             diag = engine.tkin64[ik] + vmean
             lam, Xk, r, nit = davidson(apply, Xk, nbt, diag, max_iter=(refine_max_iter or dav_max_iter or 25),
                                        tol=tol_k, nconv=nb, weight=weights(ik), apply_lo=apply_lo,
-                                       max_sub=(refine_max_sub or 2 * nbt))
+                                       max_sub=(refine_max_sub or 2 * nbt), first_rr=refine_first_rr)
             dav_iters += nit
             return lam, Xk, r, nit
         if use_cheb and not cold and ritz[ik] is not None:
