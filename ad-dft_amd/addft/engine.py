@@ -258,7 +258,7 @@ class Engine:
         out = plan.hpsi_local(psi, tk, vs)
         if fp32 and nl_bf16 and self.PB[ik] is not None:
             pb = self.PB[ik]
-            out.add_(pb.apply(torch.matmul(pb.becp_t(psi), D)))
+            pb.apply_add(torch.matmul(pb.becp_t(psi), D), out)
             return out
         becp_t = inner(psi, P)                          # (nb, nproj) = <beta|psi>^T
         out.add_(torch.matmul(torch.matmul(becp_t, D), P))

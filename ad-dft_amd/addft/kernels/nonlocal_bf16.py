@@ -97,12 +97,22 @@ class ProjectorBF16x3:
         C = part.sum(dim=0)
         return torch.complex(C[:, :self.np], C[:, self.np:])
 
-    def apply(self, S: torch.Tensor) -> torch.Tensor:
-        """S (nb, nproj) complex64 -> S P (nb, nG) complex64."""
+    def _split_s(self, S: torch.Tensor) -> torch.Tensor:
         nb = S.shape[0]
         Sr = torch.cat([S.real, S.imag], dim=1).to(torch.float32).contiguous()
         Sc = torch.empty((nb, 6 * self.np), dtype=torch.bfloat16, device=S.device)
         _lib.check(_lib.lib().addft_split_bf16x3(_lib.ptr(Sr), nb, 2 * self.np, 6 * self.np, _lib.ptr(Sc),
                                                  _lib.stream()), "addft_split_bf16x3")
-        out = torch.mm(Sc, self.V3, out_dtype=torch.float32)
+        return Sc
+
+    def apply_add(self, S: torch.Tensor, out: torch.Tensor) -> torch.Tensor:
+        """out (nb, nG) complex64 += S P, accumulated by the GEMM itself
+        (beta = 1 on the FP32 view of out: no separate product buffer)."""
+        o = torch.view_as_real(out).view(S.shape[0], 2 * self.ng)
+        torch.addmm(o, self._split_s(S), self.V3, out_dtype=torch.float32, out=o)
+        return out
+
+    def apply(self, S: torch.Tensor) -> torch.Tensor:
+        """S (nb, nproj) complex64 -> S P (nb, nG) complex64."""
+        out = torch.mm(self._split_s(S), self.V3, out_dtype=torch.float32)
         return torch.view_as_complex(out.view(S.shape[0], self.ng, 2))

@@ -55,3 +55,19 @@ def test_split_bf16x3_matches_torch_rounding():
     assert torch.equal(out[:, k:2 * k].view(torch.int16), hi.view(torch.int16))
     assert torch.equal(out[:, 2 * k:3 * k].view(torch.int16), lo.view(torch.int16))
     assert bool((out[:, 3 * k:] == 7.0).all())
+
+
+def test_apply_add_accumulates_into_output():
+    """ProjectorBF16x3.apply_add(S, out) equals out + apply(S)."""
+    import torch
+    from addft.kernels.nonlocal_bf16 import ProjectorBF16x3
+    g = torch.Generator(device="cpu").manual_seed(6)
+    npj, ng, nb = 40, 3001, 17
+    P = torch.complex(torch.randn(npj, ng, generator=g), torch.randn(npj, ng, generator=g)).cuda()
+    S = torch.complex(torch.randn(nb, npj, generator=g), torch.randn(nb, npj, generator=g)).cuda()
+    base = torch.complex(torch.randn(nb, ng, generator=g), torch.randn(nb, ng, generator=g)).cuda()
+    pb = ProjectorBF16x3(P)
+    ref = base + pb.apply(S)
+    got = pb.apply_add(S, base.clone())
+    torch.cuda.synchronize()
+    assert ((got - ref).abs().max() / ref.abs().max()).item() < 1e-6
