@@ -261,7 +261,7 @@ class Engine:
             pb.apply_add(torch.matmul(pb.becp_t(psi), D), out)
             return out
         becp_t = inner(psi, P)                          # (nb, nproj) = <beta|psi>^T
-        out.add_(torch.matmul(torch.matmul(becp_t, D), P))
+        torch.addmm(out, torch.matmul(becp_t, D), P, out=out)   # accumulated by the GEMM (beta = 1)
         return out
 
     def hpsi_batch(self, psi3, vs, fp32=False):
@@ -277,7 +277,7 @@ class Engine:
         self.hpsi_bands["fp32" if fp32 else "fp64"] += int(nk * m)
         out = plan.hpsi_local_kbatch(psi3.reshape(nk * m, ng), tk, m, vs).view(nk, m, ng)
         becp_t = torch.bmm(psi3, PH)                    # (nk, m, nproj)
-        out.add_(torch.bmm(torch.matmul(becp_t, D), P))
+        torch.baddbmm(out, torch.matmul(becp_t, D), P, out=out)
         return out
 
     def density_batch(self, psi3, occ, fp32=False):
