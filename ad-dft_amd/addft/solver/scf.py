@@ -161,7 +161,7 @@ def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, 
               switch_tol=1e-5, verbose=False, rho0=None, polish=4, dav_max_iter=None, log=None,
               nbuffer=None, eigensolver="auto", cheb_degree=None, refine=True, reset_on_switch=True,
               refine_max_sub=None, cheb_passes=None, batch_k=True, cold_max_sub=None, dav_factor=0.1,
-              refine_dav_factor=None, cold_max_iter=4, polish_gain_max=1.0, psi0=None, graphs=True,
+              refine_dav_factor=None, cold_max_iter=4, polish_gain_max=0.3, psi0=None, graphs=True,
               cheb_rr_all=True, refine_max_iter=2, mix_history=8, pulay_gg0=1.0, refine_nl_bf16=False,
               refine_first_rr=False):
     """Run the SCF on engine's current geometry.  Returns (SCFState with
@@ -455,9 +455,13 @@ def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, 
     # kept is the evaluation with the smallest residual.  Where the map
     # contracts this is exactly the reference's sequence.
     # Gain of the unmixed map along the last mixed step,
-    # |F(n_k) - F(n_k-1)| / |n_k - n_k-1|: above 1 an unmixed step is
-    # expected to grow the residual (and would be discarded by the rule
-    # above), so only the eigenvectors of H[rho] are computed.
+    # |F(n_k) - F(n_k-1)| / |n_k - n_k-1|: above polish_gain_max (0.3) an
+    # unmixed step is expected to grow the residual (and would be discarded
+    # by the rule above), so only the eigenvectors of H[rho] are computed.
+    # The estimate runs low on charge-sloshing cells (the Kerker-filtered
+    # mixed steps avoid the long wavelengths that the unmixed map amplifies:
+    # at 216 atoms a gain of 0.3-0.8 came with unmixed steps that grew the
+    # residual), hence the margin below 1.
     gain = None
     if len(pairs) == 2:
         dn = float(torch.linalg.vector_norm(pairs[1][0] - pairs[0][0]))
