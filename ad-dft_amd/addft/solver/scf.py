@@ -6,7 +6,7 @@ residual sum|dn| Omega/N/N_e, mixing that never enters the derivatives, and
 up to four unmixed "polish" steps at the end.  The eigen-solve is always the
 matrix-free Davidson (the reference switches to it above nG = 3000; below,
 its dense eigh gives the same fixed point).  Mixers:
-  * Kerker-preconditioned Pulay (alpha 0.7, q0^2 = pulay_gg0 = 1.0) for more than two
+  * Kerker-preconditioned Pulay (alpha 0.7, q0^2 = pulay_gg0: 1.0 from 32 atoms, else 0.5) for more than two
     atoms -- the reference uses the plain Kerker step there (scf.py:574-581),
     which drifts on the metallic 64/216-atom cells; `mixer="kerker"` keeps it;
   * simplified Broyden for two atoms or fewer (scf.py:285-304, 431-436);
@@ -161,8 +161,8 @@ def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, 
               switch_tol=1e-5, verbose=False, rho0=None, polish=4, dav_max_iter=None, log=None,
               nbuffer=None, eigensolver="auto", cheb_degree=None, refine=True, reset_on_switch=True,
               refine_max_sub=None, cheb_passes=None, batch_k=True, cold_max_sub=None, dav_factor=0.1,
-              refine_dav_factor=None, cold_max_iter=4, polish_gain_max=0.3, psi0=None, graphs=True,
-              cheb_rr_all=True, refine_max_iter=2, mix_history=8, pulay_gg0=1.0, refine_nl_bf16=False,
+              refine_dav_factor=None, cold_max_iter=4, polish_gain_max=None, psi0=None, graphs=True,
+              cheb_rr_all=True, refine_max_iter=2, mix_history=8, pulay_gg0=None, refine_nl_bf16=False,
               refine_first_rr=False):
     """Run the SCF on engine's current geometry.  Returns (SCFState with
     stick-order psi list in `psi` and plane-major rho, info dict).
@@ -208,6 +208,16 @@ def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, 
             X[ik] = torch.cat([p, X[ik][p.shape[0]:]], dim=0).contiguous()
     diis = DIIS(8, beta) if mixer == "diis" else None
     broy = Broyden(8) if mixer == "broyden" else None
+    if polish_gain_max is None:
+        # unmixed polish steps: skipped above an estimated gain of 0.3 on the
+        # large (charge-sloshing) cells, where the estimate runs low; the small
+        # cells keep the reference's steps while the map contracts
+        polish_gain_max = 0.3 if engine.nat >= 32 else 1.0
+    if pulay_gg0 is None:
+        # Kerker q0^2 of the Pulay mixer: 1.0 on the large (charge-sloshing)
+        # cells, 0.5 on small ones (measured: 64 and 216 atoms converge in
+        # fewer maps with 1.0, the 8-atom k-point cell with 0.5)
+        pulay_gg0 = 1.0 if engine.nat >= 32 else 0.5
     pulay = PulayKerker(engine, mix_history, alpha, pulay_gg0) if mixer == "pulay" else None
     prev = 1e9
     res = 1.0
@@ -455,7 +465,7 @@ def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, 
     # kept is the evaluation with the smallest residual.  Where the map
     # contracts this is exactly the reference's sequence.
     # Gain of the unmixed map along the last mixed step,
-    # |F(n_k) - F(n_k-1)| / |n_k - n_k-1|: above polish_gain_max (0.3) an
+    # |F(n_k) - F(n_k-1)| / |n_k - n_k-1|: above polish_gain_max an
     # unmixed step is expected to grow the residual (and would be discarded
     # by the rule above), so only the eigenvectors of H[rho] are computed.
     # The estimate runs low on charge-sloshing cells (the Kerker-filtered
