@@ -94,13 +94,15 @@ def orthonormalise(Y):
     return Y
 
 
-def chfsi_step(apply, X, degree, b_upper, lam, HX=None, rr=True):
+def chfsi_step(apply, X, degree, b_upper, lam, HX=None, rr=True, residuals=True):
     """One filter + Rayleigh-Ritz step.  X: (nbt, nG) current Ritz vectors
     with Ritz values lam (ascending).  Returns (lam, X, HX, residual norms).
     rr=False stops after the orthonormalisation and returns (lam, Y, None,
     None): a rotation within the block leaves its span -- all that the next
     filter acts on -- unchanged, so the passes between the first (which
-    checks convergence) and the last of one SCF map need no eigensolve."""
+    checks convergence) and the last of one SCF map need no eigensolve.
+    residuals=False (a single pass whose convergence is not tested) skips
+    the rotation of H Y and the residual norms: returns (lam, X, None, None)."""
     a = float(lam[-1])
     a0 = float(lam[0])
     Y = filter_block(apply, X, HX, degree, a, b_upper, a0)
@@ -113,6 +115,8 @@ def chfsi_step(apply, X, degree, b_upper, lam, HX=None, rr=True):
     w, U = eigh(T)
     Ut = U.T.to(Y.dtype)
     X = torch.matmul(Ut, Y)
+    if not residuals:
+        return w, X.contiguous(), None, None
     HX = torch.matmul(Ut, HY)
     R = HX - w.to(X.real.dtype)[:, None] * X
     rn = torch.linalg.vector_norm(R.to(torch.complex128), dim=1)

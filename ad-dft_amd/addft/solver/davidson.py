@@ -134,7 +134,7 @@ def dense_lowest(apply, nG, nb, dtype, device):
 
 
 def davidson(apply, X0, nb, diag, max_sub=None, max_iter=40, tol=1e-9, min_iter=1, nconv=None, weight=None,
-             apply_lo=None, first_rr=True):
+             apply_lo=None, first_rr=True, orth_x0=True):
     """Return (eig (nb,) float64, X (nb, nG), max residual norm, iterations).
 
     apply: (m, nG) -> (m, nG) in X0's dtype.  diag: (nG,) float64.
@@ -160,7 +160,8 @@ def davidson(apply, X0, nb, diag, max_sub=None, max_iter=40, tol=1e-9, min_iter=
     if max_sub is None:
         max_sub = max(48, 4 * nb)
     max_sub = min(max_sub, nG // 2)
-    X = _orth(X0.contiguous())
+    # orth_x0=False: X0 is known orthonormal (the previous call's Ritz vectors)
+    X = _orth(X0.contiguous()) if orth_x0 else X0.contiguous()
     if X.shape[0] < nb:
         raise ValueError("initial block is rank deficient")
     HX = apply(X)

@@ -232,6 +232,7 @@ def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, 
     use_cheb = eigensolver == "chebyshev" or (eigensolver == "auto" and nbt >= 32
                                                and engine.ng > max(DENSE_MAX_NG, 4 * nbt))
     ritz = [None] * engine.nk
+    refined = [False] * engine.nk     # a refinement map has produced orthonormal FP64 vectors
     bounds = {}
     # all k-points in one batch (same sphere, too large for the dense path)
     batched = batch_k and engine.nk > 1 and engine.ng > max(DENSE_MAX_NG, 4 * nbt)
@@ -313,7 +314,9 @@ def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, 
             diag = engine.tkin64[ik] + vmean
             lam, Xk, r, nit = davidson(apply, Xk, nbt, diag, max_iter=(refine_max_iter or dav_max_iter or 25),
                                        tol=tol_k, nconv=nb, weight=weights(ik), apply_lo=apply_lo,
-                                       max_sub=(refine_max_sub or 2 * nbt), first_rr=refine_first_rr)
+                                       max_sub=(refine_max_sub or 2 * nbt), first_rr=refine_first_rr,
+                                       orth_x0=not refined[ik])
+            refined[ik] = True
             dav_iters += nit
             return lam, Xk, r, nit
         if use_cheb and not cold and ritz[ik] is not None:
@@ -329,9 +332,14 @@ def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, 
                 # Rayleigh-Ritz after the first pass (convergence check) and the
                 # last; every pass in the open-ended tight mode
                 rr = cheb_rr_all or npass > cheb_passes + 1 or nit == 1 or nit == npass - 1
-                lam, Xk, HXk, rn = chfsi_step(apply, Xk, cheb_degree, bounds[key], lam, HX=HXk, rr=rr)
+                need = npass > 2          # residuals decide an early exit only with several passes
+                lam, Xk, HXk, rn = chfsi_step(apply, Xk, cheb_degree, bounds[key], lam, HX=HXk, rr=rr,
+                                              residuals=need)
                 if not rr:
                     continue
+                if not need:
+                    r = float("nan")
+                    break
                 r = float((rn[:nb] * wk[:nb]).max())
                 if r < tol_k:
                     break
