@@ -166,22 +166,35 @@ def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, 
               refine_first_rr=False):
     """Run the SCF on engine's current geometry.  Returns (SCFState with
     stick-order psi list in `psi` and plane-major rho, info dict).
-    The eigensolve of each map stops at dav_factor x the density residual
-    (refine_dav_factor in the FP64 phase of the mixed form), after at most
-    cheb_passes filters of degree cheb_degree (default: one degree-14 filter
-    and one Rayleigh-Ritz per map, measured fastest for both precisions at
-    216 atoms; three degree-8 passes when all k-points are batched, measured
-    fastest on the 64-k-point cell)
-    or refine_max_iter refinement-Davidson steps (FP64 phase of the mixed
-    form: the FP32 eigenvectors are corrected a little at every map while
-    the density converges, not to the full tolerance at each).  The first map
-    starts from plane waves with at most cold_max_iter (4) Davidson iterations in
-    a subspace of cold_max_sub (default 2 nbt) vectors: the eigenvectors of
-    the trial density need only be rough, the later maps refine them.
-    psi0 (per k, stick order) with rho0 warm-starts the SCF from a nearby
-    solution (e.g. a displaced geometry): no cold start; rows beyond psi0's
-    are filled with plane waves.  graphs: with several k-points batched,
-    replay the Chebyshev filter as a captured graph (solver/graphs.py)."""
+
+    Convergence: density residual sum|dn| Omega/N/N_e below tol in an FP64 map.
+    Mixing: mixer ("auto": Kerker-preconditioned Pulay above two atoms,
+      Broyden otherwise; "diis", "broyden", "kerker"), alpha, gg0 (plain
+      Kerker), pulay_gg0 (None: 1.0 from 32 atoms, else 0.5), mix_history,
+      beta (DIIS).
+    Eigensolve per map, to dav_factor x the density residual
+      (refine_dav_factor in the FP64 phase of the mixed form):
+      * first map from plane waves: at most cold_max_iter Davidson
+        iterations in cold_max_sub (None: 2 nbt) vectors -- rough vectors
+        suffice, the later maps refine them;
+      * then Chebyshev-filtered subspace iteration: cheb_passes filters of
+        degree cheb_degree (None: one degree-14 filter per map; three
+        degree-8 passes when k-points are batched), with a Rayleigh-Ritz
+        after every pass (cheb_rr_all) or only the first and last;
+      * FP64 phase of the mixed form (refine): at most refine_max_iter
+        refinement-Davidson steps in refine_max_sub vectors with FP32
+        correction products (BF16x3 ones with refine_nl_bf16), the first
+        residuals from Rayleigh quotients unless refine_first_rr.
+    Mixed precision: FP32 maps until the residual reaches switch_tol (or
+      stalls), then FP64; reset_on_switch clears the mixer history.
+    Polish: up to `polish` unmixed map steps while they reduce the residual,
+      skipped when the estimated map gain exceeds polish_gain_max (None: 0.3
+      from 32 atoms, else 1.0).
+    Warm start: rho0 (renormalised to the electron count) and psi0 (per k,
+      stick order; missing rows filled with plane waves) replace the cold map.
+    Batching: batch_k (all k-points as one block on the shared sphere) and
+      graphs (replay the batched Chebyshev filter as a captured graph,
+      solver/graphs.py)."""
     st = engine.st
     nb = st.nband
     dev = engine.dev
