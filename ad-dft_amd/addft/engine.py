@@ -224,6 +224,24 @@ class Engine:
         self.ewald_cache = None
         self.geo = cell
 
+    def atomic_density(self, sigma=1.2):
+        """A starting density: one Gaussian valence charge Z_val
+        exp(-|r - R_I|^2 / 2 sigma^2) / (2 pi sigma^2)^(3/2) per atom, summed
+        on the density sphere (rho(G) = Z_val e^{-sigma^2 G^2 / 2} / Omega
+        sum_I e^{-iG.R_I}, the structure-factor kernel of V_loc) and
+        normalised to the electron count; plane-major.  Positive wherever
+        the Gaussians are resolved (their transform is negligible at the
+        sphere's edge)."""
+        ngr = self.st.basis.nG_rho
+        q2 = self.grho_d.pow(2).sum(dim=1)
+        tab = (self.zval * torch.exp(-0.5 * sigma * sigma * q2)).contiguous()
+        g = torch.empty(ngr, dtype=torch.complex128, device=self.dev)
+        check(lib().addft_vloc_g(ngr, self.nat, ptr(self.grho_d), ptr(tab), ptr(self.pos_d), 1.0 / self.omega,
+                                 ptr(g), stream()), "addft_vloc_g")
+        rho = self.plan_rho.to_real(g, 1.0)
+        rho = torch.clamp(rho, min=1e-10)
+        return rho * (self.st.nelec / (float(rho.sum()) * self.omega / self.nfft))
+
     # ---------------------------------------------------- potentials
     def hartree(self, rho_pl, want_strain=False):
         """(v_H planar, E_H, strain sums) on the psi-sphere (hartree.py:21-48)."""

@@ -163,7 +163,7 @@ def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, 
               refine_max_sub=None, cheb_passes=None, batch_k=True, cold_max_sub=None, dav_factor=0.1,
               refine_dav_factor=None, cold_max_iter=4, polish_gain_max=None, psi0=None, graphs=True,
               cheb_rr_all=True, refine_max_iter=2, mix_history=8, pulay_gg0=None, refine_nl_bf16=False,
-              refine_first_rr=False):
+              refine_first_rr=False, rho_init="uniform", rho_sigma=1.2):
     """Run the SCF on engine's current geometry.  Returns (SCFState with
     stick-order psi list in `psi` and plane-major rho, info dict).
 
@@ -190,6 +190,8 @@ def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, 
     Polish: up to `polish` unmixed map steps while they reduce the residual,
       skipped when the estimated map gain exceeds polish_gain_max (None: 0.3
       from 32 atoms, else 1.0).
+    Start: rho_init "uniform" (the reference's N/Omega) or "atomic" (Gaussian
+      valence charges of width rho_sigma per atom, Engine.atomic_density).
     Warm start: rho0 (renormalised to the electron count) and psi0 (per k,
       stick order; missing rows filled with plane waves) replace the cold map.
     Batching: batch_k (all k-points as one block on the shared sphere) and
@@ -202,7 +204,9 @@ def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, 
         mixer = "pulay" if engine.nat > 2 else "broyden"
     fp32 = engine.mixed
     cdt = torch.complex64 if fp32 else torch.complex128
-    if rho0 is None:
+    if rho0 is None and rho_init == "atomic":
+        rho = engine.atomic_density(rho_sigma)
+    elif rho0 is None:
         rho = torch.full((engine.nfft,), st.nelec / engine.omega, dtype=torch.float64, device=dev)
     else:
         # a density from another cell (a strained geometry) holds a different
