@@ -107,6 +107,31 @@ def measured_traffic():
     return float(m.group(1)) if m else None
 
 
+def nonlocal_measure(addft, kw, steps, torch):
+    """HIP-event times of the FP32-phase projector products at the workload's
+    size: <beta|psi> (split-K batched GEMM over 6 nG) and the accumulation
+    of S P into an Hpsi block, each three bf16 products with FP32
+    accumulation.  Algorithmic flops per product: 2 m (6 nG)(2 nproj)."""
+    from addft.solver.scf import buffer_bands
+    system = addft.setup(**kw)
+    eng = addft.api._engine(system.static, system.geo)
+    pb = eng.PB[0]
+    m = system.static.nband + buffer_bands(system.static.nband)
+    g = torch.Generator(device="cpu").manual_seed(0)
+    psi = torch.complex(torch.randn(m, eng.ng, generator=g), torch.randn(m, eng.ng, generator=g)).cuda()
+    S = torch.complex(torch.randn(m, eng.nproj, generator=g), torch.randn(m, eng.nproj, generator=g)).cuda()
+    out = torch.zeros_like(psi)
+    flops = 2.0 * m * (6 * eng.ng) * (2 * eng.nproj)
+    res = {}
+    for name, fn in (("becp", lambda: pb.becp_t(psi)), ("apply_add", lambda: pb.apply_add(S, out))):
+        ms = timed(fn, steps, 3, torch)
+        res[name] = {"ms": round(ms, 4), "TF_per_s": round(flops / (ms * 1e-3) / 1e12, 1),
+                     "frac_of_bf16_dense_peak": round(flops / (ms * 1e-3) / 2.5e15, 4)}
+    return {"metric": "BF16x3 projector products, effective bf16 TF/s (3 products per complex product)",
+            "config": {"workload": f"{len(kw['frac'])}-atom cell, {m} bands, nG {eng.ng}, nproj {eng.nproj}"},
+            "flops_per_product": flops, "peak_TF_per_s": 2500.0, "products": res}
+
+
 def one_geometry(addft, system, tol, torch):
     """Per-geometry tables + SCF + forces + stress, timed on the host clock
     with the device synchronised on both sides."""
@@ -246,6 +271,9 @@ def main():
                          "and 2 stress components of the workload, against the analytic forces and stress")
     ap.add_argument("--fd-only", choices=["all", "forces", "strain"], default="all")
     ap.add_argument("--fd-h", type=float, default=1e-3, help="step of the --fd-check differences")
+    ap.add_argument("--nonlocal-only", action="store_true",
+                    help="only the BF16x3 projector products of the FP32 phase (the workload's projectors, "
+                         "nband + buffer random bands); for profiling runs")
     ap.add_argument("--hpsi-only", action="store_true",
                     help="only the local-Hpsi roofline measurement (and a 1 GiB device copy that calibrates "
                          "the FETCH_SIZE/WRITE_SIZE counters); for profiling runs")
@@ -271,6 +299,10 @@ def main():
         tol = args.fp64_tol if args.fd_check == "fp64" else args.mixed_tol
         print(json.dumps(fd_check(addft, testsystems, args.workload, args.fd_check, tol, np, torch,
                                   h=args.fd_h, only=args.fd_only)), flush=True)
+        return
+
+    if args.nonlocal_only:
+        print(json.dumps(nonlocal_measure(addft, kw, args.hpsi_steps, torch)), flush=True)
         return
 
     if args.hpsi_only:
