@@ -8,7 +8,7 @@ raises NativeUnavailable when the HIP library has not been built.
 """
 
 from .geometry.cell import Cell, make_cell, strained, volume  # noqa: F401
-from .geometry.kmesh import gamma, monkhorst_pack  # noqa: F401
+from .geometry.kmesh import gamma, monkhorst_pack, time_reversal  # noqa: F401
 from .geometry.psp import Pseudo, from_upf, from_upf_text, synthetic  # noqa: F401
 from .runtime import units  # noqa: F401
 
@@ -27,5 +27,5 @@ def __getattr__(name):
 
 __all__ = ["System", "setup", "scf", "energy", "energy_explicit", "energy_and_forces", "stress",
            "energy_decomposition", "make_energy", "energy_forces_stress", "Cell", "strained", "volume",
-           "gamma", "monkhorst_pack", "Pseudo", "synthetic", "from_upf", "units", "delta_q", "delta_q_grad",
+           "gamma", "monkhorst_pack", "time_reversal", "Pseudo", "synthetic", "from_upf", "units", "delta_q", "delta_q_grad",
            "make_gaussian_weight"]

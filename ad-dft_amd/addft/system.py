@@ -54,7 +54,7 @@ def make_static(cell: Cell, psp: Pseudo, kmesh: KMesh, ecut: float, nband: int, 
     if precision not in ("fp64", "mixed"):
         raise ValueError(f"unknown precision {precision!r}")
     kpts = kmesh.cartesian(cell.lattice)
-    basis = make_basis(cell.lattice, ecut, density_factor, kpts=kpts)
+    basis = make_basis(cell.lattice, ecut, density_factor, kpts=kmesh.shell_cartesian(cell.lattice))
     nelec = float(np.sum(psp.zval[spec_idx]))
     return Static(basis=basis, kpts=kpts, kweights=np.asarray(kmesh.weights, dtype=np.float64), psp=psp,
                   spec_idx=spec_idx, nband=int(nband), spin_degeneracy=2.0, sigma=float(sigma),

@@ -54,3 +54,30 @@ def test_batched_scf_matches_per_k():
     assert abs(F1 - F2) < 1e-9
     assert np.abs(f1 - f2).max() < 1e-7
     assert np.abs(s1 - s2).max() < 1e-8
+
+
+def test_time_reversal_mesh_matches_full_mesh():
+    """The strained 8-atom cell with the unshifted 4x4x4 mesh: the 36-point
+    time-reversal-reduced mesh gives the 64-point mesh's free energy, forces
+    and stress on the same (inversion-closed) shell, and E(k) = E(-k)."""
+    import addft
+    from addft import testsystems
+    from addft.geometry.kmesh import inversion_closed, time_reversal, time_reversal_pairs
+    m = addft.monkhorst_pack((4, 4, 4))
+    kw = dict(testsystems.config("c4"), ecut=6.0)
+    out = {}
+    for name, km in (("full", inversion_closed(m)), ("reduced", time_reversal(m))):
+        s = addft.setup(**dict(kw, kmesh=km))
+        print(f"time-reversal mesh test: {name} nk={len(km.kfrac)}", flush=True)
+        F, f, sig, info = addft.energy_forces_stress(s, tol=1e-11)
+        eig = addft.scf(s, tol=1e-11).eig.cpu().numpy() if name == "full" else None
+        out[name] = (F, f, sig, eig, info["seconds"])
+    (F1, f1, s1, e1, t1), (F2, f2, s2, _, t2) = out["full"], out["reduced"]
+    print(f"full {F1:.12f} ({t1:.2f}s)  reduced {F2:.12f} ({t2:.2f}s)  "
+          f"df {np.abs(f1 - f2).max():.2e}  ds {np.abs(s1 - s2).max():.2e}", flush=True)
+    assert abs(F1 - F2) < 1e-9
+    assert np.abs(f1 - f2).max() < 1e-7
+    assert np.abs(s1 - s2).max() < 1e-8
+    for i, j in time_reversal_pairs(m):
+        if j >= 0:
+            assert np.abs(e1[i] - e1[j]).max() < 1e-8, (i, j)
