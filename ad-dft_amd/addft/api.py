@@ -72,6 +72,17 @@ def setup(lattice, frac, psp: Pseudo | None = None, ecut: float = 8.0, nband: in
     return System(geo=geo, static=static)
 
 
+def setup_from_case(case_dir, psp=None, precision: str = "fp64") -> System:
+    """System from an ABACUS case directory (INPUT / KPT / STRU and the UPF
+    they name) (api.py:48-51 -> adapters/abacus/io.py:214-238)."""
+    try:
+        from adapters.abacus.io import setup_from_case as _from_case
+    except ImportError as e:  # the adapters package sits beside addft
+        raise ImportError("the `adapters` package is not importable; put the directory holding "
+                          "`addft` and `adapters` on sys.path") from e
+    return _from_case(case_dir, psp=psp, precision=precision)
+
+
 class _Solved(NamedTuple):
     engine: Engine
     psi: list           # stick order, per k, complex128 (nb, nG)
