@@ -40,3 +40,14 @@ def test_error_path_without_device():
     assert rc == -1 and b"dtype" in L.addft_last_error()
     rc = L.addft_pack_scatter(0, None, None, 1, 4, 8, None, None)
     assert rc == -1
+
+
+def test_autograd_wrappers_import_without_gpu():
+    """The autograd wrappers load on a host without a GPU; each defines its
+    own backward."""
+    import torch
+    from addft.kernels import autograd as AG
+    assert len(AG.FUNCTIONS) == 7
+    for fn in AG.FUNCTIONS:
+        assert issubclass(fn, torch.autograd.Function)
+        assert fn.backward is not torch.autograd.Function.backward
