@@ -132,6 +132,39 @@ def nonlocal_measure(addft, kw, steps, torch):
             "flops_per_product": flops, "peak_TF_per_s": 2500.0, "products": res}
 
 
+def subspace_measure(addft, kw, steps, torch):
+    """HIP-event times of the block products of the eigensolver at the
+    workload's size: the Gram / Rayleigh-Ritz reduction X Y^H over nG (split
+    into tuned G slices, addft.solver.linalg.inner) and the rotation Q X, in
+    complex64 over nbt bands (the FP32 phase) and in complex128 over 2 nbt
+    (the FP64 refinement's Rayleigh-Ritz).  A complex product of an (M x K)
+    and a (K x N) block is 8 M N K real flops."""
+    from addft.solver import linalg
+    from addft.solver.scf import buffer_bands
+    system = addft.setup(**kw)
+    eng = addft.api._engine(system.static, system.geo)
+    nbt = system.static.nband + buffer_bands(system.static.nband)
+    ng = eng.ng
+    g = torch.Generator(device="cpu").manual_seed(0)
+    res = {}
+    for dt, m, peak in ((torch.complex64, nbt, 157.3), (torch.complex128, 2 * nbt, 78.6)):
+        X = torch.complex(torch.randn(m, ng, generator=g), torch.randn(m, ng, generator=g)).to(dt).cuda()
+        Y = torch.complex(torch.randn(m, ng, generator=g), torch.randn(m, ng, generator=g)).to(dt).cuda()
+        Q = torch.complex(torch.randn(m, m, generator=g), torch.randn(m, m, generator=g)).to(dt).cuda()
+        flops = 8.0 * m * m * ng
+        name = "c64" if dt == torch.complex64 else "c128"
+        for op, fn in (("inner", lambda: linalg.inner(X, Y)), ("rotate", lambda: torch.matmul(Q, X))):
+            ms = timed(fn, steps, 3, torch)
+            res[f"{op}_{name}_{m}"] = {"ms": round(ms, 4), "TF_per_s": round(flops / (ms * 1e-3) / 1e12, 1),
+                                      "frac_of_peak": round(flops / (ms * 1e-3) / (peak * 1e12), 4),
+                                      "peak_TF_per_s": peak}
+        del X, Y, Q
+    return {"metric": "eigensolver block products, TF/s (8 M N K real flops per complex product)",
+            "config": {"workload": f"{len(kw['frac'])}-atom cell, nG {ng}, {nbt} bands (complex64), "
+                                   f"{2 * nbt} (complex128 refinement)"},
+            "products": res}
+
+
 def one_geometry(addft, system, tol, torch):
     """Per-geometry tables + SCF + forces + stress, timed on the host clock
     with the device synchronised on both sides."""
@@ -271,6 +304,8 @@ def main():
                          "and 2 stress components of the workload, against the analytic forces and stress")
     ap.add_argument("--fd-only", choices=["all", "forces", "strain"], default="all")
     ap.add_argument("--fd-h", type=float, default=1e-3, help="step of the --fd-check differences")
+    ap.add_argument("--subspace-only", action="store_true",
+                    help="time the eigensolver's block products (Gram/Rayleigh-Ritz reduction, rotation) only")
     ap.add_argument("--nonlocal-only", action="store_true",
                     help="only the BF16x3 projector products of the FP32 phase (the workload's projectors, "
                          "nband + buffer random bands); for profiling runs")
@@ -303,6 +338,10 @@ def main():
 
     if args.nonlocal_only:
         print(json.dumps(nonlocal_measure(addft, kw, args.hpsi_steps, torch)), flush=True)
+        return
+
+    if args.subspace_only:
+        print(json.dumps(subspace_measure(addft, kw, args.hpsi_steps, torch)), flush=True)
         return
 
     if args.hpsi_only:
