@@ -63,12 +63,15 @@ def _engine(static: Static, geo: Cell) -> Engine:
 
 def setup(lattice, frac, psp: Pseudo | None = None, ecut: float = 8.0, nband: int = 4,
           kmesh: KMesh | None = None, sigma: float = 0.05, density_factor: float = 4.0,
-          eta_ewald: float = 0.9, spec_idx=None, precision: str = "fp64") -> System:
-    """api.py:35-45."""
+          eta_ewald: float = 0.9, spec_idx=None, precision: str = "fp64", shells: str = "union") -> System:
+    """api.py:35-45.  shells="per_k": each k-point on its own sphere
+    |k + G|^2 <= 2 ecut (unequal plane-wave counts, padded to the union
+    shell in storage); "union" (default): the reference's union shell."""
     geo = make_cell(lattice, frac)
     psp = synthetic() if psp is None else psp
     static = make_static(geo, psp, kmesh or gamma(), ecut, nband, spec_idx=spec_idx, sigma=sigma,
-                         density_factor=density_factor, eta_ewald=eta_ewald, precision=precision)
+                         density_factor=density_factor, eta_ewald=eta_ewald, precision=precision,
+                         shells=shells)
     return System(geo=geo, static=static)
 
 

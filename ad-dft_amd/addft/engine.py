@@ -74,6 +74,15 @@ class Engine:
         self.miller_d = torch.as_tensor(np.ascontiguousarray(b.miller, dtype=np.int32), device=self.dev)
         self.miller_rho_d = torch.as_tensor(np.ascontiguousarray(b.miller_rho, dtype=np.int32), device=self.dev)
         self.perm_rho_d = torch.as_tensor(self.plan_rho.perm, device=self.dev)
+        # per-k shells: 0/1 masks in stick order applied to every H psi (None: union shell)
+        self.kmask64 = self.kmask32 = None
+        if static.kshell is not None:
+            km = np.ascontiguousarray(np.asarray(static.kshell, dtype=np.float64)[:, self.perm])
+            self.kmask64 = torch.as_tensor(km, device=self.dev)
+            self.kmask32 = self.kmask64.to(torch.float32) if self.mixed else None
+            self.npw = km.sum(axis=1).astype(np.int64)
+        else:
+            self.npw = np.full(self.nk, self.ng, dtype=np.int64)
         self._prep_psp()
         self._work = {}
         self.geo = None
@@ -277,10 +286,23 @@ class Engine:
         if fp32 and nl_bf16 and self.PB[ik] is not None:
             pb = self.PB[ik]
             pb.apply_add(torch.matmul(pb.becp_t(psi), D), out)
-            return out
-        becp_t = inner(psi, P)                          # (nb, nproj) = <beta|psi>^T
-        torch.addmm(out, torch.matmul(becp_t, D), P, out=out)   # accumulated by the GEMM (beta = 1)
-        return out
+        else:
+            becp_t = inner(psi, P)                      # (nb, nproj) = <beta|psi>^T
+            torch.addmm(out, torch.matmul(becp_t, D), P, out=out)   # accumulated by the GEMM (beta = 1)
+        return self.restrict(ik, out)
+
+    def restrict(self, ik, x):
+        """Zero the components of x (.., nG) outside k-point ik's own shell
+        (in place; a no-op on the union shell)."""
+        if self.kmask64 is None:
+            return x
+        return x.mul_(self.kmask64[ik] if x.real.dtype == torch.float64 else self.kmask32[ik])
+
+    def restrict_all(self, x3):
+        """restrict() for every k of an (nk, m, nG) block."""
+        if self.kmask64 is None:
+            return x3
+        return x3.mul_((self.kmask64 if x3.real.dtype == torch.float64 else self.kmask32)[:, None, :])
 
     def hpsi_batch(self, psi3, vs, fp32=False):
         """T_k + V + V_NL,k on psi3 (nk, m, nG) for all k-points at once: one
@@ -296,7 +318,7 @@ class Engine:
         out = plan.hpsi_local_kbatch(psi3.reshape(nk * m, ng), tk, m, vs).view(nk, m, ng)
         becp_t = torch.bmm(psi3, PH)                    # (nk, m, nproj)
         torch.baddbmm(out, torch.matmul(becp_t, D), P, out=out)
-        return out
+        return self.restrict_all(out)
 
     def density_batch(self, psi3, occ, fp32=False):
         """rho for psi3 (nk, nb, nG) and occ (nk, nb) in one launch."""

@@ -237,7 +237,7 @@ class Response:
             psi = self.psi[ik]
             eps = self.eig[ik]
             f = self.occ[ik]
-            Vpsi = e.plan64.hpsi_local(psi, self.zero_t, vs)
+            Vpsi = e.restrict(ik, e.plan64.hpsi_local(psi, self.zero_t, vs))
             Lam = inner(psi, Vpsi, conj_a=True)           # Lam[m, n] = <m|dV|n>
             Lam = 0.5 * (Lam + Lam.conj().T)
             de = Lam.diagonal().real

@@ -149,11 +149,16 @@ def buffer_bands(nb):
 W_MIN = 1e-3
 
 
-def initial_guess(engine, nb, dtype):
-    """The nb lowest plane waves of the reference order (scf.py:355-360)."""
+def initial_guess(engine, nb, dtype, ik=0):
+    """The nb lowest plane waves of the reference order (scf.py:355-360); on
+    per-k shells, the nb lowest |k + G| of k-point ik's own shell."""
     X = torch.zeros((nb, engine.ng), dtype=dtype, device=engine.dev)
     rows = torch.arange(nb, device=engine.dev)
-    X[rows, engine.iperm_d[:nb]] = 1.0
+    if engine.kmask64 is None:
+        X[rows, engine.iperm_d[:nb]] = 1.0
+    else:
+        t = torch.where(engine.kmask64[ik] > 0, engine.tkin64[ik], torch.full_like(engine.tkin64[ik], float("inf")))
+        X[rows, torch.argsort(t, stable=True)[:nb]] = 1.0
     return X
 
 
@@ -218,7 +223,7 @@ def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, 
             rho *= st.nelec / q
     nbuf = buffer_bands(nb) if nbuffer is None else int(nbuffer)
     nbt = min(nb + nbuf, engine.ng)
-    X = [initial_guess(engine, nbt, cdt) for _ in range(engine.nk)]
+    X = [initial_guess(engine, nbt, cdt, ik) for ik in range(engine.nk)]
     if psi0 is not None:
         for ik in range(engine.nk):
             p = torch.as_tensor(psi0[ik]).to(device=dev, dtype=cdt)[:nbt]

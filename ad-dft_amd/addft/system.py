@@ -30,6 +30,10 @@ class Static(NamedTuple):
     nelec: float
     eta_ewald: float
     precision: str = "fp64"   # "fp64" or "mixed"
+    # per-k plane-wave shells: (nk, nG) bool in reference G order, True where
+    # |k + G|^2 <= 2 ecut (1 + 1e-12) at the setup lattice; None: every k
+    # uses the whole union shell (the reference rule, basis.py:111-114)
+    kshell: object = None
 
 
 class SCFState(NamedTuple):
@@ -44,8 +48,10 @@ class SCFState(NamedTuple):
 
 def make_static(cell: Cell, psp: Pseudo, kmesh: KMesh, ecut: float, nband: int, spec_idx=None,
                 sigma: float = 0.01, density_factor: float = 4.0, eta_ewald: float = 0.9,
-                precision: str = "fp64") -> Static:
-    """system.py:47-69."""
+                precision: str = "fp64", shells: str = "union") -> Static:
+    """system.py:47-69.  shells="per_k" restricts each k-point to its own
+    sphere |k + G|^2 <= 2 ecut inside the union shell (frozen with the basis:
+    strained geometries keep the setup membership)."""
     if spec_idx is None:
         spec_idx = np.zeros((cell.natom,), dtype=np.int32)
     spec_idx = np.asarray(spec_idx, dtype=np.int32)
@@ -56,6 +62,21 @@ def make_static(cell: Cell, psp: Pseudo, kmesh: KMesh, ecut: float, nband: int, 
     kpts = kmesh.cartesian(cell.lattice)
     basis = make_basis(cell.lattice, ecut, density_factor, kpts=kmesh.shell_cartesian(cell.lattice))
     nelec = float(np.sum(psp.zval[spec_idx]))
+    if shells not in ("union", "per_k"):
+        raise ValueError(f"unknown shells {shells!r} (union or per_k)")
+    kshell = per_k_shells(cell.lattice, basis.miller, kpts, ecut) if shells == "per_k" else None
+    if kshell is not None and int(kshell.sum(axis=1).min()) < int(nband):
+        raise ValueError("a per-k shell holds fewer plane waves than nband")
     return Static(basis=basis, kpts=kpts, kweights=np.asarray(kmesh.weights, dtype=np.float64), psp=psp,
                   spec_idx=spec_idx, nband=int(nband), spin_degeneracy=2.0, sigma=float(sigma),
-                  nelec=nelec, eta_ewald=float(eta_ewald), precision=precision)
+                  nelec=nelec, eta_ewald=float(eta_ewald), precision=precision, kshell=kshell)
+
+
+def per_k_shells(lattice, miller, kpts, ecut) -> np.ndarray:
+    """(nk, nG) membership |k + G|^2 <= 2 ecut (1 + 1e-12) of the union
+    shell's G vectors, the reference's sphere test (basis.py:111-114) applied
+    per k."""
+    bmat = 2.0 * np.pi * np.linalg.inv(np.asarray(lattice, dtype=np.float64)).T
+    g = np.asarray(miller, dtype=np.float64) @ bmat
+    cut = 2.0 * float(ecut) * (1.0 + 1e-12)
+    return np.stack([np.sum((g + np.asarray(k)) ** 2, axis=1) <= cut for k in np.asarray(kpts)])
