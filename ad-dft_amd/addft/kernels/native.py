@@ -83,6 +83,37 @@ def xc_fxc(rho: torch.Tensor, w: torch.Tensor | None = None) -> torch.Tensor:
     return out
 
 
+# --------------------------------------------------- Chebyshev filter step
+def cheb_combine(a: torch.Tensor, b: torch.Tensor, c: torch.Tensor | None, alpha: float = 0.0,
+                 beta: float = 0.0, gamma: float = 0.0, coef: torch.Tensor | None = None,
+                 out: torch.Tensor | None = None) -> torch.Tensor:
+    """alpha a + beta b + gamma c in one pass (c may be None).  coef: an
+    (nblk, 3) float64 device tensor of per-block (alpha, beta, gamma) over
+    the leading dimension of a (nblk = a.shape[0]), replacing the scalars.
+    out may be a, b or c (in place)."""
+    _require(a)
+    _require(b)
+    if c is not None:
+        _require(c)
+    for t in (b, c, out):
+        if t is not None and (t.shape != a.shape or t.dtype != a.dtype):
+            raise ValueError("cheb_combine operands must share shape and dtype")
+    if out is None:
+        out = torch.empty_like(a)
+    else:
+        _require(out)
+    nblk = 1
+    if coef is not None:
+        _require(coef, torch.float64)
+        nblk = int(a.shape[0])
+        if tuple(coef.shape) != (nblk, 3):
+            raise ValueError("coef must be (a.shape[0], 3)")
+    check(lib().addft_cheb_combine(_dtype_code(a), nblk, a.numel() // max(nblk, 1), ptr(coef), float(alpha),
+                                   float(beta), float(gamma), ptr(a), ptr(b), ptr(c), ptr(out), stream()),
+          "addft_cheb_combine")
+    return out
+
+
 # ------------------------------------------------------------- real grids
 def to_planar(grid: torch.Tensor, shape) -> torch.Tensor:
     """row-major (i1,i2,i3) float64 -> plane-major (i3,i1,i2)."""

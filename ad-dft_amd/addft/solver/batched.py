@@ -21,6 +21,8 @@ import numpy as np
 import torch
 from threadpoolctl import threadpool_limits
 
+from .graphs import _recurrence
+
 
 # Batches of small Hermitian eigenproblems (nk matrices of a few hundred rows)
 # are solved with host LAPACK: the device eigensolver runs them one matrix at
@@ -155,27 +157,11 @@ def chfsi_step(apply, X, degree, b_up, lam, HX=None, filt=None, rr=True):
     abc[k] = (largest Ritz value, smallest, upper bound).  Returns (lam, X,
     HX, residual norms); rr=False returns (lam, orthonormalised block, None,
     None) as solver/chebyshev.py::chfsi_step."""
-    rd = X.real.dtype
     if HX is None:
         HX = apply(X)
-    if filt is not None:
-        abc = torch.stack([lam[:, -1].to(torch.float64), lam[:, 0].to(torch.float64),
-                           b_up.to(torch.float64)], dim=1)
-        Y = filt(X, HX, abc)
-    else:
-        a = lam[:, -1].to(rd)[:, None, None]
-        a0 = lam[:, 0].to(rd)[:, None, None]
-        b = b_up.to(rd)[:, None, None]
-        e = 0.5 * (b - a)
-        c = 0.5 * (b + a)
-        sigma = e / (a0 - c)
-        tau = 2.0 / sigma
-        Y = (HX - c * X) * (sigma / e)
-        Xm = X
-        for _ in range(2, degree + 1):
-            sigma_new = 1.0 / (tau - sigma)
-            Ynew = (apply(Y) - c * Y) * (2.0 * sigma_new / e) - (sigma * sigma_new) * Xm
-            Xm, Y, sigma = Y, Ynew, sigma_new
+    abc = torch.stack([lam[:, -1].to(torch.float64), lam[:, 0].to(torch.float64),
+                       b_up.to(torch.float64)], dim=1)
+    Y = filt(X, HX, abc) if filt is not None else _recurrence(apply, X, HX, abc, degree)
     Y = orth(Y)
     if not rr:
         return lam, Y, None, None

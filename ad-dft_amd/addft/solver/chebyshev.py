@@ -17,6 +17,7 @@ from __future__ import annotations
 
 import torch
 
+from ..kernels.native import cheb_combine
 from .davidson import eigh
 from .linalg import inner
 
@@ -59,11 +60,15 @@ def filter_block(apply, X, HX, degree, a, b, a0):
     c = 0.5 * (b + a)
     sigma = e / (a0 - c)
     tau = 2.0 / sigma
-    HX = apply(X) if HX is None else HX
-    Y = (HX - c * X) * (sigma / e)
+    X = X.contiguous()
+    HX = apply(X) if HX is None else HX.contiguous()
+    Y = cheb_combine(HX, X, None, sigma / e, -c * sigma / e)
     for _ in range(2, degree + 1):
         sigma_new = 1.0 / (tau - sigma)
-        Ynew = (apply(Y) - c * Y) * (2.0 * sigma_new / e) - (sigma * sigma_new) * X
+        al = 2.0 * sigma_new / e
+        HY = apply(Y)
+        # (H Y - c Y) al - sigma sigma_new X, written over H Y
+        Ynew = cheb_combine(HY, Y, X, al, -c * al, -sigma * sigma_new, out=HY)
         X, Y, sigma = Y, Ynew, sigma_new
     return Y
 

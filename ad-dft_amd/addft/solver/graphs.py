@@ -30,21 +30,32 @@ def enabled() -> bool:
     return os.environ.get("ADDFT_GRAPHS", "1") != "0" and torch.cuda.is_available()
 
 
+def _coef(al, be, ga):
+    """(nk, 3) float64 per-k coefficients of one combination step."""
+    return torch.stack([al.reshape(-1), be.reshape(-1), ga.reshape(-1)], dim=1).to(torch.float64).contiguous()
+
+
 def _recurrence(apply, X, HX, abc, degree):
-    """Scaled Chebyshev filter (Zhou-Saad Alg. 3.2) with per-k windows."""
+    """Scaled Chebyshev filter (Zhou-Saad Alg. 3.2) with per-k windows; each
+    step is one fused pass (native.cheb_combine) written over H Y, with the
+    per-k coefficients computed on the device."""
+    from ..kernels.native import cheb_combine
     rd = X.real.dtype
-    a = abc[:, 0, None, None].to(rd)
-    a0 = abc[:, 1, None, None].to(rd)
-    b = abc[:, 2, None, None].to(rd)
+    a = abc[:, 0].to(rd)
+    a0 = abc[:, 1].to(rd)
+    b = abc[:, 2].to(rd)
     e = 0.5 * (b - a)
     c = 0.5 * (b + a)
     sigma = e / (a0 - c)
     tau = 2.0 / sigma
-    Y = (HX - c * X) * (sigma / e)
+    X, HX = X.contiguous(), HX.contiguous()
+    Y = cheb_combine(HX, X, None, coef=_coef(sigma / e, -c * sigma / e, torch.zeros_like(e)))
     Xm = X
     for _ in range(2, degree + 1):
         sigma_new = 1.0 / (tau - sigma)
-        Ynew = (apply(Y) - c * Y) * (2.0 * sigma_new / e) - (sigma * sigma_new) * Xm
+        al = 2.0 * sigma_new / e
+        HY = apply(Y)
+        Ynew = cheb_combine(HY, Y, Xm, coef=_coef(al, -c * al, -sigma * sigma_new), out=HY)
         Xm, Y, sigma = Y, Ynew, sigma_new
     return Y
 

@@ -66,6 +66,20 @@ int addft_pack_gather(int dtype, const void* d_grid, const int32_t* d_idx,
                       int64_t batch, int64_t ng, int64_t nfft, void* d_coeffs,
                       void* stream);
 
+/* ------------------------------------------------- Chebyshev filter step
+ * out = alpha a + beta b + gamma c over nblk blocks of n complex elements
+ * each (dtype ADDFT_C64 or ADDFT_C128), with (alpha, beta, gamma) per block
+ * from d_coef (nblk x 3 doubles, device) or, when d_coef is NULL, the three
+ * scalars for every block.  d_c may be NULL (no third term).  d_out may
+ * alias d_a, d_b or d_c.  The three-term recurrence of the Chebyshev-
+ * filtered subspace iteration that stands in for most Davidson steps here;
+ * no counterpart in the reference, whose eigensolver is the block Davidson
+ * of solver/davidson.py:32-77.
+ * errors: bad dtype, nblk outside [0, 65535], n < 0, null a, b or out.    */
+int addft_cheb_combine(int dtype, int64_t nblk, int64_t n, const double* d_coef,
+                       double alpha, double beta, double gamma, const void* d_a,
+                       const void* d_b, const void* d_c, void* d_out, void* stream);
+
 /* ------------------------------------------------------------ BF16x3 split
  * FP32 rows x (rows, k) -> bf16 rows [bf16(x) | bf16(x) | bf16(x - bf16(x))]
  * (rows, ld), round to nearest even; columns [3k, ld) untouched.  The
