@@ -87,9 +87,27 @@ def exported_symbols():
     return tuple(_SIGS)
 
 
+# The reference's per-primitive switches (kernels/pack.py:68, xc_lda.py:66,
+# eigh.py:40): "0" selects its JAX implementation instead of the native one.
+# Pack and XC exist here only as HIP kernels, so "0" is refused rather than
+# ignored; the native dense eigensolver the EIGH switch governs has no use on
+# this path (the Davidson/ChFSI eigensolver replaces the dense Hamiltonian),
+# so that switch is accepted either way.
+_NO_FALLBACK = ("ADDFT_NATIVE_PACK", "ADDFT_NATIVE_XC")
+
+
+def check_switches():
+    for name in _NO_FALLBACK:
+        if os.environ.get(name, "1") == "0":
+            raise NativeUnavailable(
+                f"{name}=0 selects the reference's JAX implementation, which this build does not have: "
+                f"its pack and XC run only as HIP kernels; unset {name}")
+
+
 def lib():
     """Load (once) and return the library, or raise NativeUnavailable."""
     global _LIB, _PATH
+    check_switches()
     if _LIB is not None:
         return _LIB
     path = lib_path()

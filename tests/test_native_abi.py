@@ -6,6 +6,8 @@ from __future__ import annotations
 import ctypes
 import re
 
+import pytest
+
 from conftest import ROOT
 
 
@@ -51,3 +53,16 @@ def test_autograd_wrappers_import_without_gpu():
     for fn in AG.FUNCTIONS:
         assert issubclass(fn, torch.autograd.Function)
         assert fn.backward is not torch.autograd.Function.backward
+
+
+@pytest.mark.parametrize("name", ["ADDFT_NATIVE_PACK", "ADDFT_NATIVE_XC"])
+def test_native_switch_off_is_refused(monkeypatch, name):
+    """The reference's "0" switches select a JAX fallback; this build has
+    none, so they raise instead of running the HIP path silently."""
+    from addft.kernels import _lib
+    monkeypatch.setenv(name, "0")
+    with pytest.raises(_lib.NativeUnavailable, match=name):
+        _lib.lib()
+    monkeypatch.setenv(name, "1")
+    monkeypatch.setenv("ADDFT_NATIVE_EIGH", "0")      # accepted: no native dense eigh on this path
+    _lib.check_switches()
