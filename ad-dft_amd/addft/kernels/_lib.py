@@ -107,9 +107,9 @@ def check_switches():
 def lib():
     """Load (once) and return the library, or raise NativeUnavailable."""
     global _LIB, _PATH
-    check_switches()
     if _LIB is not None:
         return _LIB
+    check_switches()
     path = lib_path()
     if not path.exists():
         raise NativeUnavailable(

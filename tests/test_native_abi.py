@@ -60,6 +60,7 @@ def test_native_switch_off_is_refused(monkeypatch, name):
     """The reference's "0" switches select a JAX fallback; this build has
     none, so they raise instead of running the HIP path silently."""
     from addft.kernels import _lib
+    monkeypatch.setattr(_lib, "_LIB", None)          # the switches are read when the library loads
     monkeypatch.setenv(name, "0")
     with pytest.raises(_lib.NativeUnavailable, match=name):
         _lib.lib()
