@@ -21,7 +21,7 @@ import numpy as np
 import torch
 from threadpoolctl import threadpool_limits
 
-from .graphs import _recurrence
+from .graphs import filter_recurrence
 
 
 # Batches of small Hermitian eigenproblems (nk matrices of a few hundred rows)
@@ -161,7 +161,7 @@ def chfsi_step(apply, X, degree, b_up, lam, HX=None, filt=None, rr=True):
         HX = apply(X)
     abc = torch.stack([lam[:, -1].to(torch.float64), lam[:, 0].to(torch.float64),
                        b_up.to(torch.float64)], dim=1)
-    Y = filt(X, HX, abc) if filt is not None else _recurrence(apply, X, HX, abc, degree)
+    Y = filt(X, HX, abc) if filt is not None else filter_recurrence(apply, X, HX, abc, degree)
     Y = orth(Y)
     if not rr:
         return lam, Y, None, None

@@ -35,7 +35,7 @@ def _coef(al, be, ga):
     return torch.stack([al.reshape(-1), be.reshape(-1), ga.reshape(-1)], dim=1).to(torch.float64).contiguous()
 
 
-def _recurrence(apply, X, HX, abc, degree):
+def filter_recurrence(apply, X, HX, abc, degree):
     """Scaled Chebyshev filter (Zhou-Saad Alg. 3.2) with per-k windows; each
     step is one fused pass (native.cheb_combine) written over H Y, with the
     per-k coefficients computed on the device."""
@@ -82,12 +82,12 @@ class FilterGraph:
         side.wait_stream(torch.cuda.current_stream())
         with torch.cuda.stream(side):
             for _ in range(2):
-                _recurrence(apply, self.X, self.HX, self.abc, self.degree)
+                filter_recurrence(apply, self.X, self.HX, self.abc, self.degree)
         torch.cuda.current_stream().wait_stream(side)
         torch.cuda.synchronize()
         self.graph = torch.cuda.CUDAGraph()
         with torch.cuda.graph(self.graph):
-            self.Y = _recurrence(apply, self.X, self.HX, self.abc, self.degree)
+            self.Y = filter_recurrence(apply, self.X, self.HX, self.abc, self.degree)
         self.replays = 0
 
     def matches(self, X, degree):
@@ -107,4 +107,4 @@ class FilterGraph:
 
 def eager_filter(hpsi, X, HX, abc, vs, degree):
     """The same filter without capture (reference for tests and fallback)."""
-    return _recurrence(lambda p: hpsi(p, vs), X, HX, abc, degree)
+    return filter_recurrence(lambda p: hpsi(p, vs), X, HX, abc, degree)

@@ -50,7 +50,7 @@ def _dense_apply(torch, n, dt, seed):
 def test_filters_match_separate_recurrence(dt, tol):
     import torch
     from addft.solver.chebyshev import filter_block
-    from addft.solver.graphs import _recurrence
+    from addft.solver.graphs import filter_recurrence
     dt = getattr(torch, dt)
     n, m, deg = 300, 24, 10
     apply = _dense_apply(torch, n, dt, 5)
@@ -78,7 +78,7 @@ def test_filters_match_separate_recurrence(dt, tol):
     HX3 = torch.stack([apply(X3[k]) for k in range(nk)])
     wins = [(1.5, -0.4, 5.0), (2.0, -0.2, 5.5), (1.0, -0.6, 4.5)]
     abc = torch.tensor(wins, dtype=torch.float64).cuda()
-    got3 = _recurrence(lambda Y: torch.stack([apply(Y[k]) for k in range(nk)]), X3, HX3, abc, deg)
+    got3 = filter_recurrence(lambda Y: torch.stack([apply(Y[k]) for k in range(nk)]), X3, HX3, abc, deg)
     for k, (ak, a0k, bk) in enumerate(wins):
         rk = separate(X3[k], HX3[k], ak, a0k, bk)
         assert ((got3[k] - rk).abs().max() / rk.abs().max()).item() < tol, k
