@@ -67,3 +67,27 @@ def test_native_switch_off_is_refused(monkeypatch, name):
     monkeypatch.setenv(name, "1")
     monkeypatch.setenv("ADDFT_NATIVE_EIGH", "0")      # accepted: no native dense eigh on this path
     _lib.check_switches()
+
+
+def test_invalid_arguments_are_reported_without_a_launch():
+    """Shape and argument errors come back as ADDFT_EINVAL with a message
+    (the reference's handlers return ffi::Error::InvalidArgument,
+    pack.cu:50-51, eigh.cu:66-68); the checks run before any HIP call, so
+    they hold on a host without a GPU."""
+    from addft.kernels import _lib
+    L = _lib.lib()
+    EINVAL = -1
+
+    def err(rc):
+        assert rc == EINVAL
+        return L.addft_last_error().decode()
+
+    assert "dtype" in err(L.addft_cheb_combine(7, 1, 10, None, 1.0, 0.0, 0.0, 1, 1, None, 1, None))
+    assert "nblk" in err(L.addft_cheb_combine(_lib.C64, 70000, 10, None, 1.0, 0.0, 0.0, 1, 1, None, 1, None))
+    assert "null" in err(L.addft_cheb_combine(_lib.C64, 1, 10, None, 1.0, 0.0, 0.0, None, 1, None, 1, None))
+    assert "even" in err(L.addft_split_bf16x3(1, 4, 3, 12, 1, None))
+    assert "ld" in err(L.addft_split_bf16x3(1, 4, 4, 10, 1, None))
+    assert "null" in err(L.addft_pack_scatter(_lib.C64, None, None, 1, 4, 8, None, None))
+    assert "size" in err(L.addft_pack_scatter(_lib.C64, 1, 1, 1, 9, 8, 1, None))
+    # empty work is a no-op, not an error
+    assert L.addft_cheb_combine(_lib.C64, 0, 10, None, 1.0, 0.0, 0.0, None, None, None, None, None) == 0
