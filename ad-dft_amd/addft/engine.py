@@ -32,7 +32,7 @@ from .kernels import _lib
 from .kernels import native
 from .kernels._lib import check, lib, ptr, stream
 from .solver.linalg import inner
-from .kernels.nonlocal_bf16 import ProjectorBF16x3
+from .kernels.nonlocal_bf16 import GammaProjectorBF16x3, ProjectorBF16x3, gamma_pairs
 
 
 def _simpson_weights(n):
@@ -74,6 +74,11 @@ class Engine:
         self.miller_d = torch.as_tensor(np.ascontiguousarray(b.miller, dtype=np.int32), device=self.dev)
         self.miller_rho_d = torch.as_tensor(np.ascontiguousarray(b.miller_rho, dtype=np.int32), device=self.dev)
         self.perm_rho_d = torch.as_tensor(self.plan_rho.perm, device=self.dev)
+        # Gamma-only: FP32-phase projector products over the half sphere (ADDFT_GAMMA_HALF=0: full sphere)
+        self.gamma_pairs = None
+        if (self.nl_bf16 and self.nk == 1 and not np.any(static.kpts[0]) and static.kshell is None
+                and os.environ.get("ADDFT_GAMMA_HALF", "1") != "0"):
+            self.gamma_pairs = gamma_pairs(np.asarray(b.miller)[self.perm])
         # per-k shells: 0/1 masks in stick order applied to every H psi (None: union shell)
         self.kmask64 = self.kmask32 = None
         if static.kshell is not None:
@@ -138,6 +143,9 @@ class Engine:
         self.prad_d = self._i32(rows[:, 1])
         self.plm_d = self._i32(rows[:, 2])
         self.patom_l = torch.as_tensor(rows[:, 0].astype(np.int64), device=self.dev)
+        # rows of odd l (lm index l*l + m): phase i in the Gamma-point half-sphere products
+        self.odd_rows_d = torch.as_tensor(np.floor(np.sqrt(rows[:, 2] + 0.5)).astype(np.int64) % 2 == 1,
+                                          device=self.dev)
         # block-diagonal D laid out (i, p, i, q) (nonlocal_op.py:25-42)
         lm = lm_table(psp.proj_l[s])
         dblk = np.zeros((self.nh, self.nh))
@@ -204,7 +212,10 @@ class Engine:
             self.P32.append(P.to(torch.complex64) if self.mixed else None)
             if self.nl_bf16:
                 nsl = max(1, min(32, (6 * ng) // 15000))
-                self.PB.append(ProjectorBF16x3(P, nslice=nsl))
+                if self.gamma_pairs is not None:
+                    self.PB.append(GammaProjectorBF16x3(P, *self.gamma_pairs, self.odd_rows_d, nslice=nsl))
+                else:
+                    self.PB.append(ProjectorBF16x3(P, nslice=nsl))
             else:
                 self.PB.append(None)
         # k-stacked tables for the batched Hpsi (several k-points on one sphere)

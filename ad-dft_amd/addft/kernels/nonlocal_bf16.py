@@ -116,3 +116,113 @@ class ProjectorBF16x3:
         """S (nb, nproj) complex64 -> S P (nb, nG) complex64."""
         out = torch.mm(self._split_s(S), self.V3, out_dtype=torch.float32)
         return torch.view_as_complex(out.view(S.shape[0], self.ng, 2))
+
+
+# ---------------------------------------------------------- Gamma point
+def gamma_pairs(miller_stick):
+    """(plus, minus) int32 stick positions pairing each G of the positive
+    half (first nonzero Miller index > 0) with -G; entry 0 is G = 0 with
+    minus = -1.  None when the sphere is not closed under G -> -G."""
+    import numpy as np
+    m = np.asarray(miller_stick, dtype=np.int64).reshape(-1, 3)
+    pos = {tuple(r): i for i, r in enumerate(m.tolist())}
+    z = pos.get((0, 0, 0))
+    if z is None:
+        return None
+    plus, minus = [z], [-1]
+    for i, r in enumerate(m.tolist()):
+        t = tuple(r)
+        if t > (0, 0, 0):
+            j = pos.get((-t[0], -t[1], -t[2]))
+            if j is None:
+                return None
+            plus.append(i)
+            minus.append(j)
+    if 2 * len(plus) - 1 != m.shape[0]:
+        return None
+    return np.asarray(plus, dtype=np.int32), np.asarray(minus, dtype=np.int32)
+
+
+class GammaProjectorBF16x3:
+    """Split-bf16 half-sphere images of the k = 0 projector matrix: the same
+    becp_t / apply_add interface as ProjectorBF16x3 for P' = Phi P, with
+    Phi = i on the rows of odd l (so that P'(-G) = conj(P'(G)); the
+    nonlocal operator P'^H D P' = P^H D P since D couples only rows of equal
+    l).  becp_t returns psi P'^H and apply_add takes S' = (psi P'^H) D.
+
+    Both products become real GEMMs over the nh pairs (G, -G) with
+    Q = P'(G): <P'|psi> from rows [c.re | d.im], [c.im | -d.re] of
+    c, d = psi(G) +- psi(-G) against W = [Qr^T; Qi^T] (addft_gamma_pair_split
+    builds the split-bf16 rows in one pass), and S' P' from
+    U = [S'.re; S'.im] [Qr | Qi] unfolded onto +-G by
+    addft_gamma_scatter_add.  Half the flops of the complex products and
+    half the projector bytes."""
+
+    def __init__(self, P: torch.Tensor, plus, minus, odd_rows: torch.Tensor, nslice: int = 32):
+        npj, ng = P.shape
+        dev = P.device
+        self.np, self.ng, self.nslice = int(npj), int(ng), int(nslice)
+        self.plus = torch.as_tensor(plus, dtype=torch.int32, device=dev)
+        self.minus = torch.as_tensor(minus, dtype=torch.int32, device=dev)
+        nh = int(self.plus.numel())
+        self.nh = nh
+        if int(self.plus.max()) >= ng or int(self.minus.max()) >= ng:
+            raise ValueError("pair table does not fit the sphere")
+        ph = torch.where(odd_rows.to(dev), torch.tensor(1j, dtype=P.dtype, device=dev),
+                         torch.tensor(1.0, dtype=P.dtype, device=dev))
+        Q = (P * ph[:, None])[:, self.plus.long()]
+        Qr = Q.real.to(torch.float32)
+        Qi = Q.imag.to(torch.float32)
+        Qi[:, 0] = 0.0                                  # G = 0: P'(0) is real
+        del Q
+        K = 2 * nh
+        self.K = K
+        self.kc = -(-3 * K // self.nslice)
+        self.kc += self.kc % 2
+        W = torch.cat([Qr.T, Qi.T], dim=0)              # (2 nh, np)
+        hi = W.to(torch.bfloat16)
+        lo = (W - hi.to(torch.float32)).to(torch.bfloat16)
+        W3 = torch.zeros((self.kc * self.nslice, npj), dtype=torch.bfloat16, device=dev)
+        W3[:K], W3[K:2 * K], W3[2 * K:3 * K] = hi, lo, hi
+        self.W3b = W3.view(self.nslice, self.kc, npj)
+        del W, hi, lo
+        self.kp = npj + npj % 2                         # even split length for S
+        V = torch.cat([Qr, Qi], dim=1)                  # (np, 2 nh)
+        vhi = V.to(torch.bfloat16)
+        vlo = (V - vhi.to(torch.float32)).to(torch.bfloat16)
+        V3 = torch.zeros((3 * self.kp, K), dtype=torch.bfloat16, device=dev)
+        V3[:npj], V3[self.kp:self.kp + npj], V3[2 * self.kp:2 * self.kp + npj] = vhi, vlo, vhi
+        self.V3 = V3
+        self._buf = None
+
+    def becp_t(self, psi: torch.Tensor) -> torch.Tensor:
+        """psi (nb, nG) complex64 -> psi P'^H (nb, nproj) complex64."""
+        nb = psi.shape[0]
+        n = self.kc * self.nslice
+        if self._buf is None or self._buf.shape[0] < 2 * nb:
+            self._buf = torch.zeros((2 * nb, n), dtype=torch.bfloat16, device=psi.device)
+        B = self._buf[:2 * nb]
+        psi = psi.contiguous()
+        _lib.check(_lib.lib().addft_gamma_pair_split(_lib.ptr(psi), nb, self.ng, _lib.ptr(self.plus),
+                                                     _lib.ptr(self.minus), self.nh, n, _lib.ptr(B),
+                                                     _lib.stream()), "addft_gamma_pair_split")
+        Xb = B.view(2 * nb, self.nslice, self.kc).transpose(0, 1)
+        C = torch.bmm(Xb, self.W3b, out_dtype=torch.float32).sum(dim=0)
+        return torch.complex(C[:nb], C[nb:])
+
+    def apply_add(self, S: torch.Tensor, out: torch.Tensor) -> torch.Tensor:
+        """out (nb, nG) complex64 += S' P'."""
+        nb = S.shape[0]
+        Sr = torch.zeros((2 * nb, self.kp), dtype=torch.float32, device=S.device)
+        Sr[:nb, :self.np] = S.real
+        Sr[nb:, :self.np] = S.imag
+        S3 = torch.empty((2 * nb, 3 * self.kp), dtype=torch.bfloat16, device=S.device)
+        _lib.check(_lib.lib().addft_split_bf16x3(_lib.ptr(Sr), 2 * nb, self.kp, 3 * self.kp, _lib.ptr(S3),
+                                                 _lib.stream()), "addft_split_bf16x3")
+        U = torch.mm(S3, self.V3, out_dtype=torch.float32)
+        if not out.is_contiguous():
+            raise ValueError("out must be contiguous")
+        _lib.check(_lib.lib().addft_gamma_scatter_add(_lib.ptr(U), nb, self.nh, _lib.ptr(self.plus),
+                                                      _lib.ptr(self.minus), self.ng, _lib.ptr(out),
+                                                      _lib.stream()), "addft_gamma_scatter_add")
+        return out

@@ -34,6 +34,69 @@ __global__ void k_split_bf16x3(const float2* __restrict__ x, int64_t rows, int64
   }
 }
 
+// Gamma-point pair split.  With P'(-G) = conj(P'(G)) the reduction over the
+// sphere folds onto the half sphere: for pair j (G_j, -G_j) with a = psi(G_j),
+// b = psi(-G_j) (b = 0 for the unpaired G = 0, j = 0), c = a + b, d = a - b,
+//   Re <P'|psi> = sum_j Qr c.re + Qi d.im,  Im <P'|psi> = sum_j Qr c.im - Qi d.re
+// (Q = P'(G_j)).  Row b of the output holds [c.re | d.im] and row nb + b
+// holds [c.im | -d.re] (length K = 2 nh), each value v written as bf16 hi
+// at column k and K + k and bf16(v - hi) at 2K + k.  One thread per (band,
+// pair); blockIdx.y is the band.
+__global__ void k_gamma_pair_split(const float2* __restrict__ psi, int64_t ng, const int32_t* __restrict__ plus,
+                                   const int32_t* __restrict__ minus, int64_t nh, int64_t nb, int64_t ld,
+                                   uint16_t* __restrict__ out) {
+  const int64_t b = blockIdx.y;
+  const float2* row = psi + b * ng;
+  uint16_t* o_re = out + b * ld;
+  uint16_t* o_im = out + (nb + b) * ld;
+  const int64_t K = 2 * nh;
+  for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < nh; j += (int64_t)gridDim.x * blockDim.x) {
+    const float2 a = row[plus[j]];
+    const int32_t mj = minus[j];
+    const float2 bb = mj >= 0 ? row[mj] : make_float2(0.f, 0.f);
+    const float v[4] = {a.x + bb.x, a.y - bb.y, a.y + bb.y, -(a.x - bb.x)};  // c.re, d.im, c.im, -d.re
+    uint16_t* dst[4] = {o_re + j, o_re + nh + j, o_im + j, o_im + nh + j};
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const uint32_t h = bf16_rne(v[t]);
+      const uint16_t lo = (uint16_t)bf16_rne(v[t] - __uint_as_float(h << 16));
+      dst[t][0] = (uint16_t)h;
+      dst[t][K] = (uint16_t)h;
+      dst[t][2 * K] = lo;
+    }
+  }
+}
+
+// Gamma-point scatter-add.  U (2 nb, 2 nh) FP32 holds [u | v] for the real
+// parts of S' (rows < nb) and the imaginary parts (rows >= nb), with
+// u = S' Qr and v = S' Qi; the half-sphere product unfolds as
+// out(G_j) += u + i v,  out(-G_j) += u - i v  (G = 0, j = 0: only the first).
+// Every sphere position is one of plus[j], minus[j] exactly once, so the
+// adds need no atomics.
+__global__ void k_gamma_scatter_add(const float* __restrict__ U, int64_t nb, int64_t nh,
+                                    const int32_t* __restrict__ plus, const int32_t* __restrict__ minus, int64_t ng,
+                                    float2* __restrict__ out) {
+  const int64_t b = blockIdx.y;
+  const float* ur_row = U + b * 2 * nh;
+  const float* ui_row = U + (nb + b) * 2 * nh;
+  float2* row = out + b * ng;
+  for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < nh; j += (int64_t)gridDim.x * blockDim.x) {
+    const float ur = ur_row[j], vr = ur_row[nh + j];
+    const float ui = ui_row[j], vi = ui_row[nh + j];
+    float2 p = row[plus[j]];
+    p.x += ur - vi;
+    p.y += ui + vr;
+    row[plus[j]] = p;
+    const int32_t mj = minus[j];
+    if (mj >= 0) {
+      float2 q = row[mj];
+      q.x += ur + vi;
+      q.y += ui - vr;
+      row[mj] = q;
+    }
+  }
+}
+
 int blocks_for(int64_t n) {
   const int64_t b = (n + 255) / 256;
   return (int)(b < 1 ? 1 : (b > 65536 ? 65536 : b));
@@ -49,6 +112,34 @@ int addft_split_bf16x3(const float* d_x, int64_t rows, int64_t k, int64_t ld, vo
   if (rows == 0 || k == 0) return ADDFT_OK;
   hipLaunchKernelGGL(k_split_bf16x3, dim3(blocks_for(rows * (k / 2))), dim3(256), 0, addft::as_stream(stream),
                      reinterpret_cast<const float2*>(d_x), rows, k / 2, ld / 2, reinterpret_cast<uint32_t*>(d_out));
+  ADDFT_LAUNCH_CHECK();
+  return ADDFT_OK;
+}
+
+int addft_gamma_pair_split(const void* d_psi, int64_t nb, int64_t ng, const int32_t* d_plus, const int32_t* d_minus,
+                           int64_t nh, int64_t ld, void* d_out, void* stream) {
+  ADDFT_CHECK_ARG(nb >= 0 && nb <= 32767 && nh >= 0 && 2 * nh <= ng + 1, "bad sizes (nb <= 32767, 2 nh <= nG + 1)");
+  ADDFT_CHECK_ARG(ld >= 6 * nh, "ld must be at least 6 nh");
+  if (nb == 0 || nh == 0) return ADDFT_OK;
+  ADDFT_CHECK_ARG(d_psi && d_plus && d_minus && d_out, "null argument");
+  int64_t gx = (nh + 255) / 256;
+  if (gx > 1024) gx = 1024;
+  hipLaunchKernelGGL(k_gamma_pair_split, dim3((unsigned)gx, (unsigned)nb), dim3(256), 0, addft::as_stream(stream),
+                     static_cast<const float2*>(d_psi), ng, d_plus, d_minus, nh, nb, ld,
+                     static_cast<uint16_t*>(d_out));
+  ADDFT_LAUNCH_CHECK();
+  return ADDFT_OK;
+}
+
+int addft_gamma_scatter_add(const float* d_u, int64_t nb, int64_t nh, const int32_t* d_plus, const int32_t* d_minus,
+                            int64_t ng, void* d_out, void* stream) {
+  ADDFT_CHECK_ARG(nb >= 0 && nb <= 32767 && nh >= 0 && 2 * nh <= ng + 1, "bad sizes (nb <= 32767, 2 nh <= nG + 1)");
+  if (nb == 0 || nh == 0) return ADDFT_OK;
+  ADDFT_CHECK_ARG(d_u && d_plus && d_minus && d_out, "null argument");
+  int64_t gx = (nh + 255) / 256;
+  if (gx > 1024) gx = 1024;
+  hipLaunchKernelGGL(k_gamma_scatter_add, dim3((unsigned)gx, (unsigned)nb), dim3(256), 0, addft::as_stream(stream),
+                     d_u, nb, nh, d_plus, d_minus, ng, static_cast<float2*>(d_out));
   ADDFT_LAUNCH_CHECK();
   return ADDFT_OK;
 }

@@ -66,6 +66,27 @@ int addft_pack_gather(int dtype, const void* d_grid, const int32_t* d_idx,
                       int64_t batch, int64_t ng, int64_t nfft, void* d_coeffs,
                       void* stream);
 
+/* ---------------------------------------------- Gamma-point half sphere
+ * At k = 0 the projector images obey P'(-G) = conj(P'(G)) once the rows of
+ * odd l are multiplied by i (the reference applies no (-i)^l factor,
+ * nonlocal_op.py:45-64, so P(-G) = (-1)^l conj(P(G))); D couples only rows
+ * of equal l and is unchanged.  Both projector products then run over the
+ * half sphere with real P' parts (half the flops of the complex products of
+ * nonlocal_op.py:67-76).  Pair j joins stick positions plus[j] and
+ * minus[j] (G_j and -G_j); j = 0 is G = 0 with minus[0] = -1.
+ *
+ * addft_gamma_pair_split: psi (nb, ng) complex64 -> bf16 rows (2 nb, ld):
+ *   row b [c.re | d.im], row nb+b [c.im | -d.re] with c, d = psi(G) +- psi(-G),
+ *   each of length K = 2 nh written as [hi | hi | lo] at columns [0, 3K);
+ *   columns [3K, ld) untouched.
+ * addft_gamma_scatter_add: U (2 nb, 2 nh) FP32 = [S'.re; S'.im] [Qr | Qi]
+ *   -> out (nb, ng) complex64 += S' P' unfolded onto both halves.
+ * errors: nb > 32767, 2 nh > ng + 1, ld < 6 nh, null pointers.          */
+int addft_gamma_pair_split(const void* d_psi, int64_t nb, int64_t ng, const int32_t* d_plus,
+                           const int32_t* d_minus, int64_t nh, int64_t ld, void* d_out, void* stream);
+int addft_gamma_scatter_add(const float* d_u, int64_t nb, int64_t nh, const int32_t* d_plus,
+                            const int32_t* d_minus, int64_t ng, void* d_out, void* stream);
+
 /* ------------------------------------------------- Chebyshev filter step
  * out = alpha a + beta b + gamma c over nblk blocks of n complex elements
  * each (dtype ADDFT_C64 or ADDFT_C128), with (alpha, beta, gamma) per block

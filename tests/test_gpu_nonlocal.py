@@ -71,3 +71,59 @@ def test_apply_add_accumulates_into_output():
     got = pb.apply_add(S, base.clone())
     torch.cuda.synchronize()
     assert ((got - ref).abs().max() / ref.abs().max()).item() < 1e-6
+
+
+@pytest.mark.parametrize("cfg", ["c2", "c3"])
+def test_gamma_half_sphere_products(cfg):
+    """GammaProjectorBF16x3 on the engine's own k = 0 projectors (three
+    projectors, l = 0, 0, 1): psi P'^H and out + S' P' against the complex128
+    products with P' = i^(l mod 2) P, at the BF16x3 bar; the odd-l phase
+    cancels in the nonlocal operator, so (psi P'^H) D P' = (psi P^H) D P."""
+    import torch
+    import addft
+    from addft import testsystems
+    from addft.kernels.nonlocal_bf16 import GammaProjectorBF16x3
+    s = addft.setup(**testsystems.config(cfg, precision="mixed"))
+    eng = addft.api._engine(s.static, s.geo)
+    assert isinstance(eng.PB[0], GammaProjectorBF16x3)
+    pb, P = eng.PB[0], eng.P64[0]
+    ph = torch.where(eng.odd_rows_d, torch.tensor(1j, dtype=P.dtype, device=P.device),
+                     torch.tensor(1.0, dtype=P.dtype, device=P.device))
+    Pp = P * ph[:, None]
+    g = torch.Generator(device="cpu").manual_seed(4)
+    nb = 40
+    psi = torch.complex(torch.randn(nb, eng.ng, generator=g, dtype=torch.float64),
+                        torch.randn(nb, eng.ng, generator=g, dtype=torch.float64)).cuda() / eng.ng ** 0.5
+    b = pb.becp_t(psi.to(torch.complex64)).to(torch.complex128)
+    bref = psi @ Pp.conj().T
+    eb = ((b - bref).abs().max() / bref.abs().max()).item()
+    S = torch.complex(torch.randn(nb, eng.nproj, generator=g, dtype=torch.float64),
+                      torch.randn(nb, eng.nproj, generator=g, dtype=torch.float64)).cuda()
+    out0 = torch.complex(torch.randn(nb, eng.ng, generator=g, dtype=torch.float64),
+                         torch.randn(nb, eng.ng, generator=g, dtype=torch.float64)).cuda()
+    o = pb.apply_add(S.to(torch.complex64), out0.to(torch.complex64).contiguous()).to(torch.complex128)
+    oref = out0 + S @ Pp
+    eo = ((o - oref).abs().max() / (S @ Pp).abs().max()).item()
+    vnl = torch.matmul(bref @ eng.D64, Pp)
+    assert ((vnl - (psi @ P.conj().T) @ eng.D64 @ P).abs().max() / vnl.abs().max()).item() < 1e-12
+    print(f"{cfg} gamma half sphere: nh {pb.nh} of nG {eng.ng}; becp rel err {eb:.2e}, apply rel err {eo:.2e}",
+          flush=True)
+    assert eb < 1e-4 and eo < 1e-4
+
+
+def test_gamma_half_sphere_scf_matches_full_sphere(monkeypatch):
+    """The mixed SCF with the half-sphere projector products reaches the
+    fixed point of the full-sphere form (both end in FP64 maps)."""
+    import numpy as np
+    import addft
+    from addft import testsystems
+    res = {}
+    for flag in ("1", "0"):
+        monkeypatch.setenv("ADDFT_GAMMA_HALF", flag)
+        s = addft.setup(**testsystems.config("c2", precision="mixed"))
+        F, f, sig, info = addft.energy_forces_stress(s, tol=1e-10)
+        res[flag] = (F, f, sig)
+    (F1, f1, s1), (F0, f0, s0) = res["1"], res["0"]
+    print(f"half {F1:.12f} full {F0:.12f} df {np.abs(f1 - f0).max():.1e}", flush=True)
+    assert abs(F1 - F0) / 8 < 1e-9
+    assert np.abs(f1 - f0).max() < 1e-6 and np.abs(s1 - s0).max() < 1e-7
