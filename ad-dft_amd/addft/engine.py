@@ -32,7 +32,7 @@ from .kernels import _lib
 from .kernels import native
 from .kernels._lib import check, lib, ptr, stream
 from .solver.linalg import inner
-from .kernels.nonlocal_bf16 import GammaProjectorBF16x3, ProjectorBF16x3, gamma_pairs
+from .kernels.nonlocal_bf16 import GammaProjector64, GammaProjectorBF16x3, ProjectorBF16x3, gamma_pairs
 
 
 def _simpson_weights(n):
@@ -74,9 +74,10 @@ class Engine:
         self.miller_d = torch.as_tensor(np.ascontiguousarray(b.miller, dtype=np.int32), device=self.dev)
         self.miller_rho_d = torch.as_tensor(np.ascontiguousarray(b.miller_rho, dtype=np.int32), device=self.dev)
         self.perm_rho_d = torch.as_tensor(self.plan_rho.perm, device=self.dev)
-        # Gamma-only: FP32-phase projector products over the half sphere (ADDFT_GAMMA_HALF=0: full sphere)
+        # Gamma-only: H psi projector products over the half sphere, FP64 and
+        # BF16x3 (ADDFT_GAMMA_HALF=0: full sphere)
         self.gamma_pairs = None
-        if (self.nl_bf16 and self.nk == 1 and not np.any(static.kpts[0]) and static.kshell is None
+        if (self.nk == 1 and not np.any(static.kpts[0]) and static.kshell is None
                 and os.environ.get("ADDFT_GAMMA_HALF", "1") != "0"):
             self.gamma_pairs = gamma_pairs(np.asarray(b.miller)[self.perm])
         # per-k shells: 0/1 masks in stick order applied to every H psi (None: union shell)
@@ -187,7 +188,7 @@ class Engine:
         check(L.addft_gvectors(ptr(self.miller_d), ptr(self.perm_d), ng, ptr(self.bmat_d), None, ptr(self.g0_d),
                                None, ptr(qn0), stream()), "addft_gvectors")
         self.kq, self.kqn, self.tkin64, self.tkin32, self.beta, self.dbeta = [], [], [], [], [], []
-        self.P64, self.P32, self.PB = [], [], []
+        self.P64, self.P32, self.PB, self.PG64 = [], [], [], []
         isq = 1.0 / math.sqrt(self.omega)
         for k in self.st.kpts:
             kk = np.ascontiguousarray(k, dtype=np.float64)
@@ -210,6 +211,8 @@ class Engine:
             self.dbeta.append(dbeta)
             self.P64.append(P)
             self.P32.append(P.to(torch.complex64) if self.mixed else None)
+            self.PG64.append(GammaProjector64(P, *self.gamma_pairs, self.odd_rows_d)
+                             if self.gamma_pairs is not None else None)
             if self.nl_bf16:
                 nsl = max(1, min(32, (6 * ng) // 15000))
                 if self.gamma_pairs is not None:
@@ -297,6 +300,9 @@ class Engine:
         if fp32 and nl_bf16 and self.PB[ik] is not None:
             pb = self.PB[ik]
             pb.apply_add(torch.matmul(pb.becp_t(psi), D), out)
+        elif not fp32 and self.PG64[ik] is not None:
+            g = self.PG64[ik]                           # k = 0: half-sphere FP64 products
+            g.apply_add(torch.matmul(g.becp_t(psi), D), out)
         else:
             becp_t = inner(psi, P)                      # (nb, nproj) = <beta|psi>^T
             torch.addmm(out, torch.matmul(becp_t, D), P, out=out)   # accumulated by the GEMM (beta = 1)

@@ -46,7 +46,8 @@ _SIGS = {
     "addft_split_bf16x3": ([_VP, _I64, _I64, _I64, _VP, _VP], _I32),
     "addft_cheb_combine": ([_I32, _I64, _I64, _VP, _D, _D, _D, _VP, _VP, _VP, _VP, _VP], _I32),
     "addft_gamma_pair_split": ([_VP, _I64, _I64, _VP, _VP, _I64, _I64, _VP, _VP], _I32),
-    "addft_gamma_scatter_add": ([_VP, _I64, _I64, _VP, _VP, _I64, _VP, _VP], _I32),
+    "addft_gamma_pair_rows": ([_VP, _I64, _I64, _VP, _VP, _I64, _VP, _VP], _I32),
+    "addft_gamma_scatter_add": ([_I32, _VP, _I64, _I64, _VP, _VP, _I64, _VP, _VP], _I32),
     "addft_pack_gather": ([_I32, _VP, _VP, _I64, _I64, _I64, _VP, _VP], _I32),
     "addft_xc_work_bytes": ([_I64], _I64),
     "addft_xc_lda": ([_VP, _I64, _VP, _VP, _VP, _VP], _I32),

@@ -222,7 +222,55 @@ class GammaProjectorBF16x3:
         U = torch.mm(S3, self.V3, out_dtype=torch.float32)
         if not out.is_contiguous():
             raise ValueError("out must be contiguous")
-        _lib.check(_lib.lib().addft_gamma_scatter_add(_lib.ptr(U), nb, self.nh, _lib.ptr(self.plus),
+        _lib.check(_lib.lib().addft_gamma_scatter_add(_lib.C64, _lib.ptr(U), nb, self.nh, _lib.ptr(self.plus),
+                                                      _lib.ptr(self.minus), self.ng, _lib.ptr(out),
+                                                      _lib.stream()), "addft_gamma_scatter_add")
+        return out
+
+
+class GammaProjector64:
+    """The FP64 form of the k = 0 half-sphere products (same P', same
+    becp_t / apply_add interface as GammaProjectorBF16x3): real FP64 GEMMs
+    over the nh pairs, exact to rounding -- half the flops of the complex128
+    products they replace."""
+
+    def __init__(self, P: torch.Tensor, plus, minus, odd_rows: torch.Tensor):
+        npj, ng = P.shape
+        dev = P.device
+        self.np, self.ng = int(npj), int(ng)
+        self.plus = torch.as_tensor(plus, dtype=torch.int32, device=dev)
+        self.minus = torch.as_tensor(minus, dtype=torch.int32, device=dev)
+        self.nh = int(self.plus.numel())
+        if int(self.plus.max()) >= ng or int(self.minus.max()) >= ng:
+            raise ValueError("pair table does not fit the sphere")
+        ph = torch.where(odd_rows.to(dev), torch.tensor(1j, dtype=P.dtype, device=dev),
+                         torch.tensor(1.0, dtype=P.dtype, device=dev))
+        Q = (P.to(torch.complex128) * ph[:, None])[:, self.plus.long()]
+        Qr = Q.real.contiguous()
+        Qi = Q.imag.clone()
+        Qi[:, 0] = 0.0
+        self.WT = torch.cat([Qr, Qi], dim=1).contiguous()     # (np, 2 nh): W^T and V at once
+
+    def becp_t(self, psi: torch.Tensor) -> torch.Tensor:
+        """psi (nb, nG) complex128 -> psi P'^H (nb, nproj) complex128."""
+        from ..solver.linalg import inner
+        nb = psi.shape[0]
+        A = torch.empty((2 * nb, 2 * self.nh), dtype=torch.float64, device=psi.device)
+        psi = psi.contiguous()
+        _lib.check(_lib.lib().addft_gamma_pair_rows(_lib.ptr(psi), nb, self.ng, _lib.ptr(self.plus),
+                                                    _lib.ptr(self.minus), self.nh, _lib.ptr(A), _lib.stream()),
+                   "addft_gamma_pair_rows")
+        C = inner(A, self.WT)                                   # A W, the reduction split over slices
+        return torch.complex(C[:nb], C[nb:])
+
+    def apply_add(self, S: torch.Tensor, out: torch.Tensor) -> torch.Tensor:
+        """out (nb, nG) complex128 += S' P'."""
+        nb = S.shape[0]
+        Sr = torch.cat([S.real, S.imag], dim=0).contiguous()
+        U = torch.matmul(Sr, self.WT)
+        if not out.is_contiguous():
+            raise ValueError("out must be contiguous")
+        _lib.check(_lib.lib().addft_gamma_scatter_add(_lib.C128, _lib.ptr(U), nb, self.nh, _lib.ptr(self.plus),
                                                       _lib.ptr(self.minus), self.ng, _lib.ptr(out),
                                                       _lib.stream()), "addft_gamma_scatter_add")
         return out

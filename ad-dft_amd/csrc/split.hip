@@ -73,27 +73,48 @@ __global__ void k_gamma_pair_split(const float2* __restrict__ psi, int64_t ng, c
 // out(G_j) += u + i v,  out(-G_j) += u - i v  (G = 0, j = 0: only the first).
 // Every sphere position is one of plus[j], minus[j] exactly once, so the
 // adds need no atomics.
-__global__ void k_gamma_scatter_add(const float* __restrict__ U, int64_t nb, int64_t nh,
+template <typename R>
+__global__ void k_gamma_scatter_add(const R* __restrict__ U, int64_t nb, int64_t nh,
                                     const int32_t* __restrict__ plus, const int32_t* __restrict__ minus, int64_t ng,
-                                    float2* __restrict__ out) {
+                                    addft::cpx<R>* __restrict__ out) {
   const int64_t b = blockIdx.y;
-  const float* ur_row = U + b * 2 * nh;
-  const float* ui_row = U + (nb + b) * 2 * nh;
-  float2* row = out + b * ng;
+  const R* ur_row = U + b * 2 * nh;
+  const R* ui_row = U + (nb + b) * 2 * nh;
+  addft::cpx<R>* row = out + b * ng;
   for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < nh; j += (int64_t)gridDim.x * blockDim.x) {
-    const float ur = ur_row[j], vr = ur_row[nh + j];
-    const float ui = ui_row[j], vi = ui_row[nh + j];
-    float2 p = row[plus[j]];
+    const R ur = ur_row[j], vr = ur_row[nh + j];
+    const R ui = ui_row[j], vi = ui_row[nh + j];
+    addft::cpx<R> p = row[plus[j]];
     p.x += ur - vi;
     p.y += ui + vr;
     row[plus[j]] = p;
     const int32_t mj = minus[j];
     if (mj >= 0) {
-      float2 q = row[mj];
+      addft::cpx<R> q = row[mj];
       q.x += ur + vi;
       q.y += ui - vr;
       row[mj] = q;
     }
+  }
+}
+
+// FP64 pair rows (no split): row b [c.re | d.im], row nb + b [c.im | -d.re]
+// of length 2 nh, for the real-GEMM form of the FP64 products.
+__global__ void k_gamma_pair_rows(const double2* __restrict__ psi, int64_t ng, const int32_t* __restrict__ plus,
+                                  const int32_t* __restrict__ minus, int64_t nh, int64_t nb,
+                                  double* __restrict__ out) {
+  const int64_t b = blockIdx.y;
+  const double2* row = psi + b * ng;
+  double* o_re = out + b * 2 * nh;
+  double* o_im = out + (nb + b) * 2 * nh;
+  for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < nh; j += (int64_t)gridDim.x * blockDim.x) {
+    const double2 a = row[plus[j]];
+    const int32_t mj = minus[j];
+    const double2 bb = mj >= 0 ? row[mj] : make_double2(0.0, 0.0);
+    o_re[j] = a.x + bb.x;
+    o_re[nh + j] = a.y - bb.y;
+    o_im[j] = a.y + bb.y;
+    o_im[nh + j] = -(a.x - bb.x);
   }
 }
 
@@ -131,15 +152,36 @@ int addft_gamma_pair_split(const void* d_psi, int64_t nb, int64_t ng, const int3
   return ADDFT_OK;
 }
 
-int addft_gamma_scatter_add(const float* d_u, int64_t nb, int64_t nh, const int32_t* d_plus, const int32_t* d_minus,
-                            int64_t ng, void* d_out, void* stream) {
+int addft_gamma_scatter_add(int dtype, const void* d_u, int64_t nb, int64_t nh, const int32_t* d_plus,
+                            const int32_t* d_minus, int64_t ng, void* d_out, void* stream) {
+  ADDFT_CHECK_ARG(dtype == ADDFT_C64 || dtype == ADDFT_C128, "dtype must be ADDFT_C64 or ADDFT_C128");
   ADDFT_CHECK_ARG(nb >= 0 && nb <= 32767 && nh >= 0 && 2 * nh <= ng + 1, "bad sizes (nb <= 32767, 2 nh <= nG + 1)");
   if (nb == 0 || nh == 0) return ADDFT_OK;
   ADDFT_CHECK_ARG(d_u && d_plus && d_minus && d_out, "null argument");
   int64_t gx = (nh + 255) / 256;
   if (gx > 1024) gx = 1024;
-  hipLaunchKernelGGL(k_gamma_scatter_add, dim3((unsigned)gx, (unsigned)nb), dim3(256), 0, addft::as_stream(stream),
-                     d_u, nb, nh, d_plus, d_minus, ng, static_cast<float2*>(d_out));
+  hipStream_t st = addft::as_stream(stream);
+  if (dtype == ADDFT_C64)
+    hipLaunchKernelGGL(k_gamma_scatter_add<float>, dim3((unsigned)gx, (unsigned)nb), dim3(256), 0, st,
+                       static_cast<const float*>(d_u), nb, nh, d_plus, d_minus, ng,
+                       static_cast<addft::cpx<float>*>(d_out));
+  else
+    hipLaunchKernelGGL(k_gamma_scatter_add<double>, dim3((unsigned)gx, (unsigned)nb), dim3(256), 0, st,
+                       static_cast<const double*>(d_u), nb, nh, d_plus, d_minus, ng,
+                       static_cast<addft::cpx<double>*>(d_out));
+  ADDFT_LAUNCH_CHECK();
+  return ADDFT_OK;
+}
+
+int addft_gamma_pair_rows(const void* d_psi, int64_t nb, int64_t ng, const int32_t* d_plus, const int32_t* d_minus,
+                          int64_t nh, void* d_out, void* stream) {
+  ADDFT_CHECK_ARG(nb >= 0 && nb <= 32767 && nh >= 0 && 2 * nh <= ng + 1, "bad sizes (nb <= 32767, 2 nh <= nG + 1)");
+  if (nb == 0 || nh == 0) return ADDFT_OK;
+  ADDFT_CHECK_ARG(d_psi && d_plus && d_minus && d_out, "null argument");
+  int64_t gx = (nh + 255) / 256;
+  if (gx > 1024) gx = 1024;
+  hipLaunchKernelGGL(k_gamma_pair_rows, dim3((unsigned)gx, (unsigned)nb), dim3(256), 0, addft::as_stream(stream),
+                     static_cast<const double2*>(d_psi), ng, d_plus, d_minus, nh, nb, static_cast<double*>(d_out));
   ADDFT_LAUNCH_CHECK();
   return ADDFT_OK;
 }

@@ -79,12 +79,17 @@ int addft_pack_gather(int dtype, const void* d_grid, const int32_t* d_idx,
  *   row b [c.re | d.im], row nb+b [c.im | -d.re] with c, d = psi(G) +- psi(-G),
  *   each of length K = 2 nh written as [hi | hi | lo] at columns [0, 3K);
  *   columns [3K, ld) untouched.
- * addft_gamma_scatter_add: U (2 nb, 2 nh) FP32 = [S'.re; S'.im] [Qr | Qi]
- *   -> out (nb, ng) complex64 += S' P' unfolded onto both halves.
- * errors: nb > 32767, 2 nh > ng + 1, ld < 6 nh, null pointers.          */
+ * addft_gamma_pair_rows: the FP64 form, psi (nb, ng) complex128 -> double
+ *   rows (2 nb, 2 nh), the same two row kinds without splitting.
+ * addft_gamma_scatter_add: U (2 nb, 2 nh) = [S'.re; S'.im] [Qr | Qi]
+ *   (FP32 for dtype ADDFT_C64, FP64 for ADDFT_C128) -> out (nb, ng) of that
+ *   complex dtype += S' P' unfolded onto both halves.
+ * errors: bad dtype, nb > 32767, 2 nh > ng + 1, ld < 6 nh, null pointers. */
 int addft_gamma_pair_split(const void* d_psi, int64_t nb, int64_t ng, const int32_t* d_plus,
                            const int32_t* d_minus, int64_t nh, int64_t ld, void* d_out, void* stream);
-int addft_gamma_scatter_add(const float* d_u, int64_t nb, int64_t nh, const int32_t* d_plus,
+int addft_gamma_pair_rows(const void* d_psi, int64_t nb, int64_t ng, const int32_t* d_plus,
+                          const int32_t* d_minus, int64_t nh, void* d_out, void* stream);
+int addft_gamma_scatter_add(int dtype, const void* d_u, int64_t nb, int64_t nh, const int32_t* d_plus,
                             const int32_t* d_minus, int64_t ng, void* d_out, void* stream);
 
 /* ------------------------------------------------- Chebyshev filter step

@@ -127,3 +127,50 @@ def test_gamma_half_sphere_scf_matches_full_sphere(monkeypatch):
     print(f"half {F1:.12f} full {F0:.12f} df {np.abs(f1 - f0).max():.1e}", flush=True)
     assert abs(F1 - F0) / 8 < 1e-9
     assert np.abs(f1 - f0).max() < 1e-6 and np.abs(s1 - s0).max() < 1e-7
+
+
+def test_gamma_half_sphere_fp64_products():
+    """GammaProjector64: the half-sphere FP64 products equal the complex128
+    products with P' = i^(l mod 2) P to rounding."""
+    import torch
+    import addft
+    from addft import testsystems
+    from addft.kernels.nonlocal_bf16 import GammaProjector64
+    s = addft.setup(**testsystems.config("c2", precision="fp64"))
+    eng = addft.api._engine(s.static, s.geo)
+    g0 = eng.PG64[0]
+    assert isinstance(g0, GammaProjector64)
+    P = eng.P64[0]
+    ph = torch.where(eng.odd_rows_d, torch.tensor(1j, dtype=P.dtype, device=P.device),
+                     torch.tensor(1.0, dtype=P.dtype, device=P.device))
+    Pp = P * ph[:, None]
+    gen = torch.Generator(device="cpu").manual_seed(5)
+    nb = 24
+    psi = torch.complex(torch.randn(nb, eng.ng, generator=gen, dtype=torch.float64),
+                        torch.randn(nb, eng.ng, generator=gen, dtype=torch.float64)).cuda()
+    S = torch.complex(torch.randn(nb, eng.nproj, generator=gen, dtype=torch.float64),
+                      torch.randn(nb, eng.nproj, generator=gen, dtype=torch.float64)).cuda()
+    b = g0.becp_t(psi)
+    bref = psi @ Pp.conj().T
+    out = torch.zeros_like(psi)
+    g0.apply_add(S, out)
+    oref = S @ Pp
+    eb = ((b - bref).abs().max() / bref.abs().max()).item()
+    eo = ((out - oref).abs().max() / oref.abs().max()).item()
+    print(f"fp64 gamma half sphere: becp {eb:.1e}, apply {eo:.1e}", flush=True)
+    assert eb < 1e-13 and eo < 1e-13
+
+
+def test_gamma_half_sphere_fp64_scf_matches_full_sphere(monkeypatch):
+    import numpy as np
+    import addft
+    from addft import testsystems
+    res = {}
+    for flag in ("1", "0"):
+        monkeypatch.setenv("ADDFT_GAMMA_HALF", flag)
+        s = addft.setup(**testsystems.config("c2", precision="fp64"))
+        res[flag] = addft.energy_forces_stress(s, tol=1e-11)[:3]
+    (F1, f1, s1), (F0, f0, s0) = res["1"], res["0"]
+    print(f"fp64 half {F1:.13f} full {F0:.13f} df {np.abs(f1 - f0).max():.1e}", flush=True)
+    assert abs(F1 - F0) / 8 < 1e-11
+    assert np.abs(f1 - f0).max() < 1e-8 and np.abs(s1 - s0).max() < 1e-9
