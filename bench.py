@@ -127,6 +127,22 @@ def nonlocal_measure(addft, kw, steps, torch):
         ms = timed(fn, steps, 3, torch)
         res[name] = {"ms": round(ms, 4), "TF_per_s": round(flops / (ms * 1e-3) / 1e12, 1),
                      "frac_of_bf16_dense_peak": round(flops / (ms * 1e-3) / 2.5e15, 4)}
+    # FP64 products (k = 0: half sphere, GammaProjector64), beside the full-sphere complex128 GEMMs
+    from addft.solver.linalg import inner
+    psi64, S64 = psi.to(torch.complex128), S.to(torch.complex128)
+    out64 = torch.zeros_like(psi64)
+    P64 = eng.P64[0]
+    flops64 = 8.0 * m * eng.ng * eng.nproj
+    fp64 = {}
+    cands = [("becp_full", lambda: inner(psi64, P64)),
+             ("apply_full", lambda: torch.addmm(out64, S64, P64, out=out64))]
+    if eng.PG64[0] is not None:
+        g64 = eng.PG64[0]
+        cands += [("becp_half", lambda: g64.becp_t(psi64)), ("apply_half", lambda: g64.apply_add(S64, out64))]
+    for name, fn in cands:
+        ms = timed(fn, steps, 3, torch)
+        fp64[name] = {"ms": round(ms, 4), "TF_per_s": round(flops64 / (ms * 1e-3) / 1e12, 1)}
+    res["fp64"] = fp64
     half = hasattr(pb, "nh")
     return {"metric": "BF16x3 projector products, effective bf16 TF/s (3 products per complex product)",
             "config": {"workload": f"{len(kw['frac'])}-atom cell, {m} bands, nG {eng.ng}, nproj {eng.nproj}"},
