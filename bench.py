@@ -185,6 +185,31 @@ def subspace_measure(addft, kw, steps, torch):
             "products": res}
 
 
+def delta_q_measure(addft, kw, torch, np):
+    """The matrix-free Delta-Q adjoint at the workload's size: SCF to 1e-10,
+    then dC/dR and dC/deps of C = sum_r w(r) rho(r) for a Gaussian weight
+    (sigma 1.5 Bohr) on atom 5, by the preconditioned inexact GMRES adjoint
+    (rtol 1e-10), timed on the host clock with the device synchronised."""
+    from addft import api, response
+    s = addft.setup(**kw)
+    st = api._solve(s, tol=1e-10)
+    R = np.asarray(kw["frac"]) @ np.asarray(kw["lattice"])
+    w = addft.make_gaussian_weight(s, R[min(5, len(R) - 1)], 1.5)
+    eng = st.engine
+    w_pl = eng.to_planar(torch.as_tensor(np.asarray(w, dtype=np.float64)).to(eng.dev))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    dR, de, diag = response.delta_q_grad(eng, st.psi, st.eig, st.occ, st.rho, w_pl,
+                                         log=lambda msg: say(f"  {time.perf_counter() - t0:7.1f} s {msg}"))
+    torch.cuda.synchronize()
+    sec = time.perf_counter() - t0
+    return {"metric": "Delta-Q adjoint gradient, s after the SCF",
+            "config": {"workload": f"{len(kw['frac'])}-atom cell, nG {eng.ng}, {kw['nband']} bands",
+                       "rtol": 1e-10, "precision": kw.get("precision", "fp64")},
+            "seconds": round(sec, 2), "max_abs_dC_dR": float(np.abs(dR).max()),
+            "diagnostics": {k: (v if isinstance(v, (int, float, str)) else str(v)) for k, v in diag.items()}}
+
+
 def one_geometry(addft, system, tol, torch):
     """Per-geometry tables + SCF + forces + stress, timed on the host clock
     with the device synchronised on both sides."""
@@ -324,6 +349,8 @@ def main():
                          "and 2 stress components of the workload, against the analytic forces and stress")
     ap.add_argument("--fd-only", choices=["all", "forces", "strain"], default="all")
     ap.add_argument("--fd-h", type=float, default=1e-3, help="step of the --fd-check differences")
+    ap.add_argument("--delta-q", action="store_true",
+                    help="time the Delta-Q adjoint gradient after an SCF at the workload's size")
     ap.add_argument("--subspace-only", action="store_true",
                     help="time the eigensolver's block products (Gram/Rayleigh-Ritz reduction, rotation) only")
     ap.add_argument("--nonlocal-only", action="store_true",
@@ -358,6 +385,10 @@ def main():
 
     if args.nonlocal_only:
         print(json.dumps(nonlocal_measure(addft, kw, args.hpsi_steps, torch)), flush=True)
+        return
+
+    if args.delta_q:
+        print(json.dumps(delta_q_measure(addft, kw, torch, np)), flush=True)
         return
 
     if args.subspace_only:
