@@ -189,7 +189,11 @@ def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, 
       * FP64 phase of the mixed form (refine): at most refine_max_iter
         refinement-Davidson steps in refine_max_sub vectors with FP32
         correction products (BF16x3 ones with refine_nl_bf16), the first
-        residuals from Rayleigh quotients unless refine_first_rr.
+        residuals from Rayleigh quotients unless refine_first_rr.  At least
+        2: one step only evaluates the current vectors (an expansion and a
+        Rayleigh-Ritz are what improve them), and a map that never improves
+        them converges the occupations alone -- a density residual that
+        falls without the state becoming self-consistent.
     Mixed precision: FP32 maps until the residual reaches switch_tol (or
       stalls), then FP64; reset_on_switch clears the mixer history.
     Polish: up to `polish` unmixed map steps while they reduce the residual,
@@ -202,6 +206,9 @@ def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, 
     Batching: batch_k (all k-points as one block on the shared sphere) and
       graphs (replay the batched Chebyshev filter as a captured graph,
       solver/graphs.py)."""
+    if refine_max_iter is not None and int(refine_max_iter) < 2:
+        raise ValueError("refine_max_iter must be at least 2: a single refinement step leaves the "
+                         "eigenvectors unchanged, and the SCF would converge the occupations only")
     st = engine.st
     nb = st.nband
     dev = engine.dev

@@ -95,3 +95,12 @@ def test_underfilled_config_saturates():
     assert np.all(state.occ > 1 - 1e-12)
     om = B.volume(geo.lattice)
     assert abs(np.sum(state.rho_r) * om / st.basis.nfft - 4.0) < 1e-8
+
+
+def test_solve_scf_refuses_a_refinement_that_cannot_improve_the_vectors():
+    """One refinement-Davidson step only evaluates the current vectors; the
+    option is rejected before any device work."""
+    import pytest
+    from addft.solver.scf import solve_scf
+    with pytest.raises(ValueError, match="refine_max_iter"):
+        solve_scf(None, refine_max_iter=1)
