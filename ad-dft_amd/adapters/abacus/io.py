@@ -147,12 +147,10 @@ _BLOCKS = ("ATOMIC_SPECIES", "NUMERICAL_ORBITAL", "LATTICE_CONSTANT", "LATTICE_V
 def _split_blocks(lines):
     blocks, cur = {}, None
     for s in lines:
-        head = s.split()[0]
+        head, *rest = s.split()
         if head in _BLOCKS:
             cur = head
-            blocks[cur] = [s.split()[1:]] if len(s.split()) > 1 else []
-            if blocks[cur] == [[]]:
-                blocks[cur] = []
+            blocks[cur] = [rest] if rest else []       # a value on the header line (LATTICE_CONSTANT 10.2)
             continue
         if cur is not None:
             blocks[cur].append(s.split())

@@ -27,6 +27,7 @@ whose FP32 partial blocks are summed.
 
 from __future__ import annotations
 
+import numpy as np
 import torch
 
 from . import _lib
@@ -123,7 +124,6 @@ def gamma_pairs(miller_stick):
     """(plus, minus) int32 stick positions pairing each G of the positive
     half (first nonzero Miller index > 0) with -G; entry 0 is G = 0 with
     minus = -1.  None when the sphere is not closed under G -> -G."""
-    import numpy as np
     m = np.asarray(miller_stick, dtype=np.int64).reshape(-1, 3)
     pos = {tuple(r): i for i, r in enumerate(m.tolist())}
     z = pos.get((0, 0, 0))
@@ -212,6 +212,8 @@ class GammaProjectorBF16x3:
 
     def apply_add(self, S: torch.Tensor, out: torch.Tensor) -> torch.Tensor:
         """out (nb, nG) complex64 += S' P'."""
+        if not out.is_contiguous():
+            raise ValueError("out must be contiguous")
         nb = S.shape[0]
         Sr = torch.zeros((2 * nb, self.kp), dtype=torch.float32, device=S.device)
         Sr[:nb, :self.np] = S.real
@@ -220,8 +222,6 @@ class GammaProjectorBF16x3:
         _lib.check(_lib.lib().addft_split_bf16x3(_lib.ptr(Sr), 2 * nb, self.kp, 3 * self.kp, _lib.ptr(S3),
                                                  _lib.stream()), "addft_split_bf16x3")
         U = torch.mm(S3, self.V3, out_dtype=torch.float32)
-        if not out.is_contiguous():
-            raise ValueError("out must be contiguous")
         _lib.check(_lib.lib().addft_gamma_scatter_add(_lib.C64, _lib.ptr(U), nb, self.nh, _lib.ptr(self.plus),
                                                       _lib.ptr(self.minus), self.ng, _lib.ptr(out),
                                                       _lib.stream()), "addft_gamma_scatter_add")
@@ -265,11 +265,11 @@ class GammaProjector64:
 
     def apply_add(self, S: torch.Tensor, out: torch.Tensor) -> torch.Tensor:
         """out (nb, nG) complex128 += S' P'."""
+        if not out.is_contiguous():
+            raise ValueError("out must be contiguous")
         nb = S.shape[0]
         Sr = torch.cat([S.real, S.imag], dim=0).contiguous()
         U = torch.matmul(Sr, self.WT)
-        if not out.is_contiguous():
-            raise ValueError("out must be contiguous")
         _lib.check(_lib.lib().addft_gamma_scatter_add(_lib.C128, _lib.ptr(U), nb, self.nh, _lib.ptr(self.plus),
                                                       _lib.ptr(self.minus), self.ng, _lib.ptr(out),
                                                       _lib.stream()), "addft_gamma_scatter_add")
