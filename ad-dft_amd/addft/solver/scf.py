@@ -167,7 +167,7 @@ def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, 
               nbuffer=None, eigensolver="auto", cheb_degree=None, refine=True, reset_on_switch=True,
               refine_max_sub=None, cheb_passes=None, batch_k=True, cold_max_sub=None, dav_factor=0.1,
               refine_dav_factor=None, cold_max_iter=4, polish_gain_max=None, psi0=None, graphs=True,
-              cheb_rr_all=True, refine_max_iter=2, mix_history=8, pulay_gg0=None, refine_nl_bf16=False,
+              cheb_rr_all=True, refine_max_iter=2, mix_history=8, pulay_gg0=None, refine_nl_bf16=True,
               refine_first_rr=False, rho_init="uniform", rho_sigma=1.2):
     """Run the SCF on engine's current geometry.  Returns (SCFState with
     stick-order psi list in `psi` and plane-major rho, info dict).
@@ -188,8 +188,9 @@ def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, 
         after every pass (cheb_rr_all) or only the first and last;
       * FP64 phase of the mixed form (refine): at most refine_max_iter
         refinement-Davidson steps in refine_max_sub vectors with FP32
-        correction products (BF16x3 ones with refine_nl_bf16), the first
-        residuals from Rayleigh quotients unless refine_first_rr.  At least
+        correction products (BF16x3 projector products unless refine_nl_bf16
+        is False), the first residuals from Rayleigh quotients unless
+        refine_first_rr.  At least
         2: one step only evaluates the current vectors (an expansion and a
         Rayleigh-Ritz are what improve them), and a map that never improves
         them converges the occupations alone -- a density residual that
