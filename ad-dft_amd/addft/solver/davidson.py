@@ -134,7 +134,7 @@ def dense_lowest(apply, nG, nb, dtype, device):
 
 
 def davidson(apply, X0, nb, diag, max_sub=None, max_iter=40, tol=1e-9, min_iter=1, nconv=None, weight=None,
-             apply_lo=None, first_rr=True, orth_x0=True):
+             apply_lo=None, first_rr=True, orth_x0=True, last_residuals=True):
     """Return (eig (nb,) float64, X (nb, nG), max residual norm, iterations).
 
     apply: (m, nG) -> (m, nG) in X0's dtype.  diag: (nG,) float64.
@@ -152,7 +152,10 @@ def davidson(apply, X0, nb, diag, max_sub=None, max_iter=40, tol=1e-9, min_iter=
     first_rr=False: the first residuals use the Rayleigh quotients of X0
     itself (no eigensolve of X0^H H X0): for X0 already close to
     eigenvectors (the previous SCF map's) the first Rayleigh-Ritz then runs
-    on the expanded block, which also rotates X0."""
+    on the expanded block, which also rotates X0.
+    last_residuals=False: the iteration that reaches max_iter returns right
+    after rotating X (no H X rotation, no residual norms; the residual
+    returned is nan) -- for callers that only use the vectors and values."""
     nconv = nb if nconv is None else int(nconv)
     nG = X0.shape[1]
     if nG <= max(DENSE_MAX_NG, 4 * nb):
@@ -182,6 +185,9 @@ def davidson(apply, X0, nb, diag, max_sub=None, max_iter=40, tol=1e-9, min_iter=
             lam = w[:nb]
             Ub = U[:, :nb].to(Q.dtype)
             X = torch.matmul(Ub.T, Q)
+            if it == max_iter and not last_residuals:
+                res = float("nan")
+                break
             HX = torch.matmul(Ub.T, HQ)
         R = HX - lam.to(X.real.dtype)[:, None] * X
         rn = torch.linalg.vector_norm(R.to(torch.complex128), dim=1)

@@ -345,7 +345,7 @@ def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, 
             lam, Xk, r, nit = davidson(apply, Xk, nbt, diag, max_iter=(refine_max_iter or dav_max_iter or 25),
                                        tol=tol_k, nconv=nb, weight=weights(ik), apply_lo=apply_lo,
                                        max_sub=(refine_max_sub or 2 * nbt), first_rr=refine_first_rr,
-                                       orth_x0=not refined[ik])
+                                       orth_x0=not refined[ik], last_residuals=verbose)
             refined[ik] = True
             dav_iters += nit
             return lam, Xk, r, nit
