@@ -73,7 +73,7 @@ def test_apply_add_accumulates_into_output():
     assert ((got - ref).abs().max() / ref.abs().max()).item() < 1e-6
 
 
-@pytest.mark.parametrize("cfg", ["c2", "c3"])
+@pytest.mark.parametrize("cfg", ["c2", "c3", "c5"])
 def test_gamma_half_sphere_products(cfg):
     """GammaProjectorBF16x3 on the engine's own k = 0 projectors (three
     projectors, l = 0, 0, 1): psi P'^H and out + S' P' against the complex128
