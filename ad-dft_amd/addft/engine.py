@@ -80,6 +80,10 @@ class Engine:
         if (self.nk == 1 and not np.any(static.kpts[0]) and static.kshell is None
                 and os.environ.get("ADDFT_GAMMA_HALF", "1") != "0"):
             self.gamma_pairs = gamma_pairs(np.asarray(b.miller)[self.perm])
+        self.gplus_d = self.gminus_d = None
+        if self.gamma_pairs is not None:
+            self.gplus_d = torch.as_tensor(self.gamma_pairs[0], device=self.dev)
+            self.gminus_d = torch.as_tensor(self.gamma_pairs[1], device=self.dev)
         # per-k shells: 0/1 masks in stick order applied to every H psi (None: union shell)
         self.kmask64 = self.kmask32 = None
         if static.kshell is not None:
@@ -287,16 +291,24 @@ class Engine:
         return plan.stage(v_pl, 1.0 / self.nfft)
 
     # ---------------------------------------------------------- operators
-    def hpsi(self, ik, psi, vs, fp32=False, nl_bf16=True):
+    def hpsi(self, ik, psi, vs, fp32=False, nl_bf16=True, real=False):
         """T + V_loc,eff + V_NL on (nb, nG) stick-order psi.  In FP32 the
         projector products are BF16x3 split GEMMs unless nl_bf16 is False
-        (complex64 GEMMs)."""
+        (complex64 GEMMs).  real=True (k = 0, rows real in real space,
+        psi(-G) = conj(psi(G))): the local part runs two bands per transform
+        (addft_gamma_pack2 / addft_gamma_unpack2)."""
         plan = self.plan32 if fp32 else self.plan64
         tk = self.tkin32[ik] if fp32 else self.tkin64[ik]
         P = self.P32[ik] if fp32 else self.P64[ik]
         D = self.D32 if fp32 else self.D64
         self.hpsi_bands["fp32" if fp32 else "fp64"] += int(psi.shape[0])
-        out = plan.hpsi_local(psi, tk, vs)
+        if real:
+            if self.gplus_d is None:
+                raise ValueError("real=True needs the k = 0 pair tables")
+            h = plan.hpsi_local(native.gamma_pack2(psi.contiguous()), tk, vs)
+            out = native.gamma_unpack2(h, psi.shape[0], self.gplus_d, self.gminus_d)
+        else:
+            out = plan.hpsi_local(psi, tk, vs)
         if fp32 and nl_bf16 and self.PB[ik] is not None:
             pb = self.PB[ik]
             pb.apply_add(torch.matmul(pb.becp_t(psi), D), out)
@@ -307,6 +319,14 @@ class Engine:
             becp_t = inner(psi, P)                      # (nb, nproj) = <beta|psi>^T
             torch.addmm(out, torch.matmul(becp_t, D), P, out=out)   # accumulated by the GEMM (beta = 1)
         return self.restrict(ik, out)
+
+    def realify(self, x):
+        """Rows of x projected onto vectors real in real space, x(G) ->
+        (x(G) + conj x(-G)) / 2 (k = 0): the even rows of addft_gamma_unpack2
+        applied to x itself.  Keeps a block real against the rounding of
+        FP32 GEMMs, whose summation order differs between G and -G."""
+        m = x.shape[0]
+        return native.gamma_unpack2(x.contiguous(), 2 * m, self.gplus_d, self.gminus_d)[0::2].contiguous()
 
     def restrict(self, ik, x):
         """Zero the components of x (.., nG) outside k-point ik's own shell

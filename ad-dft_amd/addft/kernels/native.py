@@ -114,6 +114,30 @@ def cheb_combine(a: torch.Tensor, b: torch.Tensor, c: torch.Tensor | None, alpha
     return out
 
 
+# ------------------------------------------- Gamma point, two bands per FFT
+def gamma_pack2(psi: torch.Tensor) -> torch.Tensor:
+    """(m, nG) vectors real in real space -> (ceil(m/2), nG) pairs psi_2b + i psi_2b+1."""
+    _require(psi)
+    m, ng = psi.shape
+    out = torch.empty(((m + 1) // 2, ng), dtype=psi.dtype, device=psi.device)
+    check(lib().addft_gamma_pack2(_dtype_code(psi), ptr(psi), m, ng, ptr(out), stream()), "addft_gamma_pack2")
+    return out
+
+
+def gamma_unpack2(h: torch.Tensor, m: int, plus: torch.Tensor, minus: torch.Tensor) -> torch.Tensor:
+    """H of the packed pairs -> (m, nG) H psi (addft_gamma_unpack2)."""
+    _require(h)
+    _require(plus, torch.int32)
+    _require(minus, torch.int32)
+    ng = h.shape[1]
+    if h.shape[0] != (m + 1) // 2:
+        raise ValueError("h must hold ceil(m/2) packed rows")
+    out = torch.empty((m, ng), dtype=h.dtype, device=h.device)
+    check(lib().addft_gamma_unpack2(_dtype_code(h), ptr(h), m, ng, ptr(plus), ptr(minus), plus.numel(), ptr(out),
+                                    stream()), "addft_gamma_unpack2")
+    return out
+
+
 # ------------------------------------------------------------- real grids
 def to_planar(grid: torch.Tensor, shape) -> torch.Tensor:
     """row-major (i1,i2,i3) float64 -> plane-major (i3,i1,i2)."""

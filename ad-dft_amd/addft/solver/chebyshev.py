@@ -73,13 +73,15 @@ def filter_block(apply, X, HX, degree, a, b, a0):
     return Y
 
 
-def orthonormalise(Y):
+def orthonormalise(Y, real=False):
     """Rows of Y orthonormal: Cholesky-QR2 on the FP64 Gram matrix, with a
     shifted first pass when Y is too ill-conditioned for plain Cholesky
     (Fukaya et al., shifted CholeskyQR, SIAM J. Sci. Comput. 42 (2020))."""
     for it in range(3):
         S = inner(Y, Y).to(torch.complex128)
         S = 0.5 * (S + S.conj().T)
+        if real:              # rows real in real space (k = 0): a real Gram matrix
+            S = S.real.to(torch.complex128)
         d = torch.sqrt(torch.clamp(S.diagonal().real, min=1e-300))
         Ss = S / (d[:, None] * d[None, :])
         L, info = torch.linalg.cholesky_ex(Ss)
@@ -99,7 +101,7 @@ def orthonormalise(Y):
     return Y
 
 
-def chfsi_step(apply, X, degree, b_upper, lam, HX=None, rr=True, residuals=True):
+def chfsi_step(apply, X, degree, b_upper, lam, HX=None, rr=True, residuals=True, real=False, proj=None):
     """One filter + Rayleigh-Ritz step.  X: (nbt, nG) current Ritz vectors
     with Ritz values lam (ascending).  Returns (lam, X, HX, residual norms).
     rr=False stops after the orthonormalisation and returns (lam, Y, None,
@@ -107,17 +109,21 @@ def chfsi_step(apply, X, degree, b_upper, lam, HX=None, rr=True, residuals=True)
     filter acts on -- unchanged, so the passes between the first (which
     checks convergence) and the last of one SCF map need no eigensolve.
     residuals=False (a single pass whose convergence is not tested) skips
-    the rotation of H Y and the residual norms: returns (lam, X, None, None)."""
+    the rotation of H Y and the residual norms: returns (lam, X, None, None).
+    real=True: X real in real space (k = 0) and apply preserving that; the
+    Gram and projected matrices are taken real so the block stays real."""
     a = float(lam[-1])
     a0 = float(lam[0])
     Y = filter_block(apply, X, HX, degree, a, b_upper, a0)
-    Y = orthonormalise(Y)
+    if proj is not None:          # real mode: drop the rounding's non-real part
+        Y = proj(Y)
+    Y = orthonormalise(Y, real)
     if not rr:
         return lam, Y, None, None
     HY = apply(Y)
     T = inner(Y, HY, conj_a=True).to(torch.complex128)
     T = 0.5 * (T + T.conj().T)
-    w, U = eigh(T)
+    w, U = eigh(T.real) if real else eigh(T)
     Ut = U.T.to(Y.dtype)
     X = torch.matmul(Ut, Y)
     if not residuals:

@@ -67,13 +67,15 @@ def eigh(T):
         torch.backends.cuda.preferred_linalg_library(prev)
 
 
-def _svqb(R, drop=1e-10):
+def _svqb(R, drop=1e-10, real=False):
     """Orthonormalise the rows of R (Stathopoulos & Wu, SIAM J. Sci. Comput.
     23 (2002), SVQB): eigen-decompose the scaled FP64 Gram matrix, drop
     directions below drop * max, and rescale.  Robust for blocks whose rows
     differ by orders of magnitude in norm or are nearly dependent."""
     S = inner(R, R).to(torch.complex128)
     S = 0.5 * (S + S.conj().T)
+    if real:          # rows real in real space: the Gram matrix is real
+        S = S.real.to(torch.complex128)
     d = torch.sqrt(torch.clamp(S.diagonal().real, min=1e-300))
     Ss = S / (d[:, None] * d[None, :])
     off = Ss - torch.diag(Ss.diagonal())
@@ -88,20 +90,25 @@ def _svqb(R, drop=1e-10):
         eye = torch.eye(L.shape[0], dtype=L.dtype, device=L.device)
         Linv = torch.linalg.solve_triangular(L, eye, upper=False) / d[None, :]
         return torch.matmul(Linv.to(R.dtype), R).contiguous()
-    w, V = eigh(Ss)
+    w, V = eigh(Ss.real) if real else eigh(Ss)
     keep = w > drop * w.max()
     Vk = V[:, keep] / torch.sqrt(w[keep])[None, :]
     M = (Vk.conj().T / d[None, :]).to(R.dtype)      # (k, m)
     return torch.matmul(M, R).contiguous()
 
 
-def _orth(R, Q=None, drop=1e-10):
+def _orth(R, Q=None, drop=1e-10, real=False):
     """Rows of R orthonormal to each other and (if given) to the rows of Q:
-    two rounds of block Gram-Schmidt against Q, each followed by SVQB."""
+    two rounds of block Gram-Schmidt against Q, each followed by SVQB.
+    real: rows real in real space, so the coefficients are real (taken so,
+    which keeps the rows real)."""
     for _ in range(2):
         if Q is not None:
-            R = R - torch.matmul(inner(R, Q), Q)
-        R = _svqb(R, drop)
+            C = inner(R, Q)
+            if real:
+                C = C.real.to(C.dtype)
+            R = R - torch.matmul(C, Q)
+        R = _svqb(R, drop, real)
         if R.shape[0] == 0:
             break
     return R
@@ -134,7 +141,7 @@ def dense_lowest(apply, nG, nb, dtype, device):
 
 
 def davidson(apply, X0, nb, diag, max_sub=None, max_iter=40, tol=1e-9, min_iter=1, nconv=None, weight=None,
-             apply_lo=None, first_rr=True, orth_x0=True, last_residuals=True):
+             apply_lo=None, first_rr=True, orth_x0=True, last_residuals=True, real=False, proj=None):
     """Return (eig (nb,) float64, X (nb, nG), max residual norm, iterations).
 
     apply: (m, nG) -> (m, nG) in X0's dtype.  diag: (nG,) float64.
@@ -155,7 +162,12 @@ def davidson(apply, X0, nb, diag, max_sub=None, max_iter=40, tol=1e-9, min_iter=
     on the expanded block, which also rotates X0.
     last_residuals=False: the iteration that reaches max_iter returns right
     after rotating X (no H X rotation, no residual norms; the residual
-    returned is nan) -- for callers that only use the vectors and values."""
+    returned is nan) -- for callers that only use the vectors and values.
+    real=True: X0 real in real space (k = 0); every projected matrix is
+    taken real, so the iteration stays real; proj (x -> its projection onto
+    real vectors) is applied to each correction block and to the result, so
+    rounding cannot build up a non-real part that the normalised residuals
+    would amplify."""
     nconv = nb if nconv is None else int(nconv)
     nG = X0.shape[1]
     if nG <= max(DENSE_MAX_NG, 4 * nb):
@@ -164,7 +176,7 @@ def davidson(apply, X0, nb, diag, max_sub=None, max_iter=40, tol=1e-9, min_iter=
         max_sub = max(48, 4 * nb)
     max_sub = min(max_sub, nG // 2)
     # orth_x0=False: X0 is known orthonormal (the previous call's Ritz vectors)
-    X = _orth(X0.contiguous()) if orth_x0 else X0.contiguous()
+    X = _orth(X0.contiguous(), real=real) if orth_x0 else X0.contiguous()
     if X.shape[0] < nb:
         raise ValueError("initial block is rank deficient")
     HX = apply(X)
@@ -173,6 +185,8 @@ def davidson(apply, X0, nb, diag, max_sub=None, max_iter=40, tol=1e-9, min_iter=
     Q, HQ = X, HX
     T = inner(Q, HQ, conj_a=True).to(torch.complex128)
     T = 0.5 * (T + T.conj().T)
+    if real:
+        T = T.real.to(torch.complex128)
     res = float("inf")
     lam = None
     it = 0
@@ -181,7 +195,7 @@ def davidson(apply, X0, nb, diag, max_sub=None, max_iter=40, tol=1e-9, min_iter=
             lam = T.diagonal().real.clone()
             X, HX = Q, HQ
         else:
-            w, U = eigh(T)
+            w, U = eigh(T.real) if real else eigh(T)
             lam = w[:nb]
             Ub = U[:, :nb].to(Q.dtype)
             X = torch.matmul(Ub.T, Q)
@@ -201,11 +215,13 @@ def davidson(apply, X0, nb, diag, max_sub=None, max_iter=40, tol=1e-9, min_iter=
         active = rw > tol
         R = precondition(R[active], lam[active], diag)
         R = R / torch.linalg.vector_norm(R.to(torch.complex128), dim=1).to(R.real.dtype)[:, None]
+        if proj is not None:
+            R = proj(R)
         if Q.shape[0] + R.shape[0] > max_sub:
             # restart on the current Ritz vectors (davidson.py:66-69)
             Q, HQ = X, HX
             T = torch.diag(lam.to(torch.complex128))
-        R = _orth(R, Q)
+        R = _orth(R, Q, real=real)
         if R.shape[0] == 0:
             break
         _finite("R", R)
@@ -218,7 +234,11 @@ def davidson(apply, X0, nb, diag, max_sub=None, max_iter=40, tol=1e-9, min_iter=
         _finite("HR", HR)
         C = inner(R, HR, conj_a=True).to(torch.complex128)
         C = 0.5 * (C + C.conj().T)
+        if real:
+            B, C = B.real.to(torch.complex128), C.real.to(torch.complex128)
         T = torch.cat([torch.cat([T, B], dim=1), torch.cat([B.conj().T, C], dim=1)], dim=0)
         Q = torch.cat([Q, R], dim=0)
         HQ = torch.cat([HQ, HR], dim=0)
+    if proj is not None:
+        X = proj(X)
     return lam.clone(), X.contiguous(), res, it

@@ -149,11 +149,35 @@ def buffer_bands(nb):
 W_MIN = 1e-3
 
 
-def initial_guess(engine, nb, dtype, ik=0):
+def initial_guess(engine, nb, dtype, ik=0, real=False):
     """The nb lowest plane waves of the reference order (scf.py:355-360); on
-    per-k shells, the nb lowest |k + G| of k-point ik's own shell."""
+    per-k shells, the nb lowest |k + G| of k-point ik's own shell.  real
+    (k = 0): the same waves as the real pairs cos and sin (G, -G), so that
+    the block is real in real space."""
     X = torch.zeros((nb, engine.ng), dtype=dtype, device=engine.dev)
     rows = torch.arange(nb, device=engine.dev)
+    if real:
+        plus, minus = engine.gamma_pairs
+        order = np.argsort(np.asarray(engine.perm)[plus], kind="stable")   # pairs in the |G| order of +G
+        h = 1.0 / np.sqrt(2.0)
+        ri, ci, vals = [], [], []
+        r = 0
+        for j in order:
+            if r >= nb:
+                break
+            p, q = int(plus[j]), int(minus[j])
+            if q < 0:                                   # G = 0
+                ri.append(r); ci.append(p); vals.append(1.0)
+                r += 1
+                continue
+            ri += [r, r]; ci += [p, q]; vals += [h, h]  # sqrt(2) cos(G.r)
+            r += 1
+            if r < nb:
+                ri += [r, r]; ci += [p, q]; vals += [-1j * h, 1j * h]   # sqrt(2) sin(G.r)
+                r += 1
+        X[torch.as_tensor(ri, device=engine.dev), torch.as_tensor(ci, device=engine.dev)] = \
+            torch.as_tensor(np.asarray(vals, dtype=np.complex128), device=engine.dev).to(dtype)
+        return X
     if engine.kmask64 is None:
         X[rows, engine.iperm_d[:nb]] = 1.0
     else:
@@ -168,7 +192,7 @@ def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, 
               refine_max_sub=None, cheb_passes=None, batch_k=True, cold_max_sub=None, dav_factor=0.1,
               refine_dav_factor=None, cold_max_iter=4, polish_gain_max=None, psi0=None, graphs=True,
               cheb_rr_all=True, refine_max_iter=2, mix_history=8, pulay_gg0=None, refine_nl_bf16=True,
-              refine_first_rr=False, rho_init="uniform", rho_sigma=1.2):
+              refine_first_rr=False, rho_init="uniform", rho_sigma=1.2, gamma_real=True):
     """Run the SCF on engine's current geometry.  Returns (SCFState with
     stick-order psi list in `psi` and plane-major rho, info dict).
 
@@ -206,7 +230,12 @@ def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, 
       stick order; missing rows filled with plane waves) replace the cold map.
     Batching: batch_k (all k-points as one block on the shared sphere) and
       graphs (replay the batched Chebyshev filter as a captured graph,
-      solver/graphs.py)."""
+      solver/graphs.py).
+    gamma_real (default on; applies at k = 0 without psi0 on spheres above the
+      dense-solver size): keep the eigenvectors real in real space (cos/sin
+      start, real projected matrices, projection of each correction block)
+      so that the local H psi runs two bands per transform; same energies,
+      forces and stress (False: general complex vectors, as the reference)."""
     if refine_max_iter is not None and int(refine_max_iter) < 2:
         raise ValueError("refine_max_iter must be at least 2: a single refinement step leaves the "
                          "eigenvectors unchanged, and the SCF would converge the occupations only")
@@ -231,7 +260,11 @@ def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, 
             rho *= st.nelec / q
     nbuf = buffer_bands(nb) if nbuffer is None else int(nbuffer)
     nbt = min(nb + nbuf, engine.ng)
-    X = [initial_guess(engine, nbt, cdt, ik) for ik in range(engine.nk)]
+    from .davidson import DENSE_MAX_NG
+    real = bool(gamma_real and engine.gamma_pairs is not None and psi0 is None
+                and engine.ng > max(DENSE_MAX_NG, 4 * nbt))
+    proj = engine.realify if real else None
+    X = [initial_guess(engine, nbt, cdt, ik, real=real) for ik in range(engine.nk)]
     if psi0 is not None:
         for ik in range(engine.nk):
             p = torch.as_tensor(psi0[ik]).to(device=dev, dtype=cdt)[:nbt]
@@ -333,26 +366,28 @@ def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, 
 
     def solve_k(ik, Xk, vs, fp32, vmean, tol_k, cold, vs_lo=None):
         nonlocal dav_iters
-        apply = lambda p: engine.hpsi(ik, p, vs, fp32=fp32)  # noqa: E731
+        apply = lambda p: engine.hpsi(ik, p, vs, fp32=fp32, real=real)  # noqa: E731
         if vs_lo is not None and not cold:
             # FP64 phase of the mixed form: exact H X once per map, FP32
             # corrections (complex64 projector GEMMs) in a Davidson whose
             # Rayleigh-Ritz couples them through the exact H X
             def apply_lo(p):
                 return engine.hpsi(ik, p.to(torch.complex64), vs_lo, fp32=True,
-                                   nl_bf16=refine_nl_bf16).to(torch.complex128)
+                                   nl_bf16=refine_nl_bf16, real=real).to(torch.complex128)
             diag = engine.tkin64[ik] + vmean
             lam, Xk, r, nit = davidson(apply, Xk, nbt, diag, max_iter=(refine_max_iter or dav_max_iter or 25),
                                        tol=tol_k, nconv=nb, weight=weights(ik), apply_lo=apply_lo,
                                        max_sub=(refine_max_sub or 2 * nbt), first_rr=refine_first_rr,
-                                       orth_x0=not refined[ik], last_residuals=verbose)
+                                       orth_x0=not refined[ik], last_residuals=verbose, real=real, proj=proj)
             refined[ik] = True
             dav_iters += nit
             return lam, Xk, r, nit
         if use_cheb and not cold and ritz[ik] is not None:
             key = (ik, fp32)
             if key not in bounds:
-                bounds[key] = 1.1 * upper_bound(apply, engine.ng, Xk.dtype, engine.dev)
+                # Lanczos from a random complex vector: the plain (not paired) H psi
+                bounds[key] = 1.1 * upper_bound(lambda p: engine.hpsi(ik, p, vs, fp32=fp32), engine.ng, Xk.dtype,
+                                                engine.dev)
             lam = ritz[ik]
             wk = weights(ik)
             nit, r = 0, float("inf")
@@ -364,7 +399,7 @@ def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, 
                 rr = cheb_rr_all or npass > cheb_passes + 1 or nit == 1 or nit == npass - 1
                 need = npass > 2          # residuals decide an early exit only with several passes
                 lam, Xk, HXk, rn = chfsi_step(apply, Xk, cheb_degree, bounds[key], lam, HX=HXk, rr=rr,
-                                              residuals=need)
+                                              residuals=need, real=real, proj=proj)
                 if not rr:
                     continue
                 if not need:
@@ -378,7 +413,8 @@ def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, 
         diag = engine.tkin64[ik] + vmean
         lam, Xk, r, nit = davidson(apply, Xk, nbt, diag, max_iter=(dav_max_iter or (cold_max_iter if cold else 25)),
                                    tol=(max(tol_k, 1e-3) if (cold and use_cheb) else tol_k), nconv=nb,
-                                   weight=weights(ik), max_sub=((cold_max_sub or 2 * nbt) if cold else None))
+                                   weight=weights(ik), max_sub=((cold_max_sub or 2 * nbt) if cold else None),
+                                   real=real, proj=proj)
         dav_iters += nit
         return lam, Xk, r, nit
 

@@ -118,6 +118,64 @@ __global__ void k_gamma_pair_rows(const double2* __restrict__ psi, int64_t ng, c
   }
 }
 
+// Two real bands per complex transform (k = 0, vectors real in real space:
+// psi(-G) = conj(psi(G))).  pack: phi_b = psi_{2b} + i psi_{2b+1} (the odd
+// partner zero past the last band).  unpack: with h = H phi and its pair
+// values h(G), h(-G), H psi_{2b} (G) = (h(G) + conj h(-G)) / 2 and
+// H psi_{2b+1} (G) = (h(G) - conj h(-G)) / (2i), both written at G and -G
+// (conjugates of each other), which also projects the result exactly onto
+// vectors real in real space.  G = 0 (minus = -1): Re h(0) and Im h(0).
+template <typename R>
+__global__ void k_gamma_pack2(const addft::cpx<R>* __restrict__ psi, int64_t m, int64_t ng,
+                              addft::cpx<R>* __restrict__ phi) {
+  const int64_t b = blockIdx.y;
+  const addft::cpx<R>* a = psi + (2 * b) * ng;
+  const bool has_b = 2 * b + 1 < m;
+  const addft::cpx<R>* c = psi + (2 * b + 1) * ng;
+  addft::cpx<R>* o = phi + b * ng;
+  for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < ng; g += (int64_t)gridDim.x * blockDim.x) {
+    const addft::cpx<R> va = a[g];
+    addft::cpx<R> r = va;
+    if (has_b) {
+      const addft::cpx<R> vb = c[g];
+      r.x = va.x - vb.y;
+      r.y = va.y + vb.x;
+    }
+    o[g] = r;
+  }
+}
+
+template <typename R>
+__global__ void k_gamma_unpack2(const addft::cpx<R>* __restrict__ h, int64_t m, int64_t ng,
+                                const int32_t* __restrict__ plus, const int32_t* __restrict__ minus, int64_t nh,
+                                addft::cpx<R>* __restrict__ out) {
+  const int64_t b = blockIdx.y;
+  const addft::cpx<R>* row = h + b * ng;
+  addft::cpx<R>* oa = out + (2 * b) * ng;
+  const bool has_b = 2 * b + 1 < m;
+  addft::cpx<R>* ob = out + (2 * b + 1) * ng;
+  const R half = (R)0.5;
+  for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < nh; j += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t p = plus[j], q = minus[j];
+    const addft::cpx<R> hp = row[p];
+    if (q < 0) {                                  // G = 0
+      oa[p] = addft::mk(hp.x, (R)0);
+      if (has_b) ob[p] = addft::mk(hp.y, (R)0);
+      continue;
+    }
+    const addft::cpx<R> hq = row[q];
+    // a(G) = (hp + conj hq) / 2,  b(G) = (hp - conj hq) / (2i)
+    const addft::cpx<R> av = addft::mk(half * (hp.x + hq.x), half * (hp.y - hq.y));
+    oa[p] = av;
+    oa[q] = addft::mk(av.x, -av.y);
+    if (has_b) {
+      const addft::cpx<R> bv = addft::mk(half * (hp.y + hq.y), half * (hq.x - hp.x));
+      ob[p] = bv;
+      ob[q] = addft::mk(bv.x, -bv.y);
+    }
+  }
+}
+
 int blocks_for(int64_t n) {
   const int64_t b = (n + 255) / 256;
   return (int)(b < 1 ? 1 : (b > 65536 ? 65536 : b));
@@ -182,6 +240,48 @@ int addft_gamma_pair_rows(const void* d_psi, int64_t nb, int64_t ng, const int32
   if (gx > 1024) gx = 1024;
   hipLaunchKernelGGL(k_gamma_pair_rows, dim3((unsigned)gx, (unsigned)nb), dim3(256), 0, addft::as_stream(stream),
                      static_cast<const double2*>(d_psi), ng, d_plus, d_minus, nh, nb, static_cast<double*>(d_out));
+  ADDFT_LAUNCH_CHECK();
+  return ADDFT_OK;
+}
+
+int addft_gamma_pack2(int dtype, const void* d_psi, int64_t m, int64_t ng, void* d_phi, void* stream) {
+  ADDFT_CHECK_ARG(dtype == ADDFT_C64 || dtype == ADDFT_C128, "dtype must be ADDFT_C64 or ADDFT_C128");
+  ADDFT_CHECK_ARG(m >= 0 && m <= 65534 && ng >= 0, "bad sizes (m <= 65534)");
+  if (m == 0 || ng == 0) return ADDFT_OK;
+  ADDFT_CHECK_ARG(d_psi && d_phi && d_psi != d_phi, "null or aliased argument");
+  const int64_t np = (m + 1) / 2;
+  int64_t gx = (ng + 255) / 256;
+  if (gx > 1024) gx = 1024;
+  hipStream_t st = addft::as_stream(stream);
+  if (dtype == ADDFT_C64)
+    hipLaunchKernelGGL(k_gamma_pack2<float>, dim3((unsigned)gx, (unsigned)np), dim3(256), 0, st,
+                       static_cast<const addft::cpx<float>*>(d_psi), m, ng, static_cast<addft::cpx<float>*>(d_phi));
+  else
+    hipLaunchKernelGGL(k_gamma_pack2<double>, dim3((unsigned)gx, (unsigned)np), dim3(256), 0, st,
+                       static_cast<const addft::cpx<double>*>(d_psi), m, ng,
+                       static_cast<addft::cpx<double>*>(d_phi));
+  ADDFT_LAUNCH_CHECK();
+  return ADDFT_OK;
+}
+
+int addft_gamma_unpack2(int dtype, const void* d_h, int64_t m, int64_t ng, const int32_t* d_plus,
+                        const int32_t* d_minus, int64_t nh, void* d_out, void* stream) {
+  ADDFT_CHECK_ARG(dtype == ADDFT_C64 || dtype == ADDFT_C128, "dtype must be ADDFT_C64 or ADDFT_C128");
+  ADDFT_CHECK_ARG(m >= 0 && m <= 65534 && nh >= 0 && 2 * nh <= ng + 1, "bad sizes (m <= 65534, 2 nh <= nG + 1)");
+  if (m == 0 || nh == 0) return ADDFT_OK;
+  ADDFT_CHECK_ARG(d_h && d_plus && d_minus && d_out && d_h != d_out, "null or aliased argument");
+  const int64_t np = (m + 1) / 2;
+  int64_t gx = (nh + 255) / 256;
+  if (gx > 1024) gx = 1024;
+  hipStream_t st = addft::as_stream(stream);
+  if (dtype == ADDFT_C64)
+    hipLaunchKernelGGL(k_gamma_unpack2<float>, dim3((unsigned)gx, (unsigned)np), dim3(256), 0, st,
+                       static_cast<const addft::cpx<float>*>(d_h), m, ng, d_plus, d_minus, nh,
+                       static_cast<addft::cpx<float>*>(d_out));
+  else
+    hipLaunchKernelGGL(k_gamma_unpack2<double>, dim3((unsigned)gx, (unsigned)np), dim3(256), 0, st,
+                       static_cast<const addft::cpx<double>*>(d_h), m, ng, d_plus, d_minus, nh,
+                       static_cast<addft::cpx<double>*>(d_out));
   ADDFT_LAUNCH_CHECK();
   return ADDFT_OK;
 }

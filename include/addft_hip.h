@@ -91,6 +91,17 @@ int addft_gamma_pair_rows(const void* d_psi, int64_t nb, int64_t ng, const int32
                           const int32_t* d_minus, int64_t nh, void* d_out, void* stream);
 int addft_gamma_scatter_add(int dtype, const void* d_u, int64_t nb, int64_t nh, const int32_t* d_plus,
                             const int32_t* d_minus, int64_t ng, void* d_out, void* stream);
+/* Two bands per transform for vectors real in real space (psi(-G) =
+ * conj(psi(G)), k = 0): addft_gamma_pack2 writes phi (ceil(m/2), ng) with
+ * phi_b = psi_{2b} + i psi_{2b+1}; after H phi (any operator real in real
+ * space, e.g. addft_hpsi_local), addft_gamma_unpack2 writes H psi (m, ng)
+ * from it through the pair tables (H psi_{2b} = (h(G) + conj h(-G))/2,
+ * H psi_{2b+1} = (h(G) - conj h(-G))/(2i)).  No reference counterpart
+ * (its Gamma point keeps one complex transform per band, basis.py:115-116).
+ * errors: bad dtype, m > 65534, 2 nh > ng + 1, null or aliased pointers. */
+int addft_gamma_pack2(int dtype, const void* d_psi, int64_t m, int64_t ng, void* d_phi, void* stream);
+int addft_gamma_unpack2(int dtype, const void* d_h, int64_t m, int64_t ng, const int32_t* d_plus,
+                        const int32_t* d_minus, int64_t nh, void* d_out, void* stream);
 
 /* ------------------------------------------------- Chebyshev filter step
  * out = alpha a + beta b + gamma c over nblk blocks of n complex elements
