@@ -160,21 +160,22 @@ def initial_guess(engine, nb, dtype, ik=0, real=False):
         plus, minus = engine.gamma_pairs
         order = np.argsort(np.asarray(engine.perm)[plus], kind="stable")   # pairs in the |G| order of +G
         h = 1.0 / np.sqrt(2.0)
-        ri, ci, vals = [], [], []
+        entries = []                                    # (row, stick position, value)
         r = 0
         for j in order:
             if r >= nb:
                 break
             p, q = int(plus[j]), int(minus[j])
             if q < 0:                                   # G = 0
-                ri.append(r); ci.append(p); vals.append(1.0)
+                entries.append((r, p, 1.0))
                 r += 1
                 continue
-            ri += [r, r]; ci += [p, q]; vals += [h, h]  # sqrt(2) cos(G.r)
+            entries += [(r, p, h), (r, q, h)]           # sqrt(2) cos(G.r)
             r += 1
             if r < nb:
-                ri += [r, r]; ci += [p, q]; vals += [-1j * h, 1j * h]   # sqrt(2) sin(G.r)
+                entries += [(r, p, -1j * h), (r, q, 1j * h)]   # sqrt(2) sin(G.r)
                 r += 1
+        ri, ci, vals = zip(*entries)
         X[torch.as_tensor(ri, device=engine.dev), torch.as_tensor(ci, device=engine.dev)] = \
             torch.as_tensor(np.asarray(vals, dtype=np.complex128), device=engine.dev).to(dtype)
         return X

@@ -75,3 +75,23 @@ def test_gamma_half_sphere_algebra_on_oracle_projectors():
     out[:, plus] += u + 1j * v
     out[:, minus[1:]] += (u - 1j * v)[:, 1:]
     assert np.allclose(out, S @ Pp, atol=1e-12)
+
+
+def test_real_initial_guess_is_orthonormal_real_and_lowest():
+    """solve_scf's k = 0 start for real eigenvectors: G = 0 and the cos/sin
+    pairs of the lowest |G|, orthonormal and exactly real in real space."""
+    from types import SimpleNamespace
+    from addft.geometry.basis import make_basis
+    from addft.kernels.nonlocal_bf16 import gamma_pairs
+    from addft.solver.scf import initial_guess
+    b = make_basis(10.26 * np.eye(3), 6.0)
+    perm = np.random.default_rng(4).permutation(b.nG)          # stick position -> reference index
+    plus, minus = gamma_pairs(b.miller[perm])
+    eng = SimpleNamespace(gamma_pairs=(plus, minus), perm=perm, ng=b.nG, dev=torch.device("cpu"), kmask64=None)
+    nb = 12
+    X = initial_guess(eng, nb, torch.complex128, real=True).numpy()
+    assert np.allclose(X @ X.conj().T, np.eye(nb), atol=1e-14)
+    assert np.array_equal(X[:, minus[1:]], np.conj(X[:, plus[1:]])) and not X[:, plus[0]].imag.any()
+    used = np.flatnonzero(np.abs(X).sum(axis=0))
+    g2 = (b.miller[perm] ** 2).sum(axis=1)                       # cubic cell: |G|^2 ~ |m|^2
+    assert g2[used].max() <= np.sort(g2)[nb]                     # only the lowest shells
