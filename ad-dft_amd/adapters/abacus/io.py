@@ -24,8 +24,7 @@ from pathlib import Path
 
 import numpy as np
 
-RY_TO_HA = 0.5
-ANG_TO_BOHR = 1.0 / 0.529177210903
+from addft.runtime.units import ANG_TO_BOHR, RY_TO_HA
 
 
 class UnmappedParameterError(KeyError):

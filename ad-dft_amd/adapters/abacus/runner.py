@@ -10,7 +10,7 @@ from pathlib import Path
 
 import numpy as np
 
-HA_TO_EV = 27.211386245988
+from addft.runtime.units import HA_TO_EV
 
 
 def _table(lines, header):

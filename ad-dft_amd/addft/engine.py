@@ -28,6 +28,7 @@ from scipy.special import erf
 
 from .geometry.cell import Cell, positions, reciprocal, volume
 from .geometry.psp import lm_table
+from .runtime.units import ry_to_ha
 from .kernels import _lib
 from .kernels import native
 from .kernels._lib import check, lib, ptr, stream
@@ -107,6 +108,11 @@ class Engine:
             self._work[key] = w
         return w
 
+    def graph_buffers(self):
+        """Workspaces a captured Hpsi addresses outside the graph's pool: the
+        sphere plans' FFT work buffers (held by the graph for its lifetime)."""
+        return [pl._work for pl in (self.plan32, self.plan64) if pl is not None]
+
     def _f64(self, a):
         return torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64), device=self.dev)
 
@@ -128,7 +134,7 @@ class Engine:
         else:
             r, rab = psp.r[s], psp.rab[s]
             w = _simpson_weights(r.shape[0])
-            v = psp.vloc_at[s] / 2.0
+            v = ry_to_ha(psp.vloc_at[s])
             z = self.zval
             self.mesh = int(r.shape[0])
             self.r_d = self._f64(r)

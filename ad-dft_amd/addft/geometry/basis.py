@@ -6,7 +6,8 @@ NumPy expressions are used so the sphere ORDER (lexsort on (g2, m0, m1, m2))
 is bit-identical, which the pack index maps depend on.
 
 The device-side layouts (stick tables, internal stick-major coefficient
-order) are built from a Basis in `addft.ops.plan`.
+order) are built from a Basis by `addft.kernels.native.SpherePlan`
+(addft_sphere_plan_create in the HIP library).
 """
 
 from __future__ import annotations

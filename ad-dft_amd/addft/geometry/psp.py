@@ -12,6 +12,7 @@ from typing import NamedTuple
 
 import numpy as np
 
+from ..runtime.units import ry_to_ha
 from .upf import parse_upf_text, read_upf
 
 
@@ -77,7 +78,7 @@ def _from_raw(raw) -> Pseudo:
     return Pseudo(kind=1, zval=np.asarray([raw["zval"]]), vloc_amp=np.zeros((1,)),
                   vloc_sigma=np.ones((1,)), proj_w=np.zeros((1, nrad)),
                   proj_b=np.ones((1, nrad)), proj_l=np.asarray(raw["lll"])[None, :],
-                  dion=np.asarray(raw["dion"])[None] / 2.0,
+                  dion=ry_to_ha(np.asarray(raw["dion"]))[None],
                   r=np.asarray(raw["r"])[None], rab=np.asarray(raw["rab"])[None],
                   vloc_at=np.asarray(raw["vloc_at"])[None], betar=betar[None],
                   nlcc=np.asarray([False]))

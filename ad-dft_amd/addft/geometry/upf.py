@@ -1,10 +1,12 @@
 """UPF v1/v2 reading, and writing of synthetic norm-conserving UPF files.
 
-`read_upf` mirrors reference `src/addft/geometry/upf.py:38-101`: the v2 (XML)
-and v1 (plain text) layouts give the same raw dict; values stay in the
-file's Rydberg units; an even declared mesh size drops its last point
-(ABACUS convention).  Nonlinear core corrections are flagged (`nlcc`) and
-refused by `psp.from_upf`.
+`read_upf` gives the raw dict of reference `src/addft/geometry/upf.py:38-101`
+(same keys) from the v2 (XML) and v1 (plain text) layouts: the v2 reader is
+driven by tables of header attributes and arrays, the v1 reader by the
+format's labelled line records.  Values stay in the file's Rydberg units;
+an even declared mesh size drops its last point (ABACUS convention).
+Nonlinear core corrections are flagged (`nlcc`) and refused by
+`psp.from_upf`.
 
 `write_synthetic_upf` builds the synthetic pseudopotentials used by the
 benchmark systems: Z_val, a local part -Z erf(r/rc)/r + A exp(-(r/rg)^2)
@@ -15,45 +17,105 @@ r_i = exp(xmin + i dx)/zmesh with rab_i = r_i dx.
 
 from __future__ import annotations
 
-import re
 from pathlib import Path
 from xml.etree import ElementTree as ET
 
 import numpy as np
 from scipy.special import erf
 
-_NUM_RE = re.compile(r"[-+]?(?:\d+\.\d*|\.\d+|\d+)(?:[EeDd][-+]?\d+)?")
+_FORTRAN_EXP = str.maketrans("Dd", "Ee")
 
 
-def _floats(text):
-    if text is None:
+def _reals(text) -> np.ndarray:
+    """Whitespace- or comma-separated reals, Fortran D exponents allowed."""
+    if not text:
         return np.zeros((0,))
-    return np.asarray([float(x.replace("D", "E").replace("d", "e"))
-                       for x in _NUM_RE.findall(text)], dtype=float)
+    return np.asarray(text.replace(",", " ").translate(_FORTRAN_EXP).split(), dtype=float)
+
+
+def _leading_reals(line: str) -> list:
+    """The numbers that open a record line, up to its first label word
+    ("   1    0     Beta    L" -> [1.0, 0.0])."""
+    out = []
+    for tok in line.replace(",", " ").split():
+        try:
+            out.append(float(tok.translate(_FORTRAN_EXP)))
+        except ValueError:
+            break
+    return out
+
+
+def _flag(value) -> bool:
+    """UPF logical attribute ("T", ".true.", "true", "1", ...)."""
+    return str(value).strip().strip(".").lower() in ("t", "true", "1", "yes")
+
+
+def _odd_mesh(n: int) -> int:
+    """The ABACUS rule: an even radial mesh drops its last point, so that
+    Simpson's rule sees an odd count."""
+    return n - 1 if n % 2 == 0 else n
+
+
+# UPF v2: raw-dict key -> element path of a radial array (cut to the odd mesh)
+_V2_RADIAL = (("r", "PP_MESH/PP_R"), ("rab", "PP_MESH/PP_RAB"), ("vloc_at", "PP_LOCAL"))
+# raw-dict key -> (PP_HEADER attribute, converter)
+_V2_HEADER = (("zval", "z_valence", float), ("nbeta", "number_of_proj", int),
+              ("nlcc", "core_correction", _flag))
+
+
+def _v2_mesh(root) -> int:
+    """Declared radial mesh: the size attribute of PP_R (the reference's
+    choice), else PP_MESH@mesh, else PP_HEADER@mesh_size, else the count of
+    values in PP_R."""
+    for path, attr in (("PP_MESH/PP_R", "size"), ("PP_MESH", "mesh"), ("PP_HEADER", "mesh_size")):
+        el = root.find(path)
+        if el is not None and attr in el.attrib:
+            return int(el.attrib[attr])
+    return int(_reals(root.findtext("PP_MESH/PP_R")).size)
+
+
+def _parse_v2(text: str) -> dict:
+    """UPF v2 (XML): header attributes and arrays through the tables above;
+    the projectors are the PP_NONLOCAL children PP_BETA.<i>, taken in the
+    order of <i>, each with its angular_momentum attribute."""
+    root = ET.fromstring(text)
+    head = root.find("PP_HEADER")
+    if head is None:
+        raise ValueError("UPF v2 file without PP_HEADER")
+    out = {key: conv(head.attrib.get(attr, "F" if conv is _flag else None))
+           for key, attr, conv in _V2_HEADER}
+    mesh = _odd_mesh(_v2_mesh(root))
+    out["mesh"] = mesh
+    for key, path in _V2_RADIAL:
+        arr = _reals(root.findtext(path))
+        if arr.size < mesh:
+            raise ValueError(f"UPF v2 {path} holds {arr.size} values, fewer than the mesh {mesh}")
+        out[key] = arr[:mesh]
+    nonlocal_el = root.find("PP_NONLOCAL")
+    betas = [] if nonlocal_el is None else sorted(
+        (el for el in nonlocal_el if el.tag.startswith("PP_BETA.")), key=lambda el: int(el.tag.split(".")[1]))
+    nb = out["nbeta"]
+    if len(betas) != nb:
+        raise ValueError(f"UPF v2 header declares {nb} projectors, found {len(betas)} PP_BETA blocks")
+    out["lll"] = np.asarray([int(el.attrib["angular_momentum"]) for el in betas], dtype=int)
+    betar = np.zeros((nb, mesh))
+    for i, el in enumerate(betas):
+        vals = _reals(el.text)[:mesh]
+        betar[i, :vals.size] = vals
+    out["betar"] = betar
+    dij = _reals(root.findtext("PP_NONLOCAL/PP_DIJ"))
+    if dij.size < nb * nb:
+        raise ValueError("UPF v2 PP_DIJ is shorter than nbeta^2")
+    out["dion"] = dij[:nb * nb].reshape(nb, nb)
+    out["rho_atc"] = _reals(root.findtext("PP_NLCC"))[:mesh] if out["nlcc"] else np.zeros(mesh)
+    return out
 
 
 def parse_upf_text(text: str) -> dict:
-    """UPF v2 XML or UPF v1 text -> dict of raw (Ry) arrays (upf.py:38-101)."""
-    if "<UPF" not in text:
-        return _parse_v1(text)
-    root = ET.fromstring(text)
-    h = root.find("PP_HEADER")
-    mesh_decl = int(root.find("PP_MESH/PP_R").attrib["size"])
-    mesh = mesh_decl - (1 if mesh_decl % 2 == 0 else 0)
-    r = _floats(root.find("PP_MESH/PP_R").text)[:mesh]
-    rab = _floats(root.find("PP_MESH/PP_RAB").text)[:mesh]
-    vloc = _floats(root.find("PP_LOCAL").text)[:mesh]
-    nbeta = int(h.attrib["number_of_proj"])
-    lll, betar = [], []
-    for ib in range(1, nbeta + 1):
-        el = root.find(f"PP_NONLOCAL/PP_BETA.{ib}")
-        lll.append(int(el.attrib["angular_momentum"]))
-        betar.append(_floats(el.text)[:mesh])
-    dij = _floats(root.find("PP_NONLOCAL/PP_DIJ").text)[:nbeta * nbeta].reshape(nbeta, nbeta)
-    nlcc = h.attrib.get("core_correction", "F") in ("T", "true", "1", "True", ".true.")
-    return dict(zval=float(h.attrib["z_valence"]), mesh=mesh, r=r, rab=rab,
-                vloc_at=vloc, nbeta=nbeta, lll=np.asarray(lll, dtype=int),
-                betar=np.asarray(betar).reshape(nbeta, mesh), dion=dij, nlcc=nlcc)
+    """UPF v2 XML or UPF v1 text -> dict of raw (Ry) arrays with the keys of
+    reference upf.py:38-101 (zval, mesh, r, rab, vloc_at, nbeta, lll, betar,
+    dion, nlcc, rho_atc)."""
+    return _parse_v2(text) if "<UPF" in text else _parse_v1(text)
 
 
 def read_upf(path) -> dict:
@@ -61,8 +123,30 @@ def read_upf(path) -> dict:
 
 
 def _v1_block(text, tag):
-    m = re.search(rf"<{tag}>(.*?)</{tag}>", text, flags=re.S)
-    return None if m is None else m.group(1)
+    """Body of the first <tag> ... </tag> block, or None."""
+    open_tag, close_tag = f"<{tag}>", f"</{tag}>"
+    a = text.find(open_tag)
+    if a < 0:
+        return None
+    b = text.find(close_tag, a)
+    if b < 0:
+        raise ValueError(f"UPF v1 block {tag} is not closed")
+    return text[a + len(open_tag):b]
+
+
+def _v1_blocks(text, tag):
+    """Bodies of every <tag> ... </tag> block, in file order."""
+    out, at = [], 0
+    open_tag, close_tag = f"<{tag}>", f"</{tag}>"
+    while True:
+        a = text.find(open_tag, at)
+        if a < 0:
+            return out
+        b = text.find(close_tag, a)
+        if b < 0:
+            raise ValueError(f"UPF v1 block {tag} is not closed")
+        out.append(text[a + len(open_tag):b])
+        at = b + len(close_tag)
 
 
 def _v1_field(header, label):
@@ -75,46 +159,69 @@ def _v1_field(header, label):
     raise ValueError(f"UPF v1 PP_HEADER has no '{label}' line")
 
 
+def _v1_beta(body, mesh):
+    """One v1 PP_BETA record: a line "index l ...", a line "kkbeta", then
+    kkbeta values of r*beta (zero-padded to the mesh); anything after them
+    (QE writes a cutoff-radius record) is ignored."""
+    lines = [ln for ln in body.splitlines() if ln.strip()]
+    head = _leading_reals(lines[0])
+    l = int(head[1])
+    kb = int(_leading_reals(lines[1])[0])
+    vals = []
+    for ln in lines[2:]:
+        if len(vals) >= kb:
+            break
+        vals.extend(_leading_reals(ln))
+    n = min(kb, mesh)
+    b = np.zeros(mesh)
+    b[:n] = vals[:n]
+    return l, b
+
+
+def _v1_dij(body, nbeta):
+    """v1 PP_DIJ: a count line, then (i, j, D_ij) records, 1-based, filled
+    symmetrically (Ry)."""
+    d = np.zeros((nbeta, nbeta))
+    if body is None:
+        return d
+    lines = [ln for ln in body.splitlines() if ln.strip()]
+    if not lines:
+        return d
+    count = int(_leading_reals(lines[0])[0])
+    for ln in lines[1:1 + count]:
+        i, j, v = _leading_reals(ln)[:3]
+        d[int(i) - 1, int(j) - 1] = d[int(j) - 1, int(i) - 1] = v
+    return d
+
+
 def _parse_v1(text: str) -> dict:
     """UPF v1 (the plain-text layout of e.g. Si.pz-vbc.UPF) -> the same raw
-    dict as the v2 reader (upf.py:61-93): Z from the header's "Z valence"
-    record, the mesh from "Number of points in mesh" (even -> one point
-    fewer, as for v2), each PP_BETA as (index, l, kkbeta, kkbeta values of
-    r*beta) zero-padded to the mesh, PP_DIJ as a count and (i, j, D_ij Ry)
-    triples filled symmetrically, NLCC from a PP_NLCC block."""
+    dict as the v2 reader: Z from the header's "Z valence" record, the mesh
+    from "Number of points in mesh" (even -> one point fewer, as for v2),
+    each PP_BETA record through _v1_beta, PP_DIJ through _v1_dij, NLCC from a
+    PP_NLCC block."""
     header = _v1_block(text, "PP_HEADER")
     if header is None:
         raise ValueError("neither a UPF v2 file nor a UPF v1 file with a PP_HEADER block")
     zval = float(_v1_field(header, "Z valence")[0])
-    mesh_decl = int(_v1_field(header, "Number of points in mesh")[0])
-    mesh = mesh_decl - (1 if mesh_decl % 2 == 0 else 0)
+    mesh = _odd_mesh(int(_v1_field(header, "Number of points in mesh")[0]))
     nbeta = int(_v1_field(header, "Number of Wavefunctions")[1])
-    r = _floats(_v1_block(text, "PP_R"))[:mesh]
-    rab = _floats(_v1_block(text, "PP_RAB"))[:mesh]
-    vloc = _floats(_v1_block(text, "PP_LOCAL"))[:mesh]
-    if not (r.size == rab.size == vloc.size == mesh):
-        raise ValueError("UPF v1 radial arrays are shorter than the declared mesh")
-    lll, betar = [], []
-    for blk in re.findall(r"<PP_BETA>(.*?)</PP_BETA>", text, flags=re.S):
-        vals = _floats(blk)
-        l, kb = int(vals[1]), int(vals[2])
-        b = np.zeros(mesh)
-        n = min(kb, mesh)
-        b[:n] = vals[3:3 + n]
-        lll.append(l)
-        betar.append(b)
-    if len(lll) != nbeta:
-        raise ValueError(f"UPF v1 header declares {nbeta} projectors, found {len(lll)} PP_BETA blocks")
-    dion = np.zeros((nbeta, nbeta))
-    dv = _floats(_v1_block(text, "PP_DIJ"))
-    if dv.size:
-        nd = int(dv[0])
-        for t in range(nd):
-            i, j, v = int(dv[1 + 3 * t]) - 1, int(dv[2 + 3 * t]) - 1, dv[3 + 3 * t]
-            dion[i, j] = dion[j, i] = v
-    return dict(zval=zval, mesh=mesh, r=r, rab=rab, vloc_at=vloc, nbeta=nbeta,
-                lll=np.asarray(lll, dtype=int), betar=np.asarray(betar).reshape(nbeta, mesh), dion=dion,
-                nlcc=_v1_block(text, "PP_NLCC") is not None)
+    radial = {}
+    for key, tag in (("r", "PP_R"), ("rab", "PP_RAB"), ("vloc_at", "PP_LOCAL")):
+        arr = _reals(_v1_block(text, tag))
+        if arr.size < mesh:
+            raise ValueError("UPF v1 radial arrays are shorter than the declared mesh")
+        radial[key] = arr[:mesh]
+    betas = [_v1_beta(body, mesh) for body in _v1_blocks(text, "PP_BETA")]
+    if len(betas) != nbeta:
+        raise ValueError(f"UPF v1 header declares {nbeta} projectors, found {len(betas)} PP_BETA blocks")
+    nlcc_body = _v1_block(text, "PP_NLCC")
+    return dict(zval=zval, mesh=mesh, **radial, nbeta=nbeta,
+                lll=np.asarray([l for l, _ in betas], dtype=int),
+                betar=np.asarray([b for _, b in betas]).reshape(nbeta, mesh),
+                dion=_v1_dij(_v1_block(text, "PP_DIJ"), nbeta),
+                nlcc=nlcc_body is not None,
+                rho_atc=_reals(nlcc_body)[:mesh] if nlcc_body is not None else np.zeros(mesh))
 
 
 def _block(arr, per_line=4):

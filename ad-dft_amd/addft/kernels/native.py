@@ -25,13 +25,18 @@ def _dtype_code(t):
         raise TypeError(f"expected complex64/complex128, got {t.dtype}") from None
 
 
-def _require(t, dtype=None, device=True):
-    if device and not t.is_cuda:
-        raise ValueError("tensor must live on the GPU")
+def _require(t, dtype=None, device=True, numel=None, what="tensor"):
+    """Host-side checks before a launch: element count (a mis-sized buffer
+    would otherwise become an out-of-bounds device access), dtype,
+    contiguity, residence on the GPU."""
+    if numel is not None and t.numel() != int(numel):
+        raise ValueError(f"{what} has {t.numel()} elements, expected {int(numel)}")
     if dtype is not None and t.dtype != dtype:
-        raise TypeError(f"expected {dtype}, got {t.dtype}")
+        raise TypeError(f"{what}: expected {dtype}, got {t.dtype}")
     if not t.is_contiguous():
-        raise ValueError("tensor must be contiguous")
+        raise ValueError(f"{what} must be contiguous")
+    if device and not t.is_cuda:
+        raise ValueError(f"{what} must live on the GPU")
 
 
 # ------------------------------------------------------------------ pack
@@ -75,9 +80,9 @@ def xc_lda(rho: torch.Tensor, want_energy: bool = True):
 
 def xc_fxc(rho: torch.Tensor, w: torch.Tensor | None = None) -> torch.Tensor:
     """f_xc (or f_xc * w) (reference xc.py:181-191, 205-208)."""
-    _require(rho, torch.float64)
     if w is not None:
-        _require(w, torch.float64)
+        _require(w, torch.float64, numel=rho.numel(), what="w")
+    _require(rho, torch.float64)
     out = torch.empty_like(rho)
     check(lib().addft_xc_lda_fxc(ptr(rho), ptr(w), rho.numel(), ptr(out), stream()), "addft_xc_lda_fxc")
     return out
@@ -117,6 +122,8 @@ def cheb_combine(a: torch.Tensor, b: torch.Tensor, c: torch.Tensor | None, alpha
 # ------------------------------------------- Gamma point, two bands per FFT
 def gamma_pack2(psi: torch.Tensor) -> torch.Tensor:
     """(m, nG) vectors real in real space -> (ceil(m/2), nG) pairs psi_2b + i psi_2b+1."""
+    if psi.dim() != 2:
+        raise ValueError("psi must be (m, nG)")
     _require(psi)
     m, ng = psi.shape
     out = torch.empty(((m + 1) // 2, ng), dtype=psi.dtype, device=psi.device)
@@ -126,12 +133,16 @@ def gamma_pack2(psi: torch.Tensor) -> torch.Tensor:
 
 def gamma_unpack2(h: torch.Tensor, m: int, plus: torch.Tensor, minus: torch.Tensor) -> torch.Tensor:
     """H of the packed pairs -> (m, nG) H psi (addft_gamma_unpack2)."""
-    _require(h)
-    _require(plus, torch.int32)
-    _require(minus, torch.int32)
+    if h.dim() != 2 or h.shape[0] != (m + 1) // 2:
+        raise ValueError("h must be (ceil(m/2), nG)")
     ng = h.shape[1]
-    if h.shape[0] != (m + 1) // 2:
-        raise ValueError("h must hold ceil(m/2) packed rows")
+    if minus.numel() != plus.numel():
+        raise ValueError(f"minus has {minus.numel()} elements, expected {plus.numel()}")
+    if 2 * plus.numel() - 1 != ng:
+        raise ValueError("pair tables must cover the sphere (2 nh - 1 = nG)")
+    _require(h)
+    _require(plus, torch.int32, what="plus")
+    _require(minus, torch.int32, what="minus")
     out = torch.empty((m, ng), dtype=h.dtype, device=h.device)
     check(lib().addft_gamma_unpack2(_dtype_code(h), ptr(h), m, ng, ptr(plus), ptr(minus), plus.numel(), ptr(out),
                                     stream()), "addft_gamma_unpack2")
@@ -141,16 +152,16 @@ def gamma_unpack2(h: torch.Tensor, m: int, plus: torch.Tensor, minus: torch.Tens
 # ------------------------------------------------------------- real grids
 def to_planar(grid: torch.Tensor, shape) -> torch.Tensor:
     """row-major (i1,i2,i3) float64 -> plane-major (i3,i1,i2)."""
-    _require(grid, torch.float64)
     n1, n2, n3 = shape
+    _require(grid, torch.float64, numel=n1 * n2 * n3, what="grid")
     out = torch.empty(n1 * n2 * n3, dtype=torch.float64, device=grid.device)
     check(lib().addft_grid_transpose(n1, n2, n3, 0, ptr(grid), ptr(out), stream()), "addft_grid_transpose")
     return out
 
 
 def from_planar(grid: torch.Tensor, shape) -> torch.Tensor:
-    _require(grid, torch.float64)
     n1, n2, n3 = shape
+    _require(grid, torch.float64, numel=n1 * n2 * n3, what="grid")
     out = torch.empty(n1 * n2 * n3, dtype=torch.float64, device=grid.device)
     check(lib().addft_grid_transpose(n1, n2, n3, 1, ptr(grid), ptr(out), stream()), "addft_grid_transpose")
     return out
@@ -235,7 +246,7 @@ class SpherePlan:
 
     # real-space staging -------------------------------------------------
     def stage(self, v_planar: torch.Tensor, scale: float) -> torch.Tensor:
-        _require(v_planar, torch.float64)
+        _require(v_planar, torch.float64, numel=self.nfft, what="v_planar")
         out = torch.empty(self.nfft, dtype=self.rdtype, device=v_planar.device)
         check(lib().addft_grid_stage(self._h, ptr(v_planar), float(scale), ptr(out), stream()),
               "addft_grid_stage")
@@ -245,12 +256,14 @@ class SpherePlan:
     def hpsi_local(self, psi: torch.Tensor, tkin: torch.Tensor, v_staged: torch.Tensor,
                    out: torch.Tensor | None = None) -> torch.Tensor:
         """T psi + V psi for (nb, nG) stick-order psi."""
-        _require(psi, self.cdtype)
-        _require(tkin, self.rdtype)
-        _require(v_staged, self.rdtype)
         nb = psi.shape[0] if psi.dim() == 2 else 1
         if psi.shape[-1] != self.ng or tkin.numel() != self.ng or v_staged.numel() != self.nfft:
             raise ValueError("shape mismatch in hpsi_local")
+        if out is not None:
+            _require(out, self.cdtype, numel=psi.numel(), what="out")
+        _require(psi, self.cdtype)
+        _require(tkin, self.rdtype)
+        _require(v_staged, self.rdtype)
         if out is None:
             out = torch.empty_like(psi)
         check(lib().addft_hpsi_local(self._h, ptr(psi), ptr(tkin), ptr(v_staged), nb, ptr(out),
@@ -269,6 +282,8 @@ class SpherePlan:
             raise ValueError("shape mismatch in hpsi_local_kbatch")
         if out is None:
             out = torch.empty_like(psi)
+        else:
+            _require(out, self.cdtype, numel=psi.numel(), what="out")
         check(lib().addft_hpsi_local_kbatch(self._h, ptr(psi), ptr(tkin), int(bands_per_k), ptr(v_staged), nb,
                                             ptr(out), ptr(self.work(nb)), stream()), "addft_hpsi_local_kbatch")
         return out
@@ -276,11 +291,13 @@ class SpherePlan:
     def density(self, psi: torch.Tensor, w: torch.Tensor, out: torch.Tensor | None = None,
                 accumulate: bool = False) -> torch.Tensor:
         """sum_b w_b |IFFT(psi_b)|^2, plane-major float64."""
-        _require(psi, self.cdtype)
-        _require(w, torch.float64)
         nb = psi.shape[0] if psi.dim() == 2 else 1
         if w.numel() != nb or psi.shape[-1] != self.ng:
             raise ValueError("shape mismatch in density")
+        if out is not None:
+            _require(out, torch.float64, numel=self.nfft, what="out")
+        _require(psi, self.cdtype)
+        _require(w, torch.float64)
         if out is None:
             out = torch.empty(self.nfft, dtype=torch.float64, device=psi.device)
             accumulate = False
@@ -290,7 +307,7 @@ class SpherePlan:
 
     def to_real(self, c: torch.Tensor, scale: float = 1.0) -> torch.Tensor:
         """scale * Re sum_G c_G e^{iGr}, plane-major float64 (stick-order c)."""
-        _require(c, torch.complex128)
+        _require(c, torch.complex128, numel=self.ng, what="c")
         out = torch.empty(self.nfft, dtype=torch.float64, device=c.device)
         check(lib().addft_sphere_to_real(self._h, ptr(c), float(scale), ptr(out), ptr(self.work(1)), stream()),
               "addft_sphere_to_real")
@@ -298,7 +315,7 @@ class SpherePlan:
 
     def to_sphere(self, grid_planar: torch.Tensor, scale: float = 1.0) -> torch.Tensor:
         """scale * sum_r f(r) e^{-iGr} in stick order."""
-        _require(grid_planar, torch.float64)
+        _require(grid_planar, torch.float64, numel=self.nfft, what="grid_planar")
         out = torch.empty(self.ng, dtype=torch.complex128, device=grid_planar.device)
         check(lib().addft_real_to_sphere(self._h, ptr(grid_planar), float(scale), ptr(out), ptr(self.work(1)),
                                          stream()), "addft_real_to_sphere")
@@ -334,13 +351,13 @@ class BoxSphere:
               "addft_box_fft")
 
     def to_real(self, c: torch.Tensor, scale: float = 1.0) -> torch.Tensor:
-        _require(c, torch.complex128)
+        _require(c, torch.complex128, numel=self.ng, what="c")
         box = pack_scatter(c, self.pidx, self.nfft)
         self._fft(box, True)
         return (box.real * scale).contiguous()
 
     def to_sphere(self, grid_planar: torch.Tensor, scale: float = 1.0) -> torch.Tensor:
-        _require(grid_planar, torch.float64)
+        _require(grid_planar, torch.float64, numel=self.nfft, what="grid_planar")
         box = grid_planar.to(torch.complex128).contiguous()
         self._fft(box, False)
         return (pack_gather(box, self.pidx) * scale).contiguous()

@@ -14,8 +14,10 @@ for the smallest cells).  Here the same response is split exactly into
     the computed states), solved by block preconditioned conjugate
     gradients with the FP64 Hpsi -- the sum over all higher eigenstates of
     the reference without forming them;
-  * the occupation change df_n = f'_n (dmu - de_n), dmu per k-point as in the
-    reference (response.py:96-103), with the sign fixed (see occ_sign).
+  * the occupation change df_n = f'_n (dmu - de_n) with one dmu for the
+    whole k mesh (the derivative of the map, whose single Fermi level is
+    bisected over all k); occ_sign="reference" gives the reference's
+    f'_n (de_n - dmu_k) with dmu per k-point (response.py:96-103).
 
 Then, with K the Hartree (psi-sphere) + f_xc kernel, J = chi0 K and
 J^T = K chi0 (both operators are symmetric on the grid), and the adjoint of
@@ -159,6 +161,8 @@ class Response:
         self.eig = eig.to(torch.float64)
         self.occ = occ.to(torch.float64)
         self.deg_tol = float(deg_tol)
+        if occ_sign not in ("exact", "reference"):
+            raise ValueError(f"unknown occ_sign {occ_sign!r}")
         self.sgn = 1.0 if occ_sign == "exact" else -1.0
         self.cg_tol = float(cg_tol)
         self.cg_maxiter = int(cg_maxiter)
@@ -226,13 +230,20 @@ class Response:
 
     # -------------------------------------------------------------- chi0
     def chi0(self, dv, keep=False):
-        """Density response (plane-major) to the local potential dv."""
+        """Density response (plane-major) to the local potential dv.
+
+        Two passes over k: the first accumulates the orbital part and keeps
+        (de_n, f'_n) per k; the second adds the occupation change
+        df_n = f'_n (dmu - de_n) with ONE dmu for the whole mesh (the map
+        bisects a single mu over sum_k w_k sum_n f, occupation.py:25-49), or
+        with the reference's per-k dmu and sign (response.py:96-103) under
+        occ_sign="reference"."""
         e = self.e
         st = e.st
         self.stats["chi0"] += 1
         vs = e.stage(dv)
         out = torch.zeros(e.nfft, dtype=torch.float64, device=e.dev)
-        kept = []
+        kept, occ_parts = [], []
         for ik in range(e.nk):
             psi = self.psi[ik]
             eps = self.eig[ik]
@@ -251,19 +262,36 @@ class Response:
                 fo = f[occ_idx]
                 dpsi[occ_idx] += self.sternheimer(ik, Vpsi[occ_idx], eps[occ_idx],
                                                   weight=fo / fo.max() if self.weighted else None)
-            if st.sigma > 1e-6:
-                fp = f * (1.0 - f) / st.sigma
-                dmu = float(torch.sum(fp * de)) / (float(torch.sum(fp)) + 1e-30)
-                df = self.sgn * fp * (dmu - de)
-            else:
-                df = torch.zeros_like(f)
             c = self.kw[ik] / e.omega
             out += self._cross_density(psi, dpsi, c * f)
-            out += e.plan64.density(psi, (c * df).contiguous())
+            fp = f * (1.0 - f) / st.sigma if st.sigma > 1e-6 else torch.zeros_like(f)
+            occ_parts.append((de, fp))
             if keep:
-                kept.append((dpsi, df))
+                kept.append(dpsi)
+        dfs = self.occupation_change(occ_parts)
+        for ik in range(e.nk):
+            if st.sigma > 1e-6:
+                c = self.kw[ik] / e.omega
+                out += e.plan64.density(self.psi[ik], (c * dfs[ik]).contiguous())
         if keep:
-            self.kept = kept
+            self.kept = list(zip(kept, dfs))
+        return out
+
+    def occupation_change(self, occ_parts):
+        """df per k from [(de, f')] per k: f'(dmu - de) with one dmu =
+        sum_k w_k sum_n f' de / sum_k w_k sum_n f' (exact), or the reference's
+        f'(de - dmu_k) with dmu_k from k alone."""
+        if self.e.st.sigma <= 1e-6:
+            return [torch.zeros_like(fp) for _, fp in occ_parts]
+        if self.sgn > 0:
+            num = sum(self.kw[ik] * float(torch.sum(fp * de)) for ik, (de, fp) in enumerate(occ_parts))
+            den = sum(self.kw[ik] * float(torch.sum(fp)) for ik, (de, fp) in enumerate(occ_parts))
+            dmu = num / (den + 1e-30)
+            return [fp * (dmu - de) for de, fp in occ_parts]
+        out = []
+        for de, fp in occ_parts:
+            dmu_k = float(torch.sum(fp * de)) / (float(torch.sum(fp)) + 1e-30)
+            out.append(fp * (de - dmu_k))
         return out
 
     def _cross_density(self, psi, phi, w):

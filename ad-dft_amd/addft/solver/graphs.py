@@ -61,9 +61,15 @@ def filter_recurrence(apply, X, HX, abc, degree):
 
 
 class FilterGraph:
-    """One captured filter.  hpsi(p, vs) -> H p for a (nk, m, nG) block."""
+    """One captured filter.  hpsi(p, vs) -> H p for a (nk, m, nG) block.
 
-    def __init__(self, hpsi, X, vs, degree):
+    held(): the device buffers the captured kernels address besides the
+    graph's own pool (e.g. the sphere plans' FFT workspaces); the graph keeps
+    a reference to each for its lifetime, so a later caller that replaces
+    such a buffer (SpherePlan.work growing) cannot free memory a replay
+    still writes."""
+
+    def __init__(self, hpsi, X, vs, degree, held=None):
         self.degree = int(degree)
         self.shape, self.dtype = tuple(X.shape), X.dtype
         self.X = X.detach().clone()
@@ -88,6 +94,7 @@ class FilterGraph:
         self.graph = torch.cuda.CUDAGraph()
         with torch.cuda.graph(self.graph):
             self.Y = filter_recurrence(apply, self.X, self.HX, self.abc, self.degree)
+        self.held = [t for t in (held() if held is not None else ()) if t is not None]
         self.replays = 0
 
     def matches(self, X, degree):

@@ -316,7 +316,8 @@ def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, 
         fg = graph_cache.get(key)
         if fg is None:
             before = engine.hpsi_bands[tag]
-            fg = fgraphs.FilterGraph(lambda p, v: engine.hpsi_batch(p, v, fp32=fp32), Xb, vs, cheb_degree)
+            fg = fgraphs.FilterGraph(lambda p, v: engine.hpsi_batch(p, v, fp32=fp32), Xb, vs, cheb_degree,
+                                     held=engine.graph_buffers)
             engine.hpsi_bands[tag] = before
             graph_cache[key] = fg
 

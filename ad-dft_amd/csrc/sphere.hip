@@ -1126,7 +1126,7 @@ int addft_sphere_plan_create(int n1, int n2, int n3, int64_t ng, const int32_t* 
   ADDFT_CHECK_ARG(out != nullptr && h_miller != nullptr, "null argument");
   ADDFT_CHECK_ARG(ng > 0 && ng < (int64_t)INT32_MAX, "ng out of range");
   ADDFT_CHECK_ARG(precision == ADDFT_C64 || precision == ADDFT_C128, "bad precision");
-  auto* p = new addft_sphere_plan_s();
+  auto* p = new addft_sphere_plan_s();  // value-initialised: every d_* pointer starts null
   std::memset((void*)&p->f1, 0, sizeof(FftFactors) * 3);
   p->n1 = n1;
   p->n2 = n2;
@@ -1262,7 +1262,7 @@ int addft_sphere_plan_create(int n1, int n2, int n3, int64_t ng, const int32_t* 
       (rc = upload_i32(scol, &p->d_scol)) || (rc = upload_i32(rowval, &p->d_rowval)) ||
       (p->lane_ok && (rc = upload_i32(rowinfo, &p->d_rowinfo))) ||
       (p->lane_z && (rc = upload_i32(a3, &p->d_a3)))) {
-    delete p;
+    addft_sphere_plan_destroy(p);  // frees the tables already uploaded (the rest are null)
     return rc;
   }
   if (precision == ADDFT_C64) {
@@ -1271,8 +1271,10 @@ int addft_sphere_plan_create(int n1, int n2, int n3, int64_t ng, const int32_t* 
         (p->lane_ok && (rc = launch_plane_lane<float>(p, nullptr, nullptr, 0, nullptr))) ||
         (p->lane_ok && (rc = launch_density_lane<float>(p, nullptr, nullptr, 0, nullptr, nullptr))) ||
         (p->lane_z && (rc = launch_z_inverse<float>(p, nullptr, nullptr, 0, nullptr))) ||
-        (p->lane_z && (rc = launch_z_forward<float, float>(p, nullptr, 1.0f, nullptr, nullptr, nullptr, 0, nullptr))))
+        (p->lane_z && (rc = launch_z_forward<float, float>(p, nullptr, 1.0f, nullptr, nullptr, nullptr, 0, nullptr)))) {
+      addft_sphere_plan_destroy(p);
       return rc;
+    }
   } else {
     if ((rc = upload_twiddles<double>(n1, &p->d_tw1)) || (rc = upload_twiddles<double>(n2, &p->d_tw2)) ||
         (rc = upload_twiddles<double>(n3, &p->d_tw3)) ||
@@ -1280,8 +1282,10 @@ int addft_sphere_plan_create(int n1, int n2, int n3, int64_t ng, const int32_t* 
         (p->lane_ok && (rc = launch_plane_lane<double>(p, nullptr, nullptr, 0, nullptr))) ||
         (p->lane_ok && (rc = launch_density_lane<double>(p, nullptr, nullptr, 0, nullptr, nullptr))) ||
         (p->lane_z && (rc = launch_z_inverse<double>(p, nullptr, nullptr, 0, nullptr))) ||
-        (p->lane_z && (rc = launch_z_forward<double, double>(p, nullptr, 1.0, nullptr, nullptr, nullptr, 0, nullptr))))
+        (p->lane_z && (rc = launch_z_forward<double, double>(p, nullptr, 1.0, nullptr, nullptr, nullptr, 0, nullptr)))) {
+      addft_sphere_plan_destroy(p);
       return rc;
+    }
   }
   *out = p;
   return ADDFT_OK;

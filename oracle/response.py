@@ -5,12 +5,15 @@ Test infrastructure only.  Restates, in NumPy FP64:
     response J dn = dF/dn . dn of one physical map n -> F(n) from the full
     dense eigensystem of H[n*], occupied columns aligned to the state by the
     polar factor of <C_occ|psi>, pair coefficients M_mn/(e_n - e_m) with the
-    degeneracy cut |e_n - e_m| > 1e-6, and the occupation change with dmu
-    per k-point (response.py:96-103).  The reference writes that change as
-    f'(de - dmu); with f = 1/(1 + exp((e - mu)/sigma)) (occupation.py:21-22)
-    it is f'(dmu - de).  occ_sign="exact" (default) uses the latter, which
-    central differences of the map confirm to 1e-12 where the reference's
-    sign is off by ~20% (partially occupied bands); occ_sign="reference"
+    degeneracy cut |e_n - e_m| > 1e-6, and the occupation change.  The
+    reference writes that change as f'(de - dmu) with dmu per k-point
+    (response.py:96-103).  The map's own derivative is f'(dmu - de)
+    (f = 1/(1 + exp((e - mu)/sigma)), occupation.py:21-22) with ONE dmu for
+    the whole mesh, since the map bisects a single mu over all k
+    (occupation.py:25-49).  occ_sign="exact" (default) uses that form,
+    which central differences of the map confirm to 1e-12 where the
+    reference's is off by ~20% (partially occupied bands) and, with several
+    k-points, where a per-k dmu is off as well; occ_sign="reference"
     reproduces the reference's formula.  With integer or saturated
     occupations (f' = 0) the two agree;
   * reference diff/implicit.py:32-92 -- (I - J)^T lam = dL/dn* and
@@ -65,8 +68,11 @@ def full_eigensystem(geo, st, state, cache=None):
 
 
 def make_response(geo, st, state, tol=1e-6, cache=None, occ_sign="exact"):
-    """dn -> J dn (response.py:33-114)."""
-    sgn = 1.0 if occ_sign == "exact" else -1.0
+    """dn -> J dn (response.py:33-114).  occ_sign="exact": the derivative of
+    the map, with one dmu over all k-points; "reference": the reference's
+    per-k dmu and occupation-term sign."""
+    if occ_sign not in ("exact", "reference"):
+        raise ValueError(f"unknown occ_sign {occ_sign!r}")
     basis = st.basis
     lat = geo.lattice
     kxc = P.xc_kernel(state.rho_r)
@@ -78,7 +84,9 @@ def make_response(geo, st, state, tol=1e-6, cache=None, occ_sign="exact"):
         dv_r = P.hartree_potential_r(basis, lat, dn_r) + kxc * dn_r
         dH = S.vmat_from_real(basis, dv_r)
         out = np.zeros(basis.nfft)
-        for ik in range(eig.shape[0]):
+        nk = eig.shape[0]
+        deps_k, fp_k = [], []
+        for ik in range(nk):
             C, eps = vec[ik], eig[ik]
             M = np.conj(C).T @ dH @ C
             M = 0.5 * (M + np.conj(M).T)
@@ -87,18 +95,30 @@ def make_response(geo, st, state, tol=1e-6, cache=None, occ_sign="exact"):
             off = np.abs(d) > tol
             coef = np.where(off, M / np.where(off, d, 1.0), 0.0)
             f = np.concatenate([state.occ[ik], np.zeros(eig.shape[-1] - nb)])
-            if st.sigma > 1e-6:
-                fp = f * (1.0 - f) / st.sigma
-                dmu = np.sum(st.kweights[ik] * fp * deps) / (np.sum(st.kweights[ik] * fp) + 1e-30)
-                df = sgn * fp * (dmu - deps)
-            else:
-                df = np.zeros_like(f)
             dc = (C @ coef)[:, :nb].T
             dpsi_r = B.psi_to_real(basis, lat, dc[None])[0]
-            pr = psi_r[ik]
-            out = out + st.kweights[ik] * st.deg * (
-                np.sum(f[:nb, None] * 2.0 * np.real(np.conj(pr) * dpsi_r), axis=0)
-                + np.sum(df[:nb, None] * np.abs(pr) ** 2, axis=0))
+            out = out + st.kweights[ik] * st.deg * np.sum(
+                f[:nb, None] * 2.0 * np.real(np.conj(psi_r[ik]) * dpsi_r), axis=0)
+            deps_k.append(deps)
+            fp_k.append(f * (1.0 - f) / st.sigma if st.sigma > 1e-6 else np.zeros_like(f))
+        if st.sigma <= 1e-6:
+            return out
+        w = np.asarray(st.kweights, dtype=float)
+        if occ_sign == "exact":
+            # one Fermi level for the whole mesh (occupation.py:25-49 bisects
+            # a single mu over sum_k w_k sum_n f): dmu from the k-summed
+            # electron count
+            dmu_all = (sum(w[ik] * np.sum(fp_k[ik] * deps_k[ik]) for ik in range(nk))
+                       / (sum(w[ik] * np.sum(fp_k[ik]) for ik in range(nk)) + 1e-30))
+        for ik in range(nk):
+            fp, deps = fp_k[ik], deps_k[ik]
+            if occ_sign == "exact":
+                df = fp * (dmu_all - deps)
+            else:
+                # the reference's per-k dmu and sign (response.py:96-103)
+                dmu = np.sum(w[ik] * fp * deps) / (np.sum(w[ik] * fp) + 1e-30)
+                df = fp * (deps - dmu)
+            out = out + w[ik] * st.deg * np.sum(df[:nb, None] * np.abs(psi_r[ik]) ** 2, axis=0)
         return out
 
     return apply

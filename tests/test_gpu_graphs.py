@@ -39,9 +39,14 @@ def test_filter_graph_replays_match_eager(fp32):
     v, _ = eng.effective_potential(rho)
     vs = eng.stage(v, mixed_stage=fp32)
     degree = 6
-    fg = graphs.FilterGraph(hpsi, block(), vs, degree)
+    fg = graphs.FilterGraph(hpsi, block(), vs, degree, held=eng.graph_buffers)
     tol = 1e-5 if fp32 else 1e-12
+    plan = eng.plan32 if fp32 else eng.plan64
     for rep in range(4):
+        if rep == 2:
+            # a larger eager call replaces the plan's workspace; the graph keeps
+            # the captured one alive, so its replays stay correct
+            plan.work(4 * eng.nk * 6 + 64)
         X = block()
         HX = hpsi(X, vs)
         abc = torch.stack([torch.full((eng.nk,), 0.5 + 0.1 * rep), torch.full((eng.nk,), -0.4),

@@ -2,10 +2,12 @@
 
 * J dn (in-set pairs + Sternheimer complement + occupation change) against
   the oracle's dense-eigensystem response (reference physics/response.py
-  :33-114 with the occupation sign fixed) -- bar 1e-8 relative;
+  :33-114), in the exact form (one dmu for the mesh, sign of the map's
+  derivative) and in the reference's literal form (per-k dmu, its sign),
+  at Gamma and on two k-points -- bar 1e-8 relative;
 * dDeltaQ/dR and dDeltaQ/deps against the oracle's implicit adjoint on the
   reference's own DeltaQ system (tests/test_forces.py:187-217) and on its
-  partially occupied variant -- bar 1e-7;
+  partially occupied variant, both forms -- bar 1e-7;
 * the 8-atom cell: dDeltaQ/dR against central differences of DeltaQ with
   the SCF re-converged on the GPU (Richardson, steps 1e-4 / 5e-5) -- bar 1e-5
   relative.
@@ -26,48 +28,61 @@ LAT = 6.0 * np.eye(3)
 FRAC = np.array([[0.0, 0.0, 0.0], [0.53, 0.47, 0.49]])
 
 
-def _oracle(nband):
-    geo, st = OS.setup(LAT, FRAC, OPS.synthetic(), 1.5, nband, sigma=0.05, density_factor=2.0)
+KFRAC2 = np.array([[0.0, 0.0, 0.0], [0.25, 0.25, 0.0]])
+KW2 = np.array([0.25, 0.75])
+
+
+def _oracle(nband, two_k=False):
+    kw = dict(kfrac=KFRAC2, kweights=KW2) if two_k else {}
+    geo, st = OS.setup(LAT, FRAC, OPS.synthetic(), 1.5, nband, sigma=0.05, density_factor=2.0, **kw)
     state = OS.solve_scf_dense(geo, st, tol=1e-14, mixer="kerker", max_iter=600)
     w = ORS.make_gaussian_weight(st.basis, OB.positions(LAT, FRAC)[1], 0.9, LAT)
     return geo, st, state, w
 
 
-def _device(nband):
+def _device(nband, two_k=False):
     import addft
-    s = addft.setup(lattice=LAT, frac=FRAC, ecut=1.5, nband=nband, density_factor=2.0, sigma=0.05)
+    from addft.geometry.kmesh import KMesh
+    kw = dict(kmesh=KMesh(kfrac=KFRAC2, weights=KW2)) if two_k else {}
+    s = addft.setup(lattice=LAT, frac=FRAC, ecut=1.5, nband=nband, density_factor=2.0, sigma=0.05, **kw)
     st = addft.scf(s, tol=1e-13)
     return addft, s, st
 
 
-@pytest.mark.parametrize("nband", [2, 6])
-def test_response_matches_oracle(nband):
+@pytest.mark.parametrize("nband,occ_sign,two_k", [(2, "exact", False), (6, "exact", False),
+                                                  (6, "reference", False), (6, "exact", True),
+                                                  (6, "reference", True)])
+def test_response_matches_oracle(nband, occ_sign, two_k):
+    """J on the reference's DeltaQ system, its partially occupied variant and
+    that variant on two k-points of unequal weight (one Fermi level: dmu
+    summed over k in the exact form, per k in the reference's form)."""
     import torch
     from addft import api
     from addft.response import Response
-    geo, ost, ostate, w = _oracle(nband)
-    addft, s, st = _device(nband)
+    geo, ost, ostate, w = _oracle(nband, two_k)
+    addft, s, st = _device(nband, two_k)
     so = api._from_state(s, st)
-    resp = Response(so.engine, so.psi, so.eig, so.occ, so.rho)
+    resp = Response(so.engine, so.psi, so.eig, so.occ, so.rho, occ_sign=occ_sign)
     dn = np.random.default_rng(1).normal(size=ost.basis.nfft) * 1e-3
-    ref = ORS.make_response(geo, ost, ostate)(dn)
+    ref = ORS.make_response(geo, ost, ostate, occ_sign=occ_sign)(dn)
     eng = so.engine
     got = eng.to_rowmajor(resp.J(eng.to_planar(torch.as_tensor(dn).to(eng.dev)))).cpu().numpy()
     err = np.abs(got - ref).max() / np.abs(ref).max()
-    print(f"nband {nband}: J rel err {err:.2e}", flush=True)
+    print(f"nband {nband} {occ_sign} two_k={two_k}: J rel err {err:.2e}", flush=True)
     assert err < 1e-8
 
 
-@pytest.mark.parametrize("nband", [2, 6])
-def test_delta_q_grad_matches_oracle(nband):
+@pytest.mark.parametrize("nband,occ_sign", [(2, "exact"), (6, "exact"), (6, "reference")])
+def test_delta_q_grad_matches_oracle(nband, occ_sign):
     geo, ost, ostate, w = _oracle(nband)
-    dR_ref, de_ref, info = ORS.delta_q_grad(geo, ost, ostate, w)
+    dR_ref, de_ref, info = ORS.delta_q_grad(geo, ost, ostate, w, occ_sign=occ_sign)
     addft, s, st = _device(nband)
     wd = addft.make_gaussian_weight(s, OB.positions(LAT, FRAC)[1], 0.9)
     assert np.abs(wd - w).max() < 1e-12
-    g = addft.delta_q_grad(s, st, wd)
+    g = addft.delta_q_grad(s, st, wd, occ_sign=occ_sign)
     err = np.abs(g.dR - dR_ref).max()
-    print(f"nband {nband}: dC/dR err {err:.2e} (|dC/dR| {np.abs(dR_ref).max():.3e}); {g.diagnostics}", flush=True)
+    print(f"nband {nband} {occ_sign}: dC/dR err {err:.2e} (|dC/dR| {np.abs(dR_ref).max():.3e}); "
+          f"{g.diagnostics}", flush=True)
     assert g.diagnostics["adjoint_residual"] < 1e-8
     assert err < 1e-7 * max(1.0, np.abs(dR_ref).max())
     assert abs(g.value - ORS.delta_q(ost.basis, LAT, ostate.rho_r, w)) < 1e-10

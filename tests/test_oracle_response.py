@@ -62,3 +62,31 @@ def test_delta_q_adjoint_matches_rescf_differences(nband, h):
                            ("eps", (0, 0), de[0, 0]), ("eps", (0, 1), de[0, 1])]:
         fd = _richardson(geo, st, w, what, idx, h)
         assert abs(got - fd) < 2e-8 + 1e-6 * abs(fd), (what, idx, got, fd)
+
+
+def _two_k_system():
+    """The partially occupied variant (nband 6, sigma 0.05) on two k-points
+    of unequal weight: the occupations of both points change through one
+    Fermi level."""
+    geo, st = S.setup(6.0 * np.eye(3), np.array([[0, 0, 0], [0.53, 0.47, 0.49]]), PS.synthetic(), 1.5, 6,
+                      kfrac=np.array([[0.0, 0.0, 0.0], [0.25, 0.25, 0.0]]), kweights=np.array([0.25, 0.75]),
+                      sigma=0.05, density_factor=2.0)
+    state = S.solve_scf_dense(geo, st, tol=1e-14, mixer="kerker", max_iter=600)
+    return geo, st, state
+
+
+def test_response_two_k_points_one_fermi_level():
+    """With several k-points the map's response has ONE dmu for the whole
+    mesh: J matches central differences of the map to 1e-6 (a per-k dmu
+    with the sign fixed is off by 3e-3 here, the reference's literal form,
+    response.py:96-103, by 2e-2)."""
+    geo, st, state = _two_k_system()
+    fp = [state.occ[ik] * (1 - state.occ[ik]) / st.sigma for ik in range(2)]
+    assert min(float(np.max(f)) for f in fp) > 0.5          # both points partially occupied
+    apply = RS.make_response(geo, st, state)
+    d = np.random.default_rng(1).normal(size=st.basis.nfft) * 1e-3
+    h = 1e-3
+    fd = (S.scf_map(geo, st, state.rho_r + h * d) - S.scf_map(geo, st, state.rho_r - h * d)) / (2 * h)
+    assert np.abs(apply(d) - fd).max() < 1e-6 * np.abs(fd).max()
+    ref = RS.make_response(geo, st, state, occ_sign="reference")(d)
+    assert np.abs(ref - fd).max() > 0.01 * np.abs(fd).max()
