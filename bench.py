@@ -49,6 +49,13 @@ def hpsi_bytes(nb, ng, nst, n3, nfft, cbytes, rbytes):
     return nb * cbytes * (2 * ng + 4 * nst * n3) + rbytes * nfft
 
 
+def hpsi_paired_bytes(nb, ng, nst, n3, nfft, cbytes, rbytes):
+    """Algorithmic HBM bytes of the k = 0 local Hpsi of nb bands real in real
+    space, two bands per transform: nb bands in, nb out, four stick round
+    trips for each of the nb/2 packed transforms, V(r) once."""
+    return nb * cbytes * 2 * ng + (nb + 1) // 2 * cbytes * 4 * nst * n3 + rbytes * nfft
+
+
 def density_bytes(nb, ng, nst, n3, nfft, cbytes):
     return nb * cbytes * (ng + 2 * nst * n3) + 16 * nfft
 
@@ -93,6 +100,25 @@ def hpsi_measure(basis, lat, nb, steps, warmup, torch, native):
         res[name] = {"hpsi_ms": ms, "hpsi_GBps": by / (ms * 1e-3) / 1e9, "hpsi_bytes": by,
                      "density_ms": msd, "density_GBps": dby / (msd * 1e-3) / 1e9,
                      "n_sticks": plan.n_sticks}
+        # the k = 0 path the SCF runs: bands real in real space, two per
+        # transform (pack, local Hpsi on nb/2 rows, unpack through the pairs)
+        from addft.kernels.nonlocal_bf16 import gamma_pairs
+        pairs = gamma_pairs(basis.miller[plan.perm])
+        if pairs is not None:
+            plus = torch.as_tensor(pairs[0], device=dev)
+            minus = torch.as_tensor(pairs[1], device=dev)
+            psr = native.gamma_unpack2(psi[:(nb + 1) // 2].contiguous(), nb, plus, minus)  # real-space-real rows
+            hp = torch.empty(((nb + 1) // 2, basis.nG), dtype=cdt, device=dev)
+            plan.work((nb + 1) // 2)
+
+            def paired():
+                plan.hpsi_local(native.gamma_pack2(psr), tk, vs, hp)
+                return native.gamma_unpack2(hp, nb, plus, minus)
+
+            msp = timed(paired, steps, warmup, torch)
+            pby = hpsi_paired_bytes(nb, basis.nG, plan.n_sticks, basis.fft_shape[2], basis.nfft, cb, rb)
+            res[name].update({"paired_hpsi_ms": msp, "paired_hpsi_GBps": pby / (msp * 1e-3) / 1e9,
+                              "paired_hpsi_bytes": pby})
         del psi, out, plan
         torch.cuda.empty_cache()
     return res
@@ -230,45 +256,98 @@ def one_geometry(addft, system, tol, torch):
     return F, forces, sig, time.perf_counter() - t0, info, dict(eng.hpsi_bands)
 
 
-def cpu_baseline(basis, lat, nproj, band_apps, nb_sample=32, nl_cols=8192):
-    """The oracle's FP64 Hpsi on the host: local part (NumPy FFTs) on a
-    sample of bands of the same cell, nonlocal part as the two projector
-    GEMMs on a slice of the G-sphere; per-band cost times the band
-    applications of H counted in the GPU mixed run.  A lower bound of a
-    CPU SCF+forces+stress (subspace algebra and the rest excluded)."""
+def cpu_cores():
+    """Host threads available to this process (the affinity mask)."""
+    return len(os.sched_getaffinity(0))
+
+
+def cpu_baseline(basis, lat, nproj, band_apps, nb_sample=32, nl_cols=8192, repeats=5):
+    """The oracle's FP64 Hpsi on the host, on all the process's cores (BLAS
+    threads and scipy.fft workers set to the affinity count): local part
+    (FFTs) on a sample of bands of the same cell, nonlocal part as the two
+    projector GEMMs on a slice of the G-sphere; per-band cost times the band
+    applications of H counted in the GPU mixed run.  Median of `repeats`
+    warm runs (after one cold run) with the interquartile range.  A lower
+    bound of a CPU SCF+forces+stress (subspace algebra and the rest
+    excluded)."""
     import numpy as np
-    from oracle import basis as OB
     from threadpoolctl import threadpool_limits
-    with threadpool_limits(1):
-        return _cpu_baseline(OB, np, lat, nproj, band_apps, nb_sample, nl_cols)
+    from oracle import basis as OB
+    n = cpu_cores()
+    with threadpool_limits(n):
+        return _cpu_baseline(OB, np, lat, nproj, band_apps, nb_sample, nl_cols, n, repeats)
 
 
-def _cpu_baseline(OB, np, lat, nproj, band_apps, nb_sample, nl_cols):
+def _cpu_baseline(OB, np, lat, nproj, band_apps, nb_sample, nl_cols, ncores, repeats):
+    import scipy.fft
     b = OB.make_basis(lat, 15.0)
     rng = np.random.default_rng(0)
     psi = rng.normal(size=(1, nb_sample, b.nG)) + 1j * rng.normal(size=(1, nb_sample, b.nG))
     v = 0.1 * rng.random(b.nfft)
-    t0 = time.perf_counter()
-    pr = OB.psi_to_real(b, lat, psi)
-    OB.real_to_psi(b, lat, pr * v[None, None, :])
-    t_loc = (time.perf_counter() - t0) / nb_sample
     P = rng.normal(size=(nproj, nl_cols)) + 1j * rng.normal(size=(nproj, nl_cols))
     ps = psi[0, :, :nl_cols]
     D = np.eye(nproj)
-    t0 = time.perf_counter()
-    bec = P.conj() @ ps.T
-    _ = (D @ bec).T @ P
-    t_nl = (time.perf_counter() - t0) / nb_sample * (b.nG / nl_cols)
-    per_band = t_loc + t_nl
-    total = per_band * band_apps
-    return {"value": round(total, 2), "unit": "s/geometry", "cores": 1, "kind": "port",
-            "sample": (f"oracle FP64 Hpsi on {nb_sample} bands of the 216-atom cell (local: NumPy FFT "
-                       f"{t_loc * 1e3:.1f} ms/band; nonlocal: projector GEMMs on {nl_cols} of {b.nG} G, "
-                       f"scaled, {t_nl * 1e3:.1f} ms/band) x {band_apps} band-applications of H counted in "
-                       f"the GPU mixed run; lower bound (subspace algebra excluded), extrapolated")}
+    runs = []
+    with scipy.fft.set_workers(ncores):
+        for _ in range(repeats + 1):
+            t0 = time.perf_counter()
+            pr = OB.psi_to_real(b, lat, psi)
+            OB.real_to_psi(b, lat, pr * v[None, None, :])
+            t_loc = (time.perf_counter() - t0) / nb_sample
+            t0 = time.perf_counter()
+            bec = P.conj() @ ps.T
+            _ = (D @ bec).T @ P
+            t_nl = (time.perf_counter() - t0) / nb_sample * (b.nG / nl_cols)
+            runs.append((t_loc, t_nl))
+    warm = np.asarray(runs[1:])
+    tot = (warm[:, 0] + warm[:, 1]) * band_apps
+    q1, med, q3 = np.percentile(tot, [25, 50, 75])
+    t_loc, t_nl = np.median(warm[:, 0]), np.median(warm[:, 1])
+    return {"value": round(float(med), 2), "unit": "s/geometry", "cores": ncores, "kind": "port",
+            "median_of": repeats, "iqr": [round(float(q1), 2), round(float(q3), 2)],
+            "cold": round(float((runs[0][0] + runs[0][1]) * band_apps), 2),
+            "sample": (f"oracle FP64 Hpsi on {nb_sample} bands of the 216-atom cell, {ncores} threads (local: "
+                       f"NumPy/SciPy FFT {t_loc * 1e3:.2f} ms/band; nonlocal: projector GEMMs on {nl_cols} of "
+                       f"{b.nG} G, scaled, {t_nl * 1e3:.2f} ms/band) x {band_apps} band-applications of H "
+                       f"counted in the GPU mixed run; lower bound (subspace algebra excluded), extrapolated")}
 
 
-def fd_check(addft, testsystems, workload, precision, tol, np, torch, h=1e-3, seed=0, only="all"):
+def cpu_full_scf(testsystems, names=("c1", "c2"), repeats=(5, 1)):
+    """The oracle's full SCF + forces + stress (FP64 dense eigensolver path,
+    the reference's choice for these sphere sizes) on the host's cores for
+    the small configurations, median of warm runs (c2 once: one run is
+    ~10-60 s).  Seconds per geometry."""
+    import numpy as np
+    from threadpoolctl import threadpool_limits
+    from addft.geometry.upf import si_upf_one_projector, si_upf_three_projectors
+    from oracle import forces as OF
+    from oracle import psp as OPS
+    from oracle import solver as OS
+    out = {}
+    n = cpu_cores()
+    with threadpool_limits(n):
+        for name, reps in zip(names, repeats):
+            kw = testsystems.config(name)
+            psp = OPS.from_upf_text(si_upf_one_projector() if name == "c1" else si_upf_three_projectors())
+            times = []
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                geo, st = OS.setup(kw["lattice"], kw["frac"], psp, kw["ecut"], kw["nband"],
+                                   sigma=kw.get("sigma", 0.05))
+                state = OS.solve_scf_dense(geo, st, tol=1e-10)
+                F = OS.mermin_free_energy(geo, st, state)
+                OF.envelope_gradient(geo, st, state)          # forces and stress
+                times.append(time.perf_counter() - t0)
+            q1, med, q3 = np.percentile(times, [25, 50, 75])
+            out[name] = {"s_per_geometry": round(float(med), 3), "runs": reps,
+                         "iqr": [round(float(q1), 3), round(float(q3), 3)], "F": float(F),
+                         "scf_iterations": int(state.n_iter), "nG": int(st.basis.nG)}
+            say(f"  CPU {name}: {med:.2f} s per geometry ({reps} runs, {n} threads)")
+    return {"cores": n, "kind": "port", "what": "oracle SCF (tol 1e-10) + forces + stress", **out}
+
+
+def fd_check(addft, testsystems, workload, precision, tol, np, torch, h=1e-3, seed=0, only="all", n_force=6,
+             strains=((0, 0), (0, 1))):
     """Central differences of the Mermin free energy, F(x + h) - F(x - h)
     over 2h, with the SCF re-converged at each displaced geometry (warm
     start from the undisplaced density and orbitals, same static basis:
@@ -289,9 +368,9 @@ def fd_check(addft, testsystems, workload, precision, tol, np, torch, h=1e-3, se
     sig = stress_from_grad(omega, 0.5 * (g.eps + g.eps.T))
     say(f"fd-check {workload} {precision}: reference SCF+gradient {time.perf_counter() - t0:.1f} s")
     rng = np.random.default_rng(seed)
-    atoms = rng.choice(len(frac), size=6, replace=False)
-    dirs = rng.integers(0, 3, size=6)
-    strains = [(0, 0), (0, 1)]
+    atoms = rng.choice(len(frac), size=6, replace=False)[:n_force]
+    dirs = rng.integers(0, 3, size=6)[:n_force]
+    strains = list(strains)
     if only == "forces":
         strains = []
     if only == "strain":
@@ -343,6 +422,11 @@ def main():
     ap.add_argument("--fp64-tol", type=float, default=1e-10)
     ap.add_argument("--hpsi-steps", type=int, default=20)
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--fp64-steps", type=int, default=3, help="timed FP64-form geometries per tolerance")
+    ap.add_argument("--fd-components", type=int, default=1,
+                    help="1: add central differences (2 force, 1 strain components) to the headline line")
+    ap.add_argument("--cpu-full", type=int, default=1,
+                    help="1: time the oracle's full SCF+forces+stress of c1 and c2 on the host")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--fd-check", choices=["fp64", "mixed"], default=None,
                     help="central differences (h = 1e-3) of the free energy on 6 seeded force components "
@@ -429,11 +513,23 @@ def main():
         f"fp64 {hp['fp64']['hpsi_ms']:.3f} ms ({hp['fp64']['hpsi_GBps']:.0f} GB/s)")
 
     ref = None
+    fp64_runs = {}
     if not args.no_fp64:
-        say(f"FP64 form, tol {args.fp64_tol:g}")
+        # the FP64 form, warmed up and timed over several geometries like the
+        # mixed form; also at the mixed form's tolerance (equal-tolerance ratio)
         sys_d = addft.setup(**testsystems.config(args.workload, precision="fp64"))
-        ref = one_geometry(addft, sys_d, args.fp64_tol, torch)
-        say(f"  F={ref[0]:.10f}  {ref[3]:.2f} s  {ref[4]['iterations']} SCF iterations")
+        say(f"FP64 form warm-up, tol {args.fp64_tol:g}")
+        one_geometry(addft, sys_d, args.fp64_tol, torch)
+        for tol in (args.fp64_tol, args.mixed_tol):
+            ts = []
+            for i in range(args.fp64_steps):
+                r = one_geometry(addft, sys_d, tol, torch)
+                ts.append(r[3])
+                say(f"  FP64 tol {tol:g} step {i + 1}/{args.fp64_steps}: F={r[0]:.10f}  {r[3]:.2f} s  "
+                    f"{r[4]['iterations']} SCF iterations")
+                if tol == args.fp64_tol:
+                    ref = r
+            fp64_runs[tol] = (float(np.mean(ts)), ts, r)
         addft.api._ENGINES.clear()
         del sys_d
         torch.cuda.empty_cache()
@@ -483,18 +579,46 @@ def main():
                                f"{nb} bands", "algorithmic_bytes": f32["hpsi_bytes"],
                      "ms_per_call": round(f32["hpsi_ms"], 4)},
         "hpsi": {p: {k: (round(v, 4) if isinstance(v, float) else v) for k, v in hp[p].items()} for p in hp},
+        # the k = 0 local Hpsi the SCF runs: two real bands per transform
+        "roofline_paired": ({"bound": "hbm", "achieved": round(f32["paired_hpsi_GBps"], 2), "peak": HBM_PEAK_GBS,
+                             "unit": "GB/s", "frac": round(f32["paired_hpsi_GBps"] / HBM_PEAK_GBS, 4),
+                             "kernel": f"gamma_pack2 + addft_hpsi_local FP32 on {(nb + 1) // 2} packed rows + "
+                                       f"gamma_unpack2, {nb} bands",
+                             "algorithmic_bytes": f32["paired_hpsi_bytes"],
+                             "ms_per_call": round(f32["paired_hpsi_ms"], 4)}
+                            if "paired_hpsi_ms" in f32 else None),
         "tuning": {k: v["choice"] for k, v in tuning.decisions().items()},
     }
     if ref is not None:
-        F64, f64, s64, t64, info64, _ = ref
-        line["fp64"] = {"s_per_geometry": round(t64, 3), "scf_iterations": info64["iterations"], "F": F64}
+        F64, f64, s64, _, info64, _ = ref
+        t64, ts64, _ = fp64_runs[args.fp64_tol]
+        t64e, ts64e, r64e = fp64_runs[args.mixed_tol]
+        line["fp64"] = {"s_per_geometry": round(t64, 3), "steps": len(ts64), "warmup": 1,
+                        "step_seconds": [round(t, 3) for t in ts64], "tol": args.fp64_tol,
+                        "scf_iterations": info64["iterations"], "F": F64}
+        line["fp64_at_mixed_tol"] = {"s_per_geometry": round(t64e, 3), "tol": args.mixed_tol,
+                                     "step_seconds": [round(t, 3) for t in ts64e],
+                                     "scf_iterations": r64e[4]["iterations"], "F": r64e[0],
+                                     "error_vs_fp64_tol": {"energy_Ha_per_atom": float(abs(r64e[0] - F64) / nat),
+                                                           "force_Ha_per_Bohr": float(np.abs(r64e[1] - f64).max())}}
         line["speedup_mixed_vs_fp64"] = round(t64 / sec, 3)
+        line["speedup_mixed_vs_fp64_equal_tol"] = round(t64e / sec, 3)
         line["error_mixed_vs_fp64"] = {"energy_Ha_per_atom": float(abs(F - F64) / nat),
                                        "force_Ha_per_Bohr": float(np.abs(forces - f64).max()),
                                        "stress_Ha_per_Bohr3": float(np.abs(sig - s64).max())}
+    if args.fd_components and args.workload == "c5":
+        say("central differences at the headline workload (mixed form, tol 1e-10, warm-started)")
+        fd = fd_check(addft, testsystems, args.workload, "mixed", 1e-10, np, torch, h=1e-3,
+                      n_force=2, strains=((0, 1),))
+        line["fd_check"] = {k: fd[k] for k in ("precision", "tol", "h", "max_force_err_Ha_per_Bohr",
+                                                "max_stress_err_Ha_per_Bohr3", "seconds")}
+        line["fd_check"]["components"] = fd["components"]
     if not args.no_cpu:
-        say("CPU baseline (oracle, bounded sample)")
+        say(f"CPU baseline (oracle, bounded sample, {cpu_cores()} threads)")
         line["cpu_baseline"] = cpu_baseline(basis, lat, nproj, bands["fp32"] + bands["fp64"])
+        if args.cpu_full:
+            say("CPU full SCF + forces + stress of the small configurations (oracle)")
+            line["cpu_full_scf"] = cpu_full_scf(testsystems)
     print(json.dumps(line), flush=True)
 
 

@@ -148,11 +148,46 @@ def _geo_perturbed(geo, what, idx, h):
     return S.Geo(lattice=B.strained(geo.lattice, eps), frac=geo.frac)
 
 
+def _map_eigenvalues(geo, st, rho_r):
+    """The lowest nband eigenvalues of H[rho] per k at this geometry (the
+    eigenvalues inside one map)."""
+    v_r, _ = S.effective_potential(geo, st, rho_r)
+    return S.eigh_lowest(S.hamiltonian(geo, st, v_r), st.nband)[0]
+
+
+def _occupation_densities(geo, st, state):
+    """|psi_nk(r)|^2 per k (nk, nband, nfft) of the state's orbitals."""
+    return np.abs(B.psi_to_real(st.basis, geo.lattice, state.psi)) ** 2
+
+
+def _occupation_change(st, occ, deps, occ_sign):
+    """df (nk, nband) for eigenvalue changes deps: f'(dmu - de) with one dmu
+    over the mesh (exact) or f'(de - dmu_k) per k (the reference's form,
+    response.py:96-103 and the JVP of its map, scf.py:147-151)."""
+    if st.sigma <= 1e-6:
+        return np.zeros_like(deps)
+    fp = occ * (1.0 - occ) / st.sigma
+    w = np.asarray(st.kweights, dtype=float)[:, None]
+    if occ_sign == "exact":
+        dmu = np.sum(w * fp * deps) / (np.sum(w * fp) + 1e-30)
+        return fp * (dmu - deps)
+    dmu_k = np.sum(w * fp * deps, axis=1, keepdims=True) / (np.sum(w * fp, axis=1, keepdims=True) + 1e-30)
+    return fp * (deps - dmu_k)
+
+
 def delta_q_grad(geo, st, state, weight, h=2e-5, occ_sign="exact"):
     """dC/dR (nat, 3) and dC/deps (3, 3) by the implicit adjoint
     (implicit.py:32-92).  The map term lam . dF/dtheta|_n* is taken by
     Richardson-extrapolated central differences of the map at fixed n*
-    (steps h and h/2, error O(h^4))."""
+    (steps h and h/2, error O(h^4)).
+
+    occ_sign="reference": the reference differentiates its map through the
+    analytic JVP of scf.py:97-197, whose occupation change carries the same
+    per-k dmu and sign as its response (scf.py:147-151), so both J and the
+    map term take the reference's form.  Its map term is the true one with
+    the occupation part Sum w g_s df |psi|^2 exchanged: the eigenvalue
+    changes de_n/dtheta come from central differences of the map's
+    eigenvalues at fixed n*."""
     basis = st.basis
     omega = B.volume(geo.lattice)
     g = omega / basis.nfft * weight
@@ -160,11 +195,20 @@ def delta_q_grad(geo, st, state, weight, h=2e-5, occ_sign="exact"):
     lam = np.linalg.solve((np.eye(basis.nfft) - J).T, g)
     resid = np.linalg.norm((np.eye(basis.nfft) - J).T @ lam - g) / np.linalg.norm(g)
     n_star = state.rho_r
+    if occ_sign == "reference" and st.sigma > 1e-6:
+        # lam . |psi_nk|^2 per (k, n), weighted w_k g_s
+        dens = _occupation_densities(geo, st, state)
+        lam_dens = np.einsum("knr,r->kn", dens, lam) * np.asarray(st.kweights)[:, None] * st.deg
 
     def central(what, idx, hh):
-        fp = S.scf_map(_geo_perturbed(geo, what, idx, hh), st, n_star)
-        fm = S.scf_map(_geo_perturbed(geo, what, idx, -hh), st, n_star)
-        return float(lam @ (fp - fm)) / (2 * hh)
+        gp, gm = _geo_perturbed(geo, what, idx, hh), _geo_perturbed(geo, what, idx, -hh)
+        out = float(lam @ (S.scf_map(gp, st, n_star) - S.scf_map(gm, st, n_star))) / (2 * hh)
+        if occ_sign == "reference" and st.sigma > 1e-6:
+            deps = (_map_eigenvalues(gp, st, n_star) - _map_eigenvalues(gm, st, n_star)) / (2 * hh)
+            swap = (_occupation_change(st, state.occ, deps, "reference")
+                    - _occupation_change(st, state.occ, deps, "exact"))
+            out += float(np.sum(swap * lam_dens))
+        return out
 
     def map_term(what, idx):
         return (4.0 * central(what, idx, h / 2) - central(what, idx, h)) / 3.0

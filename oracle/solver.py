@@ -104,7 +104,7 @@ def hamiltonian(geo, st, v_r, cache=None):
     for ik, k in enumerate(st.kpts):
         t = kinetic_diag(st.basis, geo.lattice, k)
         pr = cache.proj[ik]
-        vnl = np.einsum("pg,pq,qh->gh", pr, cache.dion, np.conj(pr))
+        vnl = pr.T @ (cache.dion @ np.conj(pr))            # sum_pq P_pg D_pq conj(P_qh)
         h = np.diag(t) + vm + vnl
         hs.append(0.5 * (h + np.conj(h.T)))
     return np.stack(hs, axis=0)

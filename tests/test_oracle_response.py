@@ -90,3 +90,29 @@ def test_response_two_k_points_one_fermi_level():
     assert np.abs(apply(d) - fd).max() < 1e-6 * np.abs(fd).max()
     ref = RS.make_response(geo, st, state, occ_sign="reference")(d)
     assert np.abs(ref - fd).max() > 0.01 * np.abs(fd).max()
+
+
+def test_occupation_part_of_the_map_term():
+    """The oracle's reference-form map term swaps the occupation part of the
+    true map term; its pieces (eigenvalue differences of the map at fixed n*,
+    the one-mesh dmu, the k weights) are pinned here: with the orbitals
+    frozen and only the occupations recomputed from the map's eigenvalues at
+    the displaced geometry, central differences of lam . n equal
+    Sum_kn df_kn (lam . w_k g_s |psi_kn|^2) with df of the exact form."""
+    from oracle import physics as P
+    geo, st, state = _two_k_system()
+    lam = np.random.default_rng(4).normal(size=st.basis.nfft)
+    dens = RS._occupation_densities(geo, st, state)
+    lam_dens = np.einsum("knr,r->kn", dens, lam) * np.asarray(st.kweights)[:, None] * st.deg
+    h = 1e-5
+    for what, idx in (("R", (1, 0)), ("eps", (0, 1))):
+        gp, gm = RS._geo_perturbed(geo, what, idx, h), RS._geo_perturbed(geo, what, idx, -h)
+        ep, em = RS._map_eigenvalues(gp, st, state.rho_r), RS._map_eigenvalues(gm, st, state.rho_r)
+        fp = P.occupations(ep, st.kweights, st.nelec, st.sigma, st.deg)[0]
+        fm = P.occupations(em, st.kweights, st.nelec, st.sigma, st.deg)[0]
+        fd = float(np.sum((fp - fm) / (2 * h) * lam_dens))
+        got = float(np.sum(RS._occupation_change(st, state.occ, (ep - em) / (2 * h), "exact") * lam_dens))
+        assert abs(got - fd) < 1e-6 * max(1.0, abs(fd)), (what, got, fd)
+        # the per-k form differs where the two points' occupations move apart
+        ref = float(np.sum(RS._occupation_change(st, state.occ, (ep - em) / (2 * h), "reference") * lam_dens))
+        assert abs(ref - fd) > 1e-4 * abs(fd)
