@@ -33,7 +33,7 @@ from .kernels import _lib
 from .kernels import native
 from .kernels._lib import check, lib, ptr, stream
 from .solver.linalg import inner
-from .kernels.nonlocal_bf16 import GammaProjector64, GammaProjectorBF16x3, ProjectorBF16x3, gamma_pairs
+from .kernels.nonlocal_bf16 import GammaProjector64, gamma_pairs
 
 
 def _simpson_weights(n):
@@ -224,11 +224,12 @@ class Engine:
             self.PG64.append(GammaProjector64(P, *self.gamma_pairs, self.odd_rows_d)
                              if self.gamma_pairs is not None else None)
             if self.nl_bf16:
-                nsl = max(1, min(32, (6 * ng) // 15000))
-                if self.gamma_pairs is not None:
-                    self.PB.append(GammaProjectorBF16x3(P, *self.gamma_pairs, self.odd_rows_d, nslice=nsl))
-                else:
-                    self.PB.append(ProjectorBF16x3(P, nslice=nsl))
+                # hand-written BF16x3 MFMA kernels (addft_nonlocal_becp / _apply_add) or
+                # split operands through hipBLASLt, whichever the tuner timed faster
+                from . import tuning
+                from .solver.scf import buffer_bands
+                nbt = self.st.nband + buffer_bands(self.st.nband)
+                self.PB.append(tuning.tune_projector(P, self.gamma_pairs, self.odd_rows_d, nbt))
             else:
                 self.PB.append(None)
         # k-stacked tables for the batched Hpsi (several k-points on one sphere)

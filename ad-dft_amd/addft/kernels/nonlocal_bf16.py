@@ -27,6 +27,8 @@ whose FP32 partial blocks are summed.
 
 from __future__ import annotations
 
+import weakref
+
 import numpy as np
 import torch
 
@@ -273,4 +275,71 @@ class GammaProjector64:
         _lib.check(_lib.lib().addft_gamma_scatter_add(_lib.C128, _lib.ptr(U), nb, self.nh, _lib.ptr(self.plus),
                                                       _lib.ptr(self.minus), self.ng, _lib.ptr(out),
                                                       _lib.stream()), "addft_gamma_scatter_add")
+        return out
+
+
+# ------------------------------------------------ hand-written MFMA kernels
+class ProjectorMFMA:
+    """The FP32-phase projector products through the library's hand-written
+    BF16x3 MFMA kernels (addft_nonlocal_becp / addft_nonlocal_apply_add):
+    the operand split, the k = 0 pair folding and (for apply_add) the
+    unfolding onto +-G all happen inside the two kernels, which read the
+    split projector images built once here.  Same becp_t / apply_add
+    interface as ProjectorBF16x3 (pairs None) and GammaProjectorBF16x3
+    (pairs = (plus, minus) and the odd-l rows: P' = Phi P)."""
+
+    def __init__(self, P: torch.Tensor, pairs=None, odd_rows: torch.Tensor | None = None):
+        npj, ng = P.shape
+        dev = P.device
+        self.np, self.ng = int(npj), int(ng)
+        self.gamma = pairs is not None
+        P = P.to(torch.complex128).contiguous()
+        if self.gamma:
+            plus = torch.as_tensor(pairs[0], dtype=torch.int32, device=dev).contiguous()
+            minus = torch.as_tensor(pairs[1], dtype=torch.int32, device=dev).contiguous()
+            odd = torch.as_tensor(odd_rows, device=dev).to(torch.int32).contiguous()
+            if odd.numel() != npj:
+                raise ValueError("odd_rows must flag every projector row")
+            if int(plus.max()) >= ng or int(minus.max()) >= ng or 2 * plus.numel() - 1 != ng:
+                raise ValueError("pair tables do not cover the sphere")
+            nh = int(plus.numel())
+        else:
+            plus = minus = odd = None
+            nh = 0
+        L = _lib.lib()
+        h = _lib.ctypes.c_void_p()
+        _lib.check(L.addft_projector_create(int(self.gamma), self.np, self.ng, _lib.ptr(P), _lib.ptr(odd),
+                                            _lib.ptr(plus), _lib.ptr(minus), nh, _lib.stream(),
+                                            _lib.ctypes.byref(h)), "addft_projector_create")
+        self._h = h
+        self._fin = weakref.finalize(self, L.addft_projector_destroy, h)
+        self._work = None
+
+    def _workspace(self, nb):
+        need = int(_lib.lib().addft_nonlocal_work_bytes(self._h, nb))
+        if self._work is None or self._work.numel() < need:
+            self._work = torch.empty(max(need, 1), dtype=torch.uint8, device="cuda")
+        return self._work
+
+    def becp_t(self, psi: torch.Tensor) -> torch.Tensor:
+        """psi (nb, nG) complex64 -> psi P^H (nb, nproj) complex64 (P' at k = 0)."""
+        if psi.dtype != torch.complex64 or psi.dim() != 2 or psi.shape[1] != self.ng:
+            raise ValueError("psi must be (nb, nG) complex64")
+        psi = psi.contiguous()
+        nb = psi.shape[0]
+        out = torch.empty((nb, self.np), dtype=torch.complex64, device=psi.device)
+        _lib.check(_lib.lib().addft_nonlocal_becp(self._h, _lib.ptr(psi), nb, self.ng, _lib.ptr(out), self.np,
+                                                  _lib.ptr(self._workspace(nb)), _lib.stream()),
+                   "addft_nonlocal_becp")
+        return out
+
+    def apply_add(self, S: torch.Tensor, out: torch.Tensor) -> torch.Tensor:
+        """out (nb, nG) complex64 += S P (S' P' at k = 0)."""
+        if S.dtype != torch.complex64 or out.dtype != torch.complex64:
+            raise ValueError("S and out must be complex64")
+        if not out.is_contiguous() or out.shape != (S.shape[0], self.ng) or S.shape[1] != self.np:
+            raise ValueError("out must be a contiguous (nb, nG) block and S (nb, nproj)")
+        S = S.contiguous()
+        _lib.check(_lib.lib().addft_nonlocal_apply_add(self._h, _lib.ptr(S), S.shape[0], self.np, _lib.ptr(out),
+                                                       self.ng, _lib.stream()), "addft_nonlocal_apply_add")
         return out

@@ -1,7 +1,9 @@
 """Autotuning of the launch choices that depend on the box, the sphere and
 the band count: the plane-pass kernel of a sphere plan (lane-group register
-FFTs or workgroup Stockham, `addft_sphere_plan_set_kernel`) and the number
-of reduction slices of the G-sphere inner products (solver/linalg.py).
+FFTs or workgroup Stockham, `addft_sphere_plan_set_kernel`), the number
+of reduction slices of the G-sphere inner products (solver/linalg.py), and
+the implementation of the FP32-phase projector products (the library's
+hand-written BF16x3 MFMA kernels or split operands through hipBLASLt).
 
 Each decision is made once per key by timing the candidates on the device
 (HIP events on the current stream, after one warm-up call) and is kept for
@@ -121,3 +123,41 @@ def pretune_inner(K: int, m: int, widths, dtypes, device):
         A = torch.zeros((m, K), dtype=dt, device=device)
         for n in widths:
             inner(A, torch.zeros((n, K), dtype=dt, device=device))
+
+
+# ------------------------------------------------------- projector products
+def tune_projector(P, pairs, odd_rows, nb: int, seed: int = 0):
+    """The FP32-phase projector products of one k-point's P for blocks of nb
+    bands: builds the hand-written MFMA form (kernels.nonlocal_bf16.
+    ProjectorMFMA) and the hipBLASLt split-GEMM form, times becp_t +
+    apply_add of each and returns the faster object.  ADDFT_NL_MFMA=1 / 0
+    forces one of them."""
+    from .kernels.nonlocal_bf16 import GammaProjectorBF16x3, ProjectorBF16x3, ProjectorMFMA
+    npj, ng = P.shape
+    force = os.environ.get("ADDFT_NL_MFMA")
+
+    def mfma():
+        return ProjectorMFMA(P, pairs, odd_rows) if pairs is not None else ProjectorMFMA(P)
+
+    def library():
+        nsl = max(1, min(32, (6 * ng) // 15000))
+        if pairs is not None:
+            return GammaProjectorBF16x3(P, *pairs, odd_rows, nslice=nsl)
+        return ProjectorBF16x3(P, nslice=nsl)
+
+    if force in ("0", "1"):
+        return mfma() if force == "1" else library()
+    if not enabled():
+        return mfma()
+    key = f"projector:{'gamma' if pairs is not None else 'full'}:{ng}:{npj}:{nb}"
+    _load()
+    if key in _DECISIONS:
+        return mfma() if _DECISIONS[key]["choice"] == "mfma" else library()
+    g = torch.Generator(device="cpu").manual_seed(seed)
+    psi = torch.complex(torch.randn(nb, ng, generator=g), torch.randn(nb, ng, generator=g)).to(P.device)
+    S = torch.complex(torch.randn(nb, npj, generator=g), torch.randn(nb, npj, generator=g)).to(P.device)
+    out = torch.zeros_like(psi)
+    objs = {"mfma": mfma(), "library": library()}
+    choice = decide(key, {name: (lambda o=o: o.apply_add(S, out) if o.becp_t(psi) is not None else None)
+                          for name, o in objs.items()})
+    return objs[choice]

@@ -149,10 +149,19 @@ def nonlocal_measure(addft, kw, steps, torch):
     out = torch.zeros_like(psi)
     flops = 2.0 * m * (6 * eng.ng) * (2 * eng.nproj)
     res = {}
-    for name, fn in (("becp", lambda: pb.becp_t(psi)), ("apply_add", lambda: pb.apply_add(S, out))):
-        ms = timed(fn, steps, 3, torch)
-        res[name] = {"ms": round(ms, 4), "TF_per_s": round(flops / (ms * 1e-3) / 1e12, 1),
-                     "frac_of_bf16_dense_peak": round(flops / (ms * 1e-3) / 2.5e15, 4)}
+    from addft.kernels.nonlocal_bf16 import GammaProjectorBF16x3, ProjectorBF16x3, ProjectorMFMA
+    half = eng.gamma_pairs is not None
+    # executed flops: 3 bf16 products of the real form, over the half sphere at k = 0
+    exe = flops / 2 if half else flops
+    lib_pb = (GammaProjectorBF16x3(eng.P64[0], *eng.gamma_pairs, eng.odd_rows_d, nslice=32) if half
+              else ProjectorBF16x3(eng.P64[0]))
+    mf = ProjectorMFMA(eng.P64[0], eng.gamma_pairs, eng.odd_rows_d) if half else ProjectorMFMA(eng.P64[0])
+    for impl, obj in (("mfma", mf), ("library", lib_pb)):
+        for name, fn in (("becp", lambda o=obj: o.becp_t(psi)), ("apply_add", lambda o=obj: o.apply_add(S, out))):
+            ms = timed(fn, steps, 3, torch)
+            res[f"{name}_{impl}"] = {"ms": round(ms, 4), "TF_per_s": round(flops / (ms * 1e-3) / 1e12, 1),
+                                     "executed_TF_per_s": round(exe / (ms * 1e-3) / 1e12, 1),
+                                     "frac_of_bf16_dense_peak_executed": round(exe / (ms * 1e-3) / 2.5e15, 4)}
     # FP64 products (k = 0: half sphere, GammaProjector64), beside the full-sphere complex128 GEMMs
     from addft.solver.linalg import inner
     psi64, S64 = psi.to(torch.complex128), S.to(torch.complex128)

@@ -48,6 +48,7 @@ enum {
 enum { ADDFT_C64 = 0, ADDFT_C128 = 1 };
 
 typedef struct addft_sphere_plan_s* addft_sphere_plan;
+typedef struct addft_projector_s* addft_projector;
 
 const char* addft_last_error(void);
 int addft_version(void);
@@ -102,6 +103,37 @@ int addft_gamma_scatter_add(int dtype, const void* d_u, int64_t nb, int64_t nh, 
 int addft_gamma_pack2(int dtype, const void* d_psi, int64_t m, int64_t ng, void* d_phi, void* stream);
 int addft_gamma_unpack2(int dtype, const void* d_h, int64_t m, int64_t ng, const int32_t* d_plus,
                         const int32_t* d_minus, int64_t nh, void* d_out, void* stream);
+
+/* ------------------------------------------ nonlocal projector products
+ * The two products of the nonlocal operator (reference
+ * physics/nonlocal_op.py:67-76, becp = conj(P) psi and vnl_apply's P s) in
+ * FP32 arithmetic as hand-written BF16x3 MFMA kernels (bf16 hi/lo split of
+ * every FP32 operand on the fly, FP32 accumulation, relative error ~1e-5).
+ * A projector object holds the split images of one k-point's projector
+ * matrix P (nproj x ng complex128, stick order), built once per geometry.
+ * gamma != 0 (k = 0): the products run over the half sphere with the rows
+ * of odd l multiplied by i (d_odd[p] != 0), pair tables as for
+ * addft_gamma_pair_split; becp then returns psi P'^H and apply_add takes
+ * S' = becp' D (P'^H D P' = P^H D P).  Creation synchronises the stream.
+ * addft_nonlocal_becp: d_psi (nb, ldpsi) complex64 -> d_becp (nb, ldb)
+ *   complex64 = psi P^H (becp^T of nonlocal_op.py:67-69); d_work:
+ *   addft_nonlocal_work_bytes(p, nb) bytes (split-K partial sums; the sum
+ *   over them is in a fixed order: deterministic).
+ * addft_nonlocal_apply_add: d_out (nb, ldo) complex64 += S P with d_s
+ *   (nb, lds) complex64 (nonlocal_op.py:70-76 with s = D becp done by the
+ *   caller).  d_out may not alias d_s.
+ * errors: null arguments, leading dimensions below nproj / ng, gamma tables
+ *   that do not cover the sphere.                                          */
+int addft_projector_create(int gamma, int64_t nproj, int64_t ng, const void* d_P, const int32_t* d_odd,
+                           const int32_t* d_plus, const int32_t* d_minus, int64_t nh, void* stream,
+                           addft_projector* out);
+int addft_projector_destroy(addft_projector p);
+int addft_projector_info(addft_projector p, int64_t* nproj, int64_t* ng, int* gamma);
+int64_t addft_nonlocal_work_bytes(addft_projector p, int64_t nb);
+int addft_nonlocal_becp(addft_projector p, const void* d_psi, int64_t nb, int64_t ldpsi, void* d_becp,
+                        int64_t ldb, void* d_work, void* stream);
+int addft_nonlocal_apply_add(addft_projector p, const void* d_s, int64_t nb, int64_t lds, void* d_out,
+                             int64_t ldo, void* stream);
 
 /* ------------------------------------------------- Chebyshev filter step
  * out = alpha a + beta b + gamma c over nblk blocks of n complex elements

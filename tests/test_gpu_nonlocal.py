@@ -85,8 +85,8 @@ def test_gamma_half_sphere_products(cfg):
     from addft.kernels.nonlocal_bf16 import GammaProjectorBF16x3
     s = addft.setup(**testsystems.config(cfg, precision="mixed"))
     eng = addft.api._engine(s.static, s.geo)
-    assert isinstance(eng.PB[0], GammaProjectorBF16x3)
-    pb, P = eng.PB[0], eng.P64[0]
+    P = eng.P64[0]
+    pb = GammaProjectorBF16x3(P, *eng.gamma_pairs, eng.odd_rows_d, nslice=max(1, min(32, 6 * eng.ng // 15000)))
     ph = torch.where(eng.odd_rows_d, torch.tensor(1j, dtype=P.dtype, device=P.device),
                      torch.tensor(1.0, dtype=P.dtype, device=P.device))
     Pp = P * ph[:, None]
