@@ -91,6 +91,8 @@ _SIGS = {
     "addft_nonlocal_work_bytes": ([_VP, _I64], _I64),
     "addft_nonlocal_becp": ([_VP, _VP, _I64, _I64, _VP, _I64, _VP, _VP], _I32),
     "addft_nonlocal_apply_add": ([_VP, _VP, _I64, _I64, _VP, _I64, _VP], _I32),
+    "addft_nonlocal_becp_noalloc": ([_VP, _VP, _I64, _I64, _VP, _I64, _VP], _I32),
+    "addft_nonlocal_noalloc_scratch_bytes": ([], _I64),
 }
 
 

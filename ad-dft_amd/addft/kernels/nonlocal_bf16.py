@@ -333,6 +333,19 @@ class ProjectorMFMA:
                    "addft_nonlocal_becp")
         return out
 
+    def becp_t_noalloc(self, psi: torch.Tensor, out: torch.Tensor) -> torch.Tensor:
+        """becp_t into a caller-provided (nb, nproj) complex64 `out`, with no
+        allocation at all (addft_nonlocal_becp_noalloc: the library's own
+        per-GPU scratch array holds the partial sums)."""
+        if psi.dtype != torch.complex64 or psi.dim() != 2 or psi.shape[1] != self.ng or not psi.is_contiguous():
+            raise ValueError("psi must be a contiguous (nb, nG) complex64 block")
+        if out.dtype != torch.complex64 or out.shape != (psi.shape[0], self.np) or not out.is_contiguous():
+            raise ValueError("out must be a contiguous (nb, nproj) complex64 block")
+        _lib.check(_lib.lib().addft_nonlocal_becp_noalloc(self._h, _lib.ptr(psi), psi.shape[0], self.ng,
+                                                          _lib.ptr(out), self.np, _lib.stream()),
+                   "addft_nonlocal_becp_noalloc")
+        return out
+
     def apply_add(self, S: torch.Tensor, out: torch.Tensor) -> torch.Tensor:
         """out (nb, nG) complex64 += S P (S' P' at k = 0)."""
         if S.dtype != torch.complex64 or out.dtype != torch.complex64:

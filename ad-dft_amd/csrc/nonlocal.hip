@@ -41,6 +41,7 @@
 #include "common.h"
 
 #include <algorithm>
+#include <mutex>
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
@@ -596,6 +597,38 @@ void becp_grid(const addft_projector_s* p, int64_t nb, int* ntm, int* ntn, int* 
   *nsplit = (int)ns;
 }
 
+// ---------------------------------------------- allocation-free becp scratch
+// The split-K partial sums of addft_nonlocal_becp_noalloc live in this
+// file-scope device array (one copy per GPU, held until the process ends).
+// Host code never uses the symbol itself as an address: the device address
+// of each GPU's copy comes from hipGetSymbolAddress, taken once per device
+// with that device current.
+constexpr size_t kNoallocScratchBytes = (size_t)64 << 20;  // 64 MiB
+
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNoallocScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+int scratch_for_current_device(void** out) {
+  int dev = 0;
+  ADDFT_HIP(hipGetDevice(&dev));
+  ADDFT_CHECK_ARG(dev >= 0 && dev < kMaxDevices, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void* ptr = nullptr;
+    ADDFT_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return ADDFT_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -708,6 +741,30 @@ int addft_nonlocal_becp(addft_projector p, const void* d_psi, int64_t nb, int64_
   int ntm, ntn, ns;
   becp_grid(p, nb, &ntm, &ntn, &ns);
   return launch_becp(p, d_psi, nb, ldpsi, d_becp, ldb, static_cast<float2*>(d_work), ntm, ntn, ns,
+                     addft::as_stream(stream));
+}
+
+int64_t addft_nonlocal_noalloc_scratch_bytes(void) { return (int64_t)kNoallocScratchBytes; }
+
+int addft_nonlocal_becp_noalloc(addft_projector p, const void* d_psi, int64_t nb, int64_t ldpsi, void* d_becp,
+                                int64_t ldb, void* stream) {
+  ADDFT_CHECK_ARG(p != nullptr && d_psi != nullptr && d_becp != nullptr, "null argument");
+  ADDFT_CHECK_ARG(nb > 0 && nb <= (int64_t)1 << 20, "nb out of range");
+  ADDFT_CHECK_ARG(ldpsi >= p->ng && ldb >= p->np, "leading dimensions too small");
+  const int64_t slice = nb * p->np * (int64_t)sizeof(float2);  // one partial-sum slice
+  if (slice > (int64_t)kNoallocScratchBytes) {
+    addft::set_error("addft_nonlocal_becp_noalloc: nb * nproj * 8 bytes exceeds the 64 MiB scratch array (" +
+                     std::to_string(slice) + " bytes); use addft_nonlocal_becp with a workspace");
+    return ADDFT_EINVAL;
+  }
+  void* scr = nullptr;
+  const int rc = scratch_for_current_device(&scr);
+  if (rc != ADDFT_OK) return rc;
+  int ntm, ntn, ns;
+  becp_grid(p, nb, &ntm, &ntn, &ns, (int64_t)kNoallocScratchBytes / slice);
+  // every slice writes its whole (nb, nproj) block before the reduction reads
+  // it, so the array needs no clearing between calls
+  return launch_becp(p, d_psi, nb, ldpsi, d_becp, ldb, static_cast<float2*>(scr), ntm, ntn, ns,
                      addft::as_stream(stream));
 }
 

@@ -134,6 +134,20 @@ int addft_nonlocal_becp(addft_projector p, const void* d_psi, int64_t nb, int64_
                         int64_t ldb, void* d_work, void* stream);
 int addft_nonlocal_apply_add(addft_projector p, const void* d_s, int64_t nb, int64_t lds, void* d_out,
                              int64_t ldo, void* stream);
+/* addft_nonlocal_becp_noalloc: the same product as addft_nonlocal_becp with
+ * no workspace argument and no allocation of any kind: the split-K partial
+ * sums go to a 64 MiB array in device memory that the library holds, one per
+ * GPU, for the life of the process (its device address is looked up once
+ * per device on the first call).  No counterpart in the reference: it is
+ * for callers that cannot allocate (HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc).  Rejects nb * nproj * 8 > 64 MiB with
+ * ADDFT_EINVAL (fewer partial slices are used when the regular count does
+ * not fit).  Calls that overlap on one GPU (several streams) race on the
+ * shared array and must be serialised by the caller.
+ * addft_nonlocal_noalloc_scratch_bytes: the array's size.                */
+int addft_nonlocal_becp_noalloc(addft_projector p, const void* d_psi, int64_t nb, int64_t ldpsi, void* d_becp,
+                                int64_t ldb, void* stream);
+int64_t addft_nonlocal_noalloc_scratch_bytes(void);
 
 /* ------------------------------------------------- Chebyshev filter step
  * out = alpha a + beta b + gamma c over nblk blocks of n complex elements
