@@ -18,7 +18,8 @@ from pathlib import Path
 _LIB = None
 _PATH = None
 
-C64, C128 = 0, 1
+C64, C128, R64 = 0, 1, 2
+EIGH_DC, EIGH_JACOBI, EIGH_DCJ = 0, 1, 2
 PLANE_LDS, PLANE_LANE = 0, 1
 
 
@@ -91,6 +92,11 @@ _SIGS = {
     "addft_nonlocal_work_bytes": ([_VP, _I64], _I64),
     "addft_nonlocal_becp": ([_VP, _VP, _I64, _I64, _VP, _I64, _VP, _VP], _I32),
     "addft_nonlocal_apply_add": ([_VP, _VP, _I64, _I64, _VP, _I64, _VP], _I32),
+    "addft_subspace_gram_work_bytes": ([_I32, _I64, _I64, _I32], _I64),
+    "addft_subspace_gram": ([_I32, _I64, _I64, _I64, _VP, _I64, _VP, _I64, _VP, _I64, _I32, _VP, _VP], _I32),
+    "addft_subspace_rotate": ([_I32, _I64, _I64, _I64, _VP, _I64, _VP, _I64, _VP, _I64, _VP], _I32),
+    "addft_subspace_eigh_work_bytes": ([_I64, _I64], _I64),
+    "addft_subspace_eigh": ([_I32, _I32, _I64, _VP, _I64, _I64, _VP, _VP, _VP, _VP], _I32),
     "addft_nonlocal_becp_noalloc": ([_VP, _VP, _I64, _I64, _VP, _I64, _VP], _I32),
     "addft_nonlocal_noalloc_scratch_bytes": ([], _I64),
 }
@@ -149,4 +155,10 @@ def stream() -> int:
 
 
 def ptr(t) -> int:
-    return 0 if t is None else int(t.data_ptr())
+    """Device address of a tensor's data.  A lazily conjugated complex view
+    (torch's conj bit) is refused: the kernels would read it unconjugated."""
+    if t is None:
+        return 0
+    if t.is_complex() and t.is_conj():
+        raise ValueError("a lazily conjugated tensor reached a native call; pass conj_physical() / resolve_conj()")
+    return int(t.data_ptr())

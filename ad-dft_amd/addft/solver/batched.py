@@ -4,9 +4,8 @@ With several k-points on one G-sphere (the reference's union shell,
 geometry/basis.py:89-121) every k-point's block has the same shape, so the
 blocks are stacked as (nk, nbt, nG) and each step of the solver is one
 launch for all of them: the Hpsi (Engine.hpsi_batch), the Gram and projected
-matrices (batched GEMM) and the Cholesky factorisations (batched device
-solver); the small eigendecompositions of the whole batch go to host LAPACK
-on a thread pool.  The algorithms are those of solver/davidson.py (block Davidson,
+matrices (batched GEMM), the Cholesky factorisations and the small
+eigendecompositions (batched device solvers).  The algorithms are those of solver/davidson.py (block Davidson,
 reference solver/davidson.py:32-77) and solver/chebyshev.py (Chebyshev-
 filtered subspace iteration) without band locking -- fixed block sizes keep
 the batch rectangular -- and give the same converged eigenpairs.
@@ -14,47 +13,26 @@ the batch rectangular -- and give the same converged eigenpairs.
 
 from __future__ import annotations
 
-import os
-from concurrent.futures import ThreadPoolExecutor
-
-import numpy as np
 import torch
-from threadpoolctl import threadpool_limits
 
 from .graphs import filter_recurrence
 
 
-# Batches of small Hermitian eigenproblems (nk matrices of a few hundred rows)
-# are solved with host LAPACK: the device eigensolver runs them one matrix at
-# a time in many tiny kernels (measured: ~13 s of 19 s of kernel time at 64
-# k-points), while a host thread pool solves the whole batch in milliseconds.
-HOST_MAX = 256
-_POOL = None
-
-
-def _pool():
-    global _POOL
-    if _POOL is None:
-        _POOL = ThreadPoolExecutor(max_workers=max(1, min(16, len(os.sched_getaffinity(0)))))
-    return _POOL
-
-
 def eigh(T):
-    """Batched Hermitian eigendecomposition.  Small matrices: one LAPACK call
-    per matrix on a host thread pool (single-threaded BLAS inside each), which
-    runs the batch in parallel; measured 3-5x faster than a sequential host
-    batch and far faster than the device solver at these sizes."""
-    if T.shape[-1] > HOST_MAX:
-        return torch.linalg.eigh(T)
-    A = T.cpu().numpy()
-    if A.shape[0] == 1 or A.shape[-1] <= 32:
-        w, U = np.linalg.eigh(A)
-    else:
-        with threadpool_limits(1):
-            parts = list(_pool().map(np.linalg.eigh, A))
-        w = np.stack([p[0] for p in parts])
-        U = np.stack([p[1] for p in parts])
-    return torch.from_numpy(w).to(T.device), torch.from_numpy(U).to(T.device)
+    """Batched Hermitian eigendecomposition of (nk, n, n) on the device:
+    rocSOLVER's batched divide-and-conquer or Jacobi solver through
+    addft_subspace_eigh (the method timed fastest for the shape), one call for
+    all k-points and no host round trip (a non-converged matrix is retried
+    alone with the single-matrix path)."""
+    from .davidson import eigh as eigh_one
+    from .linalg import eigh_device, eigh_method
+    w, U, info = eigh_device(T, eigh_method(T))
+    bad = torch.nonzero(info != 0).flatten()
+    if bad.numel():
+        w, U = w.clone(), U.clone()
+        for i in bad.tolist():
+            w[i], U[i] = eigh_one(T[i])
+    return w, U
 
 
 def chol_inverse(S):

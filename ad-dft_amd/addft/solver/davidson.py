@@ -34,8 +34,9 @@ def _finite(name, t):
 
 
 def eigh(T):
-    """FP64 Hermitian eigensolve.  hipSOLVER's solver can report
-    non-convergence on (nearly) degenerate input; retry once with a 1e-14
+    """FP64 Hermitian (or real symmetric) eigensolve through the library's
+    rocSOLVER entry.  On reported non-convergence ((nearly) degenerate
+    input) it retries with hipSOLVER through torch, then with a 1e-14
     relative diagonal jitter (which leaves the eigenpairs unchanged at the
     accuracy used here), then with the MAGMA solver."""
     def ok(wv):
@@ -43,6 +44,11 @@ def eigh(T):
 
     if not bool(torch.isfinite(T).all()):
         raise FloatingPointError("non-finite projected matrix")
+    # rocSOLVER through the library (addft_subspace_eigh), method tuned per size
+    from .linalg import eigh_device, eigh_method
+    w, U, info = eigh_device(T, eigh_method(T))
+    if not bool((info != 0).any()) and ok((w, U)):
+        return w, U
     try:
         wv = torch.linalg.eigh(T)
         if ok(wv):

@@ -87,3 +87,52 @@ def inner(A, B, conj_a=False):
     if conj_a:
         A, B = A.conj(), B.conj()
     return inner_sliced(A, B, _choose_slices(A, B))
+
+
+# ---------------------------------------------------- small eigensolves
+_EIGH_METHODS = ("dc", "jacobi", "dcj")
+
+
+def eigh_device(T, method="dc"):
+    """(w ascending, U with eigenvectors as columns, info) of a Hermitian
+    complex128 or real symmetric float64 matrix, or of a batch (b, n, n) of
+    them, through addft_subspace_eigh (rocSOLVER divide and conquer, Jacobi
+    or divide and conquer with Jacobi, batched over the leading dimension).
+    T is not modified.  No host synchronisation: info (b,) int32 stays on the
+    device."""
+    from ..kernels import _lib
+    batched = T.dim() == 3
+    A = (T if batched else T.unsqueeze(0)).contiguous().clone()
+    b, n = A.shape[0], A.shape[-1]
+    real = not A.is_complex()
+    if real:
+        A = A.to(torch.float64)
+    elif A.dtype != torch.complex128:
+        A = A.to(torch.complex128)
+    w = torch.empty((b, n), dtype=torch.float64, device=A.device)
+    info = torch.empty(b, dtype=torch.int32, device=A.device)
+    L = _lib.lib()
+    work = torch.empty(int(L.addft_subspace_eigh_work_bytes(n, b)), dtype=torch.uint8, device=A.device)
+    code = {"dc": _lib.EIGH_DC, "jacobi": _lib.EIGH_JACOBI, "dcj": _lib.EIGH_DCJ}[method]
+    _lib.check(L.addft_subspace_eigh(_lib.R64 if real else _lib.C128, code, n, _lib.ptr(A), n * n, b,
+                                     _lib.ptr(w), _lib.ptr(info), _lib.ptr(work), _lib.stream()),
+               "addft_subspace_eigh")
+    # conj_physical: a materialised conjugate (a lazy conj view would reach
+    # raw-pointer consumers unconjugated)
+    U = A.transpose(1, 2) if real else torch.conj_physical(A).transpose(1, 2)
+    if not batched:
+        return w[0], U[0], info
+    return w, U, info
+
+
+def eigh_method(T):
+    """The fastest addft_subspace_eigh method for matrices of T's size, kind
+    and batch count, timed once on T itself (addft.tuning); "dc" when tuning
+    is off or the matrix is small."""
+    from .. import tuning
+    n = T.shape[-1]
+    b = T.shape[0] if T.dim() == 3 else 1
+    if not tuning.enabled() or n < 64 or torch.cuda.is_current_stream_capturing():
+        return "dc"
+    key = f"eigh:{'real' if not T.is_complex() else 'complex'}:{n}:{b}"
+    return tuning.decide(key, {m: (lambda m=m: eigh_device(T, m)) for m in _EIGH_METHODS})

@@ -45,7 +45,8 @@ enum {
   ADDFT_EUNSUPPORTED = -3
 };
 
-enum { ADDFT_C64 = 0, ADDFT_C128 = 1 };
+enum { ADDFT_C64 = 0, ADDFT_C128 = 1, ADDFT_R64 = 2 };
+enum { ADDFT_EIGH_DC = 0, ADDFT_EIGH_JACOBI = 1, ADDFT_EIGH_DCJ = 2 };
 
 typedef struct addft_sphere_plan_s* addft_sphere_plan;
 typedef struct addft_projector_s* addft_projector;
@@ -148,6 +149,32 @@ int addft_nonlocal_apply_add(addft_projector p, const void* d_s, int64_t nb, int
 int addft_nonlocal_becp_noalloc(addft_projector p, const void* d_psi, int64_t nb, int64_t ldpsi, void* d_becp,
                                 int64_t ldb, void* stream);
 int64_t addft_nonlocal_noalloc_scratch_bytes(void);
+
+/* -------------------------------------------- Rayleigh-Ritz building blocks
+ * The subspace algebra of the block eigensolvers (reference
+ * solver/davidson.py:32-77, solver/eigh.py:30-41 and native/eigh.cu's
+ * zheevdx FFI target addft_eigh_lowest, eigh.cu:155-160), row-major
+ * complex blocks, stream ordered, rocBLAS / rocSOLVER underneath (one
+ * handle per thread and device, owned by the library).
+ * addft_subspace_gram: C (m x n, ldc) = A B^H for A (m x K, lda), B (n x K,
+ *   ldb), the reduction split into nslice slices (partial sums in d_work:
+ *   addft_subspace_gram_work_bytes bytes, summed in a fixed order).
+ * addft_subspace_rotate: Y (m x K, ldy) = Q X, Q (m x n, ldq), X (n x K, ldx).
+ * addft_subspace_eigh: eigenpairs of batch symmetric (ADDFT_R64) or
+ *   Hermitian (ADDFT_C128) row-major n x n matrices (stride stride_a
+ *   elements), in place: row i = eigenvector of w[i] (conjugated for
+ *   complex), w ascending (batch x n), info per matrix on the device;
+ *   method ADDFT_EIGH_DC / _JACOBI / _DCJ; d_work:
+ *   addft_subspace_eigh_work_bytes(n, batch) bytes.                       */
+int64_t addft_subspace_gram_work_bytes(int dtype, int64_t m, int64_t n, int nslice);
+int addft_subspace_gram(int dtype, int64_t m, int64_t n, int64_t K, const void* d_a, int64_t lda,
+                        const void* d_b, int64_t ldb, void* d_c, int64_t ldc, int nslice, void* d_work,
+                        void* stream);
+int addft_subspace_rotate(int dtype, int64_t m, int64_t n, int64_t K, const void* d_q, int64_t ldq,
+                          const void* d_x, int64_t ldx, void* d_y, int64_t ldy, void* stream);
+int64_t addft_subspace_eigh_work_bytes(int64_t n, int64_t batch);
+int addft_subspace_eigh(int dtype, int method, int64_t n, void* d_a, int64_t stride_a, int64_t batch, double* d_w,
+                        int32_t* d_info, void* d_work, void* stream);
 
 /* ------------------------------------------------- Chebyshev filter step
  * out = alpha a + beta b + gamma c over nblk blocks of n complex elements
