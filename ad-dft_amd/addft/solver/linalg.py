@@ -126,13 +126,15 @@ def eigh_device(T, method="dc"):
 
 
 def eigh_method(T):
-    """The fastest addft_subspace_eigh method for matrices of T's size, kind
-    and batch count, timed once on T itself (addft.tuning); "dc" when tuning
-    is off or the matrix is small."""
+    """The fastest addft_subspace_eigh method for matrices of T's kind, size
+    class (n rounded up to 128) and batch count, timed once on T itself
+    (addft.tuning); "dc" when tuning is off or for single matrices below
+    n = 256."""
     from .. import tuning
     n = T.shape[-1]
     b = T.shape[0] if T.dim() == 3 else 1
-    if not tuning.enabled() or n < 64 or torch.cuda.is_current_stream_capturing():
+    if not tuning.enabled() or (n < 256 and b < 8) or torch.cuda.is_current_stream_capturing():
         return "dc"
-    key = f"eigh:{'real' if not T.is_complex() else 'complex'}:{n}:{b}"
+    # one decision per size class (the Davidson subspace grows in small steps)
+    key = f"eigh:{'real' if not T.is_complex() else 'complex'}:{(n + 127) // 128 * 128}:{b}"
     return tuning.decide(key, {m: (lambda m=m: eigh_device(T, m)) for m in _EIGH_METHODS})
