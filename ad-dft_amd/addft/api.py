@@ -63,15 +63,18 @@ def _engine(static: Static, geo: Cell) -> Engine:
 
 def setup(lattice, frac, psp: Pseudo | None = None, ecut: float = 8.0, nband: int = 4,
           kmesh: KMesh | None = None, sigma: float = 0.05, density_factor: float = 4.0,
-          eta_ewald: float = 0.9, spec_idx=None, precision: str = "fp64", shells: str = "union") -> System:
+          eta_ewald: float = 0.9, spec_idx=None, precision: str = "fp64", shells: str = "union",
+          xc: str = "lda") -> System:
     """api.py:35-45.  shells="per_k": each k-point on its own sphere
     |k + G|^2 <= 2 ecut (unequal plane-wave counts, padded to the union
-    shell in storage); "union" (default): the reference's union shell."""
+    shell in storage); "union" (default): the reference's union shell.
+    xc="gga_model": the reference's model GGA (physics/xc.py:137-150) with
+    its potential from the spectral gradient and divergence."""
     geo = make_cell(lattice, frac)
     psp = synthetic() if psp is None else psp
     static = make_static(geo, psp, kmesh or gamma(), ecut, nband, spec_idx=spec_idx, sigma=sigma,
                          density_factor=density_factor, eta_ewald=eta_ewald, precision=precision,
-                         shells=shells)
+                         shells=shells, xc=xc)
     return System(geo=geo, static=static)
 
 

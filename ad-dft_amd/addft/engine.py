@@ -36,6 +36,10 @@ from .solver.linalg import inner
 from .kernels.nonlocal_bf16 import GammaProjector64, gamma_pairs
 
 
+# enhancement coefficient of the model GGA (xc.py:145)
+GGA_MU = 0.1234
+
+
 def _simpson_weights(n):
     """psp.py:66-70."""
     w = np.ones((n,))
@@ -287,9 +291,39 @@ class Engine:
         v_h = self.plan64.to_real(vh_g, 1.0)
         return v_h, out
 
+    def grad_density(self, f_pl):
+        """(3, nfft) spectral gradient of a real plane-major grid on the
+        psi-sphere: sum_G i G_a f(G) e^{iGr}, f(G) = fftn(f)/N gathered
+        there (xc.py:78-85)."""
+        f_g = self.plan64.to_sphere(f_pl, 1.0 / self.nfft)
+        return torch.stack([self.plan64.to_real(f_g * (1j * self.g0_d[:, a]), 1.0) for a in range(3)])
+
+    def divergence(self, v3):
+        """Spectral divergence of a (3, nfft) plane-major field on the
+        psi-sphere, the negative adjoint of grad_density (xc.py:88-93)."""
+        tot = None
+        for a in range(3):
+            t = self.plan64.to_sphere(v3[a].contiguous(), 1.0 / self.nfft) * (1j * self.g0_d[:, a])
+            tot = t if tot is None else tot + t
+        return self.plan64.to_real(tot, 1.0)
+
+    def xc(self, rho_pl):
+        """(v_xc, raw sum of the energy density, GGA fields or None) for the
+        functional of the Static.  gga_model: v = f_n - div(2 f_sigma grad n),
+        the functional derivative (N/Omega) dE_xc/dn_j of E_xc = (Omega/N)
+        sum f(n, |grad n|^2) (xc.py:148-172)."""
+        if self.st.xc == "lda":
+            v, e = native.xc_lda(rho_pl)
+            return v, e, None
+        g = self.grad_density(rho_pl)
+        sigma = (g * g).sum(dim=0)
+        fn, fs, e = native.xc_gga_model(rho_pl, sigma, GGA_MU)
+        v = fn - self.divergence(2.0 * fs[None, :] * g)
+        return v, e, {"grad": g, "sigma": sigma, "fn": fn, "fs": fs}
+
     def effective_potential(self, rho_pl):
         v_h, hout = self.hartree(rho_pl)
-        vxc, exc = native.xc_lda(rho_pl)
+        vxc, exc, _ = self.xc(rho_pl)
         v = self.vloc_pl + v_h + vxc
         return v, {"v_h": v_h, "v_xc": vxc, "E_H": 0.5 * self.omega * hout[0], "exc_sum": exc}
 
@@ -476,7 +510,7 @@ class Engine:
             b = self.becp(ik, psi)
             e_nl += float((wf * (b.conj() * (self.D64 @ b)).real.sum(dim=0)).sum())
         v_h, hout = self.hartree(rho_pl)
-        vxc, exc = native.xc_lda(rho_pl)
+        _, exc, _ = self.xc(rho_pl)
         e_h = 0.5 * self.omega * float(hout[0])
         e_xc = dv * float(exc)
         e_loc_psi = dv * float((self.vloc_pl * rho_psi).sum())
@@ -513,7 +547,11 @@ class Engine:
         if "exchange" in active or "correlation" in active:
             n = torch.clamp(rho_pl, min=1e-14)
             ex = -0.75 * torch.pow(3.0 * n / math.pi, 1.0 / 3.0)
-            _, exc = native.xc_lda(rho_pl)
+            _, exc, gga = self.xc(rho_pl)
+            if gga is not None:
+                # both parts carry the enhancement factor 1 + mu s^2
+                kf = torch.pow(3.0 * math.pi ** 2 * n, 1.0 / 3.0)
+                ex = ex * (1.0 + GGA_MU * torch.clamp(gga["sigma"], min=0.0) / (4.0 * kf * kf * n * n))
             e_x = dv * float((n * ex).sum())
             if "exchange" in active:
                 out["exchange"] = e_x
@@ -550,11 +588,19 @@ class Engine:
         ho = hout.cpu().numpy()
         e_h = 0.5 * self.omega * ho[0]
         parts["hartree"] = (np.zeros((nat, 3)), -np.eye(3) * e_h + 0.5 * self.omega * ho[1:].reshape(3, 3))
-        # XC strain
-        vxc, exc = native.xc_lda(rho_psi)
+        # XC strain: (E_xc - (Omega/N) sum n v_xc) delta for LDA; the model
+        # GGA's strained gradients add -2 sigma f_sigma to the isotropic sum
+        # and -(Omega/N) sum 2 f_sigma d_a n d_b n
         dv = self.omega / self.nfft
+        vxc, exc, gga = self.xc(rho_psi)
         e_xc = dv * float(exc)
-        parts["xc"] = (np.zeros((nat, 3)), np.eye(3) * (e_xc - dv * float((vxc * rho_psi).sum())))
+        if gga is None:
+            parts["xc"] = (np.zeros((nat, 3)), np.eye(3) * (e_xc - dv * float((vxc * rho_psi).sum())))
+        else:
+            g, fs2 = gga["grad"], 2.0 * gga["fs"]
+            iso = e_xc - dv * float((rho_psi * gga["fn"] + gga["sigma"] * fs2).sum())
+            aniso = torch.matmul(g * fs2[None, :], g.T).cpu().numpy()
+            parts["xc"] = (np.zeros((nat, 3)), np.eye(3) * iso - dv * aniso)
         # nonlocal
         fn = np.zeros((nat, 3))
         en = np.zeros((3, 3))

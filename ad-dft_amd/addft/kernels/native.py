@@ -78,6 +78,21 @@ def xc_lda(rho: torch.Tensor, want_energy: bool = True):
     return vxc, exc
 
 
+def xc_gga_model(rho: torch.Tensor, sigma: torch.Tensor, mu: float):
+    """(df/dn, df/dsigma, raw sum f) of the model GGA (reference xc.py
+    :137-145) for float64 density and |grad n|^2 grids."""
+    _require(sigma, torch.float64, numel=rho.numel(), what="sigma")
+    _require(rho, torch.float64)
+    n = rho.numel()
+    fn = torch.empty_like(rho)
+    fs = torch.empty_like(rho)
+    work = torch.empty(int(lib().addft_xc_work_bytes(n)) // 8 + 1, dtype=torch.float64, device=rho.device)
+    exc = torch.empty((), dtype=torch.float64, device=rho.device)
+    check(lib().addft_xc_gga_model(ptr(rho), ptr(sigma), n, float(mu), ptr(fn), ptr(fs), ptr(exc), ptr(work),
+                                   stream()), "addft_xc_gga_model")
+    return fn, fs, exc
+
+
 def xc_fxc(rho: torch.Tensor, w: torch.Tensor | None = None) -> torch.Tensor:
     """f_xc (or f_xc * w) (reference xc.py:181-191, 205-208)."""
     if w is not None:

@@ -163,6 +163,9 @@ class Response:
         self.deg_tol = float(deg_tol)
         if occ_sign not in ("exact", "reference"):
             raise ValueError(f"unknown occ_sign {occ_sign!r}")
+        if st.xc != "lda":
+            raise NotImplementedError("the response kernel is the local LDA f_xc (response.py:50); "
+                                      f"xc={st.xc!r} has no response")
         self.sgn = 1.0 if occ_sign == "exact" else -1.0
         self.cg_tol = float(cg_tol)
         self.cg_maxiter = int(cg_maxiter)

@@ -18,6 +18,9 @@ from .geometry.kmesh import KMesh
 from .geometry.psp import Pseudo
 
 
+XC_FUNCTIONALS = ("lda", "gga_model")
+
+
 class Static(NamedTuple):
     basis: Basis
     kpts: np.ndarray          # (nk, 3) Cartesian, fixed at setup (system.py:55)
@@ -34,6 +37,9 @@ class Static(NamedTuple):
     # |k + G|^2 <= 2 ecut (1 + 1e-12) at the setup lattice; None: every k
     # uses the whole union shell (the reference rule, basis.py:111-114)
     kshell: object = None
+    # exchange-correlation functional: "lda" (PZ81, xc.py:51-73) or
+    # "gga_model" (n eps_LDA (1 + 0.1234 s^2), xc.py:137-150)
+    xc: str = "lda"
 
 
 class SCFState(NamedTuple):
@@ -48,7 +54,7 @@ class SCFState(NamedTuple):
 
 def make_static(cell: Cell, psp: Pseudo, kmesh: KMesh, ecut: float, nband: int, spec_idx=None,
                 sigma: float = 0.01, density_factor: float = 4.0, eta_ewald: float = 0.9,
-                precision: str = "fp64", shells: str = "union") -> Static:
+                precision: str = "fp64", shells: str = "union", xc: str = "lda") -> Static:
     """system.py:47-69.  shells="per_k" restricts each k-point to its own
     sphere |k + G|^2 <= 2 ecut inside the union shell (frozen with the basis:
     strained geometries keep the setup membership)."""
@@ -59,6 +65,8 @@ def make_static(cell: Cell, psp: Pseudo, kmesh: KMesh, ecut: float, nband: int, 
         raise NotImplementedError("only one species is supported (system.py:53-54)")
     if precision not in ("fp64", "mixed"):
         raise ValueError(f"unknown precision {precision!r}")
+    if xc not in XC_FUNCTIONALS:
+        raise ValueError(f"unknown xc {xc!r} (one of {', '.join(XC_FUNCTIONALS)})")
     kpts = kmesh.cartesian(cell.lattice)
     basis = make_basis(cell.lattice, ecut, density_factor, kpts=kmesh.shell_cartesian(cell.lattice))
     nelec = float(np.sum(psp.zval[spec_idx]))
@@ -69,7 +77,7 @@ def make_static(cell: Cell, psp: Pseudo, kmesh: KMesh, ecut: float, nband: int, 
         raise ValueError("a per-k shell holds fewer plane waves than nband")
     return Static(basis=basis, kpts=kpts, kweights=np.asarray(kmesh.weights, dtype=np.float64), psp=psp,
                   spec_idx=spec_idx, nband=int(nband), spin_degeneracy=2.0, sigma=float(sigma),
-                  nelec=nelec, eta_ewald=float(eta_ewald), precision=precision, kshell=kshell)
+                  nelec=nelec, eta_ewald=float(eta_ewald), precision=precision, kshell=kshell, xc=xc)
 
 
 def per_k_shells(lattice, miller, kpts, ecut) -> np.ndarray:

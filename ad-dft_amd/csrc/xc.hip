@@ -94,6 +94,37 @@ __global__ void k_xc_fxc(const double* __restrict__ rho, const double* __restric
   }
 }
 
+// Model GGA of reference physics/xc.py:137-145 (a test functional, not PBE):
+// f(n, sigma) = n eps_xc^LDA(n) (1 + mu s^2), s^2 = sigma / (4 k_F^2 n^2),
+// k_F = (3 pi^2 n)^(1/3), sigma = |grad n|^2, n floored at 1e-14 as for LDA.
+//   f_n     = v_xc^LDA (1 + mu s^2) - (8/3) mu n eps_xc s^2 / n
+//   f_sigma = mu n eps_xc / (4 k_F^2 n^2)
+// The caller forms v_xc = f_n - div(2 f_sigma grad n) with spectral
+// derivatives; the raw sum of f is reduced as for LDA.
+__global__ void __launch_bounds__(kThreads)
+k_xc_gga_model(const double* __restrict__ rho, const double* __restrict__ sigma, int64_t n, double mu,
+               double* __restrict__ fn, double* __restrict__ fs, double* __restrict__ partial) {
+  __shared__ double s[kThreads];
+  double acc = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
+    const Lda p = lda_point(rho[i], false);
+    const double nn = fmax(rho[i], 1e-14);
+    const double kf = cbrt(3.0 * kPi * kPi * nn);
+    const double den = 4.0 * kf * kf * nn * nn;
+    const double s2 = fmax(sigma[i], 0.0) / den;
+    fn[i] = p.v * (1.0 + mu * s2) - (8.0 / 3.0) * mu * p.exc_density * s2 / nn;
+    fs[i] = mu * p.exc_density / den;
+    acc += p.exc_density * (1.0 + mu * s2);
+  }
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int off = kThreads / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off) s[threadIdx.x] += s[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = s[0];
+}
+
 int nblocks(int64_t n) {
   int64_t b = (n + kThreads - 1) / kThreads;
   return (int)(b < 1 ? 1 : (b > kMaxBlocks ? kMaxBlocks : b));
@@ -111,6 +142,21 @@ int addft_xc_lda(const double* rho, int64_t n, double* vxc, double* exc, void* w
   const int nb = nblocks(n);
   double* partial = reinterpret_cast<double*>(work);
   hipLaunchKernelGGL(k_xc_lda, dim3(nb), dim3(kThreads), 0, st, rho, n, vxc, partial);
+  ADDFT_LAUNCH_CHECK();
+  if (exc) {
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kThreads), 0, st, partial, nb, exc);
+    ADDFT_LAUNCH_CHECK();
+  }
+  return ADDFT_OK;
+}
+
+int addft_xc_gga_model(const double* rho, const double* sigma, int64_t n, double mu, double* fn, double* fs,
+                       double* exc, void* work, void* stream) {
+  ADDFT_CHECK_ARG(rho && sigma && fn && fs && work && n > 0, "bad argument");
+  hipStream_t st = addft::as_stream(stream);
+  const int nb = nblocks(n);
+  double* partial = reinterpret_cast<double*>(work);
+  hipLaunchKernelGGL(k_xc_gga_model, dim3(nb), dim3(kThreads), 0, st, rho, sigma, n, mu, fn, fs, partial);
   ADDFT_LAUNCH_CHECK();
   if (exc) {
     hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kThreads), 0, st, partial, nb, exc);

@@ -209,6 +209,15 @@ int addft_split_bf16x3(const float* d_x, int64_t rows, int64_t k, int64_t ld,
 int64_t addft_xc_work_bytes(int64_t n);
 int addft_xc_lda(const double* d_rho, int64_t n, double* d_vxc, double* d_exc,
                  void* d_work, void* stream);
+/* Model GGA of reference physics/xc.py:137-145 (a test functional, not
+ * PBE): f = n eps_xc^LDA(n) (1 + mu s^2), s^2 = sigma / (4 k_F^2 n^2),
+ * k_F = (3 pi^2 n)^(1/3), n floored at 1e-14; the reference's mu is 0.1234.
+ * d_sigma: |grad n|^2 per point.  Writes d_fn = df/dn and d_fs = df/dsigma
+ * (v_xc = f_n - div(2 f_sigma grad n), formed by the caller with spectral
+ * derivatives, xc.py:78-98) and the raw sum of f to *d_exc (deterministic,
+ * may be NULL).  d_work: addft_xc_work_bytes(n) bytes.                   */
+int addft_xc_gga_model(const double* d_rho, const double* d_sigma, int64_t n, double mu, double* d_fn,
+                       double* d_fs, double* d_exc, void* d_work, void* stream);
 /* f_xc = d^2(n eps)/dn^2 times an optional vector: d_out = fxc * d_w (d_w
  * NULL -> fxc).  The backward of v_xc (reference xc.py:195-210).       */
 int addft_xc_lda_fxc(const double* d_rho, const double* d_w, int64_t n,

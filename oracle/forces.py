@@ -19,6 +19,7 @@ from __future__ import annotations
 import numpy as np
 
 from . import basis as B
+from . import gga as G
 from . import physics as P
 from .psp import beta_q, dbeta_q, dvloc_q, lm_table, vloc_q
 
@@ -111,10 +112,14 @@ def envelope_gradient(geo, st, state):
     eh = -np.eye(3) * e_h + 0.5 * om * np.einsum("g,ga,gb->ab", 4.0 * np.pi * n2 * 2.0 / q2s ** 2, gw, gw)
     parts["hartree"] = (np.zeros((nat, 3)), eh)
 
-    # XC (strain only, isotropic)
-    e_xc = P.xc_energy(basis, lat, rho)
-    vxc_int = om / basis.nfft * float(np.sum(P.xc_potential(rho) * rho))
-    parts["xc"] = (np.zeros((nat, 3)), np.eye(3) * (e_xc - vxc_int))
+    # XC (strain only): isotropic for LDA; the model GGA adds the
+    # -(Omega/N) sum 2 f_sigma d_a n d_b n term of the strained gradients
+    if getattr(st, "xc", "lda") == "gga_model":
+        parts["xc"] = (np.zeros((nat, 3)), G.strain_derivative(basis, lat, rho))
+    else:
+        e_xc = P.xc_energy(basis, lat, rho)
+        vxc_int = om / basis.nfft * float(np.sum(P.xc_potential(rho) * rho))
+        parts["xc"] = (np.zeros((nat, 3)), np.eye(3) * (e_xc - vxc_int))
 
     # nonlocal
     dion = P.dion_full(psp, nat)

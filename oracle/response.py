@@ -73,6 +73,8 @@ def make_response(geo, st, state, tol=1e-6, cache=None, occ_sign="exact"):
     per-k dmu and occupation-term sign."""
     if occ_sign not in ("exact", "reference"):
         raise ValueError(f"unknown occ_sign {occ_sign!r}")
+    if getattr(st, "xc", "lda") != "lda":
+        raise NotImplementedError("the response kernel is the local LDA f_xc (response.py:50)")
     basis = st.basis
     lat = geo.lattice
     kxc = P.xc_kernel(state.rho_r)

@@ -13,6 +13,7 @@ from dataclasses import dataclass, field, replace
 import numpy as np
 
 from . import basis as B
+from . import gga as G
 from . import physics as P
 
 
@@ -29,6 +30,7 @@ class Static:
     nelec: float
     eta_ewald: float = 0.9
     deg: float = 2.0
+    xc: str = "lda"            # "lda" (PZ81) or "gga_model" (xc.py:137-150)
 
 
 @dataclass
@@ -50,8 +52,10 @@ class State:
 
 
 def setup(lattice, frac, psp, ecut, nband, kfrac=None, kweights=None,
-          sigma=0.05, density_factor=4.0, eta_ewald=0.9):
+          sigma=0.05, density_factor=4.0, eta_ewald=0.9, xc="lda"):
     """api.py:35-45 + system.py:47-69 (single species)."""
+    if xc not in ("lda", "gga_model"):
+        raise ValueError(f"unknown xc {xc!r}")
     lattice = np.asarray(lattice, dtype=float)
     frac = np.asarray(frac, dtype=float)
     if kfrac is None:
@@ -61,7 +65,7 @@ def setup(lattice, frac, psp, ecut, nband, kfrac=None, kweights=None,
     nat = frac.shape[0]
     st = Static(basis=basis, kpts=kpts, kweights=np.asarray(kweights, dtype=float),
                 psp=psp, nat=nat, nband=int(nband), sigma=float(sigma),
-                nelec=float(nat * psp.zval[0]), eta_ewald=float(eta_ewald))
+                nelec=float(nat * psp.zval[0]), eta_ewald=float(eta_ewald), xc=xc)
     return Geo(lattice=lattice, frac=frac), st
 
 
@@ -127,8 +131,22 @@ def effective_potential(geo, st, rho_r, cache=None):
     v_loc = (P.local_potential_r(st.psp, st.basis, geo.lattice, geo.frac)
              if cache is None else cache.v_loc_r)
     v_h = P.hartree_potential_r(st.basis, geo.lattice, rho_r)
-    v_xc = P.xc_potential(rho_r)
+    v_xc = xc_potential(geo, st, rho_r)
     return v_loc + v_h + v_xc, (v_loc, v_h, v_xc)
+
+
+def xc_energy(geo, st, rho_r):
+    """E_xc of the functional st.xc (xc.py:112-117, 148-152)."""
+    if st.xc == "gga_model":
+        return G.xc_energy(st.basis, geo.lattice, rho_r)
+    return P.xc_energy(st.basis, geo.lattice, rho_r)
+
+
+def xc_potential(geo, st, rho_r):
+    """v_xc = (N/Omega) dE_xc/dn_j of st.xc (xc.py:152-172)."""
+    if st.xc == "gga_model":
+        return G.xc_potential(st.basis, geo.lattice, rho_r)
+    return P.xc_potential(rho_r)
 
 
 def eigh_lowest(H, nband):
@@ -348,10 +366,17 @@ def energy_terms(geo, st, state, active=ALL_TAGS):
         out["local"] = _expect_vmat(st, state, vloc)
     if "hartree" in active:
         out["hartree"] = P.hartree_energy(st.basis, geo.lattice, state.rho_r)
-    if "exchange" in active:
-        out["exchange"] = P.exchange_energy(st.basis, geo.lattice, state.rho_r)
-    if "correlation" in active:
-        out["correlation"] = P.correlation_energy(st.basis, geo.lattice, state.rho_r)
+    if st.xc == "gga_model" and ("exchange" in active or "correlation" in active):
+        e_x, e_c = G.xc_energy_split(st.basis, geo.lattice, state.rho_r)
+        if "exchange" in active:
+            out["exchange"] = e_x
+        if "correlation" in active:
+            out["correlation"] = e_c
+    elif "exchange" in active or "correlation" in active:
+        if "exchange" in active:
+            out["exchange"] = P.exchange_energy(st.basis, geo.lattice, state.rho_r)
+        if "correlation" in active:
+            out["correlation"] = P.correlation_energy(st.basis, geo.lattice, state.rho_r)
     if "nonlocal" in active:
         proj = P.projectors(st.psp, st.basis, geo.lattice, geo.frac, st.kpts)
         out["nonlocal"] = P.nonlocal_energy(proj, P.dion_full(st.psp, st.nat), state.psi,
@@ -381,8 +406,8 @@ def energy_variational(geo, st, state, cache=None):
     """E_band - E_H + E_xc - <v_xc> + E_II (energy.py:134-153)."""
     e_band = band_energy(geo, st, state, cache)
     e_h = P.hartree_energy(st.basis, geo.lattice, state.rho_r)
-    e_xc = P.xc_energy(st.basis, geo.lattice, state.rho_r)
-    e_dc = _expect_vmat(st, state, P.xc_potential(state.rho_r))
+    e_xc = xc_energy(geo, st, state.rho_r)
+    e_dc = _expect_vmat(st, state, xc_potential(geo, st, state.rho_r))
     e_ii = P.ewald(geo.lattice, geo.frac, np.full(st.nat, st.psp.zval[0]), eta=st.eta_ewald)
     return e_band - e_h + e_xc - e_dc + e_ii
 
