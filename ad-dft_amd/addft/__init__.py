@@ -28,4 +28,4 @@ def __getattr__(name):
 __all__ = ["System", "setup", "setup_from_case", "scf", "energy", "energy_explicit", "energy_and_forces", "stress",
            "energy_decomposition", "make_energy", "energy_forces_stress", "Cell", "strained", "volume",
            "gamma", "monkhorst_pack", "time_reversal", "Pseudo", "synthetic", "from_upf", "units", "delta_q", "delta_q_grad",
-           "make_gaussian_weight"]
+           "make_gaussian_weight", "ablate", "ablate_all", "Ablation"]

@@ -26,6 +26,7 @@ from .geometry.kmesh import KMesh, gamma
 from .geometry.psp import Pseudo, synthetic
 from .solver.scf import solve_scf
 from .system import SCFState, Static, make_static
+from .diff.ablation import Ablation, ablate, ablate_all  # noqa: F401  (energy-term ablation harness)
 
 ALL_TAGS = ("kinetic", "local", "hartree", "exchange", "correlation", "nonlocal", "ewald")
 

@@ -620,6 +620,30 @@ class Engine:
         de_tot = sum(v[1] for v in parts.values())
         return dR_tot, de_tot, parts
 
+    def exchange_strain(self, rho_psi):
+        """dE_x/deps (3, 3) of the exchange part alone, rho = rho(psi) frozen:
+        E_x = (Omega/N) sum n eps_x (1 + mu s^2) (mu = 0 for LDA), so with
+        f_x(n, sigma) its point function the derivative is
+        delta_ab [E_x - (Omega/N) sum (n df_x/dn + 2 sigma df_x/dsigma)]
+        - (Omega/N) sum 2 df_x/dsigma d_a n d_b n.  The correlation part is
+        the XC derivative minus this one."""
+        dv = self.omega / self.nfft
+        n = torch.clamp(rho_psi, min=1e-14)
+        ex = -0.75 * torch.pow(3.0 * n / math.pi, 1.0 / 3.0)
+        if self.st.xc == "lda":
+            e_x = dv * float((n * ex).sum())
+            return np.eye(3) * (e_x - dv * float((rho_psi * (4.0 / 3.0) * ex).sum()))
+        g = self.grad_density(rho_psi)
+        sigma = torch.clamp((g * g).sum(dim=0), min=0.0)
+        kf2 = torch.pow(3.0 * math.pi ** 2 * n, 2.0 / 3.0)
+        s2 = sigma / (4.0 * kf2 * n * n)
+        fx_n = (4.0 / 3.0) * ex * (1.0 + GGA_MU * s2) - (8.0 / 3.0) * GGA_MU * ex * s2
+        fx_s = GGA_MU * ex / (4.0 * kf2 * n)
+        e_x = dv * float((n * ex * (1.0 + GGA_MU * s2)).sum())
+        iso = e_x - dv * float((rho_psi * fx_n + 2.0 * sigma * fx_s).sum())
+        aniso = torch.matmul(g * (2.0 * fx_s)[None, :], g.T).cpu().numpy()
+        return np.eye(3) * iso - dv * aniso
+
     def local_gradient(self, rho_pl):
         """(dE/dR (nat, 3), dE/deps (3, 3)) of E_loc = (Omega/N) sum V_loc rho
         at fixed rho (plane-major), i.e. Tr[dV_loc/dtheta rho]."""
