@@ -197,23 +197,25 @@ def parse_stru(path) -> dict:
 
 
 def setup_from_case(case_dir, psp=None, precision: str = "fp64"):
-    """INPUT / KPT / STRU (+ the UPF named in STRU, found in pseudo_dir
-    relative to the case) -> `addft.System` (io.py:214-238).  One species.
-    `psp` replaces the UPF (for cases whose pseudopotential file is not at
-    hand).  `smearing_method fixed` gives sigma = 0 (integer filling); every
+    """INPUT / KPT / STRU (+ the UPF files named in STRU, found in pseudo_dir
+    relative to the case) -> `addft.System` (io.py:214-238).  Species follow
+    the ATOMIC_SPECIES order (the reference takes one species only).  `psp`
+    replaces the UPF files (for cases whose pseudopotentials are not at
+    hand): a Pseudo holding every species, or one single-species Pseudo per
+    species in that order.  `smearing_method fixed` gives sigma = 0 (integer filling); every
     other method gives Fermi-Dirac occupations (the one smearing here) at
     `smearing_sigma`, 0.05 Ha when absent, as the reference does
     (io.py:234-235)."""
     from addft.api import setup
     from addft.geometry.kmesh import KMesh, gamma, monkhorst_pack
-    from addft.geometry.psp import from_upf
+    from addft.geometry.psp import Pseudo, combine_species, from_upf
 
     case = Path(case_dir)
     inp = parse_input(case / "INPUT")
     kpt = parse_kpt(case / "KPT")
     stru = parse_stru(case / "STRU")
-    if len(set(stru["species"])) != 1 or len(stru["labels"]) != 1:
-        raise NotImplementedError("one species per case")
+    labels = stru["labels"]
+    spec_idx = np.asarray([labels.index(x) for x in stru["species"]], dtype=np.int32)
     if inp.get("spin.nspin", 1) != 1:
         raise NotImplementedError(f"nspin = {inp['spin.nspin']}: only spin-restricted runs")
     xc = str(inp.get("xc.name", "lda")).lower()
@@ -222,7 +224,13 @@ def setup_from_case(case_dir, psp=None, precision: str = "fp64"):
     if "basis.ecut_ha" not in inp:
         raise ValueError(f"{case / 'INPUT'}: no ecutwfc")
     if psp is None:
-        psp = from_upf(case / inp.get("pseudo_dir", ".") / stru["pseudo_files"][0])
+        if any(f is None for f in stru["pseudo_files"]):
+            raise ValueError(f"{case / 'STRU'}: ATOMIC_SPECIES names no pseudopotential file for some species")
+        psp = [from_upf(case / inp.get("pseudo_dir", ".") / f) for f in stru["pseudo_files"]]
+    if not isinstance(psp, Pseudo):          # a sequence of single-species pseudopotentials
+        psp = psp[0] if len(psp) == 1 else combine_species(*psp)
+    if int(psp.nspec) != len(labels):
+        raise ValueError(f"{len(labels)} species in STRU, {int(psp.nspec)} in the pseudopotential")
     if kpt["mode"] == "gamma":
         km = gamma()
     elif kpt["mode"] == "mp":
@@ -232,4 +240,4 @@ def setup_from_case(case_dir, psp=None, precision: str = "fp64"):
     method = str(inp.get("occupation.method", "gaussian")).lower()
     sigma = 0.0 if method == "fixed" else float(inp.get("occupation.sigma_ha", 0.05))
     return setup(lattice=stru["lattice"], frac=stru["frac"], psp=psp, ecut=inp["basis.ecut_ha"],
-                 nband=inp.get("nband", 6), kmesh=km, sigma=sigma, precision=precision)
+                 nband=inp.get("nband", 6), kmesh=km, sigma=sigma, precision=precision, spec_idx=spec_idx)

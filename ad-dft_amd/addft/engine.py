@@ -124,36 +124,54 @@ class Engine:
         return torch.as_tensor(np.ascontiguousarray(a, dtype=np.int32), device=self.dev)
 
     def _prep_psp(self):
-        psp, s = self.st.psp, int(self.st.spec_idx[0])
+        """Per-species radial and local-potential inputs, the projector row
+        table and the block-diagonal D.  Atom I uses species spec_idx[I];
+        the radial tables of all species are stacked (species s owns rows
+        off_s .. off_s + nrad_s - 1 of the beta tables)."""
+        psp = self.st.psp
+        spec = np.asarray(self.st.spec_idx, dtype=np.int64)
         self.kind = int(psp.kind)
-        self.nrad = int(psp.nrad)
-        self.l_d = self._i32(psp.proj_l[s])
-        self.zval = float(psp.zval[s])
-        if self.kind == 0:
-            self.pw_d = self._f64(psp.proj_w[s])
-            self.pb_d = self._f64(psp.proj_b[s])
-            self.mesh = 0
-            self.r_d = self.f_d = self.aux_d = None
-            self.vamp, self.vsig, self.g0 = float(psp.vloc_amp[s]), float(psp.vloc_sigma[s]), 0.0
-        else:
-            r, rab = psp.r[s], psp.rab[s]
-            w = _simpson_weights(r.shape[0])
-            v = ry_to_ha(psp.vloc_at[s])
-            z = self.zval
-            self.mesh = int(r.shape[0])
-            self.r_d = self._f64(r)
-            self.f_d = self._f64(np.stack([w * rab * psp.betar[s][ib] * r for ib in range(self.nrad)]))
-            self.aux_d = self._f64(w * rab * (r * v + z * erf(r)))
-            self.g0 = float(4.0 * np.pi * np.sum(w * rab * r * (r * v + z)))
-            self.pw_d = self.pb_d = None
-            self.vamp = self.vsig = 0.0
+        self.spec = spec
+        self.species = []
+        off = 0
+        for s in range(psp.nspec):
+            nr = psp.nrad_of(s)
+            lrow = np.asarray(psp.proj_l[s][:nr], dtype=np.int32)
+            sp = {"nrad": nr, "off": off, "l_d": self._i32(lrow), "zval": float(psp.zval[s]),
+                  "atoms": np.nonzero(spec == s)[0]}
+            if self.kind == 0:
+                sp.update(pw_d=self._f64(psp.proj_w[s][:nr]), pb_d=self._f64(psp.proj_b[s][:nr]), mesh=0,
+                          r_d=None, f_d=None, aux_d=None, vamp=float(psp.vloc_amp[s]),
+                          vsig=float(psp.vloc_sigma[s]), g0=0.0)
+            else:
+                n = psp.mesh_of(s)
+                r, rab = psp.r[s][:n], psp.rab[s][:n]
+                w = _simpson_weights(n)
+                v = ry_to_ha(psp.vloc_at[s][:n])
+                z = sp["zval"]
+                f = np.stack([w * rab * psp.betar[s][ib][:n] * r for ib in range(nr)]) if nr else np.zeros((0, n))
+                sp.update(pw_d=None, pb_d=None, mesh=n, r_d=self._f64(r), f_d=self._f64(f),
+                          aux_d=self._f64(w * rab * (r * v + z * erf(r))),
+                          g0=float(4.0 * np.pi * np.sum(w * rab * r * (r * v + z))), vamp=0.0, vsig=0.0)
+            self.species.append(sp)
+            off += nr
+        self.nrad_total = off
         rows = []
+        blocks = []
         for i in range(self.nat):
-            for ib, l, m in lm_table(psp.proj_l[s]):
-                rows.append((i, ib, l * l + m))
+            s = int(spec[i])
+            lm = lm_table(psp.proj_l[s])
+            for ib, l, m in lm:
+                rows.append((i, self.species[s]["off"] + ib, l * l + m))
+            # D block laid out (p, q) over this atom's rows (nonlocal_op.py:25-42)
+            d = np.zeros((len(lm), len(lm)))
+            for p, (ib, l, m) in enumerate(lm):
+                for q, (jb, l2, m2) in enumerate(lm):
+                    if l == l2 and m == m2:
+                        d[p, q] = psp.dion[s, ib, jb]
+            blocks.append(d)
         rows = np.asarray(rows, dtype=np.int32).reshape(-1, 3)
         self.nproj = rows.shape[0]
-        self.nh = int(psp.nh[s])
         self.patom_d = self._i32(rows[:, 0])
         self.prad_d = self._i32(rows[:, 1])
         self.plm_d = self._i32(rows[:, 2])
@@ -161,27 +179,44 @@ class Engine:
         # rows of odd l (lm index l*l + m): phase i in the Gamma-point half-sphere products
         self.odd_rows_d = torch.as_tensor(np.floor(np.sqrt(rows[:, 2] + 0.5)).astype(np.int64) % 2 == 1,
                                           device=self.dev)
-        # block-diagonal D laid out (i, p, i, q) (nonlocal_op.py:25-42)
-        lm = lm_table(psp.proj_l[s])
-        dblk = np.zeros((self.nh, self.nh))
-        for p, (ib, l, m) in enumerate(lm):
-            for q, (jb, l2, m2) in enumerate(lm):
-                if l == l2 and m == m2:
-                    dblk[p, q] = psp.dion[s, ib, jb]
-        self.dblk = dblk
-        D = np.kron(np.eye(self.nat), dblk)
+        D = np.zeros((self.nproj, self.nproj))
+        o = 0
+        for d in blocks:
+            D[o:o + d.shape[0], o:o + d.shape[0]] = d
+            o += d.shape[0]
         self.D64 = torch.as_tensor(D, dtype=torch.complex128, device=self.dev)
         self.D32 = self.D64.to(torch.complex64)
-        self.z_d = self._f64(np.asarray(self.st.psp.zval)[self.st.spec_idx])
+        self.z_d = self._f64(np.asarray(psp.zval)[spec])
 
     # ------------------------------------------------------------ geometry
     def _radial(self, qn, n):
-        beta = torch.empty((self.nrad, n), dtype=torch.float64, device=self.dev)
+        """beta~ and d beta~/dq of every species' radial projectors at |q|,
+        stacked (nrad_total, n)."""
+        beta = torch.empty((max(self.nrad_total, 1), n), dtype=torch.float64, device=self.dev)
         dbeta = torch.empty_like(beta)
-        check(lib().addft_beta_table(self.kind, self.nrad, ptr(self.l_d), ptr(self.pw_d), ptr(self.pb_d),
-                                     self.mesh, ptr(self.r_d), ptr(self.f_d), ptr(qn), n, ptr(beta), ptr(dbeta),
-                                     stream()), "addft_beta_table")
+        for sp in self.species:
+            if sp["nrad"] == 0:
+                continue
+            o = sp["off"] * n
+            check(lib().addft_beta_table(self.kind, sp["nrad"], ptr(sp["l_d"]), ptr(sp["pw_d"]), ptr(sp["pb_d"]),
+                                         sp["mesh"], ptr(sp["r_d"]), ptr(sp["f_d"]), ptr(qn), n,
+                                         beta.data_ptr() + 8 * o, dbeta.data_ptr() + 8 * o, stream()),
+                  "addft_beta_table")
         return beta, dbeta
+
+    def _structure_sum(self, tables, inv_omega):
+        """sum_I v_{s(I)}(G) e^{-iG.R_I} * inv_omega on the density sphere for
+        per-species tables v_s (one addft_vloc_g launch per species)."""
+        ngr = self.st.basis.nG_rho
+        out = torch.zeros(ngr, dtype=torch.complex128, device=self.dev)
+        part = torch.empty_like(out)
+        for sp, tab in zip(self.species, tables):
+            if len(sp["atoms"]) == 0:
+                continue
+            check(lib().addft_vloc_g(ngr, len(sp["atoms"]), ptr(self.grho_d), ptr(tab), ptr(sp["pos_d"]),
+                                     inv_omega, ptr(part), stream()), "addft_vloc_g")
+            out += part
+        return out
 
     def set_geometry(self, cell: Cell):
         """Build G tables, projectors and V_loc(r) for this geometry."""
@@ -250,14 +285,14 @@ class Engine:
         self.qnrho_d = torch.empty(ngr, dtype=torch.float64, device=self.dev)
         check(L.addft_gvectors(ptr(self.miller_rho_d), ptr(self.perm_rho_d), ngr, ptr(self.bmat_d), None,
                                ptr(self.grho_d), None, ptr(self.qnrho_d), stream()), "addft_gvectors")
-        self.vq_d = torch.empty(ngr, dtype=torch.float64, device=self.dev)
-        self.dvq_d = torch.empty_like(self.vq_d)
-        check(L.addft_vloc_table(self.kind, self.vamp, self.vsig, self.zval, self.g0, self.mesh, ptr(self.r_d),
-                                 ptr(self.aux_d), ptr(self.qnrho_d), ngr, ptr(self.vq_d), ptr(self.dvq_d), stream()),
-              "addft_vloc_table")
-        vg = torch.empty(ngr, dtype=torch.complex128, device=self.dev)
-        check(L.addft_vloc_g(ngr, self.nat, ptr(self.grho_d), ptr(self.vq_d), ptr(self.pos_d), 1.0 / self.omega,
-                             ptr(vg), stream()), "addft_vloc_g")
+        for sp in self.species:
+            sp["pos_d"] = self._f64(self.pos[sp["atoms"]].reshape(-1, 3))
+            sp["vq_d"] = torch.empty(ngr, dtype=torch.float64, device=self.dev)
+            sp["dvq_d"] = torch.empty_like(sp["vq_d"])
+            check(L.addft_vloc_table(self.kind, sp["vamp"], sp["vsig"], sp["zval"], sp["g0"], sp["mesh"],
+                                     ptr(sp["r_d"]), ptr(sp["aux_d"]), ptr(self.qnrho_d), ngr, ptr(sp["vq_d"]),
+                                     ptr(sp["dvq_d"]), stream()), "addft_vloc_table")
+        vg = self._structure_sum([sp["vq_d"] for sp in self.species], 1.0 / self.omega)
         self.vloc_pl = self.plan_rho.to_real(vg, 1.0)
         self.ewald_cache = None
         self.geo = cell
@@ -270,12 +305,9 @@ class Engine:
         normalised to the electron count; plane-major.  Positive wherever
         the Gaussians are resolved (their transform is negligible at the
         sphere's edge)."""
-        ngr = self.st.basis.nG_rho
         q2 = self.grho_d.pow(2).sum(dim=1)
-        tab = (self.zval * torch.exp(-0.5 * sigma * sigma * q2)).contiguous()
-        g = torch.empty(ngr, dtype=torch.complex128, device=self.dev)
-        check(lib().addft_vloc_g(ngr, self.nat, ptr(self.grho_d), ptr(tab), ptr(self.pos_d), 1.0 / self.omega,
-                                 ptr(g), stream()), "addft_vloc_g")
+        gauss = torch.exp(-0.5 * sigma * sigma * q2)
+        g = self._structure_sum([(sp["zval"] * gauss).contiguous() for sp in self.species], 1.0 / self.omega)
         rho = self.plan_rho.to_real(g, 1.0)
         rho = torch.clamp(rho, min=1e-10)
         return rho * (self.st.nelec / (float(rho.sum()) * self.omega / self.nfft))
@@ -650,14 +682,23 @@ class Engine:
         L = lib()
         nat = self.nat
         rt = self.plan_rho.to_sphere(rho_pl, 1.0 / self.nfft)
-        lout = torch.empty(3 * nat + 10, dtype=torch.float64, device=self.dev)
         wb = int(L.addft_backward_work_bytes(max(nat, 1)))
-        check(L.addft_local_backward(self.st.basis.nG_rho, nat, ptr(self.grho_d), ptr(self.qnrho_d), ptr(self.vq_d),
-                                     ptr(self.dvq_d), ptr(rt), ptr(self.pos_d), ptr(lout),
-                                     ptr(self.work("bwd", wb)), stream()), "addft_local_backward")
-        lo = lout.cpu().numpy()
-        e_loc = lo[3 * nat]
-        return lo[:3 * nat].reshape(nat, 3), -np.eye(3) * e_loc + lo[3 * nat + 1:].reshape(3, 3)
+        forces = np.zeros((nat, 3))
+        e_loc = 0.0
+        strain = np.zeros((3, 3))
+        for sp in self.species:
+            na = len(sp["atoms"])
+            if na == 0:
+                continue
+            lout = torch.empty(3 * na + 10, dtype=torch.float64, device=self.dev)
+            check(L.addft_local_backward(self.st.basis.nG_rho, na, ptr(self.grho_d), ptr(self.qnrho_d),
+                                         ptr(sp["vq_d"]), ptr(sp["dvq_d"]), ptr(rt), ptr(sp["pos_d"]), ptr(lout),
+                                         ptr(self.work("bwd", wb)), stream()), "addft_local_backward")
+            lo = lout.cpu().numpy()
+            forces[sp["atoms"]] = lo[:3 * na].reshape(na, 3)
+            e_loc += lo[3 * na]
+            strain += lo[3 * na + 1:].reshape(3, 3)
+        return forces, -np.eye(3) * e_loc + strain
 
     def nonlocal_gradient(self, ik, Y):
         """2 Re sum_G conj(d beta_p/dtheta) Y_p summed per atom: (nat, 3) and

@@ -61,8 +61,12 @@ def make_static(cell: Cell, psp: Pseudo, kmesh: KMesh, ecut: float, nband: int, 
     if spec_idx is None:
         spec_idx = np.zeros((cell.natom,), dtype=np.int32)
     spec_idx = np.asarray(spec_idx, dtype=np.int32)
-    if int(psp.nspec) != 1:
-        raise NotImplementedError("only one species is supported (system.py:53-54)")
+    # several species (geometry.psp.combine_species): the reference refuses
+    # them (system.py:53-54); here each atom takes its own species' tables
+    if spec_idx.shape != (cell.natom,):
+        raise ValueError(f"spec_idx must have one entry per atom ({cell.natom}), got shape {spec_idx.shape}")
+    if spec_idx.size and (int(spec_idx.min()) < 0 or int(spec_idx.max()) >= int(psp.nspec)):
+        raise ValueError(f"spec_idx out of range for {int(psp.nspec)} species")
     if precision not in ("fp64", "mixed"):
         raise ValueError(f"unknown precision {precision!r}")
     if xc not in XC_FUNCTIONALS:

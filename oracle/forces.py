@@ -21,6 +21,7 @@ import numpy as np
 from . import basis as B
 from . import gga as G
 from . import physics as P
+from . import solver as S
 from .psp import beta_q, dbeta_q, dvloc_q, lm_table, vloc_q
 
 
@@ -91,14 +92,21 @@ def envelope_gradient(geo, st, state):
     grho = B.gvectors(basis.miller_rho, lat)
     qn = B.safe_norm(grho)
     rt = _rho_tilde(basis, rho, basis.miller_rho)
-    amp = vloc_q(psp, 0, qn)
-    damp = dvloc_q(psp, 0, qn)
+    # each atom with its own species' v~(q) (one table per species)
+    tabs = {}
+    for i in range(nat):
+        pi = S.atom_psp(st, i)
+        if id(pi) not in tabs:
+            tabs[id(pi)] = (vloc_q(pi, 0, qn), dvloc_q(pi, 0, qn))
+    amp = np.stack([tabs[id(S.atom_psp(st, i))][0] for i in range(nat)], axis=1)    # (nG, nat)
+    damp = np.stack([tabs[id(S.atom_psp(st, i))][1] for i in range(nat)], axis=1)
     ph = np.exp(-1j * (grho @ r.T))                # (nG, nat)
-    sf = ph.sum(axis=1)
-    e_loc = float(np.real(np.sum(amp * sf * np.conj(rt))))
-    fl = np.real(np.einsum("g,gi,gc->ic", amp * np.conj(rt), -1j * ph, grho))
+    sf = (amp * ph).sum(axis=1)
+    dsf = (damp * ph).sum(axis=1)
+    e_loc = float(np.real(np.sum(sf * np.conj(rt))))
+    fl = np.real(np.einsum("g,gi,gc->ic", np.conj(rt), -1j * amp * ph, grho))
     inv_q = np.where(qn > 0, 1.0 / np.where(qn > 0, qn, 1.0), 0.0)
-    el = -np.eye(3) * e_loc + np.real(np.einsum("g,ga,gb->ab", damp * sf * np.conj(rt) * (-inv_q), grho, grho))
+    el = -np.eye(3) * e_loc + np.real(np.einsum("g,ga,gb->ab", dsf * np.conj(rt) * (-inv_q), grho, grho))
     parts["local"] = (fl, el)
 
     # Hartree (strain only), psi-sphere
@@ -121,13 +129,19 @@ def envelope_gradient(geo, st, state):
         vxc_int = om / basis.nfft * float(np.sum(P.xc_potential(rho) * rho))
         parts["xc"] = (np.zeros((nat, 3)), np.eye(3) * (e_xc - vxc_int))
 
-    # nonlocal
-    dion = P.dion_full(psp, nat)
-    nh = psp.nh()
+    # nonlocal (rows atom-major, each atom's block from its species)
+    dion = S.dion(st)
+    row_atom = np.concatenate([np.full(S.atom_psp(st, i).nh(), i) for i in range(nat)])
     fn = np.zeros((nat, 3))
     en = np.zeros((3, 3))
     for ik, k in enumerate(st.kpts):
-        Pm, dPR, dPe = projector_derivatives(psp, basis, lat, frac, k)
+        if st.species:
+            per = [projector_derivatives(S.atom_psp(st, i), basis, lat, frac[i:i + 1], k) for i in range(nat)]
+            Pm = np.concatenate([x[0] for x in per], axis=0)
+            dPR = np.concatenate([x[1] for x in per], axis=1)
+            dPe = np.concatenate([x[2] for x in per], axis=2)
+        else:
+            Pm, dPR, dPe = projector_derivatives(psp, basis, lat, frac, k)
         psi = state.psi[ik]
         w = st.kweights[ik] * state.occ[ik] * st.deg
         b = np.conj(Pm) @ psi.T                    # (nproj, nb)
@@ -135,7 +149,7 @@ def envelope_gradient(geo, st, state):
         for a in range(3):
             dba = np.conj(dPR[a]) @ psi.T
             contrib = 2.0 * np.real(np.conj(db) * dba) * w[None, :]   # (nproj, nb)
-            fn[:, a] += contrib.sum(axis=1).reshape(nat, nh).sum(axis=1)
+            np.add.at(fn[:, a], row_atom, contrib.sum(axis=1))
         for mu in range(3):
             for nu in range(3):
                 dbe = np.conj(dPe[mu, nu]) @ psi.T
@@ -143,7 +157,7 @@ def envelope_gradient(geo, st, state):
     parts["nonlocal"] = (fn, en)
 
     # Ewald
-    _, fe, ee = P.ewald(lat, frac, np.full(nat, psp.zval[0]), eta=st.eta_ewald, want_grad=True)
+    _, fe, ee = S.ewald_energy(geo, st, want_grad=True)
     parts["ewald"] = (fe, ee)
 
     for v in parts.values():

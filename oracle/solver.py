@@ -31,6 +31,10 @@ class Static:
     eta_ewald: float = 0.9
     deg: float = 2.0
     xc: str = "lda"            # "lda" (PZ81) or "gga_model" (xc.py:137-150)
+    # several species: one single-species Pseudo per species and the
+    # species index of each atom (empty / None: every atom is `psp`)
+    species: tuple = ()
+    spec: np.ndarray = None
 
 
 @dataclass
@@ -52,10 +56,21 @@ class State:
 
 
 def setup(lattice, frac, psp, ecut, nband, kfrac=None, kweights=None,
-          sigma=0.05, density_factor=4.0, eta_ewald=0.9, xc="lda"):
-    """api.py:35-45 + system.py:47-69 (single species)."""
+          sigma=0.05, density_factor=4.0, eta_ewald=0.9, xc="lda", spec_idx=None):
+    """api.py:35-45 + system.py:47-69.  psp: one Pseudo, or a sequence of
+    single-species Pseudos indexed by spec_idx (nat,) -- the reference
+    refuses several species (system.py:53-54); here each atom carries its
+    own species' local potential, projectors, D and valence charge."""
     if xc not in ("lda", "gga_model"):
         raise ValueError(f"unknown xc {xc!r}")
+    species = ()
+    spec = None
+    if isinstance(psp, (list, tuple)):
+        species = tuple(psp)
+        spec = np.asarray(spec_idx if spec_idx is not None else np.zeros(len(frac)), dtype=np.int64)
+        if spec.shape != (np.asarray(frac).shape[0],) or spec.min() < 0 or spec.max() >= len(species):
+            raise ValueError("spec_idx must give a species index for every atom")
+        psp = species[0]
     lattice = np.asarray(lattice, dtype=float)
     frac = np.asarray(frac, dtype=float)
     if kfrac is None:
@@ -65,8 +80,55 @@ def setup(lattice, frac, psp, ecut, nband, kfrac=None, kweights=None,
     nat = frac.shape[0]
     st = Static(basis=basis, kpts=kpts, kweights=np.asarray(kweights, dtype=float),
                 psp=psp, nat=nat, nband=int(nband), sigma=float(sigma),
-                nelec=float(nat * psp.zval[0]), eta_ewald=float(eta_ewald), xc=xc)
+                nelec=0.0, eta_ewald=float(eta_ewald), xc=xc, species=species, spec=spec)
+    st = replace(st, nelec=float(np.sum(atom_charges(st))))
     return Geo(lattice=lattice, frac=frac), st
+
+
+# ------------------------------------------------------------ species
+def atom_psp(st, i):
+    """The Pseudo of atom i."""
+    return st.species[int(st.spec[i])] if st.species else st.psp
+
+
+def atom_charges(st):
+    """(nat,) valence charges Z_I."""
+    return np.array([float(atom_psp(st, i).zval[0]) for i in range(st.nat)])
+
+
+def local_potential_r(geo, st):
+    """V_loc(r) = sum_I of each atom's species term (local.py:21-39)."""
+    if not st.species:
+        return P.local_potential_r(st.psp, st.basis, geo.lattice, geo.frac)
+    vg = sum(P.local_potential_g(atom_psp(st, i), st.basis, geo.lattice, geo.frac[i:i + 1])
+             for i in range(st.nat))
+    return B.real_from_density_g(st.basis, vg)
+
+
+def projectors(geo, st):
+    """(nk, sum_I nh_I, nG), rows atom-major as in nonlocal_op.py:45-64."""
+    if not st.species:
+        return P.projectors(st.psp, st.basis, geo.lattice, geo.frac, st.kpts)
+    return np.concatenate([P.projectors(atom_psp(st, i), st.basis, geo.lattice, geo.frac[i:i + 1], st.kpts)
+                           for i in range(st.nat)], axis=1)
+
+
+def dion(st):
+    """Block-diagonal D, one block per atom (nonlocal_op.py:25-42)."""
+    if not st.species:
+        return P.dion_full(st.psp, st.nat)
+    blocks = [P.dion_full(atom_psp(st, i), 1) for i in range(st.nat)]
+    n = sum(b.shape[0] for b in blocks)
+    d = np.zeros((n, n))
+    o = 0
+    for b in blocks:
+        d[o:o + b.shape[0], o:o + b.shape[0]] = b
+        o += b.shape[0]
+    return d
+
+
+def ewald_energy(geo, st, want_grad=False):
+    return P.ewald(geo.lattice, geo.frac, atom_charges(st), eta=st.eta_ewald, want_grad=want_grad)
 
 
 # ------------------------------------------------------------ Hamiltonian
@@ -95,9 +157,7 @@ class Cache:
 
 
 def build_cache(geo, st):
-    return Cache(v_loc_r=P.local_potential_r(st.psp, st.basis, geo.lattice, geo.frac),
-                 proj=P.projectors(st.psp, st.basis, geo.lattice, geo.frac, st.kpts),
-                 dion=P.dion_full(st.psp, st.nat))
+    return Cache(v_loc_r=local_potential_r(geo, st), proj=projectors(geo, st), dion=dion(st))
 
 
 def hamiltonian(geo, st, v_r, cache=None):
@@ -128,8 +188,7 @@ def hpsi(geo, st, psi, v_r, cache=None):
 
 def effective_potential(geo, st, rho_r, cache=None):
     """v_loc + v_H + v_xc (scf.py:81-92)."""
-    v_loc = (P.local_potential_r(st.psp, st.basis, geo.lattice, geo.frac)
-             if cache is None else cache.v_loc_r)
+    v_loc = local_potential_r(geo, st) if cache is None else cache.v_loc_r
     v_h = P.hartree_potential_r(st.basis, geo.lattice, rho_r)
     v_xc = xc_potential(geo, st, rho_r)
     return v_loc + v_h + v_xc, (v_loc, v_h, v_xc)
@@ -362,7 +421,7 @@ def energy_terms(geo, st, state, active=ALL_TAGS):
     if "kinetic" in active:
         out["kinetic"] = _kinetic(geo, st, state)
     if "local" in active:
-        vloc = P.local_potential_r(st.psp, st.basis, geo.lattice, geo.frac)
+        vloc = local_potential_r(geo, st)
         out["local"] = _expect_vmat(st, state, vloc)
     if "hartree" in active:
         out["hartree"] = P.hartree_energy(st.basis, geo.lattice, state.rho_r)
@@ -378,11 +437,10 @@ def energy_terms(geo, st, state, active=ALL_TAGS):
         if "correlation" in active:
             out["correlation"] = P.correlation_energy(st.basis, geo.lattice, state.rho_r)
     if "nonlocal" in active:
-        proj = P.projectors(st.psp, st.basis, geo.lattice, geo.frac, st.kpts)
-        out["nonlocal"] = P.nonlocal_energy(proj, P.dion_full(st.psp, st.nat), state.psi,
+        out["nonlocal"] = P.nonlocal_energy(projectors(geo, st), dion(st), state.psi,
                                             state.occ, st.kweights, st.deg)
     if "ewald" in active:
-        out["ewald"] = P.ewald(geo.lattice, geo.frac, np.full(st.nat, st.psp.zval[0]), eta=st.eta_ewald)
+        out["ewald"] = ewald_energy(geo, st)
     return out
 
 
@@ -408,7 +466,7 @@ def energy_variational(geo, st, state, cache=None):
     e_h = P.hartree_energy(st.basis, geo.lattice, state.rho_r)
     e_xc = xc_energy(geo, st, state.rho_r)
     e_dc = _expect_vmat(st, state, xc_potential(geo, st, state.rho_r))
-    e_ii = P.ewald(geo.lattice, geo.frac, np.full(st.nat, st.psp.zval[0]), eta=st.eta_ewald)
+    e_ii = ewald_energy(geo, st)
     return e_band - e_h + e_xc - e_dc + e_ii
 
 

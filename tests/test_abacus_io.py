@@ -125,8 +125,13 @@ def test_setup_from_case_refusals(tmp_path):
     two = ("ATOMIC_SPECIES\nSi 1.0 Si.upf\nO 16.0 O.upf\n\nLATTICE_CONSTANT\n10.2\n\nLATTICE_VECTORS\n"
            "0.0 0.5 0.5\n0.5 0.0 0.5\n0.5 0.5 0.0\n\nATOMIC_POSITIONS\nDirect\n\n"
            "Si\n0.0\n1\n0 0 0\nO\n0.0\n1\n0.25 0.25 0.25\n")
-    with pytest.raises(NotImplementedError, match="one species"):
+    with pytest.raises(ValueError, match="2 species in STRU, 1 in the pseudopotential"):
         addft.setup_from_case(_case_copy(tmp_path / "a", stru=two), psp=psp)
+    # two species, one pseudopotential each in ATOMIC_SPECIES order
+    s2 = addft.setup_from_case(_case_copy(tmp_path / "a2", stru=two),
+                               psp=[addft.synthetic(zval=(4.0,)), addft.synthetic(zval=(6.0,))])
+    assert list(s2.static.spec_idx) == [0, 1] and s2.static.nelec == pytest.approx(10.0)
+    assert s2.static.psp.nspec == 2
     with pytest.raises(NotImplementedError, match="nspin"):
         addft.setup_from_case(_case_copy(tmp_path / "b", extra_input="nspin 2\n"), psp=psp)
     with pytest.raises(NotImplementedError, match="LDA"):
