@@ -103,6 +103,9 @@ class Engine:
         self.geo = None
         # band-applications of H by precision (a work counter for reports)
         self.hpsi_bands = {"fp32": 0, "fp64": 0}
+        # (sum of the eigenvalues, buffer vectors per k, real) of the last SCF
+        # solution on this engine: the response solver's deflation space
+        self.scf_buffer = None
 
     # ------------------------------------------------------------- helpers
     def work(self, key, nbytes):
@@ -295,6 +298,7 @@ class Engine:
         vg = self._structure_sum([sp["vq_d"] for sp in self.species], 1.0 / self.omega)
         self.vloc_pl = self.plan_rho.to_real(vg, 1.0)
         self.ewald_cache = None
+        self.scf_buffer = None
         self.geo = cell
 
     def atomic_density(self, sigma=1.2):

@@ -569,7 +569,11 @@ def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, 
             break
         rho = rho_new
     rho, X, eig, occ, mu, res = best
+    # the eigensolver's buffer vectors above nband: approximate eigenvectors
+    # of the next states, kept for the response solver's deflation space
+    buffer = [x[nb:].to(torch.complex128).contiguous() for x in X] if nbt > nb else None
     X = [x[:nb].contiguous() for x in X]
+    engine.scf_buffer = (float(eig.sum()), buffer, bool(real))
     info = {"iterations": it, "residual_history": hist, "davidson_iterations": dav_iters,
             "fp64_from_iteration": info_switch, "map_end_seconds": tstamps, "map_gain": gain,
             "polish_steps": n_polish,
