@@ -189,6 +189,13 @@ class Engine:
             o += d.shape[0]
         self.D64 = torch.as_tensor(D, dtype=torch.complex128, device=self.dev)
         self.D32 = self.D64.to(torch.complex64)
+        # one species present: D is nat copies of one (nh, nh) block, applied
+        # as a GEMM over (rows * nat, nh) instead of the dense (nproj, nproj)
+        present = sorted({int(x) for x in spec})
+        self.dblk = None
+        if len(present) == 1 and self.nproj:
+            blk = torch.as_tensor(blocks[0], device=self.dev)
+            self.dblk = {torch.complex128: blk.to(torch.complex128), torch.complex64: blk.to(torch.complex64)}
         self.z_d = self._f64(np.asarray(psp.zval)[spec])
 
     # ------------------------------------------------------------ geometry
@@ -377,7 +384,6 @@ class Engine:
         plan = self.plan32 if fp32 else self.plan64
         tk = self.tkin32[ik] if fp32 else self.tkin64[ik]
         P = self.P32[ik] if fp32 else self.P64[ik]
-        D = self.D32 if fp32 else self.D64
         self.hpsi_bands["fp32" if fp32 else "fp64"] += int(psi.shape[0])
         if real:
             if self.gplus_d is None:
@@ -388,14 +394,21 @@ class Engine:
             out = plan.hpsi_local(psi, tk, vs)
         if fp32 and nl_bf16 and self.PB[ik] is not None:
             pb = self.PB[ik]
-            pb.apply_add(torch.matmul(pb.becp_t(psi), D), out)
+            pb.apply_add(self.dmul(pb.becp_t(psi)), out)
         elif not fp32 and self.PG64[ik] is not None:
             g = self.PG64[ik]                           # k = 0: half-sphere FP64 products
-            g.apply_add(torch.matmul(g.becp_t(psi), D), out)
+            g.apply_add(self.dmul(g.becp_t(psi)), out)
         else:
             becp_t = inner(psi, P)                      # (nb, nproj) = <beta|psi>^T
-            torch.addmm(out, torch.matmul(becp_t, D), P, out=out)   # accumulated by the GEMM (beta = 1)
+            torch.addmm(out, self.dmul(becp_t), P, out=out)   # accumulated by the GEMM (beta = 1)
         return self.restrict(ik, out)
+
+    def dmul(self, b):
+        """b @ D for projector coefficients b (..., nproj) in rows."""
+        if self.dblk is not None:
+            nh = self.dblk[b.dtype].shape[0]
+            return torch.matmul(b.reshape(-1, nh), self.dblk[b.dtype]).reshape(b.shape)
+        return torch.matmul(b, self.D32 if b.dtype == torch.complex64 else self.D64)
 
     def realify(self, x):
         """Rows of x projected onto vectors real in real space, x(G) ->
@@ -427,11 +440,10 @@ class Engine:
         tk = self.tk32_all if fp32 else self.tk64_all
         P = self.P32_all if fp32 else self.P64_all
         PH = self.PH32_all if fp32 else self.PH64_all
-        D = self.D32 if fp32 else self.D64
         self.hpsi_bands["fp32" if fp32 else "fp64"] += int(nk * m)
         out = plan.hpsi_local_kbatch(psi3.reshape(nk * m, ng), tk, m, vs).view(nk, m, ng)
         becp_t = torch.bmm(psi3, PH)                    # (nk, m, nproj)
-        torch.baddbmm(out, torch.matmul(becp_t, D), P, out=out)
+        torch.baddbmm(out, self.dmul(becp_t), P, out=out)
         return self.restrict_all(out)
 
     def density_batch(self, psi3, occ, fp32=False):

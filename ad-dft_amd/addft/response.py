@@ -487,45 +487,69 @@ class Response:
     def _pcg32(self, ik, r0, eps, tau):
         """Preconditioned CG in FP32 for A d_n = r0_n with A = Q' (H - e_n) Q',
         Q' projecting out the computed states and the deflation space, all
-        bands at once; band n stops at |r_n| <= tau_n |Q' r0_n|."""
+        bands at once; band n stops at |r_n| <= tau_n |Q' r0_n|.  The
+        working arrays hold the unconverged bands only (compacted when bands
+        converge), the diagonal preconditioner is tabulated once per solve,
+        and the per-band dot products and updates run on the real views.
+        (Lifting the computed space by a shift instead of projecting it out,
+        the alpha P_v device of perturbation theory, saves one projection
+        per iteration but stalls where the top computed states are inexact
+        and nearly degenerate with the complement.)"""
         e = self.e
         basis = self.basis32[ik]
-        eps32 = eps.to(torch.float32)
         diag = (e.tkin64[ik] + self.vmean).to(torch.float32)
 
         def proj(V):
-            V = V.contiguous().clone()
-            V = self._sub(V, self._coef(V, basis), basis)
+            V = self._sub(V.contiguous(), self._coef(V.contiguous(), basis), basis)
             return e.realify(V) if self.real else V
 
-        r = proj(r0)
+        def rv(t):
+            return torch.view_as_real(t).reshape(t.shape[0], -1)
+
+        def rdot(a, b):
+            return (rv(a) * rv(b)).sum(dim=1)
+
+        r = proj(r0.clone())
+        n = r.shape[0]
+        out = torch.zeros_like(r)
+        idx = torch.arange(n, device=r.device)
+        ec = eps.to(torch.float32)
+        # 1 / (ABACUS preconditioner), per band and G (solver/davidson.py)
+        xg = torch.abs(diag[None, :] - ec[:, None])
+        minv = 1.0 / torch.clamp(0.5 * (1.0 + xg + torch.sqrt(1.0 + (xg - 1.0) ** 2)), min=1e-6)
+        del xg
+        stop2 = (tau.to(torch.float32) ** 2) * rdot(r, r)
         x = torch.zeros_like(r)
-        stop = tau.to(torch.float32) * torch.linalg.vector_norm(r, dim=1)
-        z = proj(precondition(r, eps, diag))
+        z = proj(r * minv)
         p = z.clone()
-        rz = torch.sum((r.conj() * z).real, dim=1)
-        active = torch.arange(r.shape[0], device=r.device)
+        rz = rdot(r, z)
         it = 0
         for it in range(1, self.cg_maxiter + 1):
-            pa = p[active]
-            Ap = proj(e.hpsi(ik, pa, self.vs32, fp32=True, real=self.real) - eps32[active, None] * pa)
-            self.stats["hpsi_bands_fp32"] += int(active.numel())
-            pAp = torch.sum((pa.conj() * Ap).real, dim=1)
-            alpha = rz[active] / pAp
-            x[active] += alpha[:, None] * pa
-            r[active] -= alpha[:, None] * Ap
-            rn = torch.linalg.vector_norm(r[active], dim=1)
-            keep = active[rn > stop[active]]
-            if keep.numel() == 0:
-                break
-            zk = proj(precondition(r[keep], eps[keep], diag))
-            rz_new = torch.sum((r[keep].conj() * zk).real, dim=1)
-            beta = rz_new / rz[keep]
-            p[keep] = zk + beta[:, None] * p[keep]
-            rz[keep] = rz_new
-            active = keep
+            Ap = e.hpsi(ik, p, self.vs32, fp32=True, real=self.real)
+            rv(Ap).addcmul_(rv(p), ec[:, None], value=-1.0)
+            Ap = proj(Ap)
+            self.stats["hpsi_bands_fp32"] += int(p.shape[0])
+            alpha = rz / rdot(p, Ap)
+            rv(x).addcmul_(rv(p), alpha[:, None])
+            rv(r).addcmul_(rv(Ap), alpha[:, None], value=-1.0)
+            done = rdot(r, r) <= stop2
+            nd = int(done.sum())
+            if nd:
+                out[idx[done]] = x[done]
+                if nd == x.shape[0]:
+                    break
+                keep = ~done
+                x, r, p, rz, ec, stop2, idx, minv = (x[keep], r[keep], p[keep], rz[keep], ec[keep], stop2[keep],
+                                                    idx[keep], minv[keep])
+            z = proj(r * minv)
+            rz_new = rdot(r, z)
+            beta = rz_new / rz
+            rv(p).mul_(beta[:, None]).add_(rv(z))
+            rz = rz_new
+        else:
+            out[idx] = x
         self.stats["cg_iterations"] += it
-        return x
+        return out
 
     def _combine(self, C, B):
         """C B for coefficient rows C (m, n_B) (real coefficients as a real
@@ -556,14 +580,14 @@ class Response:
             eps = self.eig[ik]
             f = self.occ[ik]
             Vpsi = e.restrict(ik, e.plan64.hpsi_local(psi, self.zero_t, vs))
-            Lam = inner(psi, Vpsi, conj_a=True)           # Lam[m, n] = <m|dV|n>
+            Lam = self._coef(Vpsi, psi).T                 # Lam[m, n] = <m|dV|n> (real at k = 0)
             Lam = 0.5 * (Lam + Lam.conj().T)
             de = Lam.diagonal().real
             d = eps[None, :] - eps[:, None]               # d[m, n] = e_n - e_m
             off = d.abs() > self.deg_tol
             coef = torch.where(off, Lam / torch.where(off, d, torch.ones_like(d)).to(Lam.dtype),
                                torch.zeros_like(Lam))
-            dpsi = torch.matmul(coef.T, psi)              # dpsi_n = sum_m coef[m, n] psi_m
+            dpsi = self._combine(coef.T.contiguous(), psi)  # dpsi_n = sum_m coef[m, n] psi_m
             occ_idx = torch.nonzero(f > self.fthr).flatten()
             if occ_idx.numel() > 0:
                 fo = f[occ_idx]
@@ -713,7 +737,7 @@ def dielectric_preconditioner(engine, q0sq=2.0):
 
 
 def delta_q_grad(engine, psis, eig, occ, rho_pl, weight_pl, rtol=1e-10, maxiter=200, precondition=True,
-                 method="gmres", log=None, relax=0.1, **resp_kw):
+                 method="gmres", log=None, relax=0.5, q0sq=2.0, **resp_kw):
     """dC/dR and dC/deps for C = (Omega/N) sum w n at the SCF fixed point n*
     (implicit.py:32-92): (I - J)^T lam = (Omega/N) w, then
     dC/dtheta = dC/dtheta|_n* + lam . dF/dtheta|_n*; the state term is 0 for
@@ -737,7 +761,7 @@ def delta_q_grad(engine, psis, eig, occ, rho_pl, weight_pl, rtol=1e-10, maxiter=
             log(f"adjoint product {resp.stats['chi0']}: cg iterations {resp.stats['cg_iterations']}")
         return out
 
-    pre = dielectric_preconditioner(engine) if precondition else None
+    pre = dielectric_preconditioner(engine, q0sq) if precondition else None
     base_tol = resp.cg_tol
     if method == "gmres":
         # inexact Krylov (Simoncini & Szyld, SIAM J. Sci. Comput. 25 (2003)
