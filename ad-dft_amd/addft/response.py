@@ -262,7 +262,7 @@ class Response:
         diag = e.tkin64[ik] + self.vmean
         k = W.shape[0]
         for step in range(self.defl_steps + 1):
-            HW = self._project(ik, e.hpsi(ik, W, self.vs))
+            HW = self._project(ik, e.hpsi(ik, W, self.vs, real=self.real))
             T = inner(W, HW, conj_a=True).to(torch.complex128)
             T = 0.5 * (T + T.conj().T)
             if self.real:
@@ -290,7 +290,7 @@ class Response:
             keep = w > 1e-10 * float(w.max())
             Z = torch.matmul((V[:, keep] / torch.sqrt(w[keep])[None, :]).T.to(Z.dtype), Z)
             B = torch.cat([W, Z]).contiguous()
-            HB = self._project(ik, e.hpsi(ik, B, self.vs))
+            HB = self._project(ik, e.hpsi(ik, B, self.vs, real=self.real))
             T = inner(B, HB, conj_a=True).to(torch.complex128)
             T = 0.5 * (T + T.conj().T)
             if self.real:
@@ -371,16 +371,21 @@ class Response:
         range(Q)), band n stopping at |r_n| <= goal_n."""
         e = self.e
         diag = e.tkin64[ik] + self.vmean
+        # rows real in real space stay real only if kept so: the paired
+        # transforms and the real projection coefficients act on the real
+        # part, and rounding would otherwise leave an unprojected imaginary
+        # part to grow
+        keep_real = e.realify if self.real else (lambda t: t)
         x = torch.zeros_like(b)
-        r = b.clone()
-        z = self._project(ik, precondition(r, eps, diag))
+        r = keep_real(b.clone())
+        z = keep_real(self._project(ik, precondition(r, eps, diag)))
         p = z.clone()
         rz = torch.sum((r.conj() * z).real, dim=1)
         active = torch.arange(b.shape[0], device=b.device)
         it = 0
         for it in range(1, self.cg_maxiter + 1):
             pa = p[active]
-            Ap = self._project(ik, e.hpsi(ik, pa, self.vs) - eps[active, None].to(pa.real.dtype) * pa)
+            Ap = self._project(ik, e.hpsi(ik, pa, self.vs, real=self.real) - eps[active, None].to(pa.real.dtype) * pa)
             self.stats["hpsi_bands"] += int(active.numel())
             pAp = torch.sum((pa.conj() * Ap).real, dim=1)
             alpha = rz[active] / pAp
@@ -390,7 +395,7 @@ class Response:
             keep = active[rn > goal[active]]
             if keep.numel() == 0:
                 break
-            zk = self._project(ik, precondition(r[keep], eps[keep], diag))
+            zk = keep_real(self._project(ik, precondition(r[keep], eps[keep], diag)))
             rz_new = torch.sum((r[keep].conj() * zk).real, dim=1)
             beta = rz_new / rz[keep]
             p[keep] = zk + beta[:, None] * p[keep]
@@ -402,8 +407,11 @@ class Response:
     def _apply64(self, ik, X, eps):
         """Q (H - e_n) Q x_n in FP64 (rows)."""
         Xq = self._project(ik, X)
+        if self.real:
+            Xq = self.e.realify(Xq)
         self.stats["hpsi_bands"] += int(X.shape[0])
-        return self._project(ik, self.e.hpsi(ik, Xq, self.vs) - eps[:, None].to(Xq.real.dtype) * Xq), Xq
+        return self._project(ik, self.e.hpsi(ik, Xq, self.vs, real=self.real)
+                             - eps[:, None].to(Xq.real.dtype) * Xq), Xq
 
     def _sternheimer_mixed(self, ik, Vpsi, eps, weight=None, max_outer=12):
         """Flexible CG in FP64 (Notay, SIAM J. Sci. Comput. 22 (2000) 1444,
@@ -416,6 +424,8 @@ class Response:
         rule whatever the inner accuracy."""
         e = self.e
         b = -self._project(ik, Vpsi)
+        if self.real:
+            b = e.realify(b)
         bn = torch.linalg.vector_norm(b, dim=1)
         if weight is not None:
             bn = bn / torch.clamp(weight.to(bn.dtype), min=1e-300)
@@ -579,7 +589,7 @@ class Response:
             psi = self.psi[ik]
             eps = self.eig[ik]
             f = self.occ[ik]
-            Vpsi = e.restrict(ik, e.plan64.hpsi_local(psi, self.zero_t, vs))
+            Vpsi = e.restrict(ik, self._local64(psi, vs))
             Lam = self._coef(Vpsi, psi).T                 # Lam[m, n] = <m|dV|n> (real at k = 0)
             Lam = 0.5 * (Lam + Lam.conj().T)
             de = Lam.diagonal().real
@@ -603,7 +613,7 @@ class Response:
         for ik in range(e.nk):
             if st.sigma > 1e-6:
                 c = self.kw[ik] / e.omega
-                out += e.plan64.density(self.psi[ik], (c * dfs[ik]).contiguous())
+                out += self._density64(self.psi[ik], (c * dfs[ik]).contiguous())
         if keep:
             self.kept = list(zip(kept, dfs))
         return out
@@ -635,7 +645,38 @@ class Response:
         a = psi + t[:, None].to(psi.dtype) * phi
         b = psi - t[:, None].to(psi.dtype) * phi
         wt = (w / (2.0 * t)).contiguous()
-        return e.plan64.density(a, wt) - e.plan64.density(b, wt)
+        return self._density64(a, wt) - self._density64(b, wt)
+
+    def _local64(self, psi, vs):
+        """V psi (FP64, no kinetic term); two bands per transform for rows
+        real in real space."""
+        e = self.e
+        if not self.real:
+            return e.plan64.hpsi_local(psi, self.zero_t, vs)
+        h = e.plan64.hpsi_local(native.gamma_pack2(psi.contiguous()), self.zero_t, vs)
+        return native.gamma_unpack2(h, psi.shape[0], e.gplus_d, e.gminus_d)
+
+    def _density64(self, psi, w):
+        """sum_n w_n |psi_n(r)|^2 (plane-major FP64).  Rows real in real
+        space: sqrt(|w_n|) psi_n packed in pairs a + i b, whose |.|^2 is
+        |w_a| a^2 + |w_b| b^2, one transform per pair (weights of each sign
+        separately)."""
+        e = self.e
+        if not self.real:
+            return e.plan64.density(psi, w)
+        out = torch.zeros(e.nfft, dtype=torch.float64, device=psi.device)
+        for sign in (1.0, -1.0):
+            idx = torch.nonzero(w * sign > 0).flatten()
+            if idx.numel() == 0:
+                continue
+            a = psi[idx] * torch.sqrt(w[idx].abs()).to(psi.real.dtype)[:, None]
+            pk = native.gamma_pack2(a.contiguous())
+            ones = torch.ones(pk.shape[0], dtype=torch.float64, device=psi.device)
+            if sign > 0:
+                out += e.plan64.density(pk, ones)
+            else:
+                out -= e.plan64.density(pk, ones)
+        return out
 
     # ------------------------------------------------------------ J, J^T
     def J(self, dn):
