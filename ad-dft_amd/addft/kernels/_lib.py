@@ -55,6 +55,9 @@ _SIGS = {
     "addft_xc_work_bytes": ([_I64], _I64),
     "addft_xc_lda": ([_VP, _I64, _VP, _VP, _VP, _VP], _I32),
     "addft_xc_lda_fxc": ([_VP, _VP, _I64, _VP, _VP], _I32),
+    "addft_cg_shift_dot": ([_I64, _I64, _VP, _VP, _VP, _VP, _VP], _I32),
+    "addft_cg_update": ([_I64, _I64, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP], _I32),
+    "addft_cg_direction": ([_I64, _I64, _VP, _VP, _VP, _VP, _VP], _I32),
     "addft_xc_gga_model": ([_VP, _VP, _I64, _D, _VP, _VP, _VP, _VP, _VP], _I32),
     "addft_sphere_plan_create": ([_I32, _I32, _I32, _I64, _VP, _I32, ctypes.POINTER(_VP)], _I32),
     "addft_sphere_plan_destroy": ([_VP], _I32),

@@ -93,6 +93,48 @@ def xc_gga_model(rho: torch.Tensor, sigma: torch.Tensor, mu: float):
     return fn, fs, exc
 
 
+# ------------------------------------------------ batched CG (Sternheimer)
+def _cg_rows(t, what):
+    _require(t, torch.complex64, what=what)
+    if t.dim() != 2:
+        raise ValueError(f"{what} must be (rows, n)")
+    return t.shape
+
+
+def cg_shift_dot(ap: torch.Tensor, p: torch.Tensor, eps: torch.Tensor) -> torch.Tensor:
+    """Ap -= eps_b p (in place); returns Re<p_b, Ap_b> per row (float32)."""
+    m, n = _cg_rows(ap, "ap")
+    _require(p, torch.complex64, numel=m * n, what="p")
+    _require(eps, torch.float32, numel=m, what="eps")
+    pap = torch.empty(m, dtype=torch.float32, device=ap.device)
+    check(lib().addft_cg_shift_dot(m, n, ptr(ap), ptr(p), ptr(eps), ptr(pap), stream()), "addft_cg_shift_dot")
+    return pap
+
+
+def cg_update(x, r, p, ap, rz, pap, minv, z) -> torch.Tensor:
+    """a = rz/pAp; x += a p; r -= a Ap; z = minv r (all in place); returns
+    |r_b|^2 per row."""
+    m, n = _cg_rows(x, "x")
+    for t, what in ((r, "r"), (p, "p"), (ap, "ap"), (z, "z")):
+        _require(t, torch.complex64, numel=m * n, what=what)
+    _require(minv, torch.float32, numel=m * n, what="minv")
+    _require(rz, torch.float32, numel=m, what="rz")
+    _require(pap, torch.float32, numel=m, what="pap")
+    rr = torch.empty(m, dtype=torch.float32, device=x.device)
+    check(lib().addft_cg_update(m, n, ptr(x), ptr(r), ptr(p), ptr(ap), ptr(rz), ptr(pap), ptr(minv), ptr(z),
+                                ptr(rr), stream()), "addft_cg_update")
+    return rr
+
+
+def cg_direction(r, z, p, rz) -> None:
+    """rz' = Re<r, z>; p = z + (rz'/rz) p; rz = rz' (p and rz in place)."""
+    m, n = _cg_rows(p, "p")
+    _require(r, torch.complex64, numel=m * n, what="r")
+    _require(z, torch.complex64, numel=m * n, what="z")
+    _require(rz, torch.float32, numel=m, what="rz")
+    check(lib().addft_cg_direction(m, n, ptr(r), ptr(z), ptr(p), ptr(rz), stream()), "addft_cg_direction")
+
+
 def xc_fxc(rho: torch.Tensor, w: torch.Tensor | None = None) -> torch.Tensor:
     """f_xc (or f_xc * w) (reference xc.py:181-191, 205-208)."""
     if w is not None:

@@ -155,6 +155,36 @@ def gmres(apply, b, rtol=1e-10, maxiter=200, restart=60, precond=None, on_residu
     return x, total
 
 
+class _HalfSphere:
+    """Rows real in real space (v(-G) = conj v(G)) on the half sphere: the
+    G of the pair table's positive half (entry 0: G = 0), scaled by sqrt(2)
+    off G = 0, so that Re sum over the half of conj(a) b is the full-sphere
+    <a|b>."""
+
+    def __init__(self, plus, minus, ng):
+        self.ng = int(ng)
+        self.p = plus.long()
+        self.p1 = plus[1:].long()
+        self.m1 = minus[1:].long()
+        nh = int(plus.numel())
+        s = torch.full((nh,), math.sqrt(2.0), dtype=torch.float32, device=plus.device)
+        s[0] = 1.0
+        self.s = s
+        self.inv_s = 1.0 / s
+
+    def to_half(self, F):
+        """(m, nG) -> (m, nh) complex64 scaled half-sphere rows."""
+        return (F[:, self.p].to(torch.complex64) * self.s).contiguous()
+
+    def to_full(self, S):
+        """(m, nh) scaled half rows -> (m, nG) complex64 full-sphere rows."""
+        t = S * self.inv_s
+        out = torch.empty((S.shape[0], self.ng), dtype=torch.complex64, device=S.device)
+        out[:, self.p] = t
+        out[:, self.m1] = t[:, 1:].conj()
+        return out
+
+
 class Response:
     """chi0, K and J at a converged state (stick-order psi per k, complex128)."""
 
@@ -211,6 +241,8 @@ class Response:
             # the FP32 solves project out the computed states and W together
             self.basis32 = [torch.cat([self.psi32[ik]] + ([self.defl[ik]["W32"]] if self.defl[ik] else []))
                             .contiguous() for ik in range(engine.nk)]
+            self.half = _HalfSphere(engine.gplus_d, engine.gminus_d, engine.ng) if self.real else None
+            self.basis32h = ([self.half.to_half(b) for b in self.basis32] if self.half is not None else None)
 
     # --------------------------------------------------------------- K
     def K(self, dn):
@@ -500,66 +532,73 @@ class Response:
         bands at once; band n stops at |r_n| <= tau_n |Q' r0_n|.  The
         working arrays hold the unconverged bands only (compacted when bands
         converge), the diagonal preconditioner is tabulated once per solve,
-        and the per-band dot products and updates run on the real views.
-        (Lifting the computed space by a shift instead of projecting it out,
-        the alpha P_v device of perturbation theory, saves one projection
-        per iteration but stalls where the top computed states are inexact
-        and nearly degenerate with the complement.)"""
+        and each iteration is three fused kernels (addft_cg_*) besides H and
+        the projections.  Rows real in real space are held on the half
+        sphere (G and -G carry conjugate values), scaled by sqrt(2) off
+        G = 0 so that plain dot products over the half give the full-sphere
+        ones: half the projection flops and vector traffic.  (Lifting the
+        computed space by a shift instead of projecting it out, the alpha
+        P_v device of perturbation theory, saves one projection per
+        iteration but stalls where the top computed states are inexact and
+        nearly degenerate with the complement.)"""
         e = self.e
-        basis = self.basis32[ik]
+        half = self.half if self.real else None
+        basis = self.basis32h[ik] if half is not None else self.basis32[ik]
         diag = (e.tkin64[ik] + self.vmean).to(torch.float32)
+        if half is not None:
+            diag = diag[half.p]
 
         def proj(V):
             V = self._sub(V.contiguous(), self._coef(V.contiguous(), basis), basis)
-            return e.realify(V) if self.real else V
+            return e.realify(V) if (self.real and half is None) else V
 
-        def rv(t):
-            return torch.view_as_real(t).reshape(t.shape[0], -1)
+        def apply_h(P):
+            if half is None:
+                return e.hpsi(ik, P, self.vs32, fp32=True, real=self.real)
+            return half.to_half(e.hpsi(ik, half.to_full(P), self.vs32, fp32=True, real=True))
 
         def rdot(a, b):
-            return (rv(a) * rv(b)).sum(dim=1)
+            ra, rb = torch.view_as_real(a).reshape(a.shape[0], -1), torch.view_as_real(b).reshape(b.shape[0], -1)
+            return (ra * rb).sum(dim=1)
 
-        r = proj(r0.clone())
+        r = proj(half.to_half(r0) if half is not None else r0.clone())
         n = r.shape[0]
         out = torch.zeros_like(r)
         idx = torch.arange(n, device=r.device)
-        ec = eps.to(torch.float32)
+        ec = eps.to(torch.float32).contiguous()
         # 1 / (ABACUS preconditioner), per band and G (solver/davidson.py)
         xg = torch.abs(diag[None, :] - ec[:, None])
-        minv = 1.0 / torch.clamp(0.5 * (1.0 + xg + torch.sqrt(1.0 + (xg - 1.0) ** 2)), min=1e-6)
+        minv = (1.0 / torch.clamp(0.5 * (1.0 + xg + torch.sqrt(1.0 + (xg - 1.0) ** 2)), min=1e-6)).contiguous()
         del xg
         stop2 = (tau.to(torch.float32) ** 2) * rdot(r, r)
         x = torch.zeros_like(r)
         z = proj(r * minv)
         p = z.clone()
-        rz = rdot(r, z)
+        rz = rdot(r, z).contiguous()
         it = 0
         for it in range(1, self.cg_maxiter + 1):
-            Ap = e.hpsi(ik, p, self.vs32, fp32=True, real=self.real)
-            rv(Ap).addcmul_(rv(p), ec[:, None], value=-1.0)
+            Ap = apply_h(p)
+            # Ap -= e p and <p, (H - e) p>, which equals <p, Q'(H - e) p> for p in range(Q')
+            pap = native.cg_shift_dot(Ap, p, ec)
             Ap = proj(Ap)
             self.stats["hpsi_bands_fp32"] += int(p.shape[0])
-            alpha = rz / rdot(p, Ap)
-            rv(x).addcmul_(rv(p), alpha[:, None])
-            rv(r).addcmul_(rv(Ap), alpha[:, None], value=-1.0)
-            done = rdot(r, r) <= stop2
+            z = torch.empty_like(r)
+            rr = native.cg_update(x, r, p, Ap, rz, pap, minv, z)
+            done = rr <= stop2
             nd = int(done.sum())
             if nd:
                 out[idx[done]] = x[done]
                 if nd == x.shape[0]:
                     break
                 keep = ~done
-                x, r, p, rz, ec, stop2, idx, minv = (x[keep], r[keep], p[keep], rz[keep], ec[keep], stop2[keep],
-                                                    idx[keep], minv[keep])
-            z = proj(r * minv)
-            rz_new = rdot(r, z)
-            beta = rz_new / rz
-            rv(p).mul_(beta[:, None]).add_(rv(z))
-            rz = rz_new
+                x, r, p, rz, ec, stop2, idx, minv, z = (x[keep], r[keep], p[keep], rz[keep], ec[keep], stop2[keep],
+                                                       idx[keep], minv[keep], z[keep])
+            z = proj(z)
+            native.cg_direction(r, z, p, rz)
         else:
             out[idx] = x
         self.stats["cg_iterations"] += it
-        return out
+        return half.to_full(out) if half is not None else out
 
     def _combine(self, C, B):
         """C B for coefficient rows C (m, n_B) (real coefficients as a real
@@ -765,20 +804,28 @@ def delta_q_value(engine, rho_pl, weight_pl):
     return engine.omega / engine.nfft * float(torch.dot(weight_pl, rho_pl))
 
 
-def dielectric_preconditioner(engine, q0sq=2.0):
-    """Approximate inverse of I - K chi0 (a metal-like dielectric,
-    1 + q0^2/G^2): the Kerker filter G^2 / (G^2 + q0^2) on G != 0, identity on
-    the mean (G = 0)."""
+def dielectric_preconditioner(engine, q0sq=2.0, eps_inf=None):
+    """Approximate inverse of I - K chi0.  Metal-like dielectric
+    1 + q0^2/G^2 (eps_inf None): the Kerker filter G^2 / (G^2 + q0^2) on
+    G != 0.  With a finite eps_inf (a semiconductor, whose dielectric function
+    tends to eps_inf at long wavelength): (G^2 + q0^2/eps_inf) / (G^2 + q0^2)
+    = 1/eps_inf + (1 - 1/eps_inf) G^2/(G^2 + q0^2).  Identity on the mean
+    (G = 0)."""
     zero = torch.zeros(engine.nfft, dtype=torch.float64, device=engine.dev)
+    c = 0.0 if eps_inf is None else 1.0 / float(eps_inf)
 
     def apply(y):
-        return engine.kerker(zero, y, 1.0, q0sq) + y.mean()
+        m = y.mean()
+        k = engine.kerker(zero, y, 1.0, q0sq)
+        if c:
+            k = (1.0 - c) * k + c * (y - m)
+        return k + m
 
     return apply
 
 
 def delta_q_grad(engine, psis, eig, occ, rho_pl, weight_pl, rtol=1e-10, maxiter=200, precondition=True,
-                 method="gmres", log=None, relax=0.5, q0sq=2.0, **resp_kw):
+                 method="gmres", log=None, relax=0.5, q0sq=2.0, eps_inf=None, **resp_kw):
     """dC/dR and dC/deps for C = (Omega/N) sum w n at the SCF fixed point n*
     (implicit.py:32-92): (I - J)^T lam = (Omega/N) w, then
     dC/dtheta = dC/dtheta|_n* + lam . dF/dtheta|_n*; the state term is 0 for
@@ -802,7 +849,7 @@ def delta_q_grad(engine, psis, eig, occ, rho_pl, weight_pl, rtol=1e-10, maxiter=
             log(f"adjoint product {resp.stats['chi0']}: cg iterations {resp.stats['cg_iterations']}")
         return out
 
-    pre = dielectric_preconditioner(engine, q0sq) if precondition else None
+    pre = dielectric_preconditioner(engine, q0sq, eps_inf) if precondition else None
     base_tol = resp.cg_tol
     if method == "gmres":
         # inexact Krylov (Simoncini & Szyld, SIAM J. Sci. Comput. 25 (2003)

@@ -190,6 +190,23 @@ int addft_cheb_combine(int dtype, int64_t nblk, int64_t n, const double* d_coef,
                        double alpha, double beta, double gamma, const void* d_a,
                        const void* d_b, const void* d_c, void* d_out, void* stream);
 
+/* ------------------------------------------- batched CG (density response)
+ * Fused steps of the FP32 preconditioned CG of the Sternheimer solves
+ * (reference physics/response.py:33-121 builds the same response from the
+ * full eigensystem; these entries have no counterpart there).  m rows of n
+ * complex64 numbers, contiguous; per-row scalars float32.
+ *   shift_dot:  Ap -= eps_b p;  pAp_b = Re<p, Ap>
+ *   update:     a = rz_b / pAp_b; x += a p; r -= a Ap; rr_b = |r|^2;
+ *               z = minv .* r (minv: m x n float32)
+ *   direction:  rz'_b = Re<r, z>; p = z + (rz'_b / rz_b) p; rz_b = rz'_b   */
+int addft_cg_shift_dot(int64_t m, int64_t n, void* d_ap, const void* d_p, const float* d_eps,
+                       float* d_pap, void* stream);
+int addft_cg_update(int64_t m, int64_t n, void* d_x, void* d_r, const void* d_p, const void* d_ap,
+                    const float* d_rz, const float* d_pap, const float* d_minv, void* d_z,
+                    float* d_rr, void* stream);
+int addft_cg_direction(int64_t m, int64_t n, const void* d_r, const void* d_z, void* d_p,
+                       float* d_rz, void* stream);
+
 /* ------------------------------------------------------------ BF16x3 split
  * FP32 rows x (rows, k) -> bf16 rows [bf16(x) | bf16(x) | bf16(x - bf16(x))]
  * (rows, ld), round to nearest even; columns [3k, ld) untouched.  The
