@@ -394,10 +394,10 @@ class Engine:
             out = plan.hpsi_local(psi, tk, vs)
         if fp32 and nl_bf16 and self.PB[ik] is not None:
             pb = self.PB[ik]
-            pb.apply_add(self.dmul(pb.becp_t(psi)), out)
+            pb.apply_add(self.dmul(pb.becp_t(psi, real=real)), out, real=real)
         elif not fp32 and self.PG64[ik] is not None:
             g = self.PG64[ik]                           # k = 0: half-sphere FP64 products
-            g.apply_add(self.dmul(g.becp_t(psi)), out)
+            g.apply_add(self.dmul(g.becp_t(psi, real=real)), out, real=real)
         else:
             becp_t = inner(psi, P)                      # (nb, nproj) = <beta|psi>^T
             torch.addmm(out, self.dmul(becp_t), P, out=out)   # accumulated by the GEMM (beta = 1)

@@ -49,6 +49,7 @@ _SIGS = {
     "addft_gamma_pair_split": ([_VP, _I64, _I64, _VP, _VP, _I64, _I64, _VP, _VP], _I32),
     "addft_gamma_pair_rows": ([_VP, _I64, _I64, _VP, _VP, _I64, _VP, _VP], _I32),
     "addft_gamma_scatter_add": ([_I32, _VP, _I64, _I64, _VP, _VP, _I64, _VP, _VP], _I32),
+    "addft_gamma_scatter_add_real": ([_I32, _VP, _I64, _I64, _VP, _VP, _I64, _VP, _VP], _I32),
     "addft_gamma_pack2": ([_I32, _VP, _I64, _I64, _VP, _VP], _I32),
     "addft_gamma_unpack2": ([_I32, _VP, _I64, _I64, _VP, _VP, _I64, _VP, _VP], _I32),
     "addft_pack_gather": ([_I32, _VP, _VP, _I64, _I64, _I64, _VP, _VP], _I32),

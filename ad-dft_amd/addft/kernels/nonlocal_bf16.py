@@ -90,8 +90,9 @@ class ProjectorBF16x3:
                    "addft_split_bf16x3")
         return B
 
-    def becp_t(self, psi: torch.Tensor) -> torch.Tensor:
-        """psi (nb, nG) complex64 -> psi P^H (nb, nproj) complex64."""
+    def becp_t(self, psi: torch.Tensor, real: bool = False) -> torch.Tensor:
+        """psi (nb, nG) complex64 -> psi P^H (nb, nproj) complex64 (`real`
+        accepted for the common interface, no effect)."""
         nb = psi.shape[0]
         X = torch.view_as_real(psi).reshape(nb, self.K)
         B = self._split_rows(X)
@@ -108,7 +109,7 @@ class ProjectorBF16x3:
                                                  _lib.stream()), "addft_split_bf16x3")
         return Sc
 
-    def apply_add(self, S: torch.Tensor, out: torch.Tensor) -> torch.Tensor:
+    def apply_add(self, S: torch.Tensor, out: torch.Tensor, real: bool = False) -> torch.Tensor:
         """out (nb, nG) complex64 += S P, accumulated by the GEMM itself
         (beta = 1 on the FP32 view of out: no separate product buffer)."""
         o = torch.view_as_real(out).view(S.shape[0], 2 * self.ng)
@@ -197,8 +198,10 @@ class GammaProjectorBF16x3:
         self.V3 = V3
         self._buf = None
 
-    def becp_t(self, psi: torch.Tensor) -> torch.Tensor:
-        """psi (nb, nG) complex64 -> psi P'^H (nb, nproj) complex64."""
+    def becp_t(self, psi: torch.Tensor, real: bool = False) -> torch.Tensor:
+        """psi (nb, nG) complex64 -> psi P'^H (nb, nproj) complex64.
+        real: rows real in real space, whose <P'|psi> are real -- only the
+        Re-part rows enter the GEMM (half its rows)."""
         nb = psi.shape[0]
         n = self.kc * self.nslice
         if self._buf is None or self._buf.shape[0] < 2 * nb:
@@ -208,25 +211,31 @@ class GammaProjectorBF16x3:
         _lib.check(_lib.lib().addft_gamma_pair_split(_lib.ptr(psi), nb, self.ng, _lib.ptr(self.plus),
                                                      _lib.ptr(self.minus), self.nh, n, _lib.ptr(B),
                                                      _lib.stream()), "addft_gamma_pair_split")
-        Xb = B.view(2 * nb, self.nslice, self.kc).transpose(0, 1)
+        rows = nb if real else 2 * nb
+        Xb = B[:rows].view(rows, self.nslice, self.kc).transpose(0, 1)
         C = torch.bmm(Xb, self.W3b, out_dtype=torch.float32).sum(dim=0)
+        if real:
+            return torch.complex(C, torch.zeros_like(C))
         return torch.complex(C[:nb], C[nb:])
 
-    def apply_add(self, S: torch.Tensor, out: torch.Tensor) -> torch.Tensor:
-        """out (nb, nG) complex64 += S' P'."""
+    def apply_add(self, S: torch.Tensor, out: torch.Tensor, real: bool = False) -> torch.Tensor:
+        """out (nb, nG) complex64 += S' P'.  real: S' real (rows real in real
+        space), the Im-part rows skipped."""
         if not out.is_contiguous():
             raise ValueError("out must be contiguous")
         nb = S.shape[0]
-        Sr = torch.zeros((2 * nb, self.kp), dtype=torch.float32, device=S.device)
+        rows = nb if real else 2 * nb
+        Sr = torch.zeros((rows, self.kp), dtype=torch.float32, device=S.device)
         Sr[:nb, :self.np] = S.real
-        Sr[nb:, :self.np] = S.imag
-        S3 = torch.empty((2 * nb, 3 * self.kp), dtype=torch.bfloat16, device=S.device)
-        _lib.check(_lib.lib().addft_split_bf16x3(_lib.ptr(Sr), 2 * nb, self.kp, 3 * self.kp, _lib.ptr(S3),
+        if not real:
+            Sr[nb:, :self.np] = S.imag
+        S3 = torch.empty((rows, 3 * self.kp), dtype=torch.bfloat16, device=S.device)
+        _lib.check(_lib.lib().addft_split_bf16x3(_lib.ptr(Sr), rows, self.kp, 3 * self.kp, _lib.ptr(S3),
                                                  _lib.stream()), "addft_split_bf16x3")
         U = torch.mm(S3, self.V3, out_dtype=torch.float32)
-        _lib.check(_lib.lib().addft_gamma_scatter_add(_lib.C64, _lib.ptr(U), nb, self.nh, _lib.ptr(self.plus),
-                                                      _lib.ptr(self.minus), self.ng, _lib.ptr(out),
-                                                      _lib.stream()), "addft_gamma_scatter_add")
+        fn = _lib.lib().addft_gamma_scatter_add_real if real else _lib.lib().addft_gamma_scatter_add
+        _lib.check(fn(_lib.C64, _lib.ptr(U), nb, self.nh, _lib.ptr(self.plus), _lib.ptr(self.minus), self.ng,
+                      _lib.ptr(out), _lib.stream()), "addft_gamma_scatter_add")
         return out
 
 
@@ -253,8 +262,9 @@ class GammaProjector64:
         Qi[:, 0] = 0.0
         self.WT = torch.cat([Qr, Qi], dim=1).contiguous()     # (np, 2 nh): W^T and V at once
 
-    def becp_t(self, psi: torch.Tensor) -> torch.Tensor:
-        """psi (nb, nG) complex128 -> psi P'^H (nb, nproj) complex128."""
+    def becp_t(self, psi: torch.Tensor, real: bool = False) -> torch.Tensor:
+        """psi (nb, nG) complex128 -> psi P'^H (nb, nproj) complex128.  real:
+        rows real in real space (real <P'|psi>), the Re-part rows only."""
         from ..solver.linalg import inner
         nb = psi.shape[0]
         A = torch.empty((2 * nb, 2 * self.nh), dtype=torch.float64, device=psi.device)
@@ -262,19 +272,22 @@ class GammaProjector64:
         _lib.check(_lib.lib().addft_gamma_pair_rows(_lib.ptr(psi), nb, self.ng, _lib.ptr(self.plus),
                                                     _lib.ptr(self.minus), self.nh, _lib.ptr(A), _lib.stream()),
                    "addft_gamma_pair_rows")
+        if real:
+            C = inner(A[:nb], self.WT)
+            return torch.complex(C, torch.zeros_like(C))
         C = inner(A, self.WT)                                   # A W, the reduction split over slices
         return torch.complex(C[:nb], C[nb:])
 
-    def apply_add(self, S: torch.Tensor, out: torch.Tensor) -> torch.Tensor:
-        """out (nb, nG) complex128 += S' P'."""
+    def apply_add(self, S: torch.Tensor, out: torch.Tensor, real: bool = False) -> torch.Tensor:
+        """out (nb, nG) complex128 += S' P' (real: S' real)."""
         if not out.is_contiguous():
             raise ValueError("out must be contiguous")
         nb = S.shape[0]
-        Sr = torch.cat([S.real, S.imag], dim=0).contiguous()
+        Sr = S.real.contiguous() if real else torch.cat([S.real, S.imag], dim=0).contiguous()
         U = torch.matmul(Sr, self.WT)
-        _lib.check(_lib.lib().addft_gamma_scatter_add(_lib.C128, _lib.ptr(U), nb, self.nh, _lib.ptr(self.plus),
-                                                      _lib.ptr(self.minus), self.ng, _lib.ptr(out),
-                                                      _lib.stream()), "addft_gamma_scatter_add")
+        fn = _lib.lib().addft_gamma_scatter_add_real if real else _lib.lib().addft_gamma_scatter_add
+        _lib.check(fn(_lib.C128, _lib.ptr(U), nb, self.nh, _lib.ptr(self.plus), _lib.ptr(self.minus), self.ng,
+                      _lib.ptr(out), _lib.stream()), "addft_gamma_scatter_add")
         return out
 
 
@@ -321,8 +334,9 @@ class ProjectorMFMA:
             self._work = torch.empty(max(need, 1), dtype=torch.uint8, device="cuda")
         return self._work
 
-    def becp_t(self, psi: torch.Tensor) -> torch.Tensor:
-        """psi (nb, nG) complex64 -> psi P^H (nb, nproj) complex64 (P' at k = 0)."""
+    def becp_t(self, psi: torch.Tensor, real: bool = False) -> torch.Tensor:
+        """psi (nb, nG) complex64 -> psi P^H (nb, nproj) complex64 (P' at k = 0;
+        `real` is accepted for the common interface and changes nothing)."""
         if psi.dtype != torch.complex64 or psi.dim() != 2 or psi.shape[1] != self.ng:
             raise ValueError("psi must be (nb, nG) complex64")
         psi = psi.contiguous()
@@ -346,7 +360,7 @@ class ProjectorMFMA:
                    "addft_nonlocal_becp_noalloc")
         return out
 
-    def apply_add(self, S: torch.Tensor, out: torch.Tensor) -> torch.Tensor:
+    def apply_add(self, S: torch.Tensor, out: torch.Tensor, real: bool = False) -> torch.Tensor:
         """out (nb, nG) complex64 += S P (S' P' at k = 0)."""
         if S.dtype != torch.complex64 or out.dtype != torch.complex64:
             raise ValueError("S and out must be complex64")

@@ -76,14 +76,16 @@ __global__ void k_gamma_pair_split(const float2* __restrict__ psi, int64_t ng, c
 template <typename R>
 __global__ void k_gamma_scatter_add(const R* __restrict__ U, int64_t nb, int64_t nh,
                                     const int32_t* __restrict__ plus, const int32_t* __restrict__ minus, int64_t ng,
-                                    addft::cpx<R>* __restrict__ out) {
+                                    addft::cpx<R>* __restrict__ out, bool real_rows) {
   const int64_t b = blockIdx.y;
   const R* ur_row = U + b * 2 * nh;
-  const R* ui_row = U + (nb + b) * 2 * nh;
+  // real S' (rows real in real space): U holds only the Re-part rows, the
+  // Im-part rows are zero and not stored
+  const R* ui_row = real_rows ? nullptr : U + (nb + b) * 2 * nh;
   addft::cpx<R>* row = out + b * ng;
   for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < nh; j += (int64_t)gridDim.x * blockDim.x) {
     const R ur = ur_row[j], vr = ur_row[nh + j];
-    const R ui = ui_row[j], vi = ui_row[nh + j];
+    const R ui = real_rows ? R(0) : ui_row[j], vi = real_rows ? R(0) : ui_row[nh + j];
     addft::cpx<R> p = row[plus[j]];
     p.x += ur - vi;
     p.y += ui + vr;
@@ -210,8 +212,8 @@ int addft_gamma_pair_split(const void* d_psi, int64_t nb, int64_t ng, const int3
   return ADDFT_OK;
 }
 
-int addft_gamma_scatter_add(int dtype, const void* d_u, int64_t nb, int64_t nh, const int32_t* d_plus,
-                            const int32_t* d_minus, int64_t ng, void* d_out, void* stream) {
+static int gamma_scatter_add(int dtype, const void* d_u, int64_t nb, int64_t nh, const int32_t* d_plus,
+                             const int32_t* d_minus, int64_t ng, void* d_out, void* stream, bool real_rows) {
   ADDFT_CHECK_ARG(dtype == ADDFT_C64 || dtype == ADDFT_C128, "dtype must be ADDFT_C64 or ADDFT_C128");
   ADDFT_CHECK_ARG(nb >= 0 && nb <= 32767 && nh >= 0 && 2 * nh <= ng + 1, "bad sizes (nb <= 32767, 2 nh <= nG + 1)");
   if (nb == 0 || nh == 0) return ADDFT_OK;
@@ -222,13 +224,23 @@ int addft_gamma_scatter_add(int dtype, const void* d_u, int64_t nb, int64_t nh, 
   if (dtype == ADDFT_C64)
     hipLaunchKernelGGL(k_gamma_scatter_add<float>, dim3((unsigned)gx, (unsigned)nb), dim3(256), 0, st,
                        static_cast<const float*>(d_u), nb, nh, d_plus, d_minus, ng,
-                       static_cast<addft::cpx<float>*>(d_out));
+                       static_cast<addft::cpx<float>*>(d_out), real_rows);
   else
     hipLaunchKernelGGL(k_gamma_scatter_add<double>, dim3((unsigned)gx, (unsigned)nb), dim3(256), 0, st,
                        static_cast<const double*>(d_u), nb, nh, d_plus, d_minus, ng,
-                       static_cast<addft::cpx<double>*>(d_out));
+                       static_cast<addft::cpx<double>*>(d_out), real_rows);
   ADDFT_LAUNCH_CHECK();
   return ADDFT_OK;
+}
+
+int addft_gamma_scatter_add(int dtype, const void* d_u, int64_t nb, int64_t nh, const int32_t* d_plus,
+                            const int32_t* d_minus, int64_t ng, void* d_out, void* stream) {
+  return gamma_scatter_add(dtype, d_u, nb, nh, d_plus, d_minus, ng, d_out, stream, false);
+}
+
+int addft_gamma_scatter_add_real(int dtype, const void* d_u, int64_t nb, int64_t nh, const int32_t* d_plus,
+                                 const int32_t* d_minus, int64_t ng, void* d_out, void* stream) {
+  return gamma_scatter_add(dtype, d_u, nb, nh, d_plus, d_minus, ng, d_out, stream, true);
 }
 
 int addft_gamma_pair_rows(const void* d_psi, int64_t nb, int64_t ng, const int32_t* d_plus, const int32_t* d_minus,

@@ -86,6 +86,9 @@ int addft_pack_gather(int dtype, const void* d_grid, const int32_t* d_idx,
  * addft_gamma_scatter_add: U (2 nb, 2 nh) = [S'.re; S'.im] [Qr | Qi]
  *   (FP32 for dtype ADDFT_C64, FP64 for ADDFT_C128) -> out (nb, ng) of that
  *   complex dtype += S' P' unfolded onto both halves.
+ * addft_gamma_scatter_add_real: the same for real S' (rows psi real in real
+ *   space, whose <P'|psi> are real): U (nb, 2 nh) = S' [Qr | Qi], the
+ *   zero Im-part rows neither stored nor multiplied.
  * errors: bad dtype, nb > 32767, 2 nh > ng + 1, ld < 6 nh, null pointers. */
 int addft_gamma_pair_split(const void* d_psi, int64_t nb, int64_t ng, const int32_t* d_plus,
                            const int32_t* d_minus, int64_t nh, int64_t ld, void* d_out, void* stream);
@@ -93,6 +96,8 @@ int addft_gamma_pair_rows(const void* d_psi, int64_t nb, int64_t ng, const int32
                           const int32_t* d_minus, int64_t nh, void* d_out, void* stream);
 int addft_gamma_scatter_add(int dtype, const void* d_u, int64_t nb, int64_t nh, const int32_t* d_plus,
                             const int32_t* d_minus, int64_t ng, void* d_out, void* stream);
+int addft_gamma_scatter_add_real(int dtype, const void* d_u, int64_t nb, int64_t nh, const int32_t* d_plus,
+                                 const int32_t* d_minus, int64_t ng, void* d_out, void* stream);
 /* Two bands per transform for vectors real in real space (psi(-G) =
  * conj(psi(G)), k = 0): addft_gamma_pack2 writes phi (ceil(m/2), ng) with
  * phi_b = psi_{2b} + i psi_{2b+1}; after H phi (any operator real in real
