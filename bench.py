@@ -220,29 +220,49 @@ def subspace_measure(addft, kw, steps, torch):
             "products": res}
 
 
-def delta_q_measure(addft, kw, torch, np):
-    """The matrix-free Delta-Q adjoint at the workload's size: SCF to 1e-10,
-    then dC/dR and dC/deps of C = sum_r w(r) rho(r) for a Gaussian weight
-    (sigma 1.5 Bohr) on atom 5, by the preconditioned inexact GMRES adjoint
-    (rtol 1e-10), timed on the host clock with the device synchronised."""
+def delta_q_measure(addft, kw, torch, np, fp64=True):
+    """The matrix-free Delta-Q adjoint at the workload's size: SCF to 1e-10
+    (mixed form), then dC/dR and dC/deps of C = sum_r w(r) rho(r) for a
+    Gaussian weight (sigma 1.5 Bohr) on atom 5 by the preconditioned inexact
+    GMRES adjoint (rtol 1e-10): the mixed-precision response (FP32
+    Sternheimer solves inside FP64 flexible CG) timed on the host clock
+    with the device synchronised, and, with fp64, the FP64 response on the
+    same state for the error of the mixed one."""
     from addft import api, response
     s = addft.setup(**kw)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
     st = api._solve(s, tol=1e-10)
+    torch.cuda.synchronize()
+    scf_sec = time.perf_counter() - t0
     R = np.asarray(kw["frac"]) @ np.asarray(kw["lattice"])
     w = addft.make_gaussian_weight(s, R[min(5, len(R) - 1)], 1.5)
     eng = st.engine
     w_pl = eng.to_planar(torch.as_tensor(np.asarray(w, dtype=np.float64)).to(eng.dev))
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    dR, de, diag = response.delta_q_grad(eng, st.psi, st.eig, st.occ, st.rho, w_pl,
-                                         log=lambda msg: say(f"  {time.perf_counter() - t0:7.1f} s {msg}"))
-    torch.cuda.synchronize()
-    sec = time.perf_counter() - t0
-    return {"metric": "Delta-Q adjoint gradient, s after the SCF",
-            "config": {"workload": f"{len(kw['frac'])}-atom cell, nG {eng.ng}, {kw['nband']} bands",
-                       "rtol": 1e-10, "precision": kw.get("precision", "fp64")},
-            "seconds": round(sec, 2), "max_abs_dC_dR": float(np.abs(dR).max()),
-            "diagnostics": {k: (v if isinstance(v, (int, float, str)) else str(v)) for k, v in diag.items()}}
+    runs = {}
+    for prec in (("mixed", "fp64") if fp64 else ("mixed",)):
+        say(f"Delta-Q adjoint, {prec} response")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        dR, de, diag = response.delta_q_grad(eng, st.psi, st.eig, st.occ, st.rho, w_pl, precision=prec,
+                                             log=lambda msg: None)
+        torch.cuda.synchronize()
+        runs[prec] = (time.perf_counter() - t0, dR, de, diag)
+        say(f"  {runs[prec][0]:.2f} s, {diag['krylov_iterations']} products, residual {diag['adjoint_residual']:.2e}")
+    sec, dR, de, diag = runs["mixed"]
+    out = {"metric": "Delta-Q adjoint gradient, s after the SCF",
+           "config": {"workload": f"{len(kw['frac'])}-atom cell, nG {eng.ng}, {kw['nband']} bands",
+                      "rtol": 1e-10, "weight": "Gaussian sigma 1.5 Bohr on atom 5"},
+           "seconds": round(sec, 2), "scf_1e-10_seconds": round(scf_sec, 2),
+           "max_abs_dC_dR": float(np.abs(dR).max()),
+           "diagnostics": {k: (v if isinstance(v, (int, float, str)) else str(v)) for k, v in diag.items()}}
+    if fp64:
+        sec64, dR64, de64, diag64 = runs["fp64"]
+        out["fp64_seconds"] = round(sec64, 2)
+        out["fp64_diagnostics"] = {k: (v if isinstance(v, (int, float, str)) else str(v)) for k, v in diag64.items()}
+        out["rel_error_vs_fp64"] = {"dC_dR": float(np.abs(dR - dR64).max() / np.abs(dR64).max()),
+                                    "dC_deps": float(np.abs(de - de64).max() / np.abs(de64).max())}
+    return out
 
 
 def one_geometry(addft, system, tol, torch):
@@ -332,9 +352,10 @@ def cpu_full_scf(testsystems, names=("c1", "c2"), repeats=(5, 1)):
     from oracle import forces as OF
     from oracle import psp as OPS
     from oracle import solver as OS
+    import scipy.fft
     out = {}
     n = cpu_cores()
-    with threadpool_limits(n):
+    with threadpool_limits(n), scipy.fft.set_workers(n):
         for name, reps in zip(names, repeats):
             kw = testsystems.config(name)
             psp = OPS.from_upf_text(si_upf_one_projector() if name == "c1" else si_upf_three_projectors())
@@ -424,8 +445,8 @@ def fd_check(addft, testsystems, workload, precision, tol, np, torch, h=1e-3, se
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1)
-    ap.add_argument("--warmup", type=int, default=0)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="c5", help="benchmark system (c2, c3, c5)")
     ap.add_argument("--mixed-tol", type=float, default=1e-7)
     ap.add_argument("--fp64-tol", type=float, default=1e-10)
@@ -443,7 +464,10 @@ def main():
     ap.add_argument("--fd-only", choices=["all", "forces", "strain"], default="all")
     ap.add_argument("--fd-h", type=float, default=1e-3, help="step of the --fd-check differences")
     ap.add_argument("--delta-q", action="store_true",
-                    help="time the Delta-Q adjoint gradient after an SCF at the workload's size")
+                    help="only the Delta-Q adjoint gradient after an SCF at the workload's size")
+    ap.add_argument("--delta-q-line", type=int, default=1,
+                    help="1: add the Delta-Q adjoint (mixed, and FP64 on the same state unless --no-fp64) "
+                         "to the headline line")
     ap.add_argument("--subspace-only", action="store_true",
                     help="time the eigensolver's block products (Gram/Rayleigh-Ritz reduction, rotation) only")
     ap.add_argument("--nonlocal-only", action="store_true",
@@ -481,7 +505,7 @@ def main():
         return
 
     if args.delta_q:
-        print(json.dumps(delta_q_measure(addft, kw, torch, np)), flush=True)
+        print(json.dumps(delta_q_measure(addft, kw, torch, np, fp64=not args.no_fp64)), flush=True)
         return
 
     if args.subspace_only:
@@ -615,6 +639,10 @@ def main():
         line["error_mixed_vs_fp64"] = {"energy_Ha_per_atom": float(abs(F - F64) / nat),
                                        "force_Ha_per_Bohr": float(np.abs(forces - f64).max()),
                                        "stress_Ha_per_Bohr3": float(np.abs(sig - s64).max())}
+    if args.delta_q_line and args.workload == "c5":
+        dq = delta_q_measure(addft, kw, torch, np, fp64=not args.no_fp64)
+        dq["ratio_to_mixed_geometry"] = round(dq["seconds"] / sec, 3)
+        line["delta_q"] = dq
     if args.fd_components and args.workload == "c5":
         say("central differences at the headline workload (mixed form, tol 1e-10, warm-started)")
         fd = fd_check(addft, testsystems, args.workload, "mixed", 1e-10, np, torch, h=1e-3,

@@ -9,6 +9,7 @@ from __future__ import annotations
 from dataclasses import dataclass
 
 import numpy as np
+import scipy.fft as sfft
 
 TWO_PI = 2.0 * np.pi
 
@@ -195,7 +196,7 @@ def psi_to_real(basis, lattice, c):
     n = basis.nfft
     grid = scatter(c, flat_index(basis.fft_shape, basis.miller), n)
     grid = grid.reshape(c.shape[:-1] + tuple(basis.fft_shape))
-    out = np.fft.ifftn(grid, axes=(-3, -2, -1)) * (n / np.sqrt(volume(lattice)))
+    out = sfft.ifftn(grid, axes=(-3, -2, -1)) * (n / np.sqrt(volume(lattice)))
     return out.reshape(c.shape[:-1] + (n,))
 
 
@@ -203,14 +204,14 @@ def real_to_psi(basis, lattice, psi_r):
     """c_G = fftn(psi_r) sqrt(Omega)/N, gathered (basis.py:247-254)."""
     n = basis.nfft
     grid = np.asarray(psi_r).reshape(psi_r.shape[:-1] + tuple(basis.fft_shape))
-    spec = np.fft.fftn(grid, axes=(-3, -2, -1)) * (np.sqrt(volume(lattice)) / n)
+    spec = sfft.fftn(grid, axes=(-3, -2, -1)) * (np.sqrt(volume(lattice)) / n)
     return gather(spec.reshape(psi_r.shape[:-1] + (n,)),
                   flat_index(basis.fft_shape, basis.miller))
 
 
 def density_g(basis, rho_r):
     """n(G) = fftn(n)/N on the psi-sphere (basis.py:257-262)."""
-    spec = np.fft.fftn(np.asarray(rho_r).reshape(basis.fft_shape)) / basis.nfft
+    spec = sfft.fftn(np.asarray(rho_r).reshape(basis.fft_shape)) / basis.nfft
     return gather(spec.reshape(-1), flat_index(basis.fft_shape, basis.miller))
 
 
@@ -218,7 +219,7 @@ def real_from_g(basis, vg, miller=None):
     """v(r) = Re(N ifftn(scatter(v_G))) (basis.py:265-277)."""
     miller = basis.miller if miller is None else miller
     grid = scatter(vg, flat_index(basis.fft_shape, miller), basis.nfft)
-    return (basis.nfft * np.fft.ifftn(grid.reshape(basis.fft_shape))).real.reshape(basis.nfft)
+    return (basis.nfft * sfft.ifftn(grid.reshape(basis.fft_shape))).real.reshape(basis.nfft)
 
 
 def real_from_density_g(basis, vg):

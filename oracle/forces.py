@@ -17,6 +17,7 @@ only.
 from __future__ import annotations
 
 import numpy as np
+import scipy.fft as sfft
 
 from . import basis as B
 from . import gga as G
@@ -26,7 +27,7 @@ from .psp import beta_q, dbeta_q, dvloc_q, lm_table, vloc_q
 
 
 def _rho_tilde(basis, rho_r, miller):
-    spec = np.fft.fftn(np.asarray(rho_r).reshape(basis.fft_shape)) / basis.nfft
+    spec = sfft.fftn(np.asarray(rho_r).reshape(basis.fft_shape)) / basis.nfft
     return spec.reshape(-1)[B.flat_index(basis.fft_shape, miller)]
 
 

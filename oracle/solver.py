@@ -11,6 +11,7 @@ from __future__ import annotations
 from dataclasses import dataclass, field, replace
 
 import numpy as np
+import scipy.fft as sfft
 
 from . import basis as B
 from . import gga as G
@@ -135,7 +136,7 @@ def ewald_energy(geo, st, want_grad=False):
 def vmat_from_real(basis, v_r):
     """<G'|V|G> = fft(V)[(m_G - m_G') mod N]/N (scf.py:36-46)."""
     n1, n2, n3 = basis.fft_shape
-    vg = np.fft.fftn(np.asarray(v_r).reshape(n1, n2, n3)) / basis.nfft
+    vg = sfft.fftn(np.asarray(v_r).reshape(n1, n2, n3)) / basis.nfft
     m = basis.miller.astype(np.int64)
     d = m[:, None, :] - m[None, :, :]
     idx = (np.mod(d[..., 0], n1) * n2 + np.mod(d[..., 1], n2)) * n3 + np.mod(d[..., 2], n3)
@@ -277,10 +278,10 @@ def q2_grid(basis, lattice):
 def kerker_mix(basis, lattice, rho_in, rho_out, alpha=0.5, gg0=1.0):
     """scf.py:215-223."""
     dn = (rho_out - rho_in).reshape(basis.fft_shape)
-    dg = np.fft.fftn(dn) / basis.nfft
+    dg = sfft.fftn(dn) / basis.nfft
     q2 = q2_grid(basis, lattice)
     filt = np.where(q2 > 1e-12, q2 / (q2 + gg0), 0.0)
-    return rho_in + alpha * (basis.nfft * np.fft.ifftn(dg * filt)).real.reshape(basis.nfft)
+    return rho_in + alpha * (basis.nfft * sfft.ifftn(dg * filt)).real.reshape(basis.nfft)
 
 
 def density_residual(basis, lattice, rho_new, rho_old, nelec):
