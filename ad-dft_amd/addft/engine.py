@@ -454,7 +454,17 @@ class Engine:
         w = ((kw[:, None] * self.st.spin_degeneracy / self.omega) * occ.to(torch.float64)).reshape(-1).contiguous()
         psi = psi3.reshape(nk * nb, ng)
         psi = psi.to(plan.cdtype) if psi.dtype != plan.cdtype else psi
+        if fp32:
+            w = w / self._norm2_fp32(psi)
         return plan.density(psi.contiguous(), w)
+
+    @staticmethod
+    def _norm2_fp32(psi):
+        """|psi_n|^2 per row of an FP32 block (tree reductions): the FP32
+        eigensolver's orthonormalisation leaves norms off by ~1e-5 (Gram
+        sums over 10^5 terms in FP32), which would otherwise put that error
+        on the electron count of the FP32 maps and stall them there."""
+        return torch.linalg.vector_norm(psi, dim=1).to(torch.float64) ** 2
 
     def becp(self, ik, psi):
         """(nproj, nb) <beta|psi> in FP64 (nonlocal_op.py:67-69)."""
@@ -467,6 +477,8 @@ class Engine:
         for ik in range(self.nk):
             w = (self.st.kweights[ik] * self.st.spin_degeneracy / self.omega) * occ[ik]
             psi = psis[ik].to(plan.cdtype) if psis[ik].dtype != plan.cdtype else psis[ik]
+            if fp32:
+                w = w / self._norm2_fp32(psi)
             rho = plan.density(psi, w.contiguous(), out=rho, accumulate=ik > 0)
         return rho
 

@@ -19,7 +19,7 @@ import torch
 
 from ..kernels.native import cheb_combine
 from .davidson import eigh
-from .linalg import inner
+from .linalg import inner, rcombine, rinner
 
 
 def upper_bound(apply, nG, dtype, device, k=10, seed=0):
@@ -78,10 +78,12 @@ def orthonormalise(Y, real=False):
     shifted first pass when Y is too ill-conditioned for plain Cholesky
     (Fukaya et al., shifted CholeskyQR, SIAM J. Sci. Comput. 42 (2020))."""
     for it in range(3):
-        S = inner(Y, Y).to(torch.complex128)
-        S = 0.5 * (S + S.conj().T)
-        if real:              # rows real in real space (k = 0): a real Gram matrix
-            S = S.real.to(torch.complex128)
+        if real:              # rows real in real space (k = 0): a real Gram matrix, real GEMMs
+            S = rinner(Y, Y).to(torch.float64)
+            S = 0.5 * (S + S.T)
+        else:
+            S = inner(Y, Y).to(torch.complex128)
+            S = 0.5 * (S + S.conj().T)
         d = torch.sqrt(torch.clamp(S.diagonal().real, min=1e-300))
         Ss = S / (d[:, None] * d[None, :])
         L, info = torch.linalg.cholesky_ex(Ss)
@@ -95,7 +97,7 @@ def orthonormalise(Y, real=False):
                 raise torch.linalg.LinAlgError("block is numerically rank deficient")
         eye = torch.eye(L.shape[0], dtype=L.dtype, device=L.device)
         Linv = torch.linalg.solve_triangular(L, eye, upper=False) / d[None, :]
-        Y = torch.matmul(Linv.to(Y.dtype), Y).contiguous()
+        Y = (rcombine(Linv, Y) if real else torch.matmul(Linv.to(Y.dtype), Y)).contiguous()
         if it >= 1 and not shifted:
             break
     return Y
@@ -121,14 +123,19 @@ def chfsi_step(apply, X, degree, b_upper, lam, HX=None, rr=True, residuals=True,
     if not rr:
         return lam, Y, None, None
     HY = apply(Y)
-    T = inner(Y, HY, conj_a=True).to(torch.complex128)
-    T = 0.5 * (T + T.conj().T)
-    w, U = eigh(T.real) if real else eigh(T)
-    Ut = U.T.to(Y.dtype)
-    X = torch.matmul(Ut, Y)
+    if real:
+        T = rinner(Y, HY).to(torch.float64)
+        T = 0.5 * (T + T.T)
+        w, U = eigh(T)
+        X = rcombine(U.T, Y)
+    else:
+        T = inner(Y, HY, conj_a=True).to(torch.complex128)
+        T = 0.5 * (T + T.conj().T)
+        w, U = eigh(T)
+        X = torch.matmul(U.T.to(Y.dtype), Y)
     if not residuals:
         return w, X.contiguous(), None, None
-    HX = torch.matmul(Ut, HY)
+    HX = rcombine(U.T, HY) if real else torch.matmul(U.T.to(Y.dtype), HY)
     R = HX - w.to(X.real.dtype)[:, None] * X
     rn = torch.linalg.vector_norm(R.to(torch.complex128), dim=1)
     return w, X.contiguous(), HX.contiguous(), rn

@@ -22,7 +22,7 @@ import os
 
 import torch
 
-from .linalg import inner
+from .linalg import inner, rcombine, rinner
 
 _CHECK = os.environ.get("ADDFT_DEBUG_FINITE", "0") == "1"
 
@@ -78,10 +78,12 @@ def _svqb(R, drop=1e-10, real=False):
     23 (2002), SVQB): eigen-decompose the scaled FP64 Gram matrix, drop
     directions below drop * max, and rescale.  Robust for blocks whose rows
     differ by orders of magnitude in norm or are nearly dependent."""
-    S = inner(R, R).to(torch.complex128)
-    S = 0.5 * (S + S.conj().T)
-    if real:          # rows real in real space: the Gram matrix is real
-        S = S.real.to(torch.complex128)
+    if real:          # rows real in real space: a real Gram matrix, real GEMMs
+        S = rinner(R, R).to(torch.float64)
+        S = 0.5 * (S + S.T)
+    else:
+        S = inner(R, R).to(torch.complex128)
+        S = 0.5 * (S + S.conj().T)
     d = torch.sqrt(torch.clamp(S.diagonal().real, min=1e-300))
     Ss = S / (d[:, None] * d[None, :])
     off = Ss - torch.diag(Ss.diagonal())
@@ -95,12 +97,12 @@ def _svqb(R, drop=1e-10, real=False):
         # fails to allocate its workspace at the 216-atom size)
         eye = torch.eye(L.shape[0], dtype=L.dtype, device=L.device)
         Linv = torch.linalg.solve_triangular(L, eye, upper=False) / d[None, :]
-        return torch.matmul(Linv.to(R.dtype), R).contiguous()
-    w, V = eigh(Ss.real) if real else eigh(Ss)
+        return (rcombine(Linv, R) if real else torch.matmul(Linv.to(R.dtype), R)).contiguous()
+    w, V = eigh(Ss)
     keep = w > drop * w.max()
     Vk = V[:, keep] / torch.sqrt(w[keep])[None, :]
-    M = (Vk.conj().T / d[None, :]).to(R.dtype)      # (k, m)
-    return torch.matmul(M, R).contiguous()
+    M = Vk.conj().T / d[None, :]                    # (k, m)
+    return (rcombine(M, R) if real else torch.matmul(M.to(R.dtype), R)).contiguous()
 
 
 def _orth(R, Q=None, drop=1e-10, real=False):
@@ -110,10 +112,7 @@ def _orth(R, Q=None, drop=1e-10, real=False):
     which keeps the rows real)."""
     for _ in range(2):
         if Q is not None:
-            C = inner(R, Q)
-            if real:
-                C = C.real.to(C.dtype)
-            R = R - torch.matmul(C, Q)
+            R = R - (rcombine(rinner(R, Q), Q) if real else torch.matmul(inner(R, Q), Q))
         R = _svqb(R, drop, real)
         if R.shape[0] == 0:
             break
@@ -189,10 +188,12 @@ def davidson(apply, X0, nb, diag, max_sub=None, max_iter=40, tol=1e-9, min_iter=
     _finite("X0", X)
     _finite("HX0", HX)
     Q, HQ = X, HX
-    T = inner(Q, HQ, conj_a=True).to(torch.complex128)
-    T = 0.5 * (T + T.conj().T)
     if real:
-        T = T.real.to(torch.complex128)
+        T = rinner(Q, HQ).to(torch.float64)
+        T = 0.5 * (T + T.T)
+    else:
+        T = inner(Q, HQ, conj_a=True).to(torch.complex128)
+        T = 0.5 * (T + T.conj().T)
     res = float("inf")
     lam = None
     it = 0
@@ -201,14 +202,18 @@ def davidson(apply, X0, nb, diag, max_sub=None, max_iter=40, tol=1e-9, min_iter=
             lam = T.diagonal().real.clone()
             X, HX = Q, HQ
         else:
-            w, U = eigh(T.real) if real else eigh(T)
+            w, U = eigh(T)
             lam = w[:nb]
-            Ub = U[:, :nb].to(Q.dtype)
-            X = torch.matmul(Ub.T, Q)
+            if real:
+                Ub = U[:, :nb].T.contiguous()
+                X = rcombine(Ub, Q)
+            else:
+                Ub = U[:, :nb].to(Q.dtype)
+                X = torch.matmul(Ub.T, Q)
             if it == max_iter and not last_residuals:
                 res = float("nan")
                 break
-            HX = torch.matmul(Ub.T, HQ)
+            HX = rcombine(Ub, HQ) if real else torch.matmul(Ub.T, HQ)
         R = HX - lam.to(X.real.dtype)[:, None] * X
         rn = torch.linalg.vector_norm(R.to(torch.complex128), dim=1)
         rw = rn if weight is None else rn * weight.to(rn.dtype)
@@ -226,22 +231,25 @@ def davidson(apply, X0, nb, diag, max_sub=None, max_iter=40, tol=1e-9, min_iter=
         if Q.shape[0] + R.shape[0] > max_sub:
             # restart on the current Ritz vectors (davidson.py:66-69)
             Q, HQ = X, HX
-            T = torch.diag(lam.to(torch.complex128))
+            T = torch.diag(lam.to(torch.float64 if real else torch.complex128))
         R = _orth(R, Q, real=real)
         if R.shape[0] == 0:
             break
         _finite("R", R)
         if apply_lo is None:
             HR = apply(R)
-            B = inner(Q, HR, conj_a=True).to(torch.complex128)
         else:
             HR = apply_lo(R)
-            B = inner(HQ, R, conj_a=True).to(torch.complex128)
         _finite("HR", HR)
-        C = inner(R, HR, conj_a=True).to(torch.complex128)
-        C = 0.5 * (C + C.conj().T)
         if real:
-            B, C = B.real.to(torch.complex128), C.real.to(torch.complex128)
+            # B[i, j] = <q_i|H r_j> (from the exact H Q when apply_lo is given)
+            B = (rinner(Q, HR) if apply_lo is None else rinner(HQ, R)).to(torch.float64)
+            C = rinner(R, HR).to(torch.float64)
+            C = 0.5 * (C + C.T)
+        else:
+            B = (inner(Q, HR, conj_a=True) if apply_lo is None else inner(HQ, R, conj_a=True)).to(torch.complex128)
+            C = inner(R, HR, conj_a=True).to(torch.complex128)
+            C = 0.5 * (C + C.conj().T)
         T = torch.cat([torch.cat([T, B], dim=1), torch.cat([B.conj().T, C], dim=1)], dim=0)
         Q = torch.cat([Q, R], dim=0)
         HQ = torch.cat([HQ, HR], dim=0)

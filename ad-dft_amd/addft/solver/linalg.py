@@ -89,6 +89,24 @@ def inner(A, B, conj_a=False):
     return inner_sliced(A, B, _choose_slices(A, B))
 
 
+def rinner(A, B):
+    """Re(A @ B^H) for complex rows A (m, K), B (n, K): one real GEMM over
+    the (re, im) views, half the flops of the complex product.  For rows
+    real in real space (k = 0, v(-G) = conj v(G)) the products are real, so
+    this is the whole of <a|b>."""
+    Ar = torch.view_as_real(A.contiguous()).reshape(A.shape[0], -1)
+    Br = torch.view_as_real(B.contiguous()).reshape(B.shape[0], -1)
+    return inner(Ar, Br)
+
+
+def rcombine(C, B):
+    """C @ B for real coefficients C (m, n) and complex rows B (n, K), as one
+    real GEMM over B's (re, im) view."""
+    Br = torch.view_as_real(B.contiguous()).reshape(B.shape[0], -1)
+    out = torch.matmul(C.to(Br.dtype).contiguous(), Br)
+    return torch.view_as_complex(out.reshape(C.shape[0], -1, 2))
+
+
 # ---------------------------------------------------- small eigensolves
 _EIGH_METHODS = ("dc", "jacobi", "dcj")
 
