@@ -95,8 +95,8 @@ _SIGS = {
     "addft_projector_destroy": ([_VP], _I32),
     "addft_projector_info": ([_VP, ctypes.POINTER(_I64), ctypes.POINTER(_I64), ctypes.POINTER(_I32)], _I32),
     "addft_nonlocal_work_bytes": ([_VP, _I64], _I64),
-    "addft_nonlocal_becp": ([_VP, _VP, _I64, _I64, _VP, _I64, _VP, _VP], _I32),
-    "addft_nonlocal_apply_add": ([_VP, _VP, _I64, _I64, _VP, _I64, _VP], _I32),
+    "addft_nonlocal_becp": ([_VP, _VP, _I64, _I64, _VP, _I64, _I32, _VP, _VP], _I32),
+    "addft_nonlocal_apply_add": ([_VP, _VP, _I64, _I64, _VP, _I64, _I32, _VP, _VP], _I32),
     "addft_subspace_gram_work_bytes": ([_I32, _I64, _I64, _I32], _I64),
     "addft_subspace_gram": ([_I32, _I64, _I64, _I64, _VP, _I64, _VP, _I64, _VP, _I64, _I32, _VP, _VP], _I32),
     "addft_subspace_rotate": ([_I32, _I64, _I64, _I64, _VP, _I64, _VP, _I64, _VP, _I64, _VP], _I32),

@@ -295,9 +295,10 @@ class GammaProjector64:
 class ProjectorMFMA:
     """The FP32-phase projector products through the library's hand-written
     BF16x3 MFMA kernels (addft_nonlocal_becp / addft_nonlocal_apply_add):
-    the operand split, the k = 0 pair folding and (for apply_add) the
-    unfolding onto +-G all happen inside the two kernels, which read the
-    split projector images built once here.  Same becp_t / apply_add
+    each call splits its FP32 operand once (with the k = 0 pair folding for
+    becp) into the workspace, the MFMA tiles read that image and the split
+    projector images built once here, and apply_add unfolds onto +-G in its
+    epilogue.  Same becp_t / apply_add
     interface as ProjectorBF16x3 (pairs None) and GammaProjectorBF16x3
     (pairs = (plus, minus) and the odd-l rows: P' = Phi P)."""
 
@@ -335,16 +336,17 @@ class ProjectorMFMA:
         return self._work
 
     def becp_t(self, psi: torch.Tensor, real: bool = False) -> torch.Tensor:
-        """psi (nb, nG) complex64 -> psi P^H (nb, nproj) complex64 (P' at k = 0;
-        `real` is accepted for the common interface and changes nothing)."""
+        """psi (nb, nG) complex64 -> psi P^H (nb, nproj) complex64 (P' at k = 0).
+        real (k = 0): rows real in real space, whose <P'|psi> are real -- one
+        operand row per band (half the flops); imaginary parts are zero."""
         if psi.dtype != torch.complex64 or psi.dim() != 2 or psi.shape[1] != self.ng:
             raise ValueError("psi must be (nb, nG) complex64")
         psi = psi.contiguous()
         nb = psi.shape[0]
         out = torch.empty((nb, self.np), dtype=torch.complex64, device=psi.device)
         _lib.check(_lib.lib().addft_nonlocal_becp(self._h, _lib.ptr(psi), nb, self.ng, _lib.ptr(out), self.np,
-                                                  _lib.ptr(self._workspace(nb)), _lib.stream()),
-                   "addft_nonlocal_becp")
+                                                  int(bool(real) and self.gamma), _lib.ptr(self._workspace(nb)),
+                                                  _lib.stream()), "addft_nonlocal_becp")
         return out
 
     def becp_t_noalloc(self, psi: torch.Tensor, out: torch.Tensor) -> torch.Tensor:
@@ -361,12 +363,15 @@ class ProjectorMFMA:
         return out
 
     def apply_add(self, S: torch.Tensor, out: torch.Tensor, real: bool = False) -> torch.Tensor:
-        """out (nb, nG) complex64 += S P (S' P' at k = 0)."""
+        """out (nb, nG) complex64 += S P (S' P' at k = 0; real: S' real, rows
+        real in real space, its imaginary part not read)."""
         if S.dtype != torch.complex64 or out.dtype != torch.complex64:
             raise ValueError("S and out must be complex64")
         if not out.is_contiguous() or out.shape != (S.shape[0], self.ng) or S.shape[1] != self.np:
             raise ValueError("out must be a contiguous (nb, nG) block and S (nb, nproj)")
         S = S.contiguous()
         _lib.check(_lib.lib().addft_nonlocal_apply_add(self._h, _lib.ptr(S), S.shape[0], self.np, _lib.ptr(out),
-                                                       self.ng, _lib.stream()), "addft_nonlocal_apply_add")
+                                                       self.ng, int(bool(real) and self.gamma),
+                                                       _lib.ptr(self._workspace(S.shape[0])), _lib.stream()),
+                   "addft_nonlocal_apply_add")
         return out

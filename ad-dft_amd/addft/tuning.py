@@ -130,8 +130,8 @@ def tune_projector(P, pairs, odd_rows, nb: int, seed: int = 0):
     """The FP32-phase projector products of one k-point's P for blocks of nb
     bands: builds the hand-written MFMA form (kernels.nonlocal_bf16.
     ProjectorMFMA) and the hipBLASLt split-GEMM form, times becp_t +
-    apply_add of each and returns the faster object.  ADDFT_NL_MFMA=1 / 0
-    forces one of them."""
+    apply_add of each (at k = 0 in the real-row and the complex form) and
+    returns the faster object.  ADDFT_NL_MFMA=1 / 0 forces one of them."""
     from .kernels.nonlocal_bf16 import GammaProjectorBF16x3, ProjectorBF16x3, ProjectorMFMA
     npj, ng = P.shape
     force = os.environ.get("ADDFT_NL_MFMA")
@@ -158,6 +158,14 @@ def tune_projector(P, pairs, odd_rows, nb: int, seed: int = 0):
     S = torch.complex(torch.randn(nb, npj, generator=g), torch.randn(nb, npj, generator=g)).to(P.device)
     out = torch.zeros_like(psi)
     objs = {"mfma": mfma(), "library": library()}
-    choice = decide(key, {name: (lambda o=o: o.apply_add(S, out) if o.becp_t(psi) is not None else None)
-                          for name, o in objs.items()})
+    # at k = 0 the SCF's bands are real in real space (real-row products);
+    # both forms are timed there, since the complex form also runs
+    forms = (True, False) if pairs is not None else (False,)
+
+    def run(o):
+        for real in forms:
+            o.apply_add(S.real.to(S.dtype) if real else S, out, real=real)
+            o.becp_t(psi, real=real)
+
+    choice = decide(key, {name: (lambda o=o: run(o)) for name, o in objs.items()})
     return objs[choice]

@@ -42,6 +42,78 @@ def say(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+_MARK = False
+
+
+def mark(torch):
+    """With --mark: one marker kernel (torch's spin kernel, ~0.1 ms) on the
+    device, synchronised on both sides, so that a rocprofv3 kernel trace can
+    be cut to the timed region (--trace-summary)."""
+    if _MARK:
+        torch.cuda.synchronize()
+        torch.cuda._sleep(100000)
+        torch.cuda.synchronize()
+
+
+# kernel categories of --trace-summary: (label, regular expression on the kernel name), first match wins
+TRACE_CATEGORIES = [
+    ("local Hpsi / density (lane FFT kernels)", r"k_z_inverse|k_z_forward|k_plane|k_reduce_density|k_stage"),
+    ("projector products, hand-written BF16x3 MFMA", r"k_becp|k_apply_add|k_split_rows|k_pair_rows"),
+    ("projector products, hipBLASLt BF16x3 GEMMs", r"Cijk_.*BSS_BH"),
+    ("Gamma pair split / unfold, BF16x3 split", r"k_gamma_pair_split|k_gamma_scatter_add|k_split"),
+    ("Gamma two-band pack / unpack", r"k_gamma_pack2|k_gamma_unpack2"),
+    ("Chebyshev step", r"k_combine"),
+    ("fused CG kernels", r"k_shift_dot|k_update|k_direction"),
+    ("eigensolver (rocSOLVER / rocBLAS: tridiagonalisation, D&C, Cholesky, triangular solves)",
+     r"rocsolver|rocblas_|latrd|sytd2|stedc|potf2|trsm|copy_mat|ormtr|larf"),
+    ("library GEMMs, complex128", r"Cijk_.*_ZB"),
+    ("library GEMMs, complex64", r"Cijk_.*_CB"),
+    ("library GEMMs, real FP64", r"Cijk_.*_DB"),
+    ("library GEMMs, real FP32", r"Cijk_.*_S_B|Cijk_.*_SB"),
+    ("other library GEMMs", r"Cijk_"),
+    ("other addft kernels", r"^(void )?\(anonymous namespace\)::k_|^k_"),
+    ("device copies", r"copyBuffer|copyImage|fillBuffer"),
+]
+
+
+def trace_summary(path, top=25):
+    """Kernel time by category from a rocprofv3 --kernel-trace CSV, cut to
+    the region between the first two marker kernels (--mark) when the trace
+    holds them.  Returns the markdown tables."""
+    import csv
+    from collections import defaultdict
+    rows = []
+    with open(path, newline="") as f:
+        rd = csv.DictReader(f)
+        nk = "Kernel_Name" if "Kernel_Name" in rd.fieldnames else "KernelName"
+        for r in rd:
+            rows.append((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), r[nk]))
+    rows.sort()
+    marker = re.compile(r"spin_kernel")
+    marks = [r for r in rows if marker.search(r[2])]
+    t0, t1 = (marks[0][1], marks[1][0]) if len(marks) >= 2 else (rows[0][0], rows[-1][1])
+    per = defaultdict(lambda: [0, 0])
+    for s, e, n in rows:
+        if s >= t0 and e <= t1 and not marker.search(n):
+            per[n][0] += 1
+            per[n][1] += e - s
+    busy = sum(v[1] for v in per.values())
+    cats = defaultdict(int)
+    for n, (_, t) in per.items():
+        cats[next((c for c, p in TRACE_CATEGORIES if re.search(p, n)), "other")] += t
+    out = [f"Region: {(t1 - t0) / 1e6:.1f} ms between the marker kernels ({len(marks)} markers in the trace); "
+           f"kernels busy {busy / 1e6:.1f} ms ({100 * busy / max(1, t1 - t0):.0f} %), "
+           f"{sum(v[0] for v in per.values())} dispatches.", "",
+           "| category | time (ms) | % of kernel time |", "|---|---:|---:|"]
+    for c, t in sorted(cats.items(), key=lambda x: -x[1]):
+        out.append(f"| {c} | {t / 1e6:.1f} | {100 * t / busy:.1f} |")
+    out += ["", "| kernel | calls | average (µs) | total (ms) | % |", "|---|---:|---:|---:|---:|"]
+    for n, (c, t) in sorted(per.items(), key=lambda x: -x[1][1])[:top]:
+        short = n if len(n) <= 90 else n[:87] + "…"
+        out.append(f"| `{short}` | {c} | {t / c / 1e3:.1f} | {t / 1e6:.2f} | {100 * t / busy:.1f} |")
+    return "\n".join(out)
+
+
 def hpsi_bytes(nb, ng, nst, n3, nfft, cbytes, rbytes):
     """Algorithmic HBM bytes of one local Hpsi: per band psi in, Hpsi out and
     four stick round trips between the z pass and the LDS plane pass; V(r)
@@ -156,12 +228,20 @@ def nonlocal_measure(addft, kw, steps, torch):
     lib_pb = (GammaProjectorBF16x3(eng.P64[0], *eng.gamma_pairs, eng.odd_rows_d, nslice=32) if half
               else ProjectorBF16x3(eng.P64[0]))
     mf = ProjectorMFMA(eng.P64[0], eng.gamma_pairs, eng.odd_rows_d) if half else ProjectorMFMA(eng.P64[0])
+    Sr = S.real.to(torch.complex64)
     for impl, obj in (("mfma", mf), ("library", lib_pb)):
-        for name, fn in (("becp", lambda o=obj: o.becp_t(psi)), ("apply_add", lambda o=obj: o.apply_add(S, out))):
+        cands = [("becp", lambda o=obj: o.becp_t(psi)), ("apply_add", lambda o=obj: o.apply_add(S, out))]
+        if half:
+            # rows real in real space (the SCF's k = 0 bands): one operand row
+            # per band, half the executed flops
+            cands += [("becp_real_rows", lambda o=obj: o.becp_t(psi, real=True)),
+                      ("apply_add_real_rows", lambda o=obj: o.apply_add(Sr, out, real=True))]
+        for name, fn in cands:
             ms = timed(fn, steps, 3, torch)
+            ex = exe / 2 if name.endswith("real_rows") else exe
             res[f"{name}_{impl}"] = {"ms": round(ms, 4), "TF_per_s": round(flops / (ms * 1e-3) / 1e12, 1),
-                                     "executed_TF_per_s": round(exe / (ms * 1e-3) / 1e12, 1),
-                                     "frac_of_bf16_dense_peak_executed": round(exe / (ms * 1e-3) / 2.5e15, 4)}
+                                     "executed_TF_per_s": round(ex / (ms * 1e-3) / 1e12, 1),
+                                     "frac_of_bf16_dense_peak_executed": round(ex / (ms * 1e-3) / 2.5e15, 4)}
     # FP64 products (k = 0: half sphere, GammaProjector64), beside the full-sphere complex128 GEMMs
     from addft.solver.linalg import inner
     psi64, S64 = psi.to(torch.complex128), S.to(torch.complex128)
@@ -220,14 +300,15 @@ def subspace_measure(addft, kw, steps, torch):
             "products": res}
 
 
-def delta_q_measure(addft, kw, torch, np, fp64=True):
+def delta_q_measure(addft, kw, torch, np, fp64=True, warmup=1):
     """The matrix-free Delta-Q adjoint at the workload's size: SCF to 1e-10
     (mixed form), then dC/dR and dC/deps of C = sum_r w(r) rho(r) for a
     Gaussian weight (sigma 1.5 Bohr) on atom 5 by the preconditioned inexact
     GMRES adjoint (rtol 1e-10): the mixed-precision response (FP32
     Sternheimer solves inside FP64 flexible CG) timed on the host clock
-    with the device synchronised, and, with fp64, the FP64 response on the
-    same state for the error of the mixed one."""
+    with the device synchronised after `warmup` untimed runs (the one-time
+    tuning of the block products it meets), and, with fp64, the FP64
+    response on the same state for the error of the mixed one."""
     from addft import api, response
     s = addft.setup(**kw)
     torch.cuda.synchronize()
@@ -240,14 +321,21 @@ def delta_q_measure(addft, kw, torch, np, fp64=True):
     eng = st.engine
     w_pl = eng.to_planar(torch.as_tensor(np.asarray(w, dtype=np.float64)).to(eng.dev))
     runs = {}
+    for i in range(warmup):
+        say(f"Delta-Q adjoint, mixed response, warm-up {i + 1}/{warmup}")
+        response.delta_q_grad(eng, st.psi, st.eig, st.occ, st.rho, w_pl, precision="mixed", log=lambda msg: None)
     for prec in (("mixed", "fp64") if fp64 else ("mixed",)):
         say(f"Delta-Q adjoint, {prec} response")
+        if prec == "mixed":
+            mark(torch)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         dR, de, diag = response.delta_q_grad(eng, st.psi, st.eig, st.occ, st.rho, w_pl, precision=prec,
                                              log=lambda msg: None)
         torch.cuda.synchronize()
         runs[prec] = (time.perf_counter() - t0, dR, de, diag)
+        if prec == "mixed":
+            mark(torch)
         say(f"  {runs[prec][0]:.2f} s, {diag['krylov_iterations']} products, residual {diag['adjoint_residual']:.2e}")
     sec, dR, de, diag = runs["mixed"]
     out = {"metric": "Delta-Q adjoint gradient, s after the SCF",
@@ -473,10 +561,22 @@ def main():
     ap.add_argument("--nonlocal-only", action="store_true",
                     help="only the BF16x3 projector products of the FP32 phase (the workload's projectors, "
                          "nband + buffer random bands); for profiling runs")
+    ap.add_argument("--delta-q-warmup", type=int, default=1, help="untimed mixed Delta-Q runs before the timed one")
+    ap.add_argument("--mark", action="store_true",
+                    help="marker kernels around the timed mixed geometries (or the timed mixed Delta-Q run), "
+                         "for --trace-summary")
+    ap.add_argument("--trace-summary", metavar="CSV", default=None,
+                    help="print the kernel-time table of a rocprofv3 --kernel-trace CSV, cut to the region "
+                         "between the --mark kernels, and exit (no GPU used)")
     ap.add_argument("--hpsi-only", action="store_true",
                     help="only the local-Hpsi roofline measurement (and a 1 GiB device copy that calibrates "
                          "the FETCH_SIZE/WRITE_SIZE counters); for profiling runs")
     args = ap.parse_args()
+    if args.trace_summary:
+        print(trace_summary(args.trace_summary))
+        return
+    global _MARK
+    _MARK = args.mark
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: there is no multi-GPU path (k-points or band blocks "
               "would shard across GPUs with one all-reduce of the density per SCF step).", file=sys.stderr)
@@ -505,7 +605,8 @@ def main():
         return
 
     if args.delta_q:
-        print(json.dumps(delta_q_measure(addft, kw, torch, np, fp64=not args.no_fp64)), flush=True)
+        print(json.dumps(delta_q_measure(addft, kw, torch, np, fp64=not args.no_fp64, warmup=args.delta_q_warmup)),
+              flush=True)
         return
 
     if args.subspace_only:
@@ -572,12 +673,14 @@ def main():
         one_geometry(addft, sys_m, args.mixed_tol, torch)
     times, last = [], None
     torch.cuda.synchronize()
+    mark(torch)
     for i in range(args.steps):
         last = one_geometry(addft, sys_m, args.mixed_tol, torch)
         times.append(last[3])
         say(f"mixed step {i + 1}/{args.steps}: F={last[0]:.10f}  {last[3]:.2f} s  "
             f"{last[4]['iterations']} SCF iterations (FP64 from {last[4].get('fp64_from_iteration')})")
     torch.cuda.synchronize()
+    mark(torch)
     sec = float(np.mean(times))
     F, forces, sig, _, info, bands = last
     nproj = int(addft.api._engine(sys_m.static, sys_m.geo).nproj)
@@ -640,7 +743,7 @@ def main():
                                        "force_Ha_per_Bohr": float(np.abs(forces - f64).max()),
                                        "stress_Ha_per_Bohr3": float(np.abs(sig - s64).max())}
     if args.delta_q_line and args.workload == "c5":
-        dq = delta_q_measure(addft, kw, torch, np, fp64=not args.no_fp64)
+        dq = delta_q_measure(addft, kw, torch, np, fp64=not args.no_fp64, warmup=args.delta_q_warmup)
         dq["ratio_to_mixed_geometry"] = round(dq["seconds"] / sec, 3)
         line["delta_q"] = dq
     if args.fd_components and args.workload == "c5":

@@ -114,7 +114,9 @@ int addft_gamma_unpack2(int dtype, const void* d_h, int64_t m, int64_t ng, const
  * The two products of the nonlocal operator (reference
  * physics/nonlocal_op.py:67-76, becp = conj(P) psi and vnl_apply's P s) in
  * FP32 arithmetic as hand-written BF16x3 MFMA kernels (bf16 hi/lo split of
- * every FP32 operand on the fly, FP32 accumulation, relative error ~1e-5).
+ * every FP32 operand, FP32 accumulation, relative error ~1e-5; the FP32
+ * operand is split once per call into the workspace, then the MFMA tiles
+ * read the split images).
  * A projector object holds the split images of one k-point's projector
  * matrix P (nproj x ng complex128, stick order), built once per geometry.
  * gamma != 0 (k = 0): the products run over the half sphere with the rows
@@ -123,13 +125,20 @@ int addft_gamma_unpack2(int dtype, const void* d_h, int64_t m, int64_t ng, const
  * S' = becp' D (P'^H D P' = P^H D P).  Creation synchronises the stream.
  * addft_nonlocal_becp: d_psi (nb, ldpsi) complex64 -> d_becp (nb, ldb)
  *   complex64 = psi P^H (becp^T of nonlocal_op.py:67-69); d_work:
- *   addft_nonlocal_work_bytes(p, nb) bytes (split-K partial sums; the sum
- *   over them is in a fixed order: deterministic).
+ *   addft_nonlocal_work_bytes(p, nb) bytes (the split psi rows and the
+ *   split-K partial sums; the sum over them is in a fixed order:
+ *   deterministic).
  * addft_nonlocal_apply_add: d_out (nb, ldo) complex64 += S P with d_s
  *   (nb, lds) complex64 (nonlocal_op.py:70-76 with s = D becp done by the
- *   caller).  d_out may not alias d_s.
+ *   caller); d_work: addft_nonlocal_work_bytes(p, nb) bytes (the split S
+ *   rows; one workspace serves both products on one stream).  d_out may
+ *   not alias d_s.
+ * real_rows != 0 (gamma objects only): the bands are real in real space
+ *   (psi(-G) = conj psi(G)), so becp = psi P'^H is real and S' is taken as
+ *   real: one operand row per band instead of two (half the flops); becp's
+ *   imaginary parts are written as zeros, S's are not read.
  * errors: null arguments, leading dimensions below nproj / ng, gamma tables
- *   that do not cover the sphere.                                          */
+ *   that do not cover the sphere, real_rows on a k != 0 object.            */
 int addft_projector_create(int gamma, int64_t nproj, int64_t ng, const void* d_P, const int32_t* d_odd,
                            const int32_t* d_plus, const int32_t* d_minus, int64_t nh, void* stream,
                            addft_projector* out);
@@ -137,12 +146,13 @@ int addft_projector_destroy(addft_projector p);
 int addft_projector_info(addft_projector p, int64_t* nproj, int64_t* ng, int* gamma);
 int64_t addft_nonlocal_work_bytes(addft_projector p, int64_t nb);
 int addft_nonlocal_becp(addft_projector p, const void* d_psi, int64_t nb, int64_t ldpsi, void* d_becp,
-                        int64_t ldb, void* d_work, void* stream);
+                        int64_t ldb, int real_rows, void* d_work, void* stream);
 int addft_nonlocal_apply_add(addft_projector p, const void* d_s, int64_t nb, int64_t lds, void* d_out,
-                             int64_t ldo, void* stream);
+                             int64_t ldo, int real_rows, void* d_work, void* stream);
 /* addft_nonlocal_becp_noalloc: the same product as addft_nonlocal_becp with
- * no workspace argument and no allocation of any kind: the split-K partial
- * sums go to a 64 MiB array in device memory that the library holds, one per
+ * no workspace argument and no allocation of any kind (one kernel that
+ * splits psi inside its tile loader, so nothing is staged): the split-K
+ * partial sums go to a 64 MiB array in device memory that the library holds, one per
  * GPU, for the life of the process (its device address is looked up once
  * per device on the first call).  No counterpart in the reference: it is
  * for callers that cannot allocate (HIP graph capture, serving loops,

@@ -70,6 +70,48 @@ def test_full_sphere_products(np_, ng, nbs):
         _check(proj, P, nb, 20 + 10 * i, f"full np {np_}")
 
 
+@pytest.mark.parametrize("name,nbs", [("c2", (5, 24, 353)), ("c3", (150, 400))])
+def test_gamma_real_row_products(name, nbs):
+    """Bands real in real space (psi(-G) = conj psi(G), built through the
+    pair tables): the real-row products (one operand row per band) against
+    complex128 products -- becp real to rounding, and out + S P for a real S
+    -- and against the general two-row kernels on the same operands.  Band
+    counts straddle the 352-band real-row tiles."""
+    import torch
+    from addft.kernels.nonlocal_bf16 import ProjectorMFMA
+    eng = _engine(name)
+    P = eng.P64[0]
+    ph = torch.where(eng.odd_rows_d, torch.tensor(1j, dtype=P.dtype, device=P.device),
+                     torch.tensor(1.0, dtype=P.dtype, device=P.device))
+    Pe = P * ph[:, None]
+    plus, minus = (torch.as_tensor(np.asarray(t), dtype=torch.long, device="cuda") for t in eng.gamma_pairs)
+    proj = ProjectorMFMA(P, eng.gamma_pairs, eng.odd_rows_d)
+    for i, nb in enumerate(nbs):
+        h = _rand((nb, plus.numel()), torch.complex128, 40 + i)
+        h[:, 0] = h[:, 0].real                                    # G = 0 is its own partner
+        psi = torch.zeros((nb, eng.ng), dtype=torch.complex128, device="cuda")
+        psi[:, plus] = h
+        keep = minus >= 0
+        psi[:, minus[keep]] = h[:, keep].conj()
+        psi32 = psi.to(torch.complex64)
+        ref = psi32.to(torch.complex128) @ Pe.conj().T
+        assert ref.imag.abs().max().item() < 1e-9 * ref.abs().max().item()
+        got = proj.becp_t(psi32, real=True)
+        err = ((got.to(torch.complex128) - ref).abs().max() / ref.abs().max()).item()
+        assert got.imag.abs().max().item() == 0.0
+        two = proj.becp_t(psi32)
+        d2 = ((got - two).abs().max() / two.abs().max()).item()
+        S = _rand((nb, P.shape[0]), torch.complex128, 50 + i).real.to(torch.complex64)
+        out0 = _rand((nb, eng.ng), torch.complex64, 60 + i)
+        out = out0.clone()
+        proj.apply_add(S, out, real=True)
+        ref2 = S.to(torch.complex128) @ Pe
+        err2 = ((out.to(torch.complex128) - out0.to(torch.complex128) - ref2).abs().max()
+                / ref2.abs().max()).item()
+        print(f"{name} real rows nb {nb}: becp {err:.2e} (vs two-row {d2:.1e}), apply {err2:.2e}", flush=True)
+        assert err < 1e-4 and err2 < 1e-4 and d2 < 1e-5, (name, nb, err, err2, d2)
+
+
 def test_mfma_matches_library_split_gemms():
     """The MFMA kernels and the hipBLASLt BF16x3 split GEMMs of the same
     half-sphere operands agree to the split's rounding."""
