@@ -584,15 +584,19 @@ class Response:
             self.stats["hpsi_bands_fp32"] += int(p.shape[0])
             z = torch.empty_like(r)
             rr = native.cg_update(x, r, p, Ap, rz, pap, minv, z)
-            done = rr <= stop2
-            nd = int(done.sum())
-            if nd:
-                out[idx[done]] = x[done]
-                if nd == x.shape[0]:
-                    break
-                keep = ~done
-                x, r, p, rz, ec, stop2, idx, minv, z = (x[keep], r[keep], p[keep], rz[keep], ec[keep], stop2[keep],
-                                                       idx[keep], minv[keep], z[keep])
+            keep = rr > stop2
+            nk = int(keep.sum())
+            if nk == 0:
+                out.index_copy_(0, idx, x)
+                break
+            # compact once a sixteenth of the block has converged: bands past
+            # their goal meanwhile keep iterating (their residual only falls),
+            # and each compaction costs one gather per working array
+            if x.shape[0] - nk >= max(1, x.shape[0] // 16):
+                out.index_copy_(0, idx, x)
+                ki = torch.nonzero(keep).flatten()
+                x, r, p, rz, ec, stop2, idx, minv, z = (t.index_select(0, ki) for t in
+                                                       (x, r, p, rz, ec, stop2, idx, minv, z))
             z = proj(z)
             native.cg_direction(r, z, p, rz)
         else:
