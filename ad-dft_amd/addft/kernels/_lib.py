@@ -19,7 +19,7 @@ _LIB = None
 _PATH = None
 
 C64, C128, R64 = 0, 1, 2
-EIGH_DC, EIGH_JACOBI, EIGH_DCJ = 0, 1, 2
+EIGH_DC, EIGH_JACOBI, EIGH_DCJ, EIGH_TRD = 0, 1, 2, 3
 PLANE_LDS, PLANE_LANE = 0, 1
 
 

@@ -109,13 +109,18 @@ def rcombine(C, B):
 
 # ---------------------------------------------------- small eigensolves
 _EIGH_METHODS = ("dc", "jacobi", "dcj")
+# one real symmetric matrix: the library's one-launch tridiagonalisation
+# before rocSOLVER's dstedc / dormtr
+_EIGH_METHODS_REAL1 = _EIGH_METHODS + ("trd",)
 
 
 def eigh_device(T, method="dc"):
     """(w ascending, U with eigenvectors as columns, info) of a Hermitian
     complex128 or real symmetric float64 matrix, or of a batch (b, n, n) of
     them, through addft_subspace_eigh (rocSOLVER divide and conquer, Jacobi
-    or divide and conquer with Jacobi, batched over the leading dimension).
+    or divide and conquer with Jacobi, batched over the leading dimension;
+    "trd" for one real matrix: the library's cooperative tridiagonalisation,
+    then rocSOLVER's tridiagonal divide and conquer).
     T is not modified.  No host synchronisation: info (b,) int32 stays on the
     device."""
     from ..kernels import _lib
@@ -131,7 +136,7 @@ def eigh_device(T, method="dc"):
     info = torch.empty(b, dtype=torch.int32, device=A.device)
     L = _lib.lib()
     work = torch.empty(int(L.addft_subspace_eigh_work_bytes(n, b)), dtype=torch.uint8, device=A.device)
-    code = {"dc": _lib.EIGH_DC, "jacobi": _lib.EIGH_JACOBI, "dcj": _lib.EIGH_DCJ}[method]
+    code = {"dc": _lib.EIGH_DC, "jacobi": _lib.EIGH_JACOBI, "dcj": _lib.EIGH_DCJ, "trd": _lib.EIGH_TRD}[method]
     _lib.check(L.addft_subspace_eigh(_lib.R64 if real else _lib.C128, code, n, _lib.ptr(A), n * n, b,
                                      _lib.ptr(w), _lib.ptr(info), _lib.ptr(work), _lib.stream()),
                "addft_subspace_eigh")
@@ -155,4 +160,27 @@ def eigh_method(T):
         return "dc"
     # one decision per size class (the Davidson subspace grows in small steps)
     key = f"eigh:{'real' if not T.is_complex() else 'complex'}:{(n + 127) // 128 * 128}:{b}"
-    return tuning.decide(key, {m: (lambda m=m: eigh_device(T, m)) for m in _EIGH_METHODS})
+    made = tuning.decisions().get(key)
+    if made is not None and made["choice"] in _EIGH_METHODS_REAL1:
+        return made["choice"]
+    methods = _EIGH_METHODS
+    if not T.is_complex() and b == 1 and _trd_agrees(T):
+        methods = _EIGH_METHODS_REAL1
+    return tuning.decide(key, {m: (lambda m=m: eigh_device(T, m)) for m in methods})
+
+
+def _trd_agrees(T):
+    """The tridiagonal path is offered to the tuner only after it reproduced
+    rocSOLVER's divide-and-conquer eigenvalues on T (1e-10 relative) with an
+    orthonormal basis (1e-10) and info 0: a method that failed could
+    otherwise win the timing."""
+    try:
+        w1, U1, i1 = eigh_device(T, "trd")
+        w0, _, i0 = eigh_device(T, "dc")
+    except RuntimeError:
+        return False
+    if bool((i1 != 0).any()) or bool((i0 != 0).any()):
+        return False
+    scale = float(w0.abs().max()) or 1.0
+    orth = torch.linalg.matrix_norm(U1.T @ U1 - torch.eye(U1.shape[0], dtype=U1.dtype, device=U1.device), ord=2)
+    return float((w1 - w0).abs().max()) <= 1e-10 * scale and float(orth) <= 1e-10

@@ -69,6 +69,246 @@ __global__ void k_sum_slices(const T* __restrict__ part, int nslice, int64_t m, 
   }
 }
 
+// ------------------------------------------------ one-launch tridiagonalisation
+// Householder reduction of a real symmetric n x n matrix to tridiagonal form
+// (the lower-storage form of LAPACK dsytd2: Q^T A Q = T, Q = H_0 ... H_{n-2},
+// H_k = I - tau_k v_k v_k^T, v_k(k+1) = 1) in ONE cooperative launch: G
+// workgroups own the matrix columns cyclically (column c in workgroup c mod G)
+// and keep them, full length, in LDS for the whole reduction.  Step k:
+//   the owner of column k (already updated by steps < k) builds v_k, tau_k
+//   and T's d_k, e_k, writes v_k for all and the reflector into A's column
+//   k below the subdiagonal (where rocSOLVER's dormtr reads it);   barrier;
+//   every workgroup forms p_i = tau_k sum_j A_ji v_j for its own columns
+//   i > k (one wavefront per column, shuffle reductions) and its share of
+//   p.v;   barrier;
+//   every workgroup forms w = p - (tau_k/2)(p.v) v and applies the rank-2
+//   update A -= v w^T + w v^T to its own columns.
+// Two grid barriers per column instead of a few kernel launches per column,
+// and no matrix traffic beyond one read.  The barrier is a counter and a
+// generation word in device memory (agent-scope atomics, release / acquire
+// fences around it so that the vectors exchanged through global memory are
+// seen fresh); every wait is bounded: a workgroup that polls past the bound
+// raises an error flag, every workgroup leaves at its next barrier, and the
+// eigensolve reports failure in its info word.
+constexpr int kTrdThreads = 256;
+constexpr int kTrdWaves = kTrdThreads / 64;
+constexpr unsigned kTrdSpinMax = 1u << 22;
+constexpr int kTrdMaxGroups = 256;  // entries of the partial-sum exchange buffer
+
+struct TrdSync {
+  unsigned count;
+  unsigned gen;
+  int err;
+  int pad;
+};
+
+__device__ bool trd_barrier(TrdSync* sy, unsigned nwg) {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  __syncthreads();
+  __shared__ int ok_s;
+  if (threadIdx.x == 0) {
+    int ok = __hip_atomic_load(&sy->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0;
+    if (ok) {
+      const unsigned g0 = __hip_atomic_load(&sy->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+      const unsigned prev = __hip_atomic_fetch_add(&sy->count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+      if (prev == nwg - 1) {
+        __hip_atomic_store(&sy->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_add(&sy->gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      } else {
+        unsigned spins = 0;
+        while (__hip_atomic_load(&sy->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g0) {
+          if (++spins > kTrdSpinMax) {
+            __hip_atomic_store(&sy->err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            ok = 0;
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+        }
+      }
+    }
+    ok_s = ok;
+  }
+  __syncthreads();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  return ok_s != 0;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+// sum over the workgroup (every thread gets it)
+__device__ double block_sum(double v, double* red) {
+  v = wave_sum(v);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) red[w] = v;
+  __syncthreads();
+  double s = 0.0;
+#pragma unroll
+  for (int q = 0; q < kTrdWaves; ++q) s += red[q];
+  __syncthreads();
+  return s;
+}
+
+// A: n x n symmetric, column-major (lda n); d (n), e (n - 1), tau (n - 1);
+// vbuf, pbuf (n), part (gridDim.x) exchange buffers; LDS: maxc columns of n,
+// then v and w (n each).
+__global__ __launch_bounds__(kTrdThreads) void k_sytrd(double* __restrict__ A, int n, int maxc, double* d,
+                                                       double* e, double* tau, double* vbuf, double* pbuf,
+                                                       double* part, TrdSync* sy) {
+  extern __shared__ double lds[];
+  double* cols = lds;
+  double* vs = lds + (size_t)maxc * n;
+  double* ws = vs + n;
+  __shared__ double red[kTrdWaves];
+  const int G = gridDim.x, g = blockIdx.x, t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const int nc = (n - g + G - 1) / G;
+  // own columns from the lower triangle (column c: A(j, c) for j >= c, A(c, j)
+  // above), so that a matrix symmetric only to rounding is read as one
+  // symmetric matrix; the barrier keeps the reflector writes below after
+  // every load
+  for (int l = 0; l < nc; ++l) {
+    const int c = g + l * G;
+    for (int j = t; j < n; j += kTrdThreads)
+      cols[(size_t)l * n + j] = j >= c ? A[(size_t)c * n + j] : A[(size_t)j * n + c];
+  }
+  if (!trd_barrier(sy, G)) return;
+  for (int k = 0; k < n - 1; ++k) {
+    const int m0 = k + 1;
+    if (k % G == g) {
+      // Householder vector of column k (dlarfg): beta = -sign(alpha) |x|
+      const double* x = cols + (size_t)(k / G) * n;
+      double s = 0.0;
+      for (int j = m0 + 1 + t; j < n; j += kTrdThreads) s += x[j] * x[j];
+      s = block_sum(s, red);
+      const double alpha = x[m0];
+      double tk = 0.0, beta = alpha, scale = 0.0;
+      if (s > 0.0) {
+        beta = -copysign(sqrt(alpha * alpha + s), alpha);
+        tk = (beta - alpha) / beta;
+        scale = 1.0 / (alpha - beta);
+      }
+      if (t == 0) {
+        d[k] = x[k];
+        e[k] = beta;
+        tau[k] = tk;
+      }
+      for (int j = m0 + t; j < n; j += kTrdThreads) {
+        const double v = j == m0 ? 1.0 : x[j] * scale;
+        vbuf[j] = v;
+        if (j > m0) A[(size_t)k * n + j] = v;
+      }
+    }
+    if (!trd_barrier(sy, G)) return;
+    const double tk = tau[k];
+    for (int j = m0 + t; j < n; j += kTrdThreads) vs[j] = vbuf[j];
+    __syncthreads();
+    // p_i for the own columns i > k: one wavefront per column
+    const int l0 = (m0 - g + G - 1) / G;  // first own column index with c >= m0
+    double pv = 0.0;
+    for (int l = l0 + wv; l < nc; l += kTrdWaves) {
+      const double* col = cols + (size_t)l * n;
+      double s = 0.0;
+      for (int j = m0 + lane; j < n; j += 64) s += col[j] * vs[j];
+      s = wave_sum(s);
+      const int c = g + l * G;
+      if (lane == 0) {
+        const double p = tk * s;
+        pbuf[c] = p;
+        pv += p * vs[c];
+      }
+    }
+    if (lane != 0) pv = 0.0;
+    pv = block_sum(pv, red);
+    if (t == 0) part[g] = pv;
+    if (!trd_barrier(sy, G)) return;
+    double sp = 0.0;
+    for (int q = t; q < G; q += kTrdThreads) sp += part[q];
+    sp = block_sum(sp, red);
+    const double gamma = 0.5 * tk * sp;
+    for (int j = m0 + t; j < n; j += kTrdThreads) ws[j] = pbuf[j] - gamma * vs[j];
+    __syncthreads();
+    for (int l = l0; l < nc; ++l) {
+      const int c = g + l * G;
+      const double wi = ws[c], vi = vs[c];
+      double* col = cols + (size_t)l * n;
+      for (int j = m0 + t; j < n; j += kTrdThreads) col[j] -= vs[j] * wi + ws[j] * vi;
+    }
+    __syncthreads();
+  }
+  if ((n - 1) % G == g && t == 0) d[n - 1] = cols[(size_t)((n - 1) / G) * n + n - 1];
+}
+
+__global__ void k_trd_init(TrdSync* sy) {
+  if (threadIdx.x == 0) {
+    sy->count = 0;
+    sy->gen = 0;
+    sy->err = 0;
+    sy->pad = 0;
+  }
+}
+
+__global__ void k_trd_report(const TrdSync* sy, int32_t* info) {
+  if (threadIdx.x == 0 && sy->err) info[0] = -1;
+}
+
+// LDS budget of one workgroup of k_sytrd (one per CU)
+constexpr size_t kTrdLdsBytes = 156 * 1024;
+
+// columns per workgroup and workgroup count for an n x n matrix; 0 if the
+// matrix does not fit the device's LDS
+void trd_shape(int64_t n, int ncu, int* maxc, int* G) {
+  const int64_t per = (int64_t)(kTrdLdsBytes / 8) - 2 * n;
+  *maxc = per > 0 ? (int)std::min<int64_t>(per / n, n) : 0;
+  *G = *maxc > 0 ? (int)((n + *maxc - 1) / *maxc) : 0;
+  if (*G > ncu || *G > kTrdMaxGroups) *G = 0;
+}
+
+int device_cu_count() {
+  int dev = 0, n = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+  return n;
+}
+
+// bytes of the tridiagonal path's workspace for one n x n matrix
+int64_t trd_work_bytes(int64_t n) { return (n * n + 5 * n + 256) * 8 + 256; }
+
+// Eigenpairs of one real symmetric matrix: k_sytrd, rocSOLVER's divide and
+// conquer on T (dstedc) and the back-transformation (dormtr), eigenvectors
+// copied into A (column i = eigenvector i, the layout dsyevd leaves).
+int eigh_trd(rocblas_handle h, hipStream_t s, int64_t n, double* A, double* w, int32_t* info, char* work) {
+  int maxc = 0, G = 0;
+  trd_shape(n, device_cu_count(), &maxc, &G);
+  ADDFT_CHECK_ARG(G > 0, "ADDFT_EIGH_TRD: matrix too large for the one-launch tridiagonalisation");
+  double* C = reinterpret_cast<double*>(work);
+  double* E = C + n * n;
+  double* tau = E + n;
+  double* vbuf = tau + n;
+  double* pbuf = vbuf + n;
+  double* part = pbuf + n;  // kTrdMaxGroups
+  TrdSync* sy = reinterpret_cast<TrdSync*>(part + kTrdMaxGroups);
+  hipLaunchKernelGGL(k_trd_init, dim3(1), dim3(64), 0, s, sy);
+  ADDFT_LAUNCH_CHECK();
+  const size_t lds = ((size_t)maxc * n + 2 * n) * sizeof(double);
+  ADDFT_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_sytrd), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)lds));
+  int nn = (int)n;
+  void* args[] = {&A, &nn, &maxc, &w, &E, &tau, &vbuf, &pbuf, &part, &sy};
+  ADDFT_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_sytrd), dim3(G), dim3(kTrdThreads), args,
+                                       (unsigned)lds, s));
+  const rocblas_int N = (rocblas_int)n;
+  ADDFT_RB(rocsolver_dstedc(h, rocblas_evect_tridiagonal, N, w, E, C, N, info));
+  ADDFT_RB(rocsolver_dormtr(h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none, N, N, A, N, tau, C,
+                            N));
+  hipLaunchKernelGGL(k_trd_report, dim3(1), dim3(64), 0, s, sy, info);
+  ADDFT_LAUNCH_CHECK();
+  ADDFT_HIP(hipMemcpyAsync(A, C, (size_t)n * n * sizeof(double), hipMemcpyDeviceToDevice, s));
+  return ADDFT_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -178,8 +418,10 @@ int addft_subspace_rotate(int dtype, int64_t m, int64_t n, int64_t K, const void
 
 int64_t addft_subspace_eigh_work_bytes(int64_t n, int64_t batch) {
   // off-diagonal of the tridiagonal form (divide and conquer), Jacobi
-  // residual and sweep count, and the info flags, per matrix
-  return batch * (n * 8 + 16 + 8) + 256;
+  // residual and sweep count, and the info flags, per matrix; for one
+  // matrix also the tridiagonal path's eigenvectors of T and exchange buffers
+  const int64_t rocsolver_bytes = batch * (n * 8 + 16 + 8) + 256;
+  return batch == 1 ? std::max(rocsolver_bytes, trd_work_bytes(n)) : rocsolver_bytes;
 }
 
 // Eigenpairs of batch symmetric (dtype ADDFT_R64) or Hermitian (ADDFT_C128)
@@ -196,8 +438,11 @@ int64_t addft_subspace_eigh_work_bytes(int64_t n, int64_t batch) {
 int addft_subspace_eigh(int dtype, int method, int64_t n, void* d_a, int64_t stride_a, int64_t batch, double* d_w,
                         int32_t* d_info, void* d_work, void* stream) {
   ADDFT_CHECK_ARG(dtype == ADDFT_R64 || dtype == ADDFT_C128, "dtype must be ADDFT_R64 or ADDFT_C128");
-  ADDFT_CHECK_ARG(method == ADDFT_EIGH_DC || method == ADDFT_EIGH_JACOBI || method == ADDFT_EIGH_DCJ,
+  ADDFT_CHECK_ARG(method == ADDFT_EIGH_DC || method == ADDFT_EIGH_JACOBI || method == ADDFT_EIGH_DCJ ||
+                      method == ADDFT_EIGH_TRD,
                   "unknown eigensolver method");
+  ADDFT_CHECK_ARG(method != ADDFT_EIGH_TRD || (dtype == ADDFT_R64 && batch == 1),
+                  "ADDFT_EIGH_TRD takes one real symmetric matrix");
   ADDFT_CHECK_ARG(n > 0 && n <= 65536 && batch > 0 && batch <= INT32_MAX && stride_a >= n * n, "bad sizes");
   ADDFT_CHECK_ARG(d_a && d_w && d_info && d_work, "null argument");
   hipStream_t s = addft::as_stream(stream);
@@ -214,6 +459,7 @@ int addft_subspace_eigh(int dtype, int method, int64_t n, void* d_a, int64_t str
   const auto asc = rocblas_esort_ascending;
   if (dtype == ADDFT_R64) {
     double* A = static_cast<double*>(d_a);
+    if (method == ADDFT_EIGH_TRD) return eigh_trd(h, s, n, A, d_w, d_info, w);
     if (batch == 1) {
       if (method == ADDFT_EIGH_DC) ADDFT_RB(rocsolver_dsyevd(h, ev, up, N, A, N, d_w, E, d_info));
       else if (method == ADDFT_EIGH_DCJ) ADDFT_RB(rocsolver_dsyevdj(h, ev, up, N, A, N, d_w, d_info));

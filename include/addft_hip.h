@@ -46,7 +46,7 @@ enum {
 };
 
 enum { ADDFT_C64 = 0, ADDFT_C128 = 1, ADDFT_R64 = 2 };
-enum { ADDFT_EIGH_DC = 0, ADDFT_EIGH_JACOBI = 1, ADDFT_EIGH_DCJ = 2 };
+enum { ADDFT_EIGH_DC = 0, ADDFT_EIGH_JACOBI = 1, ADDFT_EIGH_DCJ = 2, ADDFT_EIGH_TRD = 3 };
 
 typedef struct addft_sphere_plan_s* addft_sphere_plan;
 typedef struct addft_projector_s* addft_projector;
@@ -179,7 +179,11 @@ int64_t addft_nonlocal_noalloc_scratch_bytes(void);
  *   Hermitian (ADDFT_C128) row-major n x n matrices (stride stride_a
  *   elements), in place: row i = eigenvector of w[i] (conjugated for
  *   complex), w ascending (batch x n), info per matrix on the device;
- *   method ADDFT_EIGH_DC / _JACOBI / _DCJ; d_work:
+ *   method ADDFT_EIGH_DC / _JACOBI / _DCJ (rocSOLVER syevd / syevj /
+ *   syevdj) or ADDFT_EIGH_TRD (one real symmetric matrix: a one-launch
+ *   cooperative Householder tridiagonalisation that keeps the matrix in
+ *   the LDS of its workgroups, then rocSOLVER's dstedc and dormtr; info
+ *   is set non-zero if its grid barrier gives up); d_work:
  *   addft_subspace_eigh_work_bytes(n, batch) bytes.                       */
 int64_t addft_subspace_gram_work_bytes(int dtype, int64_t m, int64_t n, int nslice);
 int addft_subspace_gram(int dtype, int64_t m, int64_t n, int64_t K, const void* d_a, int64_t lda,

@@ -69,3 +69,39 @@ def test_subspace_eigh(real, batch, method):
     assert res.abs().max().item() < 1e-10 * wr.abs().max().item()
     eye = torch.eye(n, dtype=U.dtype, device="cuda")
     assert (U.conj().transpose(1, 2) @ U - eye).abs().max().item() < 1e-11
+
+
+@pytest.mark.parametrize("n", [17, 96, 257, 528, 1056])
+def test_subspace_eigh_trd(n):
+    """ADDFT_EIGH_TRD (the one-launch cooperative tridiagonalisation, then
+    rocSOLVER's dstedc / dormtr) on one real symmetric matrix: eigenvalues
+    against torch (1e-11 relative), eigen-residuals (1e-10), orthonormal
+    vectors (1e-11); sizes that leave some workgroups one column more than
+    others.  A second matrix has a cluster of 8-fold near-degenerate levels
+    (split by 1e-9) and is symmetric only to rounding: one triangle is read
+    (the column-major lower one, i.e. the row-major upper one), so its result
+    equals that triangle's symmetric matrix's."""
+    import torch
+    from addft.solver.linalg import eigh_device
+    g = torch.Generator(device="cpu").manual_seed(n)
+    X = torch.randn(n, n, generator=g, dtype=torch.float64)
+    mats = [0.5 * (X + X.T)]
+    Q, _ = torch.linalg.qr(torch.randn(n, n, generator=g, dtype=torch.float64))
+    lev = torch.linspace(-2.0, 2.0, n, dtype=torch.float64)
+    lev[: min(8, n)] = -3.0 + 1e-9 * torch.arange(min(8, n), dtype=torch.float64)
+    S = (Q * lev) @ Q.T
+    noisy = S + 1e-16 * torch.randn(n, n, generator=g, dtype=torch.float64)  # not exactly symmetric
+    mats.append(noisy)
+    for i, T in enumerate(mats):
+        T = T.cuda().contiguous()
+        Ts = torch.triu(T) + torch.triu(T, 1).T                               # the triangle read, mirrored
+        w, U, info = eigh_device(T, "trd")
+        assert int(info.item()) == 0, (n, i)
+        wr = torch.linalg.eigvalsh(Ts)
+        scale = wr.abs().max().item()
+        print(f"n {n} matrix {i}: eigenvalue err {(w - wr).abs().max().item() / scale:.2e}", flush=True)
+        assert ((w - wr).abs().max() / scale).item() < 1e-11
+        res = Ts @ U - U * w[None, :]
+        assert res.abs().max().item() < 1e-10 * scale
+        eye = torch.eye(n, dtype=U.dtype, device="cuda")
+        assert (U.T @ U - eye).abs().max().item() < 1e-11
