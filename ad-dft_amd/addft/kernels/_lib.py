@@ -19,7 +19,7 @@ _LIB = None
 _PATH = None
 
 C64, C128, R64 = 0, 1, 2
-EIGH_DC, EIGH_JACOBI, EIGH_DCJ, EIGH_TRD = 0, 1, 2, 3
+EIGH_DC, EIGH_JACOBI, EIGH_DCJ, EIGH_TRD, EIGH_LDS = 0, 1, 2, 3, 4
 PLANE_LDS, PLANE_LANE = 0, 1
 
 
@@ -102,6 +102,7 @@ _SIGS = {
     "addft_subspace_rotate": ([_I32, _I64, _I64, _I64, _VP, _I64, _VP, _I64, _VP, _I64, _VP], _I32),
     "addft_subspace_eigh_work_bytes": ([_I64, _I64], _I64),
     "addft_subspace_eigh": ([_I32, _I32, _I64, _VP, _I64, _I64, _VP, _VP, _VP, _VP], _I32),
+    "addft_subspace_chol_inv": ([_I32, _I64, _VP, _I64, _I64, _VP, _VP, _VP], _I32),
     "addft_nonlocal_becp_noalloc": ([_VP, _VP, _I64, _I64, _VP, _I64, _VP], _I32),
     "addft_nonlocal_noalloc_scratch_bytes": ([], _I64),
 }

@@ -37,8 +37,14 @@ def eigh(T):
 
 def chol_inverse(S):
     """L^-1 of the Cholesky factors of the Hermitian batch S, or None when a
-    factorisation fails (device solver: batched factorisation and triangular
-    solves are fast there, unlike the batched eigensolver)."""
+    factorisation fails: for n <= 64 one launch of the library's
+    one-workgroup-per-matrix factorisation and inversion
+    (addft_subspace_chol_inv), else the device solver's batched
+    factorisation and triangular solve."""
+    from .linalg import _EIGH_SMALL_MAX, chol_inverse_small
+    if S.shape[-1] <= _EIGH_SMALL_MAX:
+        Linv, info = chol_inverse_small(S)
+        return None if bool((info != 0).any()) else Linv
     L, info = torch.linalg.cholesky_ex(S)
     if bool((info != 0).any()):
         return None

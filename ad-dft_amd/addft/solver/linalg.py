@@ -109,6 +109,9 @@ def rcombine(C, B):
 
 # ---------------------------------------------------- small eigensolves
 _EIGH_METHODS = ("dc", "jacobi", "dcj")
+# n <= 64 (the per-k-point blocks of the batched solver): the library's
+# one-workgroup-per-matrix Jacobi in LDS
+_EIGH_SMALL_MAX = 64
 # one real symmetric matrix: the library's one-launch tridiagonalisation
 # before rocSOLVER's dstedc / dormtr
 _EIGH_METHODS_REAL1 = _EIGH_METHODS + ("trd",)
@@ -136,7 +139,8 @@ def eigh_device(T, method="dc"):
     info = torch.empty(b, dtype=torch.int32, device=A.device)
     L = _lib.lib()
     work = torch.empty(int(L.addft_subspace_eigh_work_bytes(n, b)), dtype=torch.uint8, device=A.device)
-    code = {"dc": _lib.EIGH_DC, "jacobi": _lib.EIGH_JACOBI, "dcj": _lib.EIGH_DCJ, "trd": _lib.EIGH_TRD}[method]
+    code = {"dc": _lib.EIGH_DC, "jacobi": _lib.EIGH_JACOBI, "dcj": _lib.EIGH_DCJ, "trd": _lib.EIGH_TRD,
+            "lds": _lib.EIGH_LDS}[method]
     _lib.check(L.addft_subspace_eigh(_lib.R64 if real else _lib.C128, code, n, _lib.ptr(A), n * n, b,
                                      _lib.ptr(w), _lib.ptr(info), _lib.ptr(work), _lib.stream()),
                "addft_subspace_eigh")
@@ -159,12 +163,15 @@ def eigh_method(T):
     if not tuning.enabled() or (n < 256 and b < 8) or torch.cuda.is_current_stream_capturing():
         return "dc"
     # one decision per size class (the Davidson subspace grows in small steps)
-    key = f"eigh:{'real' if not T.is_complex() else 'complex'}:{(n + 127) // 128 * 128}:{b}"
+    cls = _EIGH_SMALL_MAX if n <= _EIGH_SMALL_MAX else (n + 127) // 128 * 128
+    key = f"eigh:{'real' if not T.is_complex() else 'complex'}:{cls}:{b}"
     made = tuning.decisions().get(key)
-    if made is not None and made["choice"] in _EIGH_METHODS_REAL1:
+    if made is not None and made["choice"] in _EIGH_METHODS_REAL1 + ("lds",):
         return made["choice"]
     methods = _EIGH_METHODS
-    if not T.is_complex() and b == 1 and _trd_agrees(T):
+    if n <= _EIGH_SMALL_MAX:
+        methods = methods + ("lds",)
+    elif not T.is_complex() and b == 1 and _trd_agrees(T):
         methods = _EIGH_METHODS_REAL1
     return tuning.decide(key, {m: (lambda m=m: eigh_device(T, m)) for m in methods})
 
@@ -184,3 +191,21 @@ def _trd_agrees(T):
     scale = float(w0.abs().max()) or 1.0
     orth = torch.linalg.matrix_norm(U1.T @ U1 - torch.eye(U1.shape[0], dtype=U1.dtype, device=U1.device), ord=2)
     return float((w1 - w0).abs().max()) <= 1e-10 * scale and float(orth) <= 1e-10
+
+
+def chol_inverse_small(S):
+    """(L^-1, info) of the Cholesky factors S = L L^H of a batch (b, n, n)
+    of Hermitian complex128 or symmetric float64 matrices with n <= 64, one
+    launch (addft_subspace_chol_inv: one workgroup per matrix in LDS); info
+    (b,) int32 on the device, k + 1 for the first non-positive pivot."""
+    from ..kernels import _lib
+    A = S.contiguous()
+    real = not A.is_complex()
+    A = A.to(torch.float64) if real else A.to(torch.complex128)
+    b, n = A.shape[0], A.shape[-1]
+    Linv = torch.empty_like(A)
+    info = torch.empty(b, dtype=torch.int32, device=A.device)
+    _lib.check(_lib.lib().addft_subspace_chol_inv(_lib.R64 if real else _lib.C128, n, _lib.ptr(A), n * n, b,
+                                                  _lib.ptr(Linv), _lib.ptr(info), _lib.stream()),
+               "addft_subspace_chol_inv")
+    return Linv, info

@@ -309,6 +309,217 @@ int eigh_trd(rocblas_handle h, hipStream_t s, int64_t n, double* A, double* w, i
   return ADDFT_OK;
 }
 
+// ------------------------------------------- small matrices in one workgroup
+// Batched eigensolves and Cholesky inverses of small matrices (n <= 64: the
+// per-k-point Rayleigh-Ritz and Gram blocks of the batched k-point solver),
+// one workgroup per matrix with the matrix in LDS: one launch per batch
+// instead of rocSOLVER's launches per matrix and column.
+constexpr int kSmallMax = 64;
+constexpr int kSmallThreads = 256;
+constexpr int kSmallLd = kSmallMax + 1;  // padded row: column accesses spread over banks
+
+// scalar helpers over double and cpx<double>
+__device__ __forceinline__ double sm_conj(double x) { return x; }
+__device__ __forceinline__ addft::cpx<double> sm_conj(addft::cpx<double> x) { return addft::mk(x.x, -x.y); }
+__device__ __forceinline__ double sm_re(double x) { return x; }
+__device__ __forceinline__ double sm_re(addft::cpx<double> x) { return x.x; }
+__device__ __forceinline__ double sm_abs(double x) { return fabs(x); }
+__device__ __forceinline__ double sm_abs(addft::cpx<double> x) { return hypot(x.x, x.y); }
+__device__ __forceinline__ double sm_mul(double a, double b) { return a * b; }
+__device__ __forceinline__ addft::cpx<double> sm_mul(addft::cpx<double> a, addft::cpx<double> b) {
+  return addft::cmul(a, b);
+}
+__device__ __forceinline__ addft::cpx<double> sm_scale(addft::cpx<double> a, double s) { return a * s; }
+__device__ __forceinline__ double sm_scale(double a, double s) { return a * s; }
+template <typename T>
+__device__ __forceinline__ T sm_unit();
+template <>
+__device__ __forceinline__ double sm_unit<double>() { return 1.0; }
+template <>
+__device__ __forceinline__ addft::cpx<double> sm_unit<addft::cpx<double>>() { return addft::mk(1.0, 0.0); }
+template <typename T>
+__device__ __forceinline__ T sm_zero() { return sm_scale(sm_unit<T>(), 0.0); }
+template <typename T>
+__device__ __forceinline__ T sm_sub(T a, T b) { return a - b; }
+template <>
+__device__ __forceinline__ double sm_sub<double>(double a, double b) { return a - b; }
+
+// Cyclic Jacobi on A (Hermitian / symmetric, row-major n x n, stride
+// elements apart per matrix): parallel rounds of the circle-method pairing
+// (n/2 disjoint rotations per round, n - 1 rounds per sweep), a rotation
+// skipped when |a_pq| <= eps sqrt(|a_pp a_qq|), sweeps until one performs
+// no rotation (at most 40).  Each rotation is J = D R with D = diag(1,
+// e^{-i phi}) taking a_pq = |a_pq| e^{i phi} to the real 2 x 2 problem and R
+// the real Jacobi rotation (tan of its angle t = sign(th) / (|th| +
+// sqrt(th^2 + 1)), th = (a_qq - a_pp) / (2 |a_pq|)); A <- J^H A J (all rows,
+// then all columns of the round), V <- V J.  Output in place: ascending w,
+// row i of A = conj(v_i) (the layout of addft_subspace_eigh); info 1 if the
+// sweeps ran out.
+template <typename T>
+__global__ __launch_bounds__(kSmallThreads) void k_jacobi_small(T* __restrict__ Ag, int n, int64_t stride,
+                                                                double* __restrict__ wg, int32_t* __restrict__ info) {
+  __shared__ T A[kSmallMax * kSmallLd];
+  __shared__ T V[kSmallMax * kSmallLd];
+  __shared__ double cs[kSmallMax / 2][2];
+  __shared__ T ph[kSmallMax / 2];
+  __shared__ int pq[kSmallMax / 2][2];
+  __shared__ int nrot;
+  __shared__ double wv[kSmallMax];
+  const int t = threadIdx.x;
+  T* Am = Ag + (int64_t)blockIdx.x * stride;
+  for (int i = t; i < n * n; i += kSmallThreads) {
+    const int r = i / n, c = i - r * n;
+    A[r * kSmallLd + c] = Am[i];
+    V[r * kSmallLd + c] = r == c ? sm_unit<T>() : sm_zero<T>();
+  }
+  __syncthreads();
+  const int npad = n + (n & 1), npair = npad / 2;
+  const double eps = 2.2204460492503131e-16;
+  int sweep = 0;
+  for (; sweep < 40; ++sweep) {
+    if (t == 0) nrot = 0;
+    __syncthreads();
+    for (int r = 0; r < npad - 1; ++r) {
+      if (t < npair) {
+        const int k = t;
+        int p = k == 0 ? 0 : ((k - 1 + r) % (npad - 1)) + 1;
+        const int kq = npad - 1 - k;
+        int q = ((kq - 1 + r) % (npad - 1)) + 1;
+        if (p > q) {
+          const int x = p;
+          p = q;
+          q = x;
+        }
+        double c = 1.0, sn = 0.0;
+        T e = sm_unit<T>();
+        bool rot = false;
+        if (q < n) {
+          const double a = sm_re(A[p * kSmallLd + p]), d = sm_re(A[q * kSmallLd + q]);
+          const T b = A[p * kSmallLd + q];
+          const double ab = sm_abs(b);
+          if (ab > eps * sqrt(fabs(a * d)) && ab > 1e-300) {
+            rot = true;
+            e = sm_scale(b, 1.0 / ab);
+            const double th = (d - a) / (2.0 * ab);
+            const double tt = (th >= 0.0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+            c = 1.0 / sqrt(tt * tt + 1.0);
+            sn = tt * c;
+          }
+        }
+        pq[k][0] = rot ? p : -1;
+        pq[k][1] = q;
+        cs[k][0] = c;
+        cs[k][1] = sn;
+        ph[k] = e;
+        if (rot) atomicAdd(&nrot, 1);
+      }
+      __syncthreads();
+      // rows p, q of A: J^H A
+      for (int idx = t; idx < npair * n; idx += kSmallThreads) {
+        const int k = idx / n, j = idx - k * n;
+        const int p = pq[k][0];
+        if (p < 0) continue;
+        const int q = pq[k][1];
+        const double c = cs[k][0], sn = cs[k][1];
+        const T e = ph[k];
+        const T rp = A[p * kSmallLd + j], rq = A[q * kSmallLd + j];
+        const T erq = sm_mul(e, rq);
+        A[p * kSmallLd + j] = sm_sub(sm_scale(rp, c), sm_scale(erq, sn));
+        A[q * kSmallLd + j] = sm_scale(rp, sn) + sm_scale(erq, c);
+      }
+      __syncthreads();
+      // columns p, q of A and V: A J, V J
+      for (int idx = t; idx < 2 * npair * n; idx += kSmallThreads) {
+        const int m = idx / (npair * n), rem = idx - m * npair * n;
+        const int k = rem / n, i = rem - k * n;
+        const int p = pq[k][0];
+        if (p < 0) continue;
+        const int q = pq[k][1];
+        T* M = m == 0 ? A : V;
+        const double c = cs[k][0], sn = cs[k][1];
+        const T ec = sm_conj(ph[k]);
+        const T cp = M[i * kSmallLd + p], cq = sm_mul(M[i * kSmallLd + q], ec);
+        M[i * kSmallLd + p] = sm_sub(sm_scale(cp, c), sm_scale(cq, sn));
+        M[i * kSmallLd + q] = sm_scale(cp, sn) + sm_scale(cq, c);
+      }
+      __syncthreads();
+    }
+    if (nrot == 0) break;
+    __syncthreads();
+  }
+  // ascending order: the rank of each eigenvalue
+  if (t < n) wv[t] = sm_re(A[t * kSmallLd + t]);
+  __syncthreads();
+  double* wo = wg + (int64_t)blockIdx.x * n;
+  for (int i = t; i < n; i += kSmallThreads) {
+    int rank = 0;
+    for (int j = 0; j < n; ++j) rank += (wv[j] < wv[i] || (wv[j] == wv[i] && j < i)) ? 1 : 0;
+    wo[rank] = wv[i];
+    for (int j = 0; j < n; ++j) Am[(int64_t)rank * n + j] = sm_conj(V[j * kSmallLd + i]);
+  }
+  if (t == 0) info[blockIdx.x] = sweep >= 40 ? 1 : 0;
+}
+
+// L^-1 of the Cholesky factor L (S = L L^H, lower) of each Hermitian /
+// symmetric n x n matrix S (row-major, stride elements apart): right-looking
+// factorisation in LDS, then one thread per column of L^-1 (forward
+// substitution on e_j).  info = k + 1 for the first non-positive pivot k
+// (the output is then not a factor).  Linv: row-major, zero above the
+// diagonal.
+template <typename T>
+__global__ __launch_bounds__(kSmallThreads) void k_chol_inv_small(const T* __restrict__ Sg, int n, int64_t stride,
+                                                                  T* __restrict__ Lg, int32_t* __restrict__ info) {
+  __shared__ T L[kSmallMax * kSmallLd];
+  __shared__ T X[kSmallMax * kSmallLd];
+  __shared__ int bad;
+  const int t = threadIdx.x;
+  const T* Sm = Sg + (int64_t)blockIdx.x * stride;
+  for (int i = t; i < n * n; i += kSmallThreads) {
+    const int r = i / n, c = i - r * n;
+    L[r * kSmallLd + c] = c <= r ? Sm[i] : sm_zero<T>();
+  }
+  if (t == 0) bad = 0;
+  __syncthreads();
+  for (int k = 0; k < n; ++k) {
+    if (t == 0) {
+      double d = sm_re(L[k * kSmallLd + k]);
+      if (!(d > 0.0)) {
+        if (!bad) bad = k + 1;
+        d = 1.0;
+      }
+      L[k * kSmallLd + k] = sm_scale(sm_unit<T>(), sqrt(d));
+    }
+    __syncthreads();
+    const double inv = 1.0 / sm_re(L[k * kSmallLd + k]);
+    for (int i = k + 1 + t; i < n; i += kSmallThreads) L[i * kSmallLd + k] = sm_scale(L[i * kSmallLd + k], inv);
+    __syncthreads();
+    const int m = n - k - 1;
+    for (int idx = t; idx < m * m; idx += kSmallThreads) {
+      const int i = k + 1 + idx / m, j = k + 1 + idx % m;
+      if (j <= i)
+        L[i * kSmallLd + j] = sm_sub(L[i * kSmallLd + j], sm_mul(L[i * kSmallLd + k], sm_conj(L[j * kSmallLd + k])));
+    }
+    __syncthreads();
+  }
+  if (t < n) {
+    const int j = t;
+    for (int i = 0; i < j; ++i) X[i * kSmallLd + j] = sm_zero<T>();
+    X[j * kSmallLd + j] = sm_scale(sm_unit<T>(), 1.0 / sm_re(L[j * kSmallLd + j]));
+    for (int i = j + 1; i < n; ++i) {
+      T acc = sm_zero<T>();
+      for (int k = j; k < i; ++k) acc = acc + sm_mul(L[i * kSmallLd + k], X[k * kSmallLd + j]);
+      X[i * kSmallLd + j] = sm_scale(acc, -1.0 / sm_re(L[i * kSmallLd + i]));
+    }
+  }
+  __syncthreads();
+  T* Lm = Lg + (int64_t)blockIdx.x * stride;
+  for (int i = t; i < n * n; i += kSmallThreads) {
+    const int r = i / n, c = i - r * n;
+    Lm[i] = X[r * kSmallLd + c];
+  }
+  if (t == 0) info[blockIdx.x] = bad;
+}
+
 }  // namespace
 
 extern "C" {
@@ -439,13 +650,24 @@ int addft_subspace_eigh(int dtype, int method, int64_t n, void* d_a, int64_t str
                         int32_t* d_info, void* d_work, void* stream) {
   ADDFT_CHECK_ARG(dtype == ADDFT_R64 || dtype == ADDFT_C128, "dtype must be ADDFT_R64 or ADDFT_C128");
   ADDFT_CHECK_ARG(method == ADDFT_EIGH_DC || method == ADDFT_EIGH_JACOBI || method == ADDFT_EIGH_DCJ ||
-                      method == ADDFT_EIGH_TRD,
+                      method == ADDFT_EIGH_TRD || method == ADDFT_EIGH_LDS,
                   "unknown eigensolver method");
   ADDFT_CHECK_ARG(method != ADDFT_EIGH_TRD || (dtype == ADDFT_R64 && batch == 1),
                   "ADDFT_EIGH_TRD takes one real symmetric matrix");
+  ADDFT_CHECK_ARG(method != ADDFT_EIGH_LDS || n <= kSmallMax, "ADDFT_EIGH_LDS takes n <= 64");
   ADDFT_CHECK_ARG(n > 0 && n <= 65536 && batch > 0 && batch <= INT32_MAX && stride_a >= n * n, "bad sizes");
   ADDFT_CHECK_ARG(d_a && d_w && d_info && d_work, "null argument");
   hipStream_t s = addft::as_stream(stream);
+  if (method == ADDFT_EIGH_LDS) {
+    if (dtype == ADDFT_R64)
+      hipLaunchKernelGGL(k_jacobi_small<double>, dim3((unsigned)batch), dim3(kSmallThreads), 0, s,
+                         static_cast<double*>(d_a), (int)n, stride_a, d_w, d_info);
+    else
+      hipLaunchKernelGGL(k_jacobi_small<addft::cpx<double>>, dim3((unsigned)batch), dim3(kSmallThreads), 0, s,
+                         static_cast<addft::cpx<double>*>(d_a), (int)n, stride_a, d_w, d_info);
+    ADDFT_LAUNCH_CHECK();
+    return ADDFT_OK;
+  }
   rocblas_handle h;
   int rc = handle_for(s, &h);
   if (rc) return rc;
@@ -489,6 +711,25 @@ int addft_subspace_eigh(int dtype, int method, int64_t n, void* d_a, int64_t str
                                                   d_info, B));
     }
   }
+  return ADDFT_OK;
+}
+
+// L^-1 of the Cholesky factors of batch small Hermitian (ADDFT_C128) or
+// symmetric (ADDFT_R64) positive definite n x n matrices (n <= 64).
+int addft_subspace_chol_inv(int dtype, int64_t n, const void* d_s, int64_t stride, int64_t batch, void* d_linv,
+                            int32_t* d_info, void* stream) {
+  ADDFT_CHECK_ARG(dtype == ADDFT_R64 || dtype == ADDFT_C128, "dtype must be ADDFT_R64 or ADDFT_C128");
+  ADDFT_CHECK_ARG(n > 0 && n <= kSmallMax && batch > 0 && batch <= INT32_MAX && stride >= n * n, "bad sizes");
+  ADDFT_CHECK_ARG(d_s && d_linv && d_info && d_s != d_linv, "null or aliased argument");
+  hipStream_t s = addft::as_stream(stream);
+  if (dtype == ADDFT_R64)
+    hipLaunchKernelGGL(k_chol_inv_small<double>, dim3((unsigned)batch), dim3(kSmallThreads), 0, s,
+                       static_cast<const double*>(d_s), (int)n, stride, static_cast<double*>(d_linv), d_info);
+  else
+    hipLaunchKernelGGL(k_chol_inv_small<addft::cpx<double>>, dim3((unsigned)batch), dim3(kSmallThreads), 0, s,
+                       static_cast<const addft::cpx<double>*>(d_s), (int)n, stride,
+                       static_cast<addft::cpx<double>*>(d_linv), d_info);
+  ADDFT_LAUNCH_CHECK();
   return ADDFT_OK;
 }
 

@@ -46,7 +46,7 @@ enum {
 };
 
 enum { ADDFT_C64 = 0, ADDFT_C128 = 1, ADDFT_R64 = 2 };
-enum { ADDFT_EIGH_DC = 0, ADDFT_EIGH_JACOBI = 1, ADDFT_EIGH_DCJ = 2, ADDFT_EIGH_TRD = 3 };
+enum { ADDFT_EIGH_DC = 0, ADDFT_EIGH_JACOBI = 1, ADDFT_EIGH_DCJ = 2, ADDFT_EIGH_TRD = 3, ADDFT_EIGH_LDS = 4 };
 
 typedef struct addft_sphere_plan_s* addft_sphere_plan;
 typedef struct addft_projector_s* addft_projector;
@@ -183,8 +183,15 @@ int64_t addft_nonlocal_noalloc_scratch_bytes(void);
  *   syevdj) or ADDFT_EIGH_TRD (one real symmetric matrix: a one-launch
  *   cooperative Householder tridiagonalisation that keeps the matrix in
  *   the LDS of its workgroups, then rocSOLVER's dstedc and dormtr; info
- *   is set non-zero if its grid barrier gives up); d_work:
- *   addft_subspace_eigh_work_bytes(n, batch) bytes.                       */
+ *   is set non-zero if its grid barrier gives up) or ADDFT_EIGH_LDS
+ *   (n <= 64, any batch, real or complex: one workgroup per matrix, cyclic
+ *   Jacobi in LDS, info 1 if 40 sweeps do not converge); d_work:
+ *   addft_subspace_eigh_work_bytes(n, batch) bytes (unused by _LDS).
+ * addft_subspace_chol_inv: L^-1 (row-major, lower) of the Cholesky factor
+ *   S = L L^H of batch small Hermitian (ADDFT_C128) / symmetric (ADDFT_R64)
+ *   matrices, n <= 64, one workgroup per matrix (the Cholesky-QR step of the
+ *   batched k-point solver; reference solver/davidson.py:32-77 orthogonalises
+ *   with jnp.linalg.qr); info = k + 1 for the first non-positive pivot.   */
 int64_t addft_subspace_gram_work_bytes(int dtype, int64_t m, int64_t n, int nslice);
 int addft_subspace_gram(int dtype, int64_t m, int64_t n, int64_t K, const void* d_a, int64_t lda,
                         const void* d_b, int64_t ldb, void* d_c, int64_t ldc, int nslice, void* d_work,
@@ -192,6 +199,8 @@ int addft_subspace_gram(int dtype, int64_t m, int64_t n, int64_t K, const void* 
 int addft_subspace_rotate(int dtype, int64_t m, int64_t n, int64_t K, const void* d_q, int64_t ldq,
                           const void* d_x, int64_t ldx, void* d_y, int64_t ldy, void* stream);
 int64_t addft_subspace_eigh_work_bytes(int64_t n, int64_t batch);
+int addft_subspace_chol_inv(int dtype, int64_t n, const void* d_s, int64_t stride, int64_t batch, void* d_linv,
+                            int32_t* d_info, void* stream);
 int addft_subspace_eigh(int dtype, int method, int64_t n, void* d_a, int64_t stride_a, int64_t batch, double* d_w,
                         int32_t* d_info, void* d_work, void* stream);
 

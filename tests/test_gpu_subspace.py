@@ -105,3 +105,58 @@ def test_subspace_eigh_trd(n):
         assert res.abs().max().item() < 1e-10 * scale
         eye = torch.eye(n, dtype=U.dtype, device="cuda")
         assert (U.T @ U - eye).abs().max().item() < 1e-11
+
+
+@pytest.mark.parametrize("real", [True, False])
+@pytest.mark.parametrize("n,batch", [(5, 1), (28, 64), (56, 64), (64, 3)])
+def test_subspace_eigh_lds(real, n, batch):
+    """ADDFT_EIGH_LDS (batched cyclic Jacobi, one workgroup per matrix in
+    LDS; the per-k-point blocks of the batched solver) against torch:
+    eigenvalues 1e-11 relative, eigen-residuals 1e-10, orthonormal vectors
+    1e-11; half the batch with an 8-fold near-degenerate cluster (split by
+    1e-10)."""
+    import torch
+    from addft.solver.linalg import eigh_device
+    X = _rand((batch, n, n), torch.complex128, 11 + n)
+    T = 0.5 * (X + X.conj().transpose(1, 2))
+    Q, _ = torch.linalg.qr(_rand((batch, n, n), torch.complex128, 12 + n))
+    lev = torch.linspace(-2.0, 2.0, n, dtype=torch.float64, device="cuda")
+    m = min(8, n)
+    lev[:m] = -3.0 + 1e-10 * torch.arange(m, dtype=torch.float64, device="cuda")
+    Dg = (Q * lev.to(Q.dtype)) @ Q.conj().transpose(1, 2)
+    T[batch // 2:] = 0.5 * (Dg[batch // 2:] + Dg[batch // 2:].conj().transpose(1, 2))
+    if real:
+        T = T.real.contiguous()
+    w, U, info = eigh_device(T, "lds")
+    assert int((info != 0).sum()) == 0
+    wr = torch.linalg.eigvalsh(T)
+    scale = wr.abs().max().item()
+    assert ((w - wr).abs().max() / scale).item() < 1e-11
+    res = T @ U - U * w[:, None, :].to(U.dtype)
+    assert res.abs().max().item() < 1e-10 * scale
+    eye = torch.eye(n, dtype=U.dtype, device="cuda")
+    assert (U.conj().transpose(1, 2) @ U - eye).abs().max().item() < 1e-11
+
+
+@pytest.mark.parametrize("real", [True, False])
+@pytest.mark.parametrize("n,batch", [(1, 2), (28, 64), (64, 5)])
+def test_subspace_chol_inv(real, n, batch):
+    """addft_subspace_chol_inv against torch's Cholesky and triangular
+    solve (1e-12 relative); a matrix with a non-positive pivot reports it
+    in its info word only."""
+    import torch
+    from addft.solver.linalg import chol_inverse_small
+    X = _rand((batch, n, 2 * n), torch.complex128, 21 + n)
+    S = X @ X.conj().transpose(1, 2) / (2 * n)
+    if real:
+        S = S.real.contiguous()
+    Linv, info = chol_inverse_small(S)
+    assert int((info != 0).sum()) == 0
+    L = torch.linalg.cholesky(S)
+    ref = torch.linalg.solve_triangular(L, torch.eye(n, dtype=S.dtype, device="cuda").expand_as(L), upper=False)
+    assert ((Linv - ref).abs().max() / ref.abs().max()).item() < 1e-12
+    if batch > 1 and n > 1:
+        S2 = S.clone()
+        S2[1, n // 2, n // 2] = -1.0
+        _, info2 = chol_inverse_small(S2)
+        assert int(info2[0]) == 0 and int(info2[1]) > 0
