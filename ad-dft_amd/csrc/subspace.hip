@@ -351,8 +351,8 @@ __device__ __forceinline__ double sm_sub<double>(double a, double b) { return a 
 // no rotation (at most 40).  Each rotation is J = D R with D = diag(1,
 // e^{-i phi}) taking a_pq = |a_pq| e^{i phi} to the real 2 x 2 problem and R
 // the real Jacobi rotation (tan of its angle t = sign(th) / (|th| +
-// sqrt(th^2 + 1)), th = (a_qq - a_pp) / (2 |a_pq|)); A <- J^H A J (all rows,
-// then all columns of the round), V <- V J.  Output in place: ascending w,
+// sqrt(th^2 + 1)), th = (a_qq - a_pp) / (2 |a_pq|)); A <- J^H A J in one
+// pass over the 2 x 2 blocks of the round's pairs, V <- V J.  Output in place: ascending w,
 // row i of A = conj(v_i) (the layout of addft_subspace_eigh); info 1 if the
 // sweeps ran out.
 template <typename T>
@@ -406,41 +406,52 @@ __global__ __launch_bounds__(kSmallThreads) void k_jacobi_small(T* __restrict__ 
             sn = tt * c;
           }
         }
-        pq[k][0] = rot ? p : -1;
-        pq[k][1] = q;
+        pq[k][0] = p;
+        pq[k][1] = q;  // q = n: the padding index of odd n (no rotation)
         cs[k][0] = c;
         cs[k][1] = sn;
         ph[k] = e;
         if (rot) atomicAdd(&nrot, 1);
       }
       __syncthreads();
-      // rows p, q of A: J^H A
-      for (int idx = t; idx < npair * n; idx += kSmallThreads) {
-        const int k = idx / n, j = idx - k * n;
-        const int p = pq[k][0];
-        if (p < 0) continue;
-        const int q = pq[k][1];
-        const double c = cs[k][0], sn = cs[k][1];
-        const T e = ph[k];
-        const T rp = A[p * kSmallLd + j], rq = A[q * kSmallLd + j];
-        const T erq = sm_mul(e, rq);
-        A[p * kSmallLd + j] = sm_sub(sm_scale(rp, c), sm_scale(erq, sn));
-        A[q * kSmallLd + j] = sm_scale(rp, sn) + sm_scale(erq, c);
+      // A <- J^H A J as 2 x 2 blocks: block (a, b) = rows of pair a x columns
+      // of pair b, read and written by one thread (pairs are disjoint, so the
+      // blocks are), J_k = [[c, s], [-s conj(e), c conj(e)]] (identity where
+      // pair k does not rotate)
+      for (int idx = t; idx < npair * npair; idx += kSmallThreads) {
+        const int ka = idx / npair, kb = idx - ka * npair;
+        const int r0 = pq[ka][0], r1 = pq[ka][1], c0 = pq[kb][0], c1 = pq[kb][1];
+        const bool v1 = r1 < n, w1 = c1 < n;
+        const T z = sm_zero<T>();
+        const T b00 = A[r0 * kSmallLd + c0];
+        const T b01 = w1 ? A[r0 * kSmallLd + c1] : z;
+        const T b10 = v1 ? A[r1 * kSmallLd + c0] : z;
+        const T b11 = v1 && w1 ? A[r1 * kSmallLd + c1] : z;
+        // T = J_a^H B: row 0 = c B0 - s e B1, row 1 = s B0 + c e B1
+        const double ca = cs[ka][0], sa = cs[ka][1];
+        const T ea = ph[ka];
+        const T eb10 = sm_mul(ea, b10), eb11 = sm_mul(ea, b11);
+        const T t00 = sm_sub(sm_scale(b00, ca), sm_scale(eb10, sa)), t01 = sm_sub(sm_scale(b01, ca), sm_scale(eb11, sa));
+        const T t10 = sm_scale(b00, sa) + sm_scale(eb10, ca), t11 = sm_scale(b01, sa) + sm_scale(eb11, ca);
+        // B' = T J_b: col 0 = c T0 - s conj(e) T1, col 1 = s T0 + c conj(e) T1
+        const double cb = cs[kb][0], sb = cs[kb][1];
+        const T eb = sm_conj(ph[kb]);
+        const T u01 = sm_mul(t01, eb), u11 = sm_mul(t11, eb);
+        A[r0 * kSmallLd + c0] = sm_sub(sm_scale(t00, cb), sm_scale(u01, sb));
+        if (w1) A[r0 * kSmallLd + c1] = sm_scale(t00, sb) + sm_scale(u01, cb);
+        if (v1) A[r1 * kSmallLd + c0] = sm_sub(sm_scale(t10, cb), sm_scale(u11, sb));
+        if (v1 && w1) A[r1 * kSmallLd + c1] = sm_scale(t10, sb) + sm_scale(u11, cb);
       }
-      __syncthreads();
-      // columns p, q of A and V: A J, V J
-      for (int idx = t; idx < 2 * npair * n; idx += kSmallThreads) {
-        const int m = idx / (npair * n), rem = idx - m * npair * n;
-        const int k = rem / n, i = rem - k * n;
-        const int p = pq[k][0];
-        if (p < 0) continue;
-        const int q = pq[k][1];
-        T* M = m == 0 ? A : V;
+      // V <- V J (rows i, pairs k)
+      for (int idx = t; idx < npair * n; idx += kSmallThreads) {
+        const int k = idx / n, i = idx - k * n;
+        const int p = pq[k][0], q = pq[k][1];
+        if (q >= n || cs[k][1] == 0.0) continue;
         const double c = cs[k][0], sn = cs[k][1];
         const T ec = sm_conj(ph[k]);
-        const T cp = M[i * kSmallLd + p], cq = sm_mul(M[i * kSmallLd + q], ec);
-        M[i * kSmallLd + p] = sm_sub(sm_scale(cp, c), sm_scale(cq, sn));
-        M[i * kSmallLd + q] = sm_scale(cp, sn) + sm_scale(cq, c);
+        const T cp = V[i * kSmallLd + p], cq = sm_mul(V[i * kSmallLd + q], ec);
+        V[i * kSmallLd + p] = sm_sub(sm_scale(cp, c), sm_scale(cq, sn));
+        V[i * kSmallLd + q] = sm_scale(cp, sn) + sm_scale(cq, c);
       }
       __syncthreads();
     }

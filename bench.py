@@ -58,7 +58,7 @@ def mark(torch):
 # kernel categories of --trace-summary: (label, regular expression on the kernel name), first match wins
 TRACE_CATEGORIES = [
     ("local Hpsi / density (lane FFT kernels)", r"k_z_inverse|k_z_forward|k_plane|k_reduce_density|k_stage"),
-    ("projector products, hand-written BF16x3 MFMA", r"k_becp|k_apply_add|k_split_rows|k_pair_rows"),
+    ("projector products, hand-written BF16x3 MFMA", r"k_becp|k_apply_staged|k_split_rows|k_pair_rows"),
     ("projector products, hipBLASLt BF16x3 GEMMs", r"Cijk_.*BSS_BH"),
     ("Gamma pair split / unfold, BF16x3 split", r"k_gamma_pair_split|k_gamma_scatter_add|k_split"),
     ("Gamma two-band pack / unpack", r"k_gamma_pack2|k_gamma_unpack2"),
