@@ -33,7 +33,7 @@ from .kernels import _lib
 from .kernels import native
 from .kernels._lib import check, lib, ptr, stream
 from .solver.linalg import inner
-from .kernels.nonlocal_bf16 import GammaProjector64, gamma_pairs
+from .kernels.nonlocal_bf16 import GammaProjector64, ProjectorMFMA, gamma_pairs
 
 
 # enhancement coefficient of the model GGA (xc.py:145)
@@ -47,6 +47,47 @@ def _simpson_weights(n):
     w[2:n - 1:2] = 2.0
     return w / 3.0
 
+
+
+class HalfRows:
+    """Rows real in real space (v(-G) = conj v(G), k = 0) held on the half
+    sphere: entry j = s_j v(G_j) over the positive half of the pair table
+    (G_0 = 0; s_0 = 1, s_j = sqrt 2), so that Re of plain dot products of
+    half rows are the full-sphere inner products and their norms the
+    full-sphere norms.  Half the storage, traffic and GEMM depth of the
+    full rows; every dtype keeps its precision."""
+
+    def __init__(self, plus, minus, ng):
+        self.ng = int(ng)
+        self.plus, self.minus = plus, minus
+        self.p = plus.long()
+        self.m1 = minus[1:].long()
+        s = torch.full((int(plus.numel()),), math.sqrt(2.0), dtype=torch.float64, device=plus.device)
+        s[0] = 1.0
+        self.s64, self.s32 = s, s.to(torch.float32)
+
+    @property
+    def nh(self):
+        return int(self.p.numel())
+
+    def _s(self, dtype):
+        return self.s64 if dtype in (torch.float64, torch.complex128) else self.s32
+
+    def to_half(self, F):
+        """(m, nG) full rows -> (m, nh) half rows, same dtype."""
+        return (F[:, self.p] * self._s(F.dtype)).contiguous()
+
+    def to_full(self, S):
+        """(m, nh) half rows -> (m, nG) full rows, same dtype."""
+        t = S / self._s(S.dtype)
+        out = torch.empty((S.shape[0], self.ng), dtype=S.dtype, device=S.device)
+        out[:, self.p] = t
+        out[:, self.m1] = t[:, 1:].conj()
+        return out
+
+    def values(self, d):
+        """A function of G on the sphere (even in G) -> its half-row values."""
+        return d[self.p].contiguous()
 
 class Engine:
     def __init__(self, static, device=None):
@@ -89,6 +130,7 @@ class Engine:
         if self.gamma_pairs is not None:
             self.gplus_d = torch.as_tensor(self.gamma_pairs[0], device=self.dev)
             self.gminus_d = torch.as_tensor(self.gamma_pairs[1], device=self.dev)
+        self.ghalf = HalfRows(self.gplus_d, self.gminus_d, self.ng) if self.gplus_d is not None else None
         # per-k shells: 0/1 masks in stick order applied to every H psi (None: union shell)
         self.kmask64 = self.kmask32 = None
         if static.kshell is not None:
@@ -383,7 +425,6 @@ class Engine:
         (addft_gamma_pack2 / addft_gamma_unpack2)."""
         plan = self.plan32 if fp32 else self.plan64
         tk = self.tkin32[ik] if fp32 else self.tkin64[ik]
-        P = self.P32[ik] if fp32 else self.P64[ik]
         self.hpsi_bands["fp32" if fp32 else "fp64"] += int(psi.shape[0])
         if real:
             if self.gplus_d is None:
@@ -392,6 +433,11 @@ class Engine:
             out = native.gamma_unpack2(h, psi.shape[0], self.gplus_d, self.gminus_d)
         else:
             out = plan.hpsi_local(psi, tk, vs)
+        self._nonlocal_add(ik, psi, out, fp32, nl_bf16, real)
+        return self.restrict(ik, out)
+
+    def _nonlocal_add(self, ik, psi, out, fp32, nl_bf16, real):
+        """out += V_NL psi (full-sphere rows)."""
         if fp32 and nl_bf16 and self.PB[ik] is not None:
             pb = self.PB[ik]
             pb.apply_add(self.dmul(pb.becp_t(psi, real=real)), out, real=real)
@@ -399,9 +445,38 @@ class Engine:
             g = self.PG64[ik]                           # k = 0: half-sphere FP64 products
             g.apply_add(self.dmul(g.becp_t(psi, real=real)), out, real=real)
         else:
+            P = self.P32[ik] if fp32 else self.P64[ik]
             becp_t = inner(psi, P)                      # (nb, nproj) = <beta|psi>^T
             torch.addmm(out, self.dmul(becp_t), P, out=out)   # accumulated by the GEMM (beta = 1)
-        return self.restrict(ik, out)
+        return out
+
+    def hpsi_half(self, ik, h, vs, fp32=False, nl_bf16=True):
+        """H psi for rows real in real space held as half rows (k = 0,
+        HalfRows): h (m, nh) -> (m, nh).  Two bands per local transform
+        straight from and to the half rows (addft_gamma_pack2_half /
+        addft_gamma_unpack2_half); the projector products read the half rows
+        and add into them (ADDFT_ROWS_HALF of the MFMA kernels, the FP64
+        half-row GEMMs), with no full-sphere copy."""
+        if self.ghalf is None or self.kmask64 is not None:
+            raise ValueError("hpsi_half needs the k = 0 pair tables")
+        plan = self.plan32 if fp32 else self.plan64
+        tk = self.tkin32[ik] if fp32 else self.tkin64[ik]
+        m = int(h.shape[0])
+        self.hpsi_bands["fp32" if fp32 else "fp64"] += m
+        h = h.contiguous()
+        hh = plan.hpsi_local(native.gamma_pack2_half(h, self.gplus_d, self.gminus_d, self.ng), tk, vs)
+        out = native.gamma_unpack2_half(hh, m, self.gplus_d, self.gminus_d)
+        pb = self.PB[ik] if (fp32 and nl_bf16) else None
+        if isinstance(pb, ProjectorMFMA):
+            pb.apply_add(self.dmul(pb.becp_t(h, real="half")), out, real="half")
+        elif not fp32 and self.PG64[ik] is not None:
+            g = self.PG64[ik]
+            g.apply_add(self.dmul(g.becp_t(h, real="half")), out, real="half")
+        else:
+            # a projector object without a half-row form: through the full sphere
+            full = self.ghalf.to_full(h)
+            out += self.ghalf.to_half(self._nonlocal_add(ik, full, torch.zeros_like(full), fp32, nl_bf16, True))
+        return out
 
     def dmul(self, b):
         """b @ D for projector coefficients b (..., nproj) in rows."""

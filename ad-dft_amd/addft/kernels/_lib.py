@@ -52,6 +52,8 @@ _SIGS = {
     "addft_gamma_scatter_add_real": ([_I32, _VP, _I64, _I64, _VP, _VP, _I64, _VP, _VP], _I32),
     "addft_gamma_pack2": ([_I32, _VP, _I64, _I64, _VP, _VP], _I32),
     "addft_gamma_unpack2": ([_I32, _VP, _I64, _I64, _VP, _VP, _I64, _VP, _VP], _I32),
+    "addft_gamma_pack2_half": ([_I32, _VP, _I64, _I64, _I64, _VP, _VP, _I64, _VP, _VP], _I32),
+    "addft_gamma_unpack2_half": ([_I32, _VP, _I64, _I64, _VP, _VP, _I64, _VP, _I64, _VP], _I32),
     "addft_pack_gather": ([_I32, _VP, _VP, _I64, _I64, _I64, _VP, _VP], _I32),
     "addft_xc_work_bytes": ([_I64], _I64),
     "addft_xc_lda": ([_VP, _I64, _VP, _VP, _VP, _VP], _I32),

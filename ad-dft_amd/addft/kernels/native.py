@@ -206,6 +206,44 @@ def gamma_unpack2(h: torch.Tensor, m: int, plus: torch.Tensor, minus: torch.Tens
     return out
 
 
+def _pair_tables(plus: torch.Tensor, minus: torch.Tensor, ng: int) -> None:
+    if minus.numel() != plus.numel():
+        raise ValueError(f"minus has {minus.numel()} elements, expected {plus.numel()}")
+    if 2 * plus.numel() - 1 != ng:
+        raise ValueError("pair tables must cover the sphere (2 nh - 1 = nG)")
+    _require(plus, torch.int32, what="plus")
+    _require(minus, torch.int32, what="minus")
+
+
+def gamma_pack2_half(h: torch.Tensor, plus: torch.Tensor, minus: torch.Tensor, ng: int) -> torch.Tensor:
+    """(m, nh) half rows (entry j = s_j psi(G_j), s_0 = 1, s_j = sqrt 2) of
+    vectors real in real space -> (ceil(m/2), nG) pairs psi_2b + i psi_2b+1
+    (addft_gamma_pack2_half)."""
+    if h.dim() != 2 or h.shape[1] != plus.numel() or h.stride(1) != 1:
+        raise ValueError("h must be (m, nh) with unit column stride")
+    _require(h)
+    _pair_tables(plus, minus, ng)
+    m = h.shape[0]
+    out = torch.empty(((m + 1) // 2, ng), dtype=h.dtype, device=h.device)
+    check(lib().addft_gamma_pack2_half(_dtype_code(h), ptr(h), m, h.stride(0), plus.numel(), ptr(plus),
+                                       ptr(minus), ng, ptr(out), stream()), "addft_gamma_pack2_half")
+    return out
+
+
+def gamma_unpack2_half(hh: torch.Tensor, m: int, plus: torch.Tensor, minus: torch.Tensor) -> torch.Tensor:
+    """H of the packed pairs -> (m, nh) half rows of H psi (addft_gamma_unpack2_half)."""
+    if hh.dim() != 2 or hh.shape[0] != (m + 1) // 2:
+        raise ValueError("hh must be (ceil(m/2), nG)")
+    ng = hh.shape[1]
+    _require(hh)
+    _pair_tables(plus, minus, ng)
+    nh = plus.numel()
+    out = torch.empty((m, nh), dtype=hh.dtype, device=hh.device)
+    check(lib().addft_gamma_unpack2_half(_dtype_code(hh), ptr(hh), m, ng, ptr(plus), ptr(minus), nh, ptr(out), nh,
+                                         stream()), "addft_gamma_unpack2_half")
+    return out
+
+
 # ------------------------------------------------------------- real grids
 def to_planar(grid: torch.Tensor, shape) -> torch.Tensor:
     """row-major (i1,i2,i3) float64 -> plane-major (i3,i1,i2)."""

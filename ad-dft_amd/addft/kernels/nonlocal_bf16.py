@@ -27,6 +27,7 @@ whose FP32 partial blocks are summed.
 
 from __future__ import annotations
 
+import math
 import weakref
 
 import numpy as np
@@ -261,12 +262,31 @@ class GammaProjector64:
         Qi = Q.imag.clone()
         Qi[:, 0] = 0.0
         self.WT = torch.cat([Qr, Qi], dim=1).contiguous()     # (np, 2 nh): W^T and V at once
+        self.WTi = None
 
-    def becp_t(self, psi: torch.Tensor, real: bool = False) -> torch.Tensor:
+    def _half_image(self):
+        """[s Qr, s Qi] interleaved per pair (np, 2 nh), s_0 = 1, s_j = sqrt 2:
+        against the real view of half rows (entry j = s_j psi(G_j)) one real
+        GEMM gives <P'|psi>, and S' times it is the half row of S' P'."""
+        if self.WTi is None:
+            nh = self.nh
+            f = torch.full((nh,), math.sqrt(2.0), dtype=torch.float64, device=self.WT.device)
+            f[0] = 1.0
+            self.WTi = torch.stack([self.WT[:, :nh] * f, self.WT[:, nh:] * f], dim=2).reshape(self.np, 2 * nh)
+            self.WTi = self.WTi.contiguous()
+        return self.WTi
+
+    def becp_t(self, psi: torch.Tensor, real=False) -> torch.Tensor:
         """psi (nb, nG) complex128 -> psi P'^H (nb, nproj) complex128.  real:
-        rows real in real space (real <P'|psi>), the Re-part rows only."""
+        rows real in real space (real <P'|psi>), the Re-part rows only;
+        real="half": psi given as half rows (nb, nh), no pair gather."""
         from ..solver.linalg import inner
         nb = psi.shape[0]
+        if real == "half":
+            if psi.dtype != torch.complex128 or psi.shape[1] != self.nh:
+                raise ValueError("half rows must be (nb, nh) complex128")
+            C = inner(torch.view_as_real(psi.contiguous()).reshape(nb, 2 * self.nh), self._half_image())
+            return torch.complex(C, torch.zeros_like(C))
         A = torch.empty((2 * nb, 2 * self.nh), dtype=torch.float64, device=psi.device)
         psi = psi.contiguous()
         _lib.check(_lib.lib().addft_gamma_pair_rows(_lib.ptr(psi), nb, self.ng, _lib.ptr(self.plus),
@@ -278,11 +298,18 @@ class GammaProjector64:
         C = inner(A, self.WT)                                   # A W, the reduction split over slices
         return torch.complex(C[:nb], C[nb:])
 
-    def apply_add(self, S: torch.Tensor, out: torch.Tensor, real: bool = False) -> torch.Tensor:
-        """out (nb, nG) complex128 += S' P' (real: S' real)."""
+    def apply_add(self, S: torch.Tensor, out: torch.Tensor, real=False) -> torch.Tensor:
+        """out (nb, nG) complex128 += S' P' (real: S' real; real="half": out
+        holds half rows (nb, nh), added in place by the GEMM)."""
         if not out.is_contiguous():
             raise ValueError("out must be contiguous")
         nb = S.shape[0]
+        if real == "half":
+            if out.dtype != torch.complex128 or out.shape != (nb, self.nh):
+                raise ValueError("out must be (nb, nh) complex128 half rows")
+            ov = torch.view_as_real(out).view(nb, 2 * self.nh)
+            torch.addmm(ov, S.real.contiguous(), self._half_image(), out=ov)
+            return out
         Sr = S.real.contiguous() if real else torch.cat([S.real, S.imag], dim=0).contiguous()
         U = torch.matmul(Sr, self.WT)
         fn = _lib.lib().addft_gamma_scatter_add_real if real else _lib.lib().addft_gamma_scatter_add
@@ -320,6 +347,7 @@ class ProjectorMFMA:
         else:
             plus = minus = odd = None
             nh = 0
+        self.nh = nh
         L = _lib.lib()
         h = _lib.ctypes.c_void_p()
         _lib.check(L.addft_projector_create(int(self.gamma), self.np, self.ng, _lib.ptr(P), _lib.ptr(odd),
@@ -335,18 +363,29 @@ class ProjectorMFMA:
             self._work = torch.empty(max(need, 1), dtype=torch.uint8, device="cuda")
         return self._work
 
-    def becp_t(self, psi: torch.Tensor, real: bool = False) -> torch.Tensor:
+    def _rows(self, real):
+        """(ADDFT_ROWS_* code, row length) of an operand mode."""
+        if real == "half":
+            if not self.gamma:
+                raise ValueError("half rows need a k = 0 (gamma) projector object")
+            return 2, self.nh
+        return int(bool(real) and self.gamma), self.ng
+
+    def becp_t(self, psi: torch.Tensor, real=False) -> torch.Tensor:
         """psi (nb, nG) complex64 -> psi P^H (nb, nproj) complex64 (P' at k = 0).
         real (k = 0): rows real in real space, whose <P'|psi> are real -- one
-        operand row per band (half the flops); imaginary parts are zero."""
-        if psi.dtype != torch.complex64 or psi.dim() != 2 or psi.shape[1] != self.ng:
-            raise ValueError("psi must be (nb, nG) complex64")
+        operand row per band (half the flops); imaginary parts are zero.
+        real="half": the same with psi given as half rows (nb, nh), entry j =
+        s_j psi(G_j) (s_0 = 1, s_j = sqrt 2)."""
+        mode, n = self._rows(real)
+        if psi.dtype != torch.complex64 or psi.dim() != 2 or psi.shape[1] != n:
+            raise ValueError(f"psi must be (nb, {n}) complex64")
         psi = psi.contiguous()
         nb = psi.shape[0]
         out = torch.empty((nb, self.np), dtype=torch.complex64, device=psi.device)
-        _lib.check(_lib.lib().addft_nonlocal_becp(self._h, _lib.ptr(psi), nb, self.ng, _lib.ptr(out), self.np,
-                                                  int(bool(real) and self.gamma), _lib.ptr(self._workspace(nb)),
-                                                  _lib.stream()), "addft_nonlocal_becp")
+        _lib.check(_lib.lib().addft_nonlocal_becp(self._h, _lib.ptr(psi), nb, n, _lib.ptr(out), self.np, mode,
+                                                  _lib.ptr(self._workspace(nb)), _lib.stream()),
+                   "addft_nonlocal_becp")
         return out
 
     def becp_t_noalloc(self, psi: torch.Tensor, out: torch.Tensor) -> torch.Tensor:
@@ -362,16 +401,17 @@ class ProjectorMFMA:
                    "addft_nonlocal_becp_noalloc")
         return out
 
-    def apply_add(self, S: torch.Tensor, out: torch.Tensor, real: bool = False) -> torch.Tensor:
+    def apply_add(self, S: torch.Tensor, out: torch.Tensor, real=False) -> torch.Tensor:
         """out (nb, nG) complex64 += S P (S' P' at k = 0; real: S' real, rows
-        real in real space, its imaginary part not read)."""
+        real in real space, its imaginary part not read; real="half": out
+        holds half rows (nb, nh), updated in place)."""
+        mode, n = self._rows(real)
         if S.dtype != torch.complex64 or out.dtype != torch.complex64:
             raise ValueError("S and out must be complex64")
-        if not out.is_contiguous() or out.shape != (S.shape[0], self.ng) or S.shape[1] != self.np:
-            raise ValueError("out must be a contiguous (nb, nG) block and S (nb, nproj)")
+        if not out.is_contiguous() or out.shape != (S.shape[0], n) or S.shape[1] != self.np:
+            raise ValueError(f"out must be a contiguous (nb, {n}) block and S (nb, nproj)")
         S = S.contiguous()
-        _lib.check(_lib.lib().addft_nonlocal_apply_add(self._h, _lib.ptr(S), S.shape[0], self.np, _lib.ptr(out),
-                                                       self.ng, int(bool(real) and self.gamma),
-                                                       _lib.ptr(self._workspace(S.shape[0])), _lib.stream()),
+        _lib.check(_lib.lib().addft_nonlocal_apply_add(self._h, _lib.ptr(S), S.shape[0], self.np, _lib.ptr(out), n,
+                                                       mode, _lib.ptr(self._workspace(S.shape[0])), _lib.stream()),
                    "addft_nonlocal_apply_add")
         return out

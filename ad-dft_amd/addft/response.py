@@ -31,7 +31,7 @@ N/Omega from their trace normalisation to the plain grid dot product.
 
 from __future__ import annotations
 
-import math
+import os
 from typing import NamedTuple
 
 import numpy as np
@@ -39,6 +39,7 @@ import torch
 
 from .kernels import _lib as _lib_mod
 from .kernels import native
+from .kernels.nonlocal_bf16 import ProjectorMFMA
 from .solver.davidson import eigh as _eigh
 from .solver.davidson import precondition
 from .solver.linalg import inner
@@ -155,36 +156,6 @@ def gmres(apply, b, rtol=1e-10, maxiter=200, restart=60, precond=None, on_residu
     return x, total
 
 
-class _HalfSphere:
-    """Rows real in real space (v(-G) = conj v(G)) on the half sphere: the
-    G of the pair table's positive half (entry 0: G = 0), scaled by sqrt(2)
-    off G = 0, so that Re sum over the half of conj(a) b is the full-sphere
-    <a|b>."""
-
-    def __init__(self, plus, minus, ng):
-        self.ng = int(ng)
-        self.p = plus.long()
-        self.p1 = plus[1:].long()
-        self.m1 = minus[1:].long()
-        nh = int(plus.numel())
-        s = torch.full((nh,), math.sqrt(2.0), dtype=torch.float32, device=plus.device)
-        s[0] = 1.0
-        self.s = s
-        self.inv_s = 1.0 / s
-
-    def to_half(self, F):
-        """(m, nG) -> (m, nh) complex64 scaled half-sphere rows."""
-        return (F[:, self.p].to(torch.complex64) * self.s).contiguous()
-
-    def to_full(self, S):
-        """(m, nh) scaled half rows -> (m, nG) complex64 full-sphere rows."""
-        t = S * self.inv_s
-        out = torch.empty((S.shape[0], self.ng), dtype=torch.complex64, device=S.device)
-        out[:, self.p] = t
-        out[:, self.m1] = t[:, 1:].conj()
-        return out
-
-
 class Response:
     """chi0, K and J at a converged state (stick-order psi per k, complex128)."""
 
@@ -215,6 +186,13 @@ class Response:
         self.kw = [float(st.kweights[ik]) * st.spin_degeneracy for ik in range(engine.nk)]
         self.stats = {"chi0": 0, "cg_iterations": 0, "hpsi_bands": 0}
         self.kept = None
+        # k = 0 with rows real in real space: every vector of the response is
+        # held as half rows (HalfRows: half the traffic, real coefficients,
+        # real GEMMs of half the depth, two bands per local transform)
+        self.real = bool(engine.nk == 1 and engine.ghalf is not None and engine.kmask64 is None
+                         and self._rows_real(self.psi[0]))
+        self.half = engine.ghalf if self.real else None
+        self.rows = [self.half.to_half(p) for p in self.psi] if self.real else self.psi
         # mixed precision: FP32 deflated PCG inside FP64 iterative refinement
         if precision not in ("auto", "fp64", "mixed"):
             raise ValueError(f"unknown precision {precision!r}")
@@ -223,11 +201,8 @@ class Response:
             raise ValueError("precision='mixed' needs an engine set up with precision='mixed'")
         if self.mixed:
             self.vs32 = engine.stage(v, mixed_stage=True)
-            self.psi32 = [p.to(torch.complex64).contiguous() for p in self.psi]
+            self.rows32 = [p.to(torch.complex64).contiguous() for p in self.rows]
             self.stats.update({"refinements": 0, "hpsi_bands_fp32": 0})
-        # k = 0 with rows real in real space: real coefficients, real GEMMs,
-        # two bands per local transform
-        self.real = bool(engine.nk == 1 and engine.gplus_d is not None and self._rows_real(self.psi[0]))
         self.defl_steps = int(defl_steps)
         self.defl_res = float(defl_res)
         self.defl = [None] * engine.nk
@@ -239,10 +214,24 @@ class Response:
                     self.defl[ik] = self._deflation_space(ik, deflation[ik])
         if self.mixed:
             # the FP32 solves project out the computed states and W together
-            self.basis32 = [torch.cat([self.psi32[ik]] + ([self.defl[ik]["W32"]] if self.defl[ik] else []))
+            self.basis32 = [torch.cat([self.rows32[ik]] + ([self.defl[ik]["W32"]] if self.defl[ik] else []))
                             .contiguous() for ik in range(engine.nk)]
-            self.half = _HalfSphere(engine.gplus_d, engine.gminus_d, engine.ng) if self.real else None
-            self.basis32h = ([self.half.to_half(b) for b in self.basis32] if self.half is not None else None)
+            # ADDFT_RESP_PROJ=mfma: on half rows the projection coefficients
+            # <b_m|v> as BF16x3 MFMA products (the projector kernels with the
+            # basis as even-parity projector rows).  Not the default: the
+            # bf16 hi + lo image of the basis differs from the basis by
+            # ~2^-18 per entry, so the projection is no longer symmetric to
+            # FP32 accuracy, and the inner CG, whose goals reach 2e-6, takes
+            # 2.5x the iterations (216 atoms: 3011 against 1225)
+            self.proj_mfma = None
+            mode = os.environ.get("ADDFT_RESP_PROJ", "gemm")
+            if mode not in ("mfma", "gemm"):
+                raise ValueError(f"ADDFT_RESP_PROJ={mode!r}: expected mfma or gemm")
+            if self.real and mode == "mfma":
+                full = torch.cat([self.psi[0]] + ([self.half.to_full(self.defl[0]["W64"])] if self.defl[0] else []))
+                odd = torch.zeros(full.shape[0], dtype=torch.int32, device=full.device)
+                self.proj_mfma = ProjectorMFMA(full, (engine.gplus_d, engine.gminus_d), odd)
+                del full
 
     # --------------------------------------------------------------- K
     def K(self, dn):
@@ -273,10 +262,9 @@ class Response:
         of H above the computed set, which deflate the slow low end of each
         band's Sternheimer operator.  Any W leaves the solution unchanged
         (deflated CG converges to the same x); a good W only speeds it."""
-        e = self.e
         W = V.to(torch.complex128).contiguous()
-        if self.real:
-            W = e.realify(W)
+        if self.half is not None:
+            W = self.half.to_half(W)
         for _ in range(2):
             W = self._project(ik, W)
             S = inner(W, W, conj_a=True).to(torch.complex128)      # S[m, n] = <w_m|w_n>
@@ -287,14 +275,12 @@ class Response:
             keep = w > 1e-8 * float(w.max())
             Wn = (U[:, keep] / torch.sqrt(w[keep])[None, :]).T.to(W.dtype)   # rows W'_j = sum_m U[m,j] w_m / sqrt
             W = torch.matmul(Wn, W).contiguous()
-            if self.real:
-                W = e.realify(W)
         # a few block-Davidson steps in range(Q) (FP64): the buffer vectors
         # come from an eigensolver that does not converge them
-        diag = e.tkin64[ik] + self.vmean
+        diag = self._diag(ik)
         k = W.shape[0]
         for step in range(self.defl_steps + 1):
-            HW = self._project(ik, e.hpsi(ik, W, self.vs, real=self.real))
+            HW = self._project(ik, self._h64(ik, W))
             T = inner(W, HW, conj_a=True).to(torch.complex128)
             T = 0.5 * (T + T.conj().T)
             if self.real:
@@ -303,8 +289,6 @@ class Response:
             Ut = U.T.to(W.dtype)
             W = torch.matmul(Ut, W).contiguous()
             HWq = torch.matmul(Ut, HW).contiguous()
-            if self.real:
-                W, HWq = e.realify(W), e.realify(HWq)
             R = HWq - theta[:, None].to(W.real.dtype) * W
             res = torch.linalg.vector_norm(R, dim=1)
             if step == self.defl_steps or float(res.max()) < 1e-3 * self.defl_res:
@@ -322,15 +306,13 @@ class Response:
             keep = w > 1e-10 * float(w.max())
             Z = torch.matmul((V[:, keep] / torch.sqrt(w[keep])[None, :]).T.to(Z.dtype), Z)
             B = torch.cat([W, Z]).contiguous()
-            HB = self._project(ik, e.hpsi(ik, B, self.vs, real=self.real))
+            HB = self._project(ik, self._h64(ik, B))
             T = inner(B, HB, conj_a=True).to(torch.complex128)
             T = 0.5 * (T + T.conj().T)
             if self.real:
                 T = T.real.to(torch.complex128)
             _, U = _eigh(T.real if self.real else T)
             W = torch.matmul(U[:, :k].T.to(B.dtype), B).contiguous()
-            if self.real:
-                W = e.realify(W)
         # residual of each W as an eigenvector of Q H Q: its coupling to the
         # rest of the complement, which the outer flexible CG absorbs
         # keep the lowest run of accurate vectors, cut at its widest gap in
@@ -379,9 +361,24 @@ class Response:
         return V.sub_(torch.matmul(C.to(V.dtype), B))
 
     def _project(self, ik, V, fp32=False):
-        psi = self.psi32[ik] if fp32 else self.psi[ik]
+        psi = self.rows32[ik] if fp32 else self.rows[ik]
         V = V.contiguous().clone()
         return self._sub(V, self._coef(V, psi), psi)
+
+    def _h64(self, ik, X):
+        """FP64 H X on the working rows (half rows at k = 0)."""
+        if self.half is not None:
+            return self.e.hpsi_half(ik, X, self.vs)
+        return self.e.hpsi(ik, X, self.vs)
+
+    def _diag(self, ik):
+        """Kinetic + mean potential on the working rows (FP64)."""
+        d = self.e.tkin64[ik] + self.vmean
+        return self.half.values(d) if self.half is not None else d
+
+    def _full(self, X):
+        """Working rows -> full-sphere rows."""
+        return self.half.to_full(X) if self.half is not None else X
 
     def sternheimer(self, ik, Vpsi, eps, weight=None):
         """x_n (rows) with Q (H - e_n) Q x_n = -Q Vpsi_n, x_n in range(Q).
@@ -401,23 +398,17 @@ class Response:
     def _cg64(self, ik, b, eps, goal):
         """Preconditioned CG in FP64 for Q (H - e_n) Q x_n = b_n (b in
         range(Q)), band n stopping at |r_n| <= goal_n."""
-        e = self.e
-        diag = e.tkin64[ik] + self.vmean
-        # rows real in real space stay real only if kept so: the paired
-        # transforms and the real projection coefficients act on the real
-        # part, and rounding would otherwise leave an unprojected imaginary
-        # part to grow
-        keep_real = e.realify if self.real else (lambda t: t)
+        diag = self._diag(ik)
         x = torch.zeros_like(b)
-        r = keep_real(b.clone())
-        z = keep_real(self._project(ik, precondition(r, eps, diag)))
+        r = b.clone()
+        z = self._project(ik, precondition(r, eps, diag))
         p = z.clone()
         rz = torch.sum((r.conj() * z).real, dim=1)
         active = torch.arange(b.shape[0], device=b.device)
         it = 0
         for it in range(1, self.cg_maxiter + 1):
             pa = p[active]
-            Ap = self._project(ik, e.hpsi(ik, pa, self.vs, real=self.real) - eps[active, None].to(pa.real.dtype) * pa)
+            Ap = self._project(ik, self._h64(ik, pa) - eps[active, None].to(pa.real.dtype) * pa)
             self.stats["hpsi_bands"] += int(active.numel())
             pAp = torch.sum((pa.conj() * Ap).real, dim=1)
             alpha = rz[active] / pAp
@@ -427,7 +418,7 @@ class Response:
             keep = active[rn > goal[active]]
             if keep.numel() == 0:
                 break
-            zk = keep_real(self._project(ik, precondition(r[keep], eps[keep], diag)))
+            zk = self._project(ik, precondition(r[keep], eps[keep], diag))
             rz_new = torch.sum((r[keep].conj() * zk).real, dim=1)
             beta = rz_new / rz[keep]
             p[keep] = zk + beta[:, None] * p[keep]
@@ -439,11 +430,8 @@ class Response:
     def _apply64(self, ik, X, eps):
         """Q (H - e_n) Q x_n in FP64 (rows)."""
         Xq = self._project(ik, X)
-        if self.real:
-            Xq = self.e.realify(Xq)
         self.stats["hpsi_bands"] += int(X.shape[0])
-        return self._project(ik, self.e.hpsi(ik, Xq, self.vs, real=self.real)
-                             - eps[:, None].to(Xq.real.dtype) * Xq), Xq
+        return self._project(ik, self._h64(ik, Xq) - eps[:, None].to(Xq.real.dtype) * Xq), Xq
 
     def _sternheimer_mixed(self, ik, Vpsi, eps, weight=None, max_outer=12):
         """Flexible CG in FP64 (Notay, SIAM J. Sci. Comput. 22 (2000) 1444,
@@ -454,10 +442,7 @@ class Response:
         FP64 H psi; the FP32 solves carry the iterations.  The outer
         residual is the FP64 one, so the result meets the FP64 stopping
         rule whatever the inner accuracy."""
-        e = self.e
         b = -self._project(ik, Vpsi)
-        if self.real:
-            b = e.realify(b)
         bn = torch.linalg.vector_norm(b, dim=1)
         if weight is not None:
             bn = bn / torch.clamp(weight.to(bn.dtype), min=1e-300)
@@ -533,35 +518,35 @@ class Response:
         working arrays hold the unconverged bands only (compacted when bands
         converge), the diagonal preconditioner is tabulated once per solve,
         and each iteration is three fused kernels (addft_cg_*) besides H and
-        the projections.  Rows real in real space are held on the half
-        sphere (G and -G carry conjugate values), scaled by sqrt(2) off
-        G = 0 so that plain dot products over the half give the full-sphere
-        ones: half the projection flops and vector traffic.  (Lifting the
+        the projections.  At k = 0 the rows are half rows (HalfRows), H acts
+        on them directly (Engine.hpsi_half) and the projections are BF16x3
+        MFMA products with the basis as the projector rows.  (Lifting the
         computed space by a shift instead of projecting it out, the alpha
         P_v device of perturbation theory, saves one projection per
         iteration but stalls where the top computed states are inexact and
         nearly degenerate with the complement.)"""
         e = self.e
-        half = self.half if self.real else None
-        basis = self.basis32h[ik] if half is not None else self.basis32[ik]
-        diag = (e.tkin64[ik] + self.vmean).to(torch.float32)
-        if half is not None:
-            diag = diag[half.p]
+        basis = self.basis32[ik]
+        diag = self._diag(ik).to(torch.float32)
+        pm = self.proj_mfma if self.half is not None else None
 
         def proj(V):
-            V = self._sub(V.contiguous(), self._coef(V.contiguous(), basis), basis)
-            return e.realify(V) if (self.real and half is None) else V
+            V = V.contiguous()
+            # V <- V - sum_m <b_m|V> b_m in place (FP32 GEMMs; the
+            # coefficients from the MFMA kernels under ADDFT_RESP_PROJ=mfma)
+            C = pm.becp_t(V, real="half").real if pm is not None else self._coef(V, basis)
+            return self._sub(V, C, basis)
 
         def apply_h(P):
-            if half is None:
-                return e.hpsi(ik, P, self.vs32, fp32=True, real=self.real)
-            return half.to_half(e.hpsi(ik, half.to_full(P), self.vs32, fp32=True, real=True))
+            if self.half is not None:
+                return e.hpsi_half(ik, P, self.vs32, fp32=True)
+            return e.hpsi(ik, P, self.vs32, fp32=True)
 
         def rdot(a, b):
             ra, rb = torch.view_as_real(a).reshape(a.shape[0], -1), torch.view_as_real(b).reshape(b.shape[0], -1)
             return (ra * rb).sum(dim=1)
 
-        r = proj(half.to_half(r0) if half is not None else r0.clone())
+        r = proj(r0.clone())
         n = r.shape[0]
         out = torch.zeros_like(r)
         idx = torch.arange(n, device=r.device)
@@ -602,7 +587,7 @@ class Response:
         else:
             out[idx] = x
         self.stats["cg_iterations"] += it
-        return half.to_full(out) if half is not None else out
+        return out
 
     def _combine(self, C, B):
         """C B for coefficient rows C (m, n_B) (real coefficients as a real
@@ -629,10 +614,12 @@ class Response:
         out = torch.zeros(e.nfft, dtype=torch.float64, device=e.dev)
         kept, occ_parts = [], []
         for ik in range(e.nk):
-            psi = self.psi[ik]
+            psi = self.rows[ik]
             eps = self.eig[ik]
             f = self.occ[ik]
-            Vpsi = e.restrict(ik, self._local64(psi, vs))
+            Vpsi = self._local64(psi, vs)
+            if self.half is None:
+                Vpsi = e.restrict(ik, Vpsi)
             Lam = self._coef(Vpsi, psi).T                 # Lam[m, n] = <m|dV|n> (real at k = 0)
             Lam = 0.5 * (Lam + Lam.conj().T)
             de = Lam.diagonal().real
@@ -656,7 +643,7 @@ class Response:
         for ik in range(e.nk):
             if st.sigma > 1e-6:
                 c = self.kw[ik] / e.omega
-                out += self._density64(self.psi[ik], (c * dfs[ik]).contiguous())
+                out += self._density64(self.rows[ik], (c * dfs[ik]).contiguous())
         if keep:
             self.kept = list(zip(kept, dfs))
         return out
@@ -681,7 +668,6 @@ class Response:
     def _cross_density(self, psi, phi, w):
         """sum_n w_n 2 Re(conj(psi_n(r)) phi_n(r)) by polarisation:
         (|psi + t phi|^2 - |psi - t phi|^2) / (2t), t_n = |psi_n| / |phi_n|."""
-        e = self.e
         pn = torch.linalg.vector_norm(psi, dim=1)
         fn = torch.linalg.vector_norm(phi, dim=1)
         t = torch.where(fn > 0, pn / torch.clamp(fn, min=1e-300), torch.ones_like(pn)).clamp(max=1e8)
@@ -691,21 +677,22 @@ class Response:
         return self._density64(a, wt) - self._density64(b, wt)
 
     def _local64(self, psi, vs):
-        """V psi (FP64, no kinetic term); two bands per transform for rows
-        real in real space."""
+        """V psi (FP64, no kinetic term) on the working rows; two bands per
+        transform for half rows."""
         e = self.e
-        if not self.real:
+        if self.half is None:
             return e.plan64.hpsi_local(psi, self.zero_t, vs)
-        h = e.plan64.hpsi_local(native.gamma_pack2(psi.contiguous()), self.zero_t, vs)
-        return native.gamma_unpack2(h, psi.shape[0], e.gplus_d, e.gminus_d)
+        h = e.plan64.hpsi_local(native.gamma_pack2_half(psi.contiguous(), e.gplus_d, e.gminus_d, e.ng),
+                                self.zero_t, vs)
+        return native.gamma_unpack2_half(h, psi.shape[0], e.gplus_d, e.gminus_d)
 
     def _density64(self, psi, w):
-        """sum_n w_n |psi_n(r)|^2 (plane-major FP64).  Rows real in real
-        space: sqrt(|w_n|) psi_n packed in pairs a + i b, whose |.|^2 is
+        """sum_n w_n |psi_n(r)|^2 (plane-major FP64) of working rows.  Half
+        rows: sqrt(|w_n|) psi_n packed in pairs a + i b, whose |.|^2 is
         |w_a| a^2 + |w_b| b^2, one transform per pair (weights of each sign
         separately)."""
         e = self.e
-        if not self.real:
+        if self.half is None:
             return e.plan64.density(psi, w)
         out = torch.zeros(e.nfft, dtype=torch.float64, device=psi.device)
         for sign in (1.0, -1.0):
@@ -713,7 +700,7 @@ class Response:
             if idx.numel() == 0:
                 continue
             a = psi[idx] * torch.sqrt(w[idx].abs()).to(psi.real.dtype)[:, None]
-            pk = native.gamma_pack2(a.contiguous())
+            pk = native.gamma_pack2_half(a.contiguous(), e.gplus_d, e.gminus_d, e.ng)
             ones = torch.ones(pk.shape[0], dtype=torch.float64, device=psi.device)
             if sign > 0:
                 out += e.plan64.density(pk, ones)
@@ -747,6 +734,7 @@ class Response:
         for ik in range(e.nk):
             psi = self.psi[ik]
             dpsi, df = self.kept[ik]
+            dpsi = self._full(dpsi)
             f = self.occ[ik]
             c = self.kw[ik] * f
             dc = self.kw[ik] * df
@@ -779,6 +767,7 @@ class Response:
         for ik in range(e.nk):
             psi = self.psi[ik]
             dpsi, df = self.kept[ik]
+            dpsi = self._full(dpsi)
             c = self.kw[ik] * self.occ[ik]
             dc = self.kw[ik] * df
             dens = ((c[:, None] * 2.0 * (psi.conj() * dpsi).real) + dc[:, None] * psi.abs() ** 2).sum(dim=0)

@@ -87,6 +87,7 @@ constexpr int kTM = 352;        // rows per tile: 176 bands
 constexpr int kWM = 176;        // rows per wave
 constexpr int kMB = 11;         // 16-row MFMA blocks per wave
 constexpr int kKT = 32;         // real K per step
+constexpr float kSqrt2f = 1.41421356237309504880f;
 constexpr int kBecpNB = 3;      // 16-column blocks per wave, becp
 constexpr int kBecpTN = 4 * 16 * kBecpNB;   // 192 projectors per tile
 constexpr int kApplyNB = 2;     // apply: one entry step (Q.re, Q.im blocks) per wave
@@ -210,7 +211,10 @@ constexpr int kPrStepStride = kPrRows * kKT + 16;  // elements
 // REAL (k = 0, rows real in real space: psi(-G) = conj psi(G)): c is real
 // and d imaginary, so the Im rows and the imaginary part of becp vanish;
 // only the Re row of each band is built (row r = b, 16 bands per block).
-template <bool GAMMA, bool REAL>
+// HALF (with REAL): psi given as half rows h, entry j = s_j psi(G_j) (s_0 =
+// 1, s_j = sqrt 2); then c.re = 2 Re psi(G_j) and d.im = 2 Im psi(G_j) off
+// G = 0, i.e. [c.re | d.im] = s_j (h_j.re, h_j.im), read without a gather.
+template <bool GAMMA, bool REAL, bool HALF = false>
 __global__ __launch_bounds__(kPrThreads) void k_pair_rows(const float2* __restrict__ psi, int64_t nb, int64_t ldpsi,
                                                           const int32_t* __restrict__ plus,
                                                           const int32_t* __restrict__ minus, int64_t npair,
@@ -233,8 +237,8 @@ __global__ __launch_bounds__(kPrThreads) void k_pair_rows(const float2* __restri
   for (int u = 0; u < 2; ++u) {
     const int64_t j = 16 * s0 + e + u;
     ok[u] = j < npair;
-    ip[u] = ok[u] ? (GAMMA ? plus[j] : (int32_t)j) : 0;
-    im[u] = (ok[u] && GAMMA) ? minus[j] : -1;
+    ip[u] = ok[u] ? ((GAMMA && !HALF) ? plus[j] : (int32_t)j) : 0;
+    im[u] = (ok[u] && GAMMA && !HALF) ? minus[j] : -1;
   }
   const int sl = e >> 4, c = e & 15;
 #pragma unroll
@@ -249,7 +253,13 @@ __global__ __launch_bounds__(kPrThreads) void k_pair_rows(const float2* __restri
       float2 a = make_float2(0.f, 0.f), m = make_float2(0.f, 0.f);
       if (b < nb && ok[u]) {
         a = row[ip[u]];
-        if (GAMMA && im[u] >= 0) m = row[im[u]];
+        if (HALF) {
+          const float f = ip[u] == 0 ? 1.f : kSqrt2f;
+          a.x *= f;
+          a.y *= f;
+        } else if (GAMMA && im[u] >= 0) {
+          m = row[im[u]];
+        }
       }
       // Re row [c.re | d.im], Im row [c.im | -d.re]
       const float v[4] = {GAMMA ? a.x + m.x : a.x, GAMMA ? a.y - m.y : a.y, GAMMA ? a.y + m.y : a.y,
@@ -508,8 +518,10 @@ __global__ __launch_bounds__(kThreads) void k_becp_staged(const uint16_t* __rest
 
 // out += S P from the staged S rows.  Grid: N tile major (the M tiles of one
 // N tile, which share its V columns, run together on one XCD: V is read from
-// memory once), XCD-ordered.  REAL: S real, one image row per band.
-template <bool GAMMA, bool REAL, int WGM>
+// memory once), XCD-ordered.  REAL: S real, one image row per band.  HALF
+// (with REAL): out holds half rows (entry jj = s_jj value(G_jj)), updated
+// in place at jj only.
+template <bool GAMMA, bool REAL, int WGM, bool HALF = false>
 __global__ __launch_bounds__(kThreads) void k_apply_staged(const uint16_t* __restrict__ A, int64_t mrows,
                                                            int64_t nb, const uint16_t* __restrict__ V, int64_t npst,
                                                            int64_t kv, const int32_t* __restrict__ plus,
@@ -537,8 +549,9 @@ __global__ __launch_bounds__(kThreads) void k_apply_staged(const uint16_t* __res
   for (int e = 0; e < kApplyNB / 2; ++e) {
     const int64_t jj = (n0 + wn * 16 * kApplyNB + 32 * e) / 2 + fr;
     if (jj >= npair) continue;
-    const int64_t pos_p = GAMMA ? plus[jj] : jj;
-    const int64_t pos_m = GAMMA ? minus[jj] : -1;
+    const int64_t pos_p = (GAMMA && !HALF) ? plus[jj] : jj;
+    const int64_t pos_m = (GAMMA && !HALF) ? minus[jj] : -1;
+    const float fh = jj == 0 ? 1.f : kSqrt2f;  // s_jj of a half row
 #pragma unroll
     for (int i = 0; i < kMB; ++i) {
       const int64_t r = (int64_t)mt * TM + wm * kWM + i * 16 + 4 * fc;
@@ -549,6 +562,12 @@ __global__ __launch_bounds__(kThreads) void k_apply_staged(const uint16_t* __res
           const float rr = acc[i][2 * e][h], ri = acc[i][2 * e + 1][h];
           float2* row = out + (r + h) * ldo;
           float2 v = row[pos_p];
+          if (HALF) {
+            v.x += fh * rr;
+            v.y += fh * ri;
+            row[pos_p] = v;
+            continue;
+          }
           v.x += rr;
           v.y += ri;
           row[pos_p] = v;
@@ -1029,7 +1048,9 @@ int addft_nonlocal_becp(addft_projector p, const void* d_psi, int64_t nb, int64_
                         int64_t ldb, int real_rows, void* d_work, void* stream) {
   ADDFT_CHECK_ARG(p != nullptr && d_psi != nullptr && d_becp != nullptr && d_work != nullptr, "null argument");
   ADDFT_CHECK_ARG(nb > 0 && nb <= (int64_t)1 << 20, "nb out of range");
-  ADDFT_CHECK_ARG(ldpsi >= p->ng && ldb >= p->np, "leading dimensions too small");
+  ADDFT_CHECK_ARG(real_rows >= ADDFT_ROWS_COMPLEX && real_rows <= ADDFT_ROWS_HALF, "bad real_rows");
+  const bool half = real_rows == ADDFT_ROWS_HALF;
+  ADDFT_CHECK_ARG(ldpsi >= (half ? p->npair : p->ng) && ldb >= p->np, "leading dimensions too small");
   ADDFT_CHECK_ARG(!real_rows || p->gamma, "real_rows needs a k = 0 (gamma) projector object");
   hipStream_t s = addft::as_stream(stream);
   const int rpb = real_rows ? 1 : 2;
@@ -1041,7 +1062,10 @@ int addft_nonlocal_becp(addft_projector p, const void* d_psi, int64_t nb, int64_
   const int64_t pr_blocks = (nb * rpb + kPrRows - 1) / kPrRows * ((p->ws + kPrSteps - 1) / kPrSteps);
   ADDFT_CHECK_ARG(pr_blocks <= INT32_MAX, "grid too large");
   const float2* psi = static_cast<const float2*>(d_psi);
-  if (real_rows)
+  if (half)
+    hipLaunchKernelGGL((k_pair_rows<true, true, true>), dim3((unsigned)pr_blocks), dim3(kPrThreads), 0, s, psi, nb,
+                       ldpsi, nullptr, nullptr, p->npair, p->kp, w.mrows, rows);
+  else if (real_rows)
     hipLaunchKernelGGL((k_pair_rows<true, true>), dim3((unsigned)pr_blocks), dim3(kPrThreads), 0, s, psi, nb, ldpsi,
                        p->d_plus, p->d_minus, p->npair, p->kp, w.mrows, rows);
   else if (p->gamma)
@@ -1089,7 +1113,9 @@ int addft_nonlocal_apply_add(addft_projector p, const void* d_s, int64_t nb, int
                              int64_t ldo, int real_rows, void* d_work, void* stream) {
   ADDFT_CHECK_ARG(p != nullptr && d_s != nullptr && d_out != nullptr && d_work != nullptr, "null argument");
   ADDFT_CHECK_ARG(nb > 0 && nb <= (int64_t)1 << 20, "nb out of range");
-  ADDFT_CHECK_ARG(lds >= p->np && ldo >= p->ng, "leading dimensions too small");
+  ADDFT_CHECK_ARG(real_rows >= ADDFT_ROWS_COMPLEX && real_rows <= ADDFT_ROWS_HALF, "bad real_rows");
+  const bool half = real_rows == ADDFT_ROWS_HALF;
+  ADDFT_CHECK_ARG(lds >= p->np && ldo >= (half ? p->npair : p->ng), "leading dimensions too small");
   ADDFT_CHECK_ARG(!real_rows || p->gamma, "real_rows needs a k = 0 (gamma) projector object");
   hipStream_t s = addft::as_stream(stream);
   const int rpb = real_rows ? 1 : 2;
@@ -1109,7 +1135,8 @@ int addft_nonlocal_apply_add(addft_projector p, const void* d_s, int64_t nb, int
   const int64_t grid = ntm * ntn;
   ADDFT_CHECK_ARG(grid <= INT32_MAX, "grid too large");
   float2* out = static_cast<float2*>(d_out);
-  auto kern = real_rows ? (w.wgm == 1 ? k_apply_staged<true, true, 1> : k_apply_staged<true, true, 2>)
+  auto kern = half ? (w.wgm == 1 ? k_apply_staged<true, true, 1, true> : k_apply_staged<true, true, 2, true>)
+              : real_rows ? (w.wgm == 1 ? k_apply_staged<true, true, 1> : k_apply_staged<true, true, 2>)
               : p->gamma ? (w.wgm == 1 ? k_apply_staged<true, false, 1> : k_apply_staged<true, false, 2>)
                          : (w.wgm == 1 ? k_apply_staged<false, false, 1> : k_apply_staged<false, false, 2>);
   hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(kThreads), 0, s, rows, w.mrows, nb, p->d_v, p->npst, p->kv,

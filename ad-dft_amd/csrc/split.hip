@@ -178,6 +178,59 @@ __global__ void k_gamma_unpack2(const addft::cpx<R>* __restrict__ h, int64_t m, 
   }
 }
 
+// The same pair of transforms for rows held on the half sphere: entry j of
+// a half row is s_j psi(G_j) over the positive half of the pair table
+// (G_0 = 0, s_0 = 1, s_j = sqrt 2), so that plain dot products of half rows
+// are the full-sphere ones.  pack writes phi_b at G_j and -G_j from the two
+// bands' half rows (psi(0) taken as real); unpack writes H psi_{2b} and
+// H psi_{2b+1} as half rows.  Both read and write each half row once,
+// coalesced; the full-sphere phi / h are reached through the tables.
+template <typename R>
+__global__ void k_gamma_pack2_half(const addft::cpx<R>* __restrict__ half, int64_t m, int64_t ldh, int64_t nh,
+                                   const int32_t* __restrict__ plus, const int32_t* __restrict__ minus,
+                                   int64_t ng, addft::cpx<R>* __restrict__ phi) {
+  const int64_t b = blockIdx.y;
+  const addft::cpx<R>* a = half + (2 * b) * ldh;
+  const bool has_b = 2 * b + 1 < m;
+  const addft::cpx<R>* c = half + (2 * b + 1) * ldh;
+  addft::cpx<R>* o = phi + b * ng;
+  const R inv = (R)0.70710678118654752440;
+  for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < nh; j += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t q = minus[j];  // -1 at G = 0
+    const R f = q < 0 ? (R)1 : inv;
+    addft::cpx<R> va = a[j], vb = has_b ? c[j] : addft::mk((R)0, (R)0);
+    va = addft::mk(f * va.x, q < 0 ? (R)0 : f * va.y);
+    vb = addft::mk(f * vb.x, q < 0 ? (R)0 : f * vb.y);
+    // phi(G) = a + i b, phi(-G) = conj a + i conj b
+    o[plus[j]] = addft::mk(va.x - vb.y, va.y + vb.x);
+    if (q >= 0) o[q] = addft::mk(va.x + vb.y, vb.x - va.y);
+  }
+}
+
+template <typename R>
+__global__ void k_gamma_unpack2_half(const addft::cpx<R>* __restrict__ h, int64_t m, int64_t ng,
+                                     const int32_t* __restrict__ plus, const int32_t* __restrict__ minus,
+                                     int64_t nh, addft::cpx<R>* __restrict__ out, int64_t ldo) {
+  const int64_t b = blockIdx.y;
+  const addft::cpx<R>* row = h + b * ng;
+  addft::cpx<R>* oa = out + (2 * b) * ldo;
+  const bool has_b = 2 * b + 1 < m;
+  addft::cpx<R>* ob = out + (2 * b + 1) * ldo;
+  const R hs = (R)0.70710678118654752440;  // s_j / 2 off G = 0
+  for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < nh; j += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t q = minus[j];
+    const addft::cpx<R> hp = row[plus[j]];
+    if (q < 0) {  // G = 0
+      oa[j] = addft::mk(hp.x, (R)0);
+      if (has_b) ob[j] = addft::mk(hp.y, (R)0);
+      continue;
+    }
+    const addft::cpx<R> hq = row[q];
+    oa[j] = addft::mk(hs * (hp.x + hq.x), hs * (hp.y - hq.y));
+    if (has_b) ob[j] = addft::mk(hs * (hp.y + hq.y), hs * (hq.x - hp.x));
+  }
+}
+
 int blocks_for(int64_t n) {
   const int64_t b = (n + 255) / 256;
   return (int)(b < 1 ? 1 : (b > 65536 ? 65536 : b));
@@ -241,6 +294,52 @@ int addft_gamma_scatter_add(int dtype, const void* d_u, int64_t nb, int64_t nh, 
 int addft_gamma_scatter_add_real(int dtype, const void* d_u, int64_t nb, int64_t nh, const int32_t* d_plus,
                                  const int32_t* d_minus, int64_t ng, void* d_out, void* stream) {
   return gamma_scatter_add(dtype, d_u, nb, nh, d_plus, d_minus, ng, d_out, stream, true);
+}
+
+int addft_gamma_pack2_half(int dtype, const void* d_half, int64_t m, int64_t ldh, int64_t nh,
+                           const int32_t* d_plus, const int32_t* d_minus, int64_t ng, void* d_phi, void* stream) {
+  ADDFT_CHECK_ARG(dtype == ADDFT_C64 || dtype == ADDFT_C128, "dtype must be ADDFT_C64 or ADDFT_C128");
+  ADDFT_CHECK_ARG(m >= 0 && m <= 65534 && nh >= 0 && ldh >= nh && 2 * nh == ng + 1,
+                  "bad sizes (m <= 65534, ldh >= nh, 2 nh = nG + 1)");
+  if (m == 0 || nh == 0) return ADDFT_OK;
+  ADDFT_CHECK_ARG(d_half && d_plus && d_minus && d_phi && d_half != d_phi, "null or aliased argument");
+  const int64_t np = (m + 1) / 2;
+  int64_t gx = (nh + 255) / 256;
+  if (gx > 1024) gx = 1024;
+  hipStream_t st = addft::as_stream(stream);
+  if (dtype == ADDFT_C64)
+    hipLaunchKernelGGL(k_gamma_pack2_half<float>, dim3((unsigned)gx, (unsigned)np), dim3(256), 0, st,
+                       static_cast<const addft::cpx<float>*>(d_half), m, ldh, nh, d_plus, d_minus, ng,
+                       static_cast<addft::cpx<float>*>(d_phi));
+  else
+    hipLaunchKernelGGL(k_gamma_pack2_half<double>, dim3((unsigned)gx, (unsigned)np), dim3(256), 0, st,
+                       static_cast<const addft::cpx<double>*>(d_half), m, ldh, nh, d_plus, d_minus, ng,
+                       static_cast<addft::cpx<double>*>(d_phi));
+  ADDFT_LAUNCH_CHECK();
+  return ADDFT_OK;
+}
+
+int addft_gamma_unpack2_half(int dtype, const void* d_h, int64_t m, int64_t ng, const int32_t* d_plus,
+                             const int32_t* d_minus, int64_t nh, void* d_out, int64_t ldo, void* stream) {
+  ADDFT_CHECK_ARG(dtype == ADDFT_C64 || dtype == ADDFT_C128, "dtype must be ADDFT_C64 or ADDFT_C128");
+  ADDFT_CHECK_ARG(m >= 0 && m <= 65534 && nh >= 0 && ldo >= nh && 2 * nh == ng + 1,
+                  "bad sizes (m <= 65534, ldo >= nh, 2 nh = nG + 1)");
+  if (m == 0 || nh == 0) return ADDFT_OK;
+  ADDFT_CHECK_ARG(d_h && d_plus && d_minus && d_out && d_h != d_out, "null or aliased argument");
+  const int64_t np = (m + 1) / 2;
+  int64_t gx = (nh + 255) / 256;
+  if (gx > 1024) gx = 1024;
+  hipStream_t st = addft::as_stream(stream);
+  if (dtype == ADDFT_C64)
+    hipLaunchKernelGGL(k_gamma_unpack2_half<float>, dim3((unsigned)gx, (unsigned)np), dim3(256), 0, st,
+                       static_cast<const addft::cpx<float>*>(d_h), m, ng, d_plus, d_minus, nh,
+                       static_cast<addft::cpx<float>*>(d_out), ldo);
+  else
+    hipLaunchKernelGGL(k_gamma_unpack2_half<double>, dim3((unsigned)gx, (unsigned)np), dim3(256), 0, st,
+                       static_cast<const addft::cpx<double>*>(d_h), m, ng, d_plus, d_minus, nh,
+                       static_cast<addft::cpx<double>*>(d_out), ldo);
+  ADDFT_LAUNCH_CHECK();
+  return ADDFT_OK;
 }
 
 int addft_gamma_pair_rows(const void* d_psi, int64_t nb, int64_t ng, const int32_t* d_plus, const int32_t* d_minus,

@@ -47,6 +47,9 @@ enum {
 
 enum { ADDFT_C64 = 0, ADDFT_C128 = 1, ADDFT_R64 = 2 };
 enum { ADDFT_EIGH_DC = 0, ADDFT_EIGH_JACOBI = 1, ADDFT_EIGH_DCJ = 2, ADDFT_EIGH_TRD = 3, ADDFT_EIGH_LDS = 4 };
+/* real_rows of the nonlocal products: complex rows, rows real in real space
+ * (full sphere), rows real in real space held on the half sphere. */
+enum { ADDFT_ROWS_COMPLEX = 0, ADDFT_ROWS_REAL = 1, ADDFT_ROWS_HALF = 2 };
 
 typedef struct addft_sphere_plan_s* addft_sphere_plan;
 typedef struct addft_projector_s* addft_projector;
@@ -109,6 +112,19 @@ int addft_gamma_scatter_add_real(int dtype, const void* d_u, int64_t nb, int64_t
 int addft_gamma_pack2(int dtype, const void* d_psi, int64_t m, int64_t ng, void* d_phi, void* stream);
 int addft_gamma_unpack2(int dtype, const void* d_h, int64_t m, int64_t ng, const int32_t* d_plus,
                         const int32_t* d_minus, int64_t nh, void* d_out, void* stream);
+/* The same for rows held on the half sphere: entry j of a half row (row
+ * stride ldh / ldo >= nh) is s_j psi(G_j) over the pair table's positive
+ * half (d_plus[j]; G_0 = 0 with d_minus[0] = -1, s_0 = 1, s_j = sqrt 2), so
+ * that Re of plain dot products of half rows are the full-sphere inner
+ * products.  addft_gamma_pack2_half writes phi (ceil(m/2), ng) from half
+ * rows (psi(0) taken as real); addft_gamma_unpack2_half writes H psi as half
+ * rows (m, ldo).  No reference counterpart (its Gamma point keeps full
+ * complex rows).  errors: bad dtype, m > 65534, 2 nh != ng + 1, ld < nh,
+ * null or aliased pointers. */
+int addft_gamma_pack2_half(int dtype, const void* d_half, int64_t m, int64_t ldh, int64_t nh,
+                           const int32_t* d_plus, const int32_t* d_minus, int64_t ng, void* d_phi, void* stream);
+int addft_gamma_unpack2_half(int dtype, const void* d_h, int64_t m, int64_t ng, const int32_t* d_plus,
+                             const int32_t* d_minus, int64_t nh, void* d_out, int64_t ldo, void* stream);
 
 /* ------------------------------------------ nonlocal projector products
  * The two products of the nonlocal operator (reference
@@ -137,6 +153,11 @@ int addft_gamma_unpack2(int dtype, const void* d_h, int64_t m, int64_t ng, const
  *   (psi(-G) = conj psi(G)), so becp = psi P'^H is real and S' is taken as
  *   real: one operand row per band instead of two (half the flops); becp's
  *   imaginary parts are written as zeros, S's are not read.
+ *   real_rows == ADDFT_ROWS_HALF: the same, with psi and out given as half
+ *   rows (entry j = s_j psi(G_j) over the pair table's positive half, s_0 =
+ *   1, s_j = sqrt 2, as for addft_gamma_pack2_half; ldpsi, ldo >= nh): the
+ *   operand rows are read and the products added in place without the
+ *   gathers and scatters of the +-G unfolding.
  * errors: null arguments, leading dimensions below nproj / ng, gamma tables
  *   that do not cover the sphere, real_rows on a k != 0 object.            */
 int addft_projector_create(int gamma, int64_t nproj, int64_t ng, const void* d_P, const int32_t* d_odd,
