@@ -557,6 +557,20 @@ class Engine:
             rho = plan.density(psi, w.contiguous(), out=rho, accumulate=ik > 0)
         return rho
 
+    def density_half(self, h, occ, fp32=False):
+        """density() for one k = 0 block of half rows (HalfRows) real in
+        real space: sqrt(w_n) psi_n packed two per transform
+        (addft_gamma_pack2_half), |a + i b|^2 = w_a a^2 + w_b b^2, so each
+        transform carries two bands."""
+        plan = self.plan32 if fp32 else self.plan64
+        w = (self.st.kweights[0] * self.st.spin_degeneracy / self.omega) * occ[0]
+        h = h.to(plan.cdtype) if h.dtype != plan.cdtype else h
+        if fp32:
+            w = w / self._norm2_fp32(h)
+        a = h * torch.sqrt(torch.clamp(w, min=0.0)).to(h.real.dtype)[:, None]
+        pk = native.gamma_pack2_half(a.contiguous(), self.gplus_d, self.gminus_d, self.ng)
+        return plan.density(pk, torch.ones(pk.shape[0], dtype=torch.float64, device=h.device))
+
     def occupations(self, eig):
         """(f, mu) for eig (nk, nb) float64 (occupation.py:25-49)."""
         st = self.st

@@ -19,7 +19,7 @@ import torch
 
 from ..kernels.native import cheb_combine
 from .davidson import eigh
-from .linalg import inner, rcombine, rinner
+from .linalg import inner, near_identity_inv_sqrt, rcombine, rinner
 
 
 def upper_bound(apply, nG, dtype, device, k=10, seed=0):
@@ -76,7 +76,10 @@ def filter_block(apply, X, HX, degree, a, b, a0):
 def orthonormalise(Y, real=False):
     """Rows of Y orthonormal: Cholesky-QR2 on the FP64 Gram matrix, with a
     shifted first pass when Y is too ill-conditioned for plain Cholesky
-    (Fukaya et al., shifted CholeskyQR, SIAM J. Sci. Comput. 42 (2020))."""
+    (Fukaya et al., shifted CholeskyQR, SIAM J. Sci. Comput. 42 (2020));
+    the second pass, whose Gram matrix is the identity to the first pass's
+    rounding, as the series Ss^(-1/2) = I - E/2 + 3 E^2 / 8 (two small GEMMs
+    instead of a Cholesky factorisation and a triangular inverse)."""
     for it in range(3):
         if real:              # rows real in real space (k = 0): a real Gram matrix, real GEMMs
             S = rinner(Y, Y).to(torch.float64)
@@ -86,6 +89,13 @@ def orthonormalise(Y, real=False):
             S = 0.5 * (S + S.conj().T)
         d = torch.sqrt(torch.clamp(S.diagonal().real, min=1e-300))
         Ss = S / (d[:, None] * d[None, :])
+        if it >= 1:
+            # second pass: Ss is the identity up to the first pass's rounding
+            M = near_identity_inv_sqrt(Ss)
+            if M is not None:
+                M = M / d[None, :]
+                Y = (rcombine(M, Y) if real else torch.matmul(M.to(Y.dtype), Y)).contiguous()
+                break
         L, info = torch.linalg.cholesky_ex(Ss)
         shifted = False
         if int(info.item()) != 0:

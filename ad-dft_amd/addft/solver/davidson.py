@@ -22,7 +22,7 @@ import os
 
 import torch
 
-from .linalg import inner, rcombine, rinner
+from .linalg import inner, near_identity_inv_sqrt, rcombine, rinner
 
 _CHECK = os.environ.get("ADDFT_DEBUG_FINITE", "0") == "1"
 
@@ -90,6 +90,11 @@ def _svqb(R, drop=1e-10, real=False):
     if float(off.abs().max()) < 1e-13:
         # already orthogonal: only rescale
         return (R / d.to(R.real.dtype)[:, None]).contiguous()
+    # nearly orthonormal (the second pass of _orth): Ss^(-1/2) as a series
+    M = near_identity_inv_sqrt(Ss)
+    if M is not None:
+        M = M / d[None, :]
+        return (rcombine(M, R) if real else torch.matmul(M.to(R.dtype), R)).contiguous()
     # well conditioned (the usual case): Cholesky of the scaled Gram matrix
     L, info = torch.linalg.cholesky_ex(Ss)
     if int(info.item()) == 0 and bool(torch.isfinite(L).all()) and float(L.diagonal().real.min()) > 1e-5:

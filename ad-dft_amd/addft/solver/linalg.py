@@ -209,3 +209,23 @@ def chol_inverse_small(S):
                                                   _lib.ptr(Linv), _lib.ptr(info), _lib.stream()),
                "addft_subspace_chol_inv")
     return Linv, info
+
+
+# Largest entry of S - I for which near_identity_inv_sqrt replaces a Cholesky
+# factorisation: the neglected third-order term is then below 1e-12.
+NEAR_IDENTITY_TOL = 1e-4
+
+
+def near_identity_inv_sqrt(Ss, tol=NEAR_IDENTITY_TOL):
+    """(I + E)^(-1/2) = I - E/2 + 3 E^2 / 8 + O(E^3) for a Gram matrix
+    Ss = I + E of unit diagonal that is already close to the identity (the
+    second pass of Cholesky-QR2, or a block that is nearly orthonormal):
+    rows M Y with M = Ss^(-1/2) are orthonormal to O(|E|^3), with two small
+    GEMMs instead of a Cholesky factorisation and a triangular inverse.
+    Returns None when max |E| > tol (one host synchronisation)."""
+    n = Ss.shape[0]
+    eye = torch.eye(n, dtype=Ss.dtype, device=Ss.device)
+    E = Ss - eye
+    if float(E.abs().max()) > tol:
+        return None
+    return eye - 0.5 * E + 0.375 * (E @ E)

@@ -262,10 +262,16 @@ def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, 
     nbuf = buffer_bands(nb) if nbuffer is None else int(nbuffer)
     nbt = min(nb + nbuf, engine.ng)
     from .davidson import DENSE_MAX_NG
-    real = bool(gamma_real and engine.gamma_pairs is not None and psi0 is None
-                and engine.ng > max(DENSE_MAX_NG, 4 * nbt))
-    proj = engine.realify if real else None
+    real = bool(gamma_real and engine.ghalf is not None and psi0 is None
+                and engine.ghalf.nh > max(DENSE_MAX_NG, 4 * nbt))
+    # k = 0, vectors real in real space: the block is held as half rows
+    # (HalfRows: half the storage, transform input and subspace GEMM depth),
+    # H applied by Engine.hpsi_half; converted back to full rows at the end
+    half = engine.ghalf if real else None
+    proj = None
     X = [initial_guess(engine, nbt, cdt, ik, real=real) for ik in range(engine.nk)]
+    if half is not None:
+        X = [half.to_half(x) for x in X]
     if psi0 is not None:
         for ik in range(engine.nk):
             p = torch.as_tensor(psi0[ik]).to(device=dev, dtype=cdt)[:nbt]
@@ -366,17 +372,28 @@ def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, 
         ritz_b[0] = lam
         return lam, Xb, r, nit
 
+    def diag_k(ik, vmean):
+        d = engine.tkin64[ik] + vmean
+        return half.values(d) if half is not None else d
+
     def solve_k(ik, Xk, vs, fp32, vmean, tol_k, cold, vs_lo=None):
         nonlocal dav_iters
-        apply = lambda p: engine.hpsi(ik, p, vs, fp32=fp32, real=real)  # noqa: E731
+        if half is not None:
+            apply = lambda p: engine.hpsi_half(ik, p, vs, fp32=fp32)  # noqa: E731
+        else:
+            apply = lambda p: engine.hpsi(ik, p, vs, fp32=fp32)  # noqa: E731
         if vs_lo is not None and not cold:
             # FP64 phase of the mixed form: exact H X once per map, FP32
             # corrections (complex64 projector GEMMs) in a Davidson whose
             # Rayleigh-Ritz couples them through the exact H X
             def apply_lo(p):
-                return engine.hpsi(ik, p.to(torch.complex64), vs_lo, fp32=True,
-                                   nl_bf16=refine_nl_bf16, real=real).to(torch.complex128)
-            diag = engine.tkin64[ik] + vmean
+                p = p.to(torch.complex64)
+                if half is not None:
+                    h = engine.hpsi_half(ik, p, vs_lo, fp32=True, nl_bf16=refine_nl_bf16)
+                else:
+                    h = engine.hpsi(ik, p, vs_lo, fp32=True, nl_bf16=refine_nl_bf16)
+                return h.to(torch.complex128)
+            diag = diag_k(ik, vmean)
             lam, Xk, r, nit = davidson(apply, Xk, nbt, diag, max_iter=(refine_max_iter or dav_max_iter or 25),
                                        tol=tol_k, nconv=nb, weight=weights(ik), apply_lo=apply_lo,
                                        max_sub=(refine_max_sub or 2 * nbt), first_rr=refine_first_rr,
@@ -412,7 +429,7 @@ def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, 
                     break
             dav_iters += nit
             return lam, Xk, r, nit
-        diag = engine.tkin64[ik] + vmean
+        diag = diag_k(ik, vmean)
         lam, Xk, r, nit = davidson(apply, Xk, nbt, diag, max_iter=(dav_max_iter or (cold_max_iter if cold else 25)),
                                    tol=(max(tol_k, 1e-3) if (cold and use_cheb) else tol_k), nconv=nb,
                                    weight=weights(ik), max_sub=((cold_max_sub or 2 * nbt) if cold else None),
@@ -470,7 +487,10 @@ def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, 
         occ, mu = engine.occupations(eig)
         for ik in range(engine.nk):
             occ_prev[ik] = occ[ik].to(torch.float64)
-        rho_out = engine.density([x[:nb] for x in Xn], occ, fp32=fp32)
+        if half is not None:
+            rho_out = engine.density_half(Xn[0][:nb], occ, fp32=fp32)
+        else:
+            rho_out = engine.density([x[:nb] for x in Xn], occ, fp32=fp32)
         return rho_out, Xn, eig, occ, mu
 
     tstamps = []
@@ -571,6 +591,8 @@ def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, 
     rho, X, eig, occ, mu, res = best
     # the eigensolver's buffer vectors above nband: approximate eigenvectors
     # of the next states, kept for the response solver's deflation space
+    if half is not None:
+        X = [half.to_full(x) for x in X]
     buffer = [x[nb:].to(torch.complex128).contiguous() for x in X] if nbt > nb else None
     X = [x[:nb].contiguous() for x in X]
     engine.scf_buffer = (float(eig.sum()), buffer, bool(real))

@@ -1,0 +1,78 @@
+"""Orthonormalisation of eigensolver blocks on the CPU: the series
+(I + E)^(-1/2) = I - E/2 + 3 E^2 / 8 used for nearly orthonormal blocks (the
+second pass of Cholesky-QR2 and of the Davidson SVQB) against the exact
+inverse square root, the refusal above its tolerance, and the resulting
+orthonormality of whole blocks (real and complex rows) after
+chebyshev.orthonormalise and davidson._orth, including blocks whose rows
+differ by orders of magnitude in norm."""
+
+import numpy as np
+import pytest
+import torch
+
+from addft.solver import chebyshev, davidson
+from addft.solver.linalg import NEAR_IDENTITY_TOL, near_identity_inv_sqrt
+
+
+def _sym_near_identity(n, eps, seed):
+    g = torch.Generator().manual_seed(seed)
+    A = torch.randn(n, n, generator=g, dtype=torch.float64)
+    E = 0.5 * (A + A.T)
+    E = E - torch.diag(E.diagonal())
+    E = eps * E / E.abs().max()
+    return torch.eye(n, dtype=torch.float64) + E
+
+
+@pytest.mark.parametrize("eps", [1e-8, 1e-6, 3e-5, 1e-4])
+def test_series_matches_inverse_sqrt(eps):
+    S = _sym_near_identity(40, eps, 1)
+    M = near_identity_inv_sqrt(S)
+    w, V = np.linalg.eigh(S.numpy())
+    ref = (V / np.sqrt(w)) @ V.T
+    err = np.abs(M.numpy() - ref).max()
+    # neglected term -5 E^3 / 16: bounded by (n eps)^3 entrywise
+    assert err < 5.0 / 16.0 * (40 * eps) ** 3 + 1e-15
+    # orthonormality of M S M
+    assert np.abs(M.numpy() @ S.numpy() @ M.numpy() - np.eye(40)).max() < 2 * (40 * eps) ** 3 + 1e-14
+
+
+def test_series_refuses_far_from_identity():
+    assert near_identity_inv_sqrt(_sym_near_identity(10, 2 * NEAR_IDENTITY_TOL, 2)) is None
+
+
+def _block(m, n, seed, cplx, spread):
+    g = torch.Generator().manual_seed(seed)
+    Y = torch.randn(m, n, generator=g, dtype=torch.float64)
+    if cplx:
+        Y = torch.complex(Y, torch.randn(m, n, generator=g, dtype=torch.float64))
+    scale = torch.logspace(0, spread, m, dtype=torch.float64)
+    return (Y * scale[:, None].to(Y.dtype)).contiguous()
+
+
+def _gram(B, real):
+    G = B @ B.conj().T
+    return G.real if real else G
+
+
+@pytest.mark.parametrize("real", [False, True])
+def test_cholesky_qr2_block_orthonormal(real):
+    """real=True: rows taken as real vectors over their (re, im) views (the
+    k = 0 form: real coefficients, real Gram matrix)."""
+    Y = _block(24, 300, 3, True, 4.0)
+    Q = chebyshev.orthonormalise(Y, real=real)
+    G = _gram(Q, real)
+    assert (G - torch.eye(24, dtype=G.dtype)).abs().max().item() < 1e-13
+    # same span: Y's rows are combinations of Q's
+    C = _gram(torch.cat([Y, Q]), real)[:24, 24:]
+    resid = Y - C.to(Y.dtype) @ Q
+    assert resid.abs().max().item() < 1e-10 * Y.abs().max().item()
+
+
+@pytest.mark.parametrize("real", [False, True])
+def test_davidson_orth_block_orthonormal(real):
+    Q0 = chebyshev.orthonormalise(_block(10, 300, 4, True, 0.0), real=real)
+    R = _block(14, 300, 5, True, 3.0)
+    R = davidson._orth(R, Q0, real=real)
+    B = torch.cat([Q0, R])
+    G = _gram(B, real)
+    assert (G - torch.eye(B.shape[0], dtype=G.dtype)).abs().max().item() < 1e-13
