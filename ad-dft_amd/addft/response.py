@@ -31,7 +31,6 @@ N/Omega from their trace normalisation to the plain grid dot product.
 
 from __future__ import annotations
 
-import os
 from typing import NamedTuple
 
 import numpy as np
@@ -39,7 +38,6 @@ import torch
 
 from .kernels import _lib as _lib_mod
 from .kernels import native
-from .kernels.nonlocal_bf16 import ProjectorMFMA
 from .solver.davidson import eigh as _eigh
 from .solver.davidson import precondition
 from .solver.linalg import inner
@@ -216,22 +214,6 @@ class Response:
             # the FP32 solves project out the computed states and W together
             self.basis32 = [torch.cat([self.rows32[ik]] + ([self.defl[ik]["W32"]] if self.defl[ik] else []))
                             .contiguous() for ik in range(engine.nk)]
-            # ADDFT_RESP_PROJ=mfma: on half rows the projection coefficients
-            # <b_m|v> as BF16x3 MFMA products (the projector kernels with the
-            # basis as even-parity projector rows).  Not the default: the
-            # bf16 hi + lo image of the basis differs from the basis by
-            # ~2^-18 per entry, so the projection is no longer symmetric to
-            # FP32 accuracy, and the inner CG, whose goals reach 2e-6, takes
-            # 2.5x the iterations (216 atoms: 3011 against 1225)
-            self.proj_mfma = None
-            mode = os.environ.get("ADDFT_RESP_PROJ", "gemm")
-            if mode not in ("mfma", "gemm"):
-                raise ValueError(f"ADDFT_RESP_PROJ={mode!r}: expected mfma or gemm")
-            if self.real and mode == "mfma":
-                full = torch.cat([self.psi[0]] + ([self.half.to_full(self.defl[0]["W64"])] if self.defl[0] else []))
-                odd = torch.zeros(full.shape[0], dtype=torch.int32, device=full.device)
-                self.proj_mfma = ProjectorMFMA(full, (engine.gplus_d, engine.gminus_d), odd)
-                del full
 
     # --------------------------------------------------------------- K
     def K(self, dn):
@@ -518,9 +500,8 @@ class Response:
         working arrays hold the unconverged bands only (compacted when bands
         converge), the diagonal preconditioner is tabulated once per solve,
         and each iteration is three fused kernels (addft_cg_*) besides H and
-        the projections.  At k = 0 the rows are half rows (HalfRows), H acts
-        on them directly (Engine.hpsi_half) and the projections are BF16x3
-        MFMA products with the basis as the projector rows.  (Lifting the
+        the projections.  At k = 0 the rows are half rows (HalfRows) and H
+        acts on them directly (Engine.hpsi_half).  (Lifting the
         computed space by a shift instead of projecting it out, the alpha
         P_v device of perturbation theory, saves one projection per
         iteration but stalls where the top computed states are inexact and
@@ -528,14 +509,15 @@ class Response:
         e = self.e
         basis = self.basis32[ik]
         diag = self._diag(ik).to(torch.float32)
-        pm = self.proj_mfma if self.half is not None else None
 
         def proj(V):
+            # V <- V - sum_m <b_m|V> b_m in place, FP32 GEMMs.  (BF16x3 MFMA
+            # products here -- the projector kernels with the basis as
+            # even-parity projector rows -- make the projection unsymmetric
+            # at the 1e-6 level through the bf16 hi + lo images, and the CG,
+            # whose goals reach 2e-6, then takes 2.5x the iterations.)
             V = V.contiguous()
-            # V <- V - sum_m <b_m|V> b_m in place (FP32 GEMMs; the
-            # coefficients from the MFMA kernels under ADDFT_RESP_PROJ=mfma)
-            C = pm.becp_t(V, real="half").real if pm is not None else self._coef(V, basis)
-            return self._sub(V, C, basis)
+            return self._sub(V, self._coef(V, basis), basis)
 
         def apply_h(P):
             if self.half is not None:
@@ -818,7 +800,7 @@ def dielectric_preconditioner(engine, q0sq=2.0, eps_inf=None):
 
 
 def delta_q_grad(engine, psis, eig, occ, rho_pl, weight_pl, rtol=1e-10, maxiter=200, precondition=True,
-                 method="gmres", log=None, relax=0.5, q0sq=2.0, eps_inf=None, **resp_kw):
+                 method="gmres", log=None, relax=1.0, q0sq=2.0, eps_inf=None, **resp_kw):
     """dC/dR and dC/deps for C = (Omega/N) sum w n at the SCF fixed point n*
     (implicit.py:32-92): (I - J)^T lam = (Omega/N) w, then
     dC/dtheta = dC/dtheta|_n* + lam . dF/dtheta|_n*; the state term is 0 for
