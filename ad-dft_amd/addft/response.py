@@ -159,7 +159,7 @@ class Response:
 
     def __init__(self, engine, psis, eig, occ, rho_pl, deg_tol=1e-6, occ_sign="exact",
                  cg_tol=1e-11, cg_maxiter=500, fthr=1e-12, weighted=True, precision="auto",
-                 deflation="auto", defl_steps=15, defl_res=1e-5):
+                 deflation="auto", defl_steps=15, defl_res=1e-5, inner_rel=0.5, inner_floor=2e-6):
         self.e = engine
         st = engine.st
         self.psi = [p.to(torch.complex128).contiguous() for p in psis]
@@ -201,6 +201,10 @@ class Response:
             self.vs32 = engine.stage(v, mixed_stage=True)
             self.rows32 = [p.to(torch.complex64).contiguous() for p in self.rows]
             self.stats.update({"refinements": 0, "hpsi_bands_fp32": 0})
+        # inner FP32 solve of each outer step: to inner_rel of the outer goal
+        # relative to the current residual, at least inner_floor (FP32 level)
+        self.inner_rel = float(inner_rel)
+        self.inner_floor = float(inner_floor)
         self.defl_steps = int(defl_steps)
         self.defl_res = float(defl_res)
         self.defl = [None] * engine.nk
@@ -443,7 +447,7 @@ class Response:
                 break
             self.stats["refinements"] += 1
             ea = eps[active]
-            tau = torch.clamp(0.3 * goal[active] / rn[active], min=2e-6, max=0.1)
+            tau = torch.clamp(self.inner_rel * goal[active] / rn[active], min=self.inner_floor, max=0.1)
             z = self._precondition_two_level(ik, r[active], ea, tau)
             fa = first[active]
             if not bool(fa.all()):
@@ -800,7 +804,7 @@ def dielectric_preconditioner(engine, q0sq=2.0, eps_inf=None):
 
 
 def delta_q_grad(engine, psis, eig, occ, rho_pl, weight_pl, rtol=1e-10, maxiter=200, precondition=True,
-                 method="gmres", log=None, relax=1.0, q0sq=2.0, eps_inf=None, **resp_kw):
+                 method="gmres", log=None, relax=2.0, q0sq=2.0, eps_inf=None, **resp_kw):
     """dC/dR and dC/deps for C = (Omega/N) sum w n at the SCF fixed point n*
     (implicit.py:32-92): (I - J)^T lam = (Omega/N) w, then
     dC/dtheta = dC/dtheta|_n* + lam . dF/dtheta|_n*; the state term is 0 for
@@ -831,7 +835,9 @@ def delta_q_grad(engine, psis, eig, occ, rho_pl, weight_pl, rtol=1e-10, maxiter=
         # 454; Bouras & Fraysse, SIAM J. Matrix Anal. Appl. 26 (2005) 660):
         # once the residual has fallen to rel, a product error of
         # rtol / rel (relative) no longer limits the attainable accuracy, so
-        # the Sternheimer solves inside chi0 stop at relax * rtol / rel
+        # the Sternheimer solves inside chi0 stop at relax * rtol / rel (the
+        # returned residual is a true one; relax 2 moved the 216-atom
+        # gradient by 3e-11 relative, within its distance to the FP64 one)
         def on_residual(rel):
             true_product["on"] = rel >= 1.0
             if relax:
