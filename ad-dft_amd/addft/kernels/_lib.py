@@ -59,6 +59,7 @@ _SIGS = {
     "addft_xc_lda": ([_VP, _I64, _VP, _VP, _VP, _VP], _I32),
     "addft_xc_lda_fxc": ([_VP, _VP, _I64, _VP, _VP], _I32),
     "addft_cg_shift_dot": ([_I64, _I64, _VP, _VP, _VP, _VP, _VP], _I32),
+    "addft_cg_shifted_rows": ([_I64, _I64, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP], _I32),
     "addft_cg_update": ([_I64, _I64, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP], _I32),
     "addft_cg_direction": ([_I64, _I64, _VP, _VP, _VP, _VP, _VP], _I32),
     "addft_xc_gga_model": ([_VP, _VP, _I64, _D, _VP, _VP, _VP, _VP, _VP], _I32),

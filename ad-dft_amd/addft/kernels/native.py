@@ -111,6 +111,20 @@ def cg_shift_dot(ap: torch.Tensor, p: torch.Tensor, eps: torch.Tensor) -> torch.
     return pap
 
 
+def cg_shifted_rows(x, r, p, q, a, dc, rz, res, fnorm) -> None:
+    """One Jacobi-preconditioned CG step of the shifted real systems
+    (C - a_j) y_j = f_j, rows j, after q = p C (addft_cg_shifted_rows);
+    updates x, r, p, rz and res (|r_j| / fnorm_j) in place."""
+    m, n = x.shape
+    for t, what in ((x, "x"), (r, "r"), (p, "p"), (q, "q")):
+        _require(t, torch.float64, numel=m * n, what=what)
+    for t, what in ((a, "a"), (rz, "rz"), (res, "res"), (fnorm, "fnorm")):
+        _require(t, torch.float64, numel=m, what=what)
+    _require(dc, torch.float64, numel=n, what="dc")
+    check(lib().addft_cg_shifted_rows(m, n, ptr(x), ptr(r), ptr(p), ptr(q), ptr(a), ptr(dc), ptr(rz), ptr(res),
+                                      ptr(fnorm), stream()), "addft_cg_shifted_rows")
+
+
 def cg_update(x, r, p, ap, rz, pap, minv, z) -> torch.Tensor:
     """a = rz/pAp; x += a p; r -= a Ap; z = minv r (all in place); returns
     |r_b|^2 per row."""

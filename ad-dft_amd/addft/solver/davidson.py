@@ -22,7 +22,7 @@ import os
 
 import torch
 
-from .linalg import inner, near_identity_inv_sqrt, rcombine, rinner
+from .linalg import inner, lowest_split_eigh, near_identity_inv_sqrt, rcombine, rinner
 
 _CHECK = os.environ.get("ADDFT_DEBUG_FINITE", "0") == "1"
 
@@ -151,7 +151,8 @@ def dense_lowest(apply, nG, nb, dtype, device):
 
 
 def davidson(apply, X0, nb, diag, max_sub=None, max_iter=40, tol=1e-9, min_iter=1, nconv=None, weight=None,
-             apply_lo=None, first_rr=True, orth_x0=True, last_residuals=True, real=False, proj=None):
+             apply_lo=None, first_rr=True, orth_x0=True, last_residuals=True, real=False, proj=None,
+             split_rr=None):
     """Return (eig (nb,) float64, X (nb, nG), max residual norm, iterations).
 
     apply: (m, nG) -> (m, nG) in X0's dtype.  diag: (nG,) float64.
@@ -173,12 +174,20 @@ def davidson(apply, X0, nb, diag, max_sub=None, max_iter=40, tol=1e-9, min_iter=
     last_residuals=False: the iteration that reaches max_iter returns right
     after rotating X (no H X rotation, no residual norms; the residual
     returned is nan) -- for callers that only use the vectors and values.
+    split_rr (real blocks; default: with first_rr=False, i.e. X0 already
+    Ritz vectors): the Rayleigh-Ritz of [X0 | corrections] through
+    linalg.lowest_split_eigh (Riccati decoupling of the corrections and a
+    refinement of the nearly diagonal remainder, GEMMs instead of a dense
+    eigensolve of the doubled block), with the dense eigensolve as the
+    fallback wherever its structure checks fail.
     real=True: X0 real in real space (k = 0); every projected matrix is
     taken real, so the iteration stays real; proj (x -> its projection onto
     real vectors) is applied to each correction block and to the result, so
     rounding cannot build up a non-real part that the normalised residuals
     would amplify."""
     nconv = nb if nconv is None else int(nconv)
+    if split_rr is None:
+        split_rr = not first_rr
     nG = X0.shape[1]
     if nG <= max(DENSE_MAX_NG, 4 * nb):
         return dense_lowest(apply, nG, nb, X0.dtype, X0.device)
@@ -207,8 +216,14 @@ def davidson(apply, X0, nb, diag, max_sub=None, max_iter=40, tol=1e-9, min_iter=
             lam = T.diagonal().real.clone()
             X, HX = Q, HQ
         else:
-            w, U = eigh(T)
-            lam = w[:nb]
+            split = lowest_split_eigh(T, nb) if (real and split_rr and T.shape[0] > nb) else None
+            if split is not None:
+                # [previous Ritz vectors | corrections] with separated spectra:
+                # the lowest nb pairs without a dense eigensolve of all of T
+                lam, U = split
+            else:
+                w, U = eigh(T)
+                lam = w[:nb]
             if real:
                 Ub = U[:, :nb].T.contiguous()
                 X = rcombine(Ub, Q)

@@ -229,3 +229,181 @@ def near_identity_inv_sqrt(Ss, tol=NEAR_IDENTITY_TOL):
     if float(E.abs().max()) > tol:
         return None
     return eye - 0.5 * E + 0.375 * (E @ E)
+
+
+def lowest_split_eigh(T, nb, tol=1e-11, max_cg=200):
+    """The nb lowest eigenpairs of a real symmetric T = [[A, B], [B^T, C]]
+    whose leading nb x nb block A is nearly diagonal (the Rayleigh quotients
+    of the previous Ritz vectors) and lies below the spectrum of C (the
+    block of the correction vectors): the Davidson refinement's projected
+    matrix.  Instead of a dense eigensolve of all of T, the invariant
+    subspace [I; Y] of the lowest nb eigenvalues is the solution of the
+    Riccati equation C Y - Y A = Y B Y - B^T, solved by a few fixed-point
+    steps, each a block CG on the shifted systems (C - a_j) y_j = f_j (all
+    columns at once: one GEMM per iteration); the nearly diagonal projected
+    matrix of that subspace is then diagonalised by the iterative refinement
+    of Ogita and Aishima (Japan J. Indust. Appl. Math. 35 (2018) 1007: a few
+    GEMMs per step, quadratic convergence), with pairs too close for it
+    (|m_ij| > |m_ii - m_jj| / 10, or values within 1e-6 |T|) diagonalised
+    exactly as clusters.
+
+    Returns (w (nb,), U (n, nb)) or None where the structure does not hold
+    -- C - mu I not positive definite for mu above A's Gershgorin bound
+    (which, since B (C - mu)^-1 B^T is positive semi-definite, guarantees
+    that exactly nb eigenvalues of T lie below mu), a CG that does not
+    converge, or a final eigen-residual above tol * |T| -- so the caller
+    falls back to a dense eigensolve."""
+    n = T.shape[0]
+    if T.is_complex() or n <= nb or T.dim() != 2:
+        return None
+    nc = n - nb
+    A, B, C = T[:nb, :nb], T[:nb, nb:], T[nb:, nb:]
+    a = A.diagonal()
+    A_off = A - torch.diag(a)
+    scale = float(T.abs().max())
+    if float(A_off.abs().max()) > 1e-3 * scale:
+        return None
+    mu = float((a + A_off.abs().sum(dim=1)).max()) + 1e-6 * scale
+    eye_c = torch.eye(nc, dtype=T.dtype, device=T.device)
+    _, info = torch.linalg.cholesky_ex(C - mu * eye_c)
+    if int(info.item()) != 0:
+        return None
+    dC = C.diagonal().contiguous()
+    Bt = B.T.contiguous()
+    a = a.contiguous()
+    # rows j of Yt = Y^T: (C - a_j) y_j = f_j, F^T = Yt B^T Yt - B + A_off Yt
+    Yt = torch.zeros((nb, nc), dtype=T.dtype, device=T.device)
+    rz = torch.empty(nb, dtype=T.dtype, device=T.device)
+    res = torch.empty_like(rz)
+    for outer in range(10):
+        Ft = Yt @ (Bt @ Yt) - B + A_off @ Yt
+        fnorm = torch.linalg.vector_norm(Ft, dim=1).clamp(min=1e-300).contiguous()
+        Xt = Yt.clone()
+        Rt = (Ft - (Xt @ C - a[:, None] * Xt)).contiguous()
+        Pt = (Rt / (dC[None, :] - a[:, None])).contiguous()
+        rz.copy_((Rt * Pt).sum(dim=1))
+        for it in range(max_cg):
+            Qt = Pt @ C
+            _shifted_cg_step(Xt, Rt, Pt, Qt, a, dC, rz, res, fnorm)
+            if it % 8 == 7 and float(res.max()) < 1e-14:
+                break
+        else:
+            return None
+        dY = float((Xt - Yt).abs().max())
+        Yt = Xt
+        if dY < 1e-12:
+            break
+    else:
+        return None
+    Y = Yt.T
+    # orthonormal basis V = [I; Y] S^-1/2 of the subspace, S = I + Y^T Y
+    eye_b = torch.eye(nb, dtype=T.dtype, device=T.device)
+    V = torch.cat([eye_b, Y])
+    for _ in range(5):            # the series until orthonormal: |E| -> |E|^3 per pass
+        E = V.T @ V - eye_b
+        e = float(E.abs().max())
+        if e < 1e-14:
+            break
+        if e > 5e-2:
+            return None
+        V = V @ (eye_b - 0.5 * E + 0.375 * (E @ E))
+    M = V.T @ (T @ V)
+    M = 0.5 * (M + M.T)
+    Xr = _refine_near_diagonal(M)
+    if Xr is None:
+        return None
+    U = V @ Xr
+    w = (U * (T @ U)).sum(dim=0)
+    order = torch.argsort(w)
+    w, U = w[order], U[:, order].contiguous()
+    res = float((T @ U - U * w[None, :]).abs().max())
+    if res > tol * max(scale, 1e-300):
+        return None
+    return w, U
+
+
+def _shifted_cg_step(x, r, p, q, a, dc, rz, res, fnorm):
+    """One Jacobi-preconditioned CG step of the row systems (C - a_j) y_j =
+    f_j after q = p C: the fused library kernel on the GPU
+    (addft_cg_shifted_rows); the same arithmetic in tensor operations for
+    tensors in host memory."""
+    if x.is_cuda:
+        from ..kernels import native
+        native.cg_shifted_rows(x, r, p, q, a, dc, rz, res, fnorm)
+        return
+    sh = dc[None, :] - a[:, None]
+    qq = q - a[:, None] * p
+    pq = (p * qq).sum(dim=1)
+    alpha = torch.where(pq > 0, rz / torch.where(pq > 0, pq, torch.ones_like(pq)), torch.zeros_like(pq))
+    x += alpha[:, None] * p
+    r -= alpha[:, None] * qq
+    z = r / sh
+    rzn = (r * z).sum(dim=1)
+    beta = torch.where(rz > 0, rzn / torch.where(rz > 0, rz, torch.ones_like(rz)), torch.zeros_like(rz))
+    p.copy_(z + beta[:, None] * p)
+    rz.copy_(rzn)
+    res.copy_(torch.linalg.vector_norm(r, dim=1) / fnorm)
+
+
+def _refine_near_diagonal(M, iters=10):
+    """Eigenvectors X of a symmetric M that is diagonal up to small entries,
+    by Jacobi-type refinement steps from X = I (the iteration of Ogita and
+    Aishima with X kept orthonormal): each step orthonormalises X (the
+    series (X^T X)^(-1/2)), diagonalises exactly, on the host, the clusters
+    of pairs too close for a perturbative rotation -- |s_ij| > |s_ii - s_jj|
+    / 10, or values within 1e-6 |M| -- and rotates by the first-order
+    angles s_ij / (s_jj - s_ii) of all other pairs (quadratic convergence).
+    None if the steps do not converge or a cluster is large."""
+    n = M.shape[0]
+    eye = torch.eye(n, dtype=M.dtype, device=M.device)
+    X = eye.clone()
+    scale = float(M.abs().max())
+    for it in range(iters):
+        for _ in range(4):        # orthonormal columns: |X^T X - I| 5e-2 -> 1e-4 -> 1e-12 -> rounding
+            E = X.T @ X - eye
+            e = float(E.abs().max())
+            if e < 1e-14:
+                break
+            if e > 5e-2:
+                return None
+            X = X @ (eye - 0.5 * E + 0.375 * (E @ E))
+        S = X.T @ (M @ X)
+        S = 0.5 * (S + S.T)
+        d = S.diagonal()
+        D = d[None, :] - d[:, None]
+        big = (S.abs() > 0.1 * D.abs()) | (D.abs() < 1e-6 * scale)
+        big.fill_diagonal_(False)
+        pairs = torch.nonzero(torch.triu(big)).cpu().numpy()
+        if len(pairs):
+            parent = list(range(n))
+
+            def find(i):
+                while parent[i] != i:
+                    parent[i] = parent[parent[i]]
+                    i = parent[i]
+                return i
+            for i, j in pairs:
+                parent[find(int(i))] = find(int(j))
+            groups = {}
+            for i in {int(v) for v in pairs.reshape(-1)}:
+                groups.setdefault(find(i), []).append(i)
+            Sh = S.cpu()
+            for g in groups.values():
+                if len(g) > 128:
+                    return None
+                g = sorted(g)
+                _, Vc = torch.linalg.eigh(Sh[g][:, g])
+                idx = torch.as_tensor(g, device=M.device)
+                X[:, idx] = X[:, idx] @ Vc.to(M.device)
+            S = X.T @ (M @ X)
+            S = 0.5 * (S + S.T)
+            d = S.diagonal()
+            D = d[None, :] - d[:, None]
+        close = D.abs() < 1e-6 * scale
+        off = torch.where(close, torch.zeros_like(S), S - torch.diag(d)).abs().max()
+        if float(off) < 1e-13 * scale:
+            return X
+        E = torch.where(close, torch.zeros_like(S), S / torch.where(close, torch.ones_like(D), D))
+        E.fill_diagonal_(0.0)
+        X = X + X @ E
+    return None

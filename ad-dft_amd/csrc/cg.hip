@@ -110,9 +110,86 @@ __global__ __launch_bounds__(kThreads) void k_direction(int64_t n, const float2*
   if (threadIdx.x == 0) rz[blockIdx.x] = acc;
 }
 
+// Shifted real systems (C - a_j) y_j = f_j for many j at once (FP64, one
+// row per j, C symmetric), the solver of the Riccati equation of the split
+// Rayleigh-Ritz: after the GEMM Q = P C (rows), one workgroup per row j
+// completes the step
+//   q = Q_j - a_j p_j;  alpha = rz_j / <p_j, q>;  x_j += alpha p_j;
+//   r_j -= alpha q;  z = r_j / (dC - a_j) (Jacobi);  rz'_j = <r_j, z>;
+//   p_j = z + (rz'_j / rz_j) p_j;  rz_j = rz'_j;  res_j = |r_j| / fnorm_j
+// in one pass (two reductions).
+constexpr int kShiftThreads = 256;
+
+__device__ __forceinline__ double wave_sum_d(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+__device__ __forceinline__ double block_sum_d(double v, double* red) {
+  v = wave_sum_d(v);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) red[wave] = v;
+  __syncthreads();
+  double s = 0.0;
+#pragma unroll
+  for (int w = 0; w < kShiftThreads / 64; ++w) s += red[w];
+  __syncthreads();
+  return s;
+}
+
+__global__ __launch_bounds__(kShiftThreads) void k_shifted_cg_rows(
+    int64_t n, double* __restrict__ x, double* __restrict__ r, double* __restrict__ p,
+    const double* __restrict__ q, const double* __restrict__ a, const double* __restrict__ dc,
+    double* __restrict__ rz, double* __restrict__ res, const double* __restrict__ fnorm) {
+  __shared__ double red[kShiftThreads / 64];
+  const int64_t row = blockIdx.x;
+  double* xr = x + row * n;
+  double* rr = r + row * n;
+  double* pr = p + row * n;
+  const double* qr = q + row * n;
+  const double aj = a[row];
+  double pq = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += kShiftThreads) pq += pr[i] * (qr[i] - aj * pr[i]);
+  pq = block_sum_d(pq, red);
+  const double rz0 = rz[row];
+  const double alpha = pq > 0.0 ? rz0 / pq : 0.0;
+  double rzn = 0.0, r2 = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += kShiftThreads) {
+    const double pv = pr[i];
+    const double qv = qr[i] - aj * pv;
+    xr[i] += alpha * pv;
+    const double rv = rr[i] - alpha * qv;
+    rr[i] = rv;
+    const double zv = rv / (dc[i] - aj);
+    rzn += rv * zv;
+    r2 += rv * rv;
+  }
+  rzn = block_sum_d(rzn, red);
+  r2 = block_sum_d(r2, red);
+  const double beta = rz0 > 0.0 ? rzn / rz0 : 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += kShiftThreads) pr[i] = rr[i] / (dc[i] - aj) + beta * pr[i];
+  if (threadIdx.x == 0) {
+    rz[row] = rzn;
+    res[row] = sqrt(r2) / fnorm[row];
+  }
+}
+
 }  // namespace
 
 extern "C" {
+
+int addft_cg_shifted_rows(int64_t m, int64_t n, double* d_x, double* d_r, double* d_p, const double* d_q,
+                          const double* d_a, const double* d_dc, double* d_rz, double* d_res,
+                          const double* d_fnorm, void* stream) {
+  ADDFT_CHECK_ARG(m >= 0 && n >= 0 && m <= INT32_MAX, "bad sizes");
+  if (m == 0 || n == 0) return ADDFT_OK;
+  ADDFT_CHECK_ARG(d_x && d_r && d_p && d_q && d_a && d_dc && d_rz && d_res && d_fnorm, "null argument");
+  hipLaunchKernelGGL(k_shifted_cg_rows, dim3((unsigned)m), dim3(kShiftThreads), 0, addft::as_stream(stream), n,
+                     d_x, d_r, d_p, d_q, d_a, d_dc, d_rz, d_res, d_fnorm);
+  ADDFT_LAUNCH_CHECK();
+  return ADDFT_OK;
+}
 
 int addft_cg_shift_dot(int64_t m, int64_t n, void* d_ap, const void* d_p, const float* d_eps, float* d_pap,
                        void* stream) {

@@ -255,6 +255,18 @@ int addft_cg_update(int64_t m, int64_t n, void* d_x, void* d_r, const void* d_p,
                     float* d_rr, void* stream);
 int addft_cg_direction(int64_t m, int64_t n, const void* d_r, const void* d_z, void* d_p,
                        float* d_rz, void* stream);
+/* addft_cg_shifted_rows: one Jacobi-preconditioned CG step of the m shifted
+ * real systems (C - a_j) y_j = f_j (FP64 rows of n, C symmetric, d_dc its
+ * diagonal), after the caller's GEMM d_q = P C:  q = Q_j - a_j p_j;
+ * alpha = rz_j / <p_j, q>; x_j += alpha p_j; r_j -= alpha q;
+ * z = r_j / (dc - a_j); rz'_j = <r_j, z>; p_j = z + (rz'_j / rz_j) p_j;
+ * rz_j = rz'_j; res_j = |r_j| / fnorm_j.  The Riccati solver of the split
+ * Rayleigh-Ritz of the Davidson refinement (no reference counterpart: the
+ * reference eigensolves the whole projected matrix, solver/davidson.py:
+ * 32-77).  Requires dc_i > a_j (C - a_j positive definite). */
+int addft_cg_shifted_rows(int64_t m, int64_t n, double* d_x, double* d_r, double* d_p, const double* d_q,
+                          const double* d_a, const double* d_dc, double* d_rz, double* d_res,
+                          const double* d_fnorm, void* stream);
 
 /* ------------------------------------------------------------ BF16x3 split
  * FP32 rows x (rows, k) -> bf16 rows [bf16(x) | bf16(x) | bf16(x - bf16(x))]

@@ -76,3 +76,47 @@ def test_davidson_orth_block_orthonormal(real):
     B = torch.cat([Q0, R])
     G = _gram(B, real)
     assert (G - torch.eye(B.shape[0], dtype=G.dtype)).abs().max().item() < 1e-13
+
+
+def _davidson_like(nb, nc, seed, gap=0.3, coupling=3e-3, degenerate=True):
+    """[[A, B], [B^T, C]]: A nearly diagonal (Ritz values, optionally with
+    near-degenerate pairs), C dense above A's spectrum, small coupling B."""
+    rng = np.random.default_rng(seed)
+    a = np.sort(rng.uniform(-0.5, 0.2, nb))
+    if degenerate:
+        a[10:13] = a[10]                      # a threefold multiplet
+        a[21] = a[20] + 1e-9                  # a nearly degenerate pair
+    A = np.diag(a) + 1e-8 * (lambda m: m + m.T)(rng.standard_normal((nb, nb)))
+    G = rng.standard_normal((nc, nc))
+    C = (0.2 + a.max() + gap) * np.eye(nc) + (G @ G.T) / nc
+    B = coupling / np.sqrt(nc) * rng.standard_normal((nb, nc))     # row norms ~ coupling
+    return torch.as_tensor(np.block([[A, B], [B.T, C]]))
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_lowest_split_eigh_matches_dense(seed):
+    """The Riccati + refinement eigensolver for the Davidson refinement's
+    block structure: lowest eigenvalues to 1e-13, eigen-residual at the
+    rounding level, orthonormal vectors, the same invariant subspace as the
+    dense solver."""
+    from addft.solver.linalg import lowest_split_eigh
+    nb, nc = 60, 50
+    T = _davidson_like(nb, nc, seed)
+    w, U = lowest_split_eigh(T, nb)
+    w0, U0 = torch.linalg.eigh(T)
+    assert (w - w0[:nb]).abs().max().item() < 1e-13
+    assert (T @ U - U * w[None, :]).abs().max().item() < 1e-13
+    assert (U.T @ U - torch.eye(nb, dtype=U.dtype)).abs().max().item() < 1e-13
+    s = torch.linalg.svdvals(U0[:, :nb].T @ U)
+    assert (1.0 - s.min()).item() < 1e-12
+
+
+def test_lowest_split_eigh_refuses_overlapping_spectra():
+    """C reaching below A's spectrum (or a far-from-diagonal A): None, so the
+    caller takes the dense eigensolver."""
+    from addft.solver.linalg import lowest_split_eigh
+    T = _davidson_like(40, 30, 3, gap=-1.5)
+    assert lowest_split_eigh(T, 40) is None
+    T = _davidson_like(40, 30, 4)
+    T[:40, :40] += 0.05 * torch.ones(40, 40, dtype=T.dtype)
+    assert lowest_split_eigh(T, 40) is None
