@@ -174,12 +174,13 @@ def davidson(apply, X0, nb, diag, max_sub=None, max_iter=40, tol=1e-9, min_iter=
     last_residuals=False: the iteration that reaches max_iter returns right
     after rotating X (no H X rotation, no residual norms; the residual
     returned is nan) -- for callers that only use the vectors and values.
-    split_rr (real blocks; default: with first_rr=False, i.e. X0 already
-    Ritz vectors): the Rayleigh-Ritz of [X0 | corrections] through
-    linalg.lowest_split_eigh (Riccati decoupling of the corrections and a
+    split_rr (real blocks; default: when X0 is given as Ritz vectors,
+    first_rr=False): each Rayleigh-Ritz over [Ritz vectors | corrections]
+    through linalg.lowest_split_eigh (Riccati decoupling of the corrections and a
     refinement of the nearly diagonal remainder, GEMMs instead of a dense
     eigensolve of the doubled block), with the dense eigensolve as the
-    fallback wherever its structure checks fail.
+    fallback wherever its structure checks fail (a leading block that is
+    not nearly diagonal is refused before any work).
     real=True: X0 real in real space (k = 0); every projected matrix is
     taken real, so the iteration stays real; proj (x -> its projection onto
     real vectors) is applied to each correction block and to the result, so
@@ -187,6 +188,9 @@ def davidson(apply, X0, nb, diag, max_sub=None, max_iter=40, tol=1e-9, min_iter=
     would amplify."""
     nconv = nb if nconv is None else int(nconv)
     if split_rr is None:
+        # from plane waves (the cold start) the corrections' spectrum reaches
+        # into the wanted one and the split solver spends 10-20 ms before
+        # its checks send it to the dense eigensolve
         split_rr = not first_rr
     nG = X0.shape[1]
     if nG <= max(DENSE_MAX_NG, 4 * nb):
