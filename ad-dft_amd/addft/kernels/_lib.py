@@ -95,6 +95,7 @@ _SIGS = {
     "addft_box_fft": ([_I32, _I32, _I32, _VP, _I32, _VP, _VP], _I32),
     "addft_kerker_mix": ([_I32, _I32, _I32, _VP, _VP, _VP, _D, _D, _VP, _VP, _VP], _I32),
     "addft_projector_create": ([_I32, _I64, _I64, _VP, _VP, _VP, _VP, _I64, _VP, ctypes.POINTER(_VP)], _I32),
+    "addft_projector_set_tiles": ([_VP, _I32], _I32),
     "addft_projector_destroy": ([_VP], _I32),
     "addft_projector_info": ([_VP, ctypes.POINTER(_I64), ctypes.POINTER(_I64), ctypes.POINTER(_I32)], _I32),
     "addft_nonlocal_work_bytes": ([_VP, _I64], _I64),

@@ -356,6 +356,13 @@ class ProjectorMFMA:
         self._h = h
         self._fin = weakref.finalize(self, L.addft_projector_destroy, h)
         self._work = None
+        self.tiles = 0
+
+    def set_tiles(self, wgm: int) -> None:
+        """Band-row height of the MFMA tiles: 0 automatic (least padding),
+        1 = 176 rows, 2 = 352 rows (addft_projector_set_tiles)."""
+        _lib.check(_lib.lib().addft_projector_set_tiles(self._h, int(wgm)), "addft_projector_set_tiles")
+        self.tiles = int(wgm)
 
     def _workspace(self, nb):
         need = int(_lib.lib().addft_nonlocal_work_bytes(self._h, nb))

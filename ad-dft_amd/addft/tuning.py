@@ -130,8 +130,10 @@ def tune_projector(P, pairs, odd_rows, nb: int, seed: int = 0):
     """The FP32-phase projector products of one k-point's P for blocks of nb
     bands: builds the hand-written MFMA form (kernels.nonlocal_bf16.
     ProjectorMFMA) and the hipBLASLt split-GEMM form, times becp_t +
-    apply_add of each (at k = 0 in the real-row and the complex form) and
-    returns the faster object.  ADDFT_NL_MFMA=1 / 0 forces one of them."""
+    apply_add of each (at k = 0 in the half-row -- real-row for the library
+    form -- and the complex form) and returns the faster object; for the
+    MFMA form also the tiles' band-row height (automatic, 176 or 352 rows).
+    ADDFT_NL_MFMA=1 / 0 forces one of the forms."""
     from .kernels.nonlocal_bf16 import GammaProjectorBF16x3, ProjectorBF16x3, ProjectorMFMA
     npj, ng = P.shape
     force = os.environ.get("ADDFT_NL_MFMA")
@@ -150,22 +152,50 @@ def tune_projector(P, pairs, odd_rows, nb: int, seed: int = 0):
     if not enabled():
         return mfma()
     key = f"projector:{'gamma' if pairs is not None else 'full'}:{ng}:{npj}:{nb}"
+    tk = f"projector_tiles:{'gamma' if pairs is not None else 'full'}:{ng}:{npj}:{nb}"
     _load()
-    if key in _DECISIONS:
-        return mfma() if _DECISIONS[key]["choice"] == "mfma" else library()
+    if key in _DECISIONS and (_DECISIONS[key]["choice"] != "mfma" or tk in _DECISIONS):
+        # decided before (this process or the cache file): no operands, no timing
+        if _DECISIONS[key]["choice"] != "mfma":
+            return library()
+        obj = mfma()
+        obj.set_tiles(int(_DECISIONS[tk]["choice"]))
+        return obj
     g = torch.Generator(device="cpu").manual_seed(seed)
     psi = torch.complex(torch.randn(nb, ng, generator=g), torch.randn(nb, ng, generator=g)).to(P.device)
     S = torch.complex(torch.randn(nb, npj, generator=g), torch.randn(nb, npj, generator=g)).to(P.device)
     out = torch.zeros_like(psi)
-    objs = {"mfma": mfma(), "library": library()}
-    # at k = 0 the SCF's bands are real in real space (real-row products);
-    # both forms are timed there, since the complex form also runs
-    forms = (True, False) if pairs is not None else (False,)
+    nh = (ng + 1) // 2
+    half = psi[:, :nh].contiguous() if pairs is not None else None
+    out_h = torch.zeros_like(half) if pairs is not None else None
 
-    def run(o):
-        for real in forms:
-            o.apply_add(S.real.to(S.dtype) if real else S, out, real=real)
-            o.becp_t(psi, real=real)
+    def run(o, mfma_form):
+        # at k = 0 the SCF and the response run half rows (the MFMA kernels'
+        # ADDFT_ROWS_HALF; the library form's nearest, real rows), and the
+        # complex form also runs
+        if pairs is not None:
+            Sr = S.real.to(S.dtype)
+            if mfma_form:
+                o.apply_add(Sr, out_h, real="half")
+                o.becp_t(half, real="half")
+            else:
+                o.apply_add(Sr, out, real=True)
+                o.becp_t(psi, real=True)
+        o.apply_add(S, out)
+        o.becp_t(psi)
 
-    choice = decide(key, {name: (lambda o=o: run(o)) for name, o in objs.items()})
-    return objs[choice]
+    if key in _DECISIONS:
+        choice = _DECISIONS[key]["choice"]
+        obj = mfma() if choice == "mfma" else library()
+    else:
+        objs = {"mfma": mfma(), "library": library()}
+        choice = decide(key, {name: (lambda o=o, name=name: run(o, name == "mfma")) for name, o in objs.items()})
+        obj = objs[choice]
+    if choice == "mfma":
+        # the tiles' band-row height: least padding (auto), 176 or 352 rows
+        def with_tiles(w):
+            obj.set_tiles(w)
+            run(obj, True)
+        best = decide(tk, {str(w): (lambda w=w: with_tiles(w)) for w in (0, 1, 2)})
+        obj.set_tiles(int(best))
+    return obj

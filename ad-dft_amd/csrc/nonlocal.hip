@@ -75,6 +75,7 @@ struct addft_projector_s {
   int32_t* d_plus;
   int32_t* d_minus;
   int ncu;
+  int wgm_force;   // 0: tile height by least padding; 1 / 2: 176- / 352-row tiles
 };
 
 namespace {
@@ -907,14 +908,15 @@ struct Work {
 
 // waves along M for a staged product of `rows` image rows: 176-row tiles
 // where they pad less than 352-row ones (e.g. 528 real rows)
-int choose_wgm(int64_t rows) {
+int choose_wgm(const addft_projector_s* p, int64_t rows) {
+  if (p->wgm_force == 1 || p->wgm_force == 2) return p->wgm_force;
   const int64_t pad2 = (rows + 2 * kWM - 1) / (2 * kWM) * (2 * kWM), pad1 = (rows + kWM - 1) / kWM * kWM;
   return pad1 < pad2 ? 1 : 2;
 }
 
 // rpb: image rows per band, 2 (Re, Im) or 1 (real rows)
 Work work_layout(const addft_projector_s* p, int64_t nb, int rpb) {
-  const int wgm = choose_wgm(rpb * nb);
+  const int wgm = choose_wgm(p, rpb * nb);
   int ntm, ntn, ns;
   becp_grid(p, rpb * nb, &ntm, &ntn, &ns, 0, wgm);
   Work w;
@@ -986,6 +988,13 @@ int addft_projector_destroy(addft_projector p) {
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   delete p;
+  return ADDFT_OK;
+}
+
+int addft_projector_set_tiles(addft_projector p, int wgm) {
+  ADDFT_CHECK_ARG(p != nullptr, "null projector");
+  ADDFT_CHECK_ARG(wgm >= 0 && wgm <= 2, "wgm must be 0, 1 or 2");
+  p->wgm_force = wgm;
   return ADDFT_OK;
 }
 

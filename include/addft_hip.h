@@ -164,6 +164,12 @@ int addft_projector_create(int gamma, int64_t nproj, int64_t ng, const void* d_P
                            const int32_t* d_plus, const int32_t* d_minus, int64_t nh, void* stream,
                            addft_projector* out);
 int addft_projector_destroy(addft_projector p);
+/* addft_projector_set_tiles: the band-row height of the MFMA tiles of both
+ * products, 0 = the height that pads the operand rows least (default),
+ * 1 = 176-row tiles, 2 = 352-row tiles (for an autotuner; the workspace size
+ * of addft_nonlocal_work_bytes follows the setting).  No reference
+ * counterpart. */
+int addft_projector_set_tiles(addft_projector p, int wgm);
 int addft_projector_info(addft_projector p, int64_t* nproj, int64_t* ng, int* gamma);
 int64_t addft_nonlocal_work_bytes(addft_projector p, int64_t nb);
 int addft_nonlocal_becp(addft_projector p, const void* d_psi, int64_t nb, int64_t ldpsi, void* d_becp,

@@ -186,3 +186,33 @@ def test_bf16x3_basis_projection_on_half_rows():
     print(f"basis projection: coefficients {cerr:.1e} of |V|, result {err:.1e}, remaining overlap {ov:.1e} "
           f"(all-BF16x3 {ov3:.1e})", flush=True)
     assert cerr < 1e-6 and err < 1e-6 and ov < 1e-6 and ov3 < 1e-6
+
+
+@pytest.mark.parametrize("nb", [150, 353])
+def test_tile_heights_agree(nb):
+    """addft_projector_set_tiles: 176- and 352-row tiles and the automatic
+    choice give the same products (to the BF16x3 rounding of the split-K
+    order), half rows and complex rows."""
+    import torch
+    from addft.kernels.nonlocal_bf16 import ProjectorMFMA
+    eng = _engine("c3")
+    proj = ProjectorMFMA(eng.P64[0], eng.gamma_pairs, eng.odd_rows_d)
+    h, full = _half(torch, eng, nb, 70 + nb, torch.complex64)
+    S = torch.randn(nb, proj.np, dtype=torch.float64, device="cuda").to(torch.complex64)
+    ref = None
+    for w in (0, 1, 2):
+        proj.set_tiles(w)
+        b_h = proj.becp_t(h, real="half")
+        b_c = proj.becp_t(full)
+        o_h = torch.zeros_like(h)
+        proj.apply_add(S, o_h, real="half")
+        o_c = torch.zeros_like(full)
+        proj.apply_add(S, o_c)
+        got = (b_h, b_c, o_h, o_c)
+        if ref is None:
+            ref = got
+            continue
+        errs = [_rel(g, r) for g, r in zip(got, ref)]
+        print(f"nb {nb} tiles {w}: {['%.1e' % e for e in errs]}", flush=True)
+        assert max(errs) < 1e-5
+    proj.set_tiles(0)
