@@ -21,6 +21,7 @@ _PATH = None
 C64, C128, R64 = 0, 1, 2
 EIGH_DC, EIGH_JACOBI, EIGH_DCJ, EIGH_TRD, EIGH_LDS = 0, 1, 2, 3, 4
 PLANE_LDS, PLANE_LANE = 0, 1
+OK, EINVAL, EINTERNAL, EUNSUPPORTED = 0, -1, -2, -3
 
 
 class NativeUnavailable(RuntimeError):
@@ -67,6 +68,7 @@ _SIGS = {
     "addft_sphere_plan_destroy": ([_VP], _I32),
     "addft_sphere_plan_info": ([_VP, ctypes.POINTER(_I64), ctypes.POINTER(_I64), _VP], _I32),
     "addft_sphere_work_bytes": ([_VP, _I64], _I64),
+    "addft_sphere_plan_set_lane_waves": ([_VP, _I32], _I32),
     "addft_sphere_plan_set_kernel": ([_VP, _I32], _I32),
     "addft_sphere_plan_get_kernel": ([_VP, ctypes.POINTER(_I32), ctypes.POINTER(_I32)], _I32),
     "addft_sphere_permute": ([_VP, _I32, _I32, _VP, _I64, _VP, _VP], _I32),

@@ -296,6 +296,7 @@ class SpherePlan:
                                              m.ctypes.data, _CDT[precision], _lib.ctypes.byref(handle)),
                   "addft_sphere_plan_create")
         self._h = handle
+        self.lane_waves = 4
         self._fin = weakref.finalize(self, L.addft_sphere_plan_destroy, handle)
         nst, nra = _lib.ctypes.c_int64(), _lib.ctypes.c_int64()
         perm = np.empty(self.ng, dtype=np.int32)
@@ -329,6 +330,16 @@ class SpherePlan:
     def set_plane_kernel(self, kind: str):
         check(lib().addft_sphere_plan_set_kernel(self._h, _lib.PLANE_LANE if kind == "lane" else _lib.PLANE_LDS),
               "addft_sphere_plan_set_kernel")
+
+    def set_lane_waves(self, waves: int) -> bool:
+        """Wavefronts per workgroup of the lane-group plane pass (4 or 8);
+        False where that variant does not fit the plan (nothing changed)."""
+        rc = lib().addft_sphere_plan_set_lane_waves(self._h, int(waves))
+        if rc == _lib.EUNSUPPORTED:
+            return False
+        check(rc, "addft_sphere_plan_set_lane_waves")
+        self.lane_waves = int(waves)
+        return True
 
     def work(self, nb: int) -> torch.Tensor:
         need = int(lib().addft_sphere_work_bytes(self._h, max(1, nb)))

@@ -87,7 +87,9 @@ def decisions() -> dict:
 # ------------------------------------------------------------ sphere plans
 def tune_plan(plan, nb: int = 32, seed: int = 0):
     """Choose the plane-pass kernel of `plan` (kernels.native.SpherePlan) by
-    timing one local Hpsi on nb random bands with each available kernel."""
+    timing one local Hpsi on nb random bands with each available kernel, and
+    for the lane-group kernel its wavefronts per workgroup (4 or 8, where
+    8 fit in LDS)."""
     if not enabled() or not plan.lane_available:
         return plan.plane_kernel
     key = f"plane:{plan.shape}:{plan.ng}:{plan.cdtype}:{nb}"
@@ -107,6 +109,15 @@ def tune_plan(plan, nb: int = 32, seed: int = 0):
 
     choice = decide(key, {"lane": run("lane"), "lds": run("lds")})
     plan.set_plane_kernel(choice if choice else current)
+    if plan.plane_kernel == "lane" and plan.set_lane_waves(8):
+        # wavefronts per (band, plane) workgroup of the lane-group pass
+        def waves(w):
+            def fn():
+                plan.set_lane_waves(w)
+                plan.hpsi_local(psi, tk, vs, out)
+            return fn
+        best = decide(f"plane_waves:{plan.shape}:{plan.ng}:{plan.cdtype}:{nb}", {"4": waves(4), "8": waves(8)})
+        plan.set_lane_waves(int(best))
     return plan.plane_kernel
 
 

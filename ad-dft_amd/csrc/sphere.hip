@@ -861,27 +861,28 @@ int set_plane_attrs_any(int ept, size_t lds) {
 }
 
 constexpr int kLaneWaves = 4;
+constexpr int kLaneWavesWide = 8;  // the alternative the autotuner times
 
 #define ADDFT_LANE_SIZES(X) \
   X(9) X(12) X(32) X(48) X(50) X(60) X(64) X(72) X(75) X(80) X(96) X(100) X(120) X(128)
 
 // Launch (or, with nb == 0, only configure the LDS limit of) the lane-group
-// plane pass for the plan's box size.
-template <typename R>
-int launch_plane_lane(const addft_sphere_plan_s* p, cpx<R>* T, const R* v, int64_t nb, hipStream_t st) {
-  const size_t lds = lane_lds_bytes<R>(p->n1, p->nra, kLaneWaves);
+// plane pass for the plan's box size, with W wavefronts per workgroup.
+template <typename R, int W>
+int launch_plane_lane_w(const addft_sphere_plan_s* p, cpx<R>* T, const R* v, int64_t nb, hipStream_t st) {
+  const size_t lds = lane_lds_bytes<R>(p->n1, p->nra, W);
   const dim3 grid((unsigned)std::max<int64_t>(nb, 1), (unsigned)p->n3);
   auto tw = reinterpret_cast<const cpx<R>*>(p->d_tw1);
   auto ri = reinterpret_cast<const int4*>(p->d_rowinfo);
 #define ADDFT_LANE_CASE(NN)                                                                         \
   case NN:                                                                                          \
     if (nb == 0) {                                                                                  \
-      ADDFT_HIP(hipFuncSetAttribute((const void*)k_plane_v_lane<R, NN, lane_width(NN), kLaneWaves>, \
+      ADDFT_HIP(hipFuncSetAttribute((const void*)k_plane_v_lane<R, NN, lane_width(NN), W>,          \
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));         \
       return ADDFT_OK;                                                                              \
     }                                                                                               \
-    hipLaunchKernelGGL((k_plane_v_lane<R, NN, lane_width(NN), kLaneWaves>), grid, dim3(64 * kLaneWaves), \
-                       lds, st, T, p->ldt, p->nra, p->a1, ri, tw, v, p->n3);                       \
+    hipLaunchKernelGGL((k_plane_v_lane<R, NN, lane_width(NN), W>), grid, dim3(64 * W), lds, st, T,  \
+                       p->ldt, p->nra, p->a1, ri, tw, v, p->n3);                                    \
     break;
   switch (p->n1) {
     ADDFT_LANE_SIZES(ADDFT_LANE_CASE)
@@ -892,6 +893,20 @@ int launch_plane_lane(const addft_sphere_plan_s* p, cpx<R>* T, const R* v, int64
 #undef ADDFT_LANE_CASE
   ADDFT_LAUNCH_CHECK();
   return ADDFT_OK;
+}
+
+// The plan's wave count (kLaneWaves, or kLaneWavesWide where it fits and was
+// chosen); nb == 0 configures every variant that fits.
+template <typename R>
+int launch_plane_lane(const addft_sphere_plan_s* p, cpx<R>* T, const R* v, int64_t nb, hipStream_t st) {
+  if (nb == 0) {
+    int rc = launch_plane_lane_w<R, kLaneWaves>(p, T, v, 0, st);
+    if (rc == ADDFT_OK && lane_lds_bytes<R>(p->n1, p->nra, kLaneWavesWide) <= kLdsMax)
+      rc = launch_plane_lane_w<R, kLaneWavesWide>(p, T, v, 0, st);
+    return rc;
+  }
+  if (p->lane_waves == kLaneWavesWide) return launch_plane_lane_w<R, kLaneWavesWide>(p, T, v, nb, st);
+  return launch_plane_lane_w<R, kLaneWaves>(p, T, v, nb, st);
 }
 
 constexpr int kZLaneWaves = 4;
@@ -1317,6 +1332,23 @@ int addft_sphere_plan_set_kernel(addft_sphere_plan p, int kernel) {
     return ADDFT_EUNSUPPORTED;
   }
   p->lane_kernel = kernel == ADDFT_PLANE_LANE;
+  return ADDFT_OK;
+}
+
+int addft_sphere_plan_set_lane_waves(addft_sphere_plan p, int waves) {
+  ADDFT_CHECK_ARG(p != nullptr, "null plan");
+  ADDFT_CHECK_ARG(waves == kLaneWaves || waves == kLaneWavesWide, "waves must be 4 or 8");
+  if (!p->lane_ok) {
+    addft::set_error("no lane-group plane kernel for this sphere and box");
+    return ADDFT_EUNSUPPORTED;
+  }
+  const size_t lds = p->precision == ADDFT_C64 ? lane_lds_bytes<float>(p->n1, p->nra, waves)
+                                               : lane_lds_bytes<double>(p->n1, p->nra, waves);
+  if (lds > kLdsMax) {
+    addft::set_error("the lane-group plane pass with this many waves does not fit in LDS");
+    return ADDFT_EUNSUPPORTED;
+  }
+  p->lane_waves = waves;
   return ADDFT_OK;
 }
 

@@ -330,6 +330,12 @@ int addft_sphere_plan_info(addft_sphere_plan plan, int64_t* n_sticks,
 #define ADDFT_PLANE_LDS 0
 #define ADDFT_PLANE_LANE 1
 int addft_sphere_plan_set_kernel(addft_sphere_plan plan, int kernel);
+/* addft_sphere_plan_set_lane_waves: wavefronts per workgroup of the
+ * lane-group plane pass, 4 (default) or 8 (one (band, plane) workgroup with
+ * twice the transforms in flight; ADDFT_EUNSUPPORTED where the plane and
+ * exchange tiles would not fit in LDS) -- for an autotuner.  No reference
+ * counterpart. */
+int addft_sphere_plan_set_lane_waves(addft_sphere_plan p, int waves);
 int addft_sphere_plan_get_kernel(addft_sphere_plan plan, int* kernel, int* lane_available);
 /* Bytes of workspace for Hpsi / density / transforms on nb bands.   */
 int64_t addft_sphere_work_bytes(addft_sphere_plan plan, int64_t nb);

@@ -325,3 +325,30 @@ def test_lane_kernel_unavailable_is_an_error():
     with pytest.raises(AddftError):
         plan.set_plane_kernel("lane")
     assert plan.plane_kernel == "lds"
+
+
+def test_plane_pass_wave_counts_agree():
+    """addft_sphere_plan_set_lane_waves: the lane-group plane pass with 4 and
+    8 wavefronts per workgroup gives the same local H psi (FP32 and FP64,
+    the 216-atom box)."""
+    import torch
+    import addft
+    from addft import testsystems
+    s = addft.setup(**testsystems.config("c5", precision="mixed"))
+    eng = addft.api._engine(s.static, s.geo)
+    for plan, tol in ((eng.plan32, 1e-6), (eng.plan64, 1e-14)):
+        if plan.plane_kernel != "lane" or not plan.set_lane_waves(8):
+            continue
+        g = torch.Generator(device="cpu").manual_seed(5)
+        psi = torch.complex(torch.randn(6, plan.ng, generator=g), torch.randn(6, plan.ng, generator=g))
+        psi = psi.to(plan.cdtype).cuda()
+        tk = torch.rand(plan.ng, generator=g, dtype=torch.float64).to(plan.rdtype).cuda()
+        vs = torch.rand(plan.nfft, generator=g, dtype=torch.float64).to(plan.rdtype).cuda()
+        out8 = torch.empty_like(psi)
+        plan.hpsi_local(psi, tk, vs, out8)
+        assert plan.set_lane_waves(4)
+        out4 = torch.empty_like(psi)
+        plan.hpsi_local(psi, tk, vs, out4)
+        err = ((out8 - out4).abs().max() / out4.abs().max()).item()
+        print(f"{plan.cdtype}: 8 vs 4 waves {err:.1e}", flush=True)
+        assert err < tol
