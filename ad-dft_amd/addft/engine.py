@@ -165,6 +165,14 @@ class Engine:
     def _f64(self, a):
         return torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64), device=self.dev)
 
+    def kweights_d(self):
+        """(nk,) float64 k-point weights on the device, uploaded once (a
+        captured SCF map reads them without a host copy)."""
+        kw = getattr(self, "_kw_d", None)
+        if kw is None:
+            kw = self._kw_d = self._f64(self.st.kweights)
+        return kw
+
     def _i32(self, a):
         return torch.as_tensor(np.ascontiguousarray(a, dtype=np.int32), device=self.dev)
 
@@ -525,7 +533,7 @@ class Engine:
         """rho for psi3 (nk, nb, nG) and occ (nk, nb) in one launch."""
         plan = self.plan32 if fp32 else self.plan64
         nk, nb, ng = psi3.shape
-        kw = torch.as_tensor(np.asarray(self.st.kweights, dtype=np.float64), device=self.dev)
+        kw = self.kweights_d()
         w = ((kw[:, None] * self.st.spin_degeneracy / self.omega) * occ.to(torch.float64)).reshape(-1).contiguous()
         psi = psi3.reshape(nk * nb, ng)
         psi = psi.to(plan.cdtype) if psi.dtype != plan.cdtype else psi
@@ -578,7 +586,7 @@ class Engine:
             return self._integer_occupation(eig), 0.0
         occ = torch.empty_like(eig)
         mu = torch.empty((), dtype=torch.float64, device=self.dev)
-        w = self._f64(st.kweights)
+        w = self.kweights_d()
         check(lib().addft_occupations(ptr(eig.contiguous()), ptr(w), self.nk, eig.shape[1], st.nelec, st.sigma,
                                       st.spin_degeneracy, ptr(occ), ptr(mu), stream()), "addft_occupations")
         return occ, mu

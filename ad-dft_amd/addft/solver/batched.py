@@ -18,15 +18,20 @@ import torch
 from .graphs import filter_recurrence
 
 
-def eigh(T):
+def eigh(T, infos=None):
     """Batched Hermitian eigendecomposition of (nk, n, n) on the device:
     rocSOLVER's batched divide-and-conquer or Jacobi solver through
     addft_subspace_eigh (the method timed fastest for the shape), one call for
     all k-points and no host round trip (a non-converged matrix is retried
-    alone with the single-matrix path)."""
+    alone with the single-matrix path).  infos (a list): inside a captured
+    map the status (nk,) is appended there instead of being read, and the
+    caller checks it after the replay."""
     from .davidson import eigh as eigh_one
     from .linalg import eigh_device, eigh_method
     w, U, info = eigh_device(T, eigh_method(T))
+    if infos is not None:
+        infos.append(info)
+        return w, U
     bad = torch.nonzero(info != 0).flatten()
     if bad.numel():
         w, U = w.clone(), U.clone()
@@ -35,15 +40,19 @@ def eigh(T):
     return w, U
 
 
-def chol_inverse(S):
+def chol_inverse(S, infos=None):
     """L^-1 of the Cholesky factors of the Hermitian batch S, or None when a
     factorisation fails: for n <= 64 one launch of the library's
     one-workgroup-per-matrix factorisation and inversion
     (addft_subspace_chol_inv), else the device solver's batched
-    factorisation and triangular solve."""
+    factorisation and triangular solve.  infos: as for eigh (n <= 64 only;
+    the factors are returned unchecked)."""
     from .linalg import _EIGH_SMALL_MAX, chol_inverse_small
     if S.shape[-1] <= _EIGH_SMALL_MAX:
         Linv, info = chol_inverse_small(S)
+        if infos is not None:
+            infos.append(info)
+            return Linv
         return None if bool((info != 0).any()) else Linv
     L, info = torch.linalg.cholesky_ex(S)
     if bool((info != 0).any()):
@@ -66,15 +75,17 @@ def _herm(T):
     return 0.5 * (T + T.conj().transpose(1, 2))
 
 
-def orth(Y, passes=2):
+def orth(Y, passes=2, infos=None):
     """Rows of every Y[k] orthonormal: Cholesky QR of the scaled FP64 Gram
     matrix (twice), with an eigen-decomposition (SVQB, floored spectrum)
-    where a Cholesky factorisation fails."""
+    where a Cholesky factorisation fails.  infos: inside a captured map the
+    factorisations' status goes there (no SVQB branch; the caller redoes
+    the map eagerly when one failed)."""
     for _ in range(passes):
         S = _herm(_gram(Y, Y).to(torch.complex128))
         d = torch.sqrt(torch.clamp(S.diagonal(dim1=1, dim2=2).real, min=1e-300))
         Ss = S / (d[:, :, None] * d[:, None, :])
-        Linv = chol_inverse(Ss)
+        Linv = chol_inverse(Ss, infos)
         if Linv is None:
             w, V = eigh(Ss)
             w = torch.clamp(w, min=1e-14 * float(w.max()))
@@ -85,9 +96,9 @@ def orth(Y, passes=2):
     return Y.contiguous()
 
 
-def rayleigh_ritz(Y, HY):
+def rayleigh_ritz(Y, HY, infos=None):
     T = _herm(_proj(Y, HY).to(torch.complex128))
-    w, U = eigh(T)
+    w, U = eigh(T, infos)
     Ut = U.transpose(1, 2).to(Y.dtype)
     return w, torch.bmm(Ut, Y), torch.bmm(Ut, HY)
 
@@ -134,22 +145,24 @@ def upper_bound(apply, nk, nG, dtype, device, steps=10, seed=0):
     return (theta + torch.linalg.vector_norm(f, dim=2)[:, 0].double().cpu()).to(device)
 
 
-def chfsi_step(apply, X, degree, b_up, lam, HX=None, filt=None, rr=True):
+def chfsi_step(apply, X, degree, b_up, lam, HX=None, filt=None, rr=True, infos=None):
     """One scaled Chebyshev filter (Zhou-Saad) + Rayleigh-Ritz per k.
     b_up (nk,), lam (nk, nbt) ascending; HX = H X when known.  filt(X, HX,
     abc) runs the filter instead (a captured graph, solver/graphs.py), with
     abc[k] = (largest Ritz value, smallest, upper bound).  Returns (lam, X,
     HX, residual norms); rr=False returns (lam, orthonormalised block, None,
-    None) as solver/chebyshev.py::chfsi_step."""
+    None) as solver/chebyshev.py::chfsi_step.  infos: status tensors of the
+    factorisations and eigensolves, collected instead of read (captured
+    maps, solver/graphs.py::MapGraph)."""
     if HX is None:
         HX = apply(X)
     abc = torch.stack([lam[:, -1].to(torch.float64), lam[:, 0].to(torch.float64),
                        b_up.to(torch.float64)], dim=1)
     Y = filt(X, HX, abc) if filt is not None else filter_recurrence(apply, X, HX, abc, degree)
-    Y = orth(Y)
+    Y = orth(Y, infos=infos)
     if not rr:
         return lam, Y, None, None
-    w, X, HX = rayleigh_ritz(Y, apply(Y))
+    w, X, HX = rayleigh_ritz(Y, apply(Y), infos)
     _, rn = residual_norms(HX, X, w)
     return w, X, HX, rn
 

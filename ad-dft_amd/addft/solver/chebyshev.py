@@ -19,6 +19,7 @@ import torch
 
 from ..kernels.native import cheb_combine
 from .davidson import eigh
+from .linalg import ritz_eigh
 from .linalg import inner, near_identity_inv_sqrt, rcombine, rinner
 
 
@@ -136,7 +137,7 @@ def chfsi_step(apply, X, degree, b_upper, lam, HX=None, rr=True, residuals=True,
     if real:
         T = rinner(Y, HY).to(torch.float64)
         T = 0.5 * (T + T.T)
-        w, U = eigh(T)
+        w, U = ritz_eigh(T)
         X = rcombine(U.T, Y)
     else:
         T = inner(Y, HY, conj_a=True).to(torch.complex128)

@@ -1,4 +1,7 @@
-"""hipGraph capture of the Chebyshev filter over all k-points.
+"""hipGraph capture of the batched Chebyshev maps over all k-points.
+
+MapGraph (below) captures a whole SCF map of the batched Chebyshev form; the
+older FilterGraph captures the filter alone.
 
 The filter p_m(H) X of one ChFSI pass (solver/batched.py::chfsi_step) is a
 fixed sequence of m batched Hpsi calls (Engine.hpsi_batch: one local launch
@@ -115,3 +118,62 @@ class FilterGraph:
 def eager_filter(hpsi, X, HX, abc, vs, degree):
     """The same filter without capture (reference for tests and fallback)."""
     return filter_recurrence(lambda p: hpsi(p, vs), X, HX, abc, degree)
+
+
+class MapGraph:
+    """One SCF map of the batched Chebyshev form -- effective potential,
+    staging, the filter passes with their orthonormalisations and
+    Rayleigh-Ritz steps, occupations and density -- captured as one graph
+    and replayed for every later map of the same precision and shape.
+
+    body(rho, X, lam, W, infos) runs once under capture and returns a dict of
+    output tensors (among them "r", the weighted residual of the last
+    Rayleigh-Ritz step).  The static inputs are refreshed by device copies
+    before each replay:
+
+      rho (nfft,)         input density, plane-major FP64
+      X   (nk, nbt, nG)   the previous map's Ritz vectors
+      lam (nk, nbt)       their Ritz values
+      W   (nk, nbt)       occupation weights of the residual norms
+
+    The status tensors the body collects in infos (Cholesky pivots,
+    eigensolver convergence) are summed inside the graph into one flag,
+    which a replay reads together with the residual in a single host
+    transfer; when it is set the caller redoes the map eagerly.  The outputs
+    live in the graph's pool and are overwritten by the next replay: callers
+    copy what they keep.  held(): device buffers outside the pool that the
+    captured kernels address (kept alive with the graph, as for FilterGraph).
+    """
+
+    def __init__(self, body, rho, X, lam, W, held=None):
+        self.shape, self.dtype = tuple(X.shape), X.dtype
+        self.rho = rho.detach().clone()
+        self.X = X.detach().clone()
+        self.lam = lam.detach().clone()
+        self.W = W.detach().clone()
+        infos = []
+        torch.cuda.synchronize()
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            self.out = body(self.rho, self.X, self.lam, self.W, infos)
+            bad = torch.zeros((), dtype=torch.float64, device=X.device)
+            for info in infos:
+                bad = bad + (info != 0).sum().to(torch.float64)
+            self.check = torch.stack([bad, self.out["r"].to(torch.float64).reshape(())])
+        self.held = [t for t in (held() if held is not None else ()) if t is not None]
+        self.n_status = len(infos)
+        self.replays = 0
+
+    def matches(self, X):
+        return tuple(X.shape) == self.shape and X.dtype == self.dtype
+
+    def __call__(self, rho, X, lam, W):
+        """Replay on these inputs: (ok, residual, outputs)."""
+        self.rho.copy_(rho)
+        self.X.copy_(X)
+        self.lam.copy_(lam)
+        self.W.copy_(W)
+        self.graph.replay()
+        self.replays += 1
+        bad, r = self.check.tolist()
+        return bad == 0.0, r, self.out

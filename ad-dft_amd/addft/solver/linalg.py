@@ -160,7 +160,7 @@ def eigh_method(T):
     from .. import tuning
     n = T.shape[-1]
     b = T.shape[0] if T.dim() == 3 else 1
-    if not tuning.enabled() or (n < 256 and b < 8) or torch.cuda.is_current_stream_capturing():
+    if not tuning.enabled() or (n < 256 and b < 8):
         return "dc"
     # one decision per size class (the Davidson subspace grows in small steps)
     cls = _EIGH_SMALL_MAX if n <= _EIGH_SMALL_MAX else (n + 127) // 128 * 128
@@ -168,6 +168,8 @@ def eigh_method(T):
     made = tuning.decisions().get(key)
     if made is not None and made["choice"] in _EIGH_METHODS_REAL1 + ("lds",):
         return made["choice"]
+    if torch.cuda.is_current_stream_capturing():
+        return "dc"                   # no timing inside a capture
     methods = _EIGH_METHODS
     if n <= _EIGH_SMALL_MAX:
         methods = methods + ("lds",)
@@ -320,6 +322,48 @@ def lowest_split_eigh(T, nb, tol=1e-11, max_cg=200):
     if res > tol * max(scale, 1e-300):
         return None
     return w, U
+
+
+# ritz_eigh: largest off-diagonal entry of T, relative to max |T_ij|, for which
+# the refinement is tried before the dense eigensolve, and the smallest n for
+# which it is tried at all (below it the dense solve is a few launches).
+RITZ_REFINE_OFF = 1e-2
+RITZ_REFINE_MIN_N = 256
+# how many projected matrices took each path (read by the benchmark)
+RITZ_STATS = {"refined": 0, "dense": 0}
+
+
+def ritz_eigh(T, tol=1e-12):
+    """Eigenpairs (w ascending, U with eigenvectors as columns) of a real
+    symmetric projected matrix T = Y^T H Y whose basis Y is the previous
+    map's Ritz vectors after a filter and an orthonormalisation: T is then
+    diagonal up to entries of the order of the change of H between maps and
+    of the previous eigensolve's residuals.  Such a T is diagonalised by
+    _refine_near_diagonal from X = I (a few n^3 GEMMs per step, clusters of
+    close values diagonalised exactly) instead of a dense eigensolve, whose
+    column-by-column tridiagonalisation is launch-bound at n of a few
+    hundred.  The dense eigensolve (davidson.eigh) is taken where T is far
+    from diagonal (largest off-diagonal entry above RITZ_REFINE_OFF |T|),
+    where the refinement fails (a cluster above 128 values, no convergence in
+    its step limit) and where the refined pairs' eigen-residual exceeds
+    tol |T|."""
+    from .davidson import eigh
+    if T.is_complex() or T.dim() != 2 or T.shape[0] < RITZ_REFINE_MIN_N:
+        return eigh(T)
+    d = T.diagonal()
+    scale = float(T.abs().max())
+    off = float((T - torch.diag(d)).abs().max())
+    if scale > 0.0 and off <= RITZ_REFINE_OFF * scale:
+        X = _refine_near_diagonal(T)
+        if X is not None:
+            w = (X * (T @ X)).sum(dim=0)
+            order = torch.argsort(w)
+            w, U = w[order].contiguous(), X[:, order].contiguous()
+            if float((T @ U - U * w[None, :]).abs().max()) <= tol * scale:
+                RITZ_STATS["refined"] += 1
+                return w, U
+    RITZ_STATS["dense"] += 1
+    return eigh(T)
 
 
 def _shifted_cg_step(x, r, p, q, a, dc, rz, res, fnorm):

@@ -36,7 +36,7 @@ from . import batched as kbatch
 from .chebyshev import chfsi_step, upper_bound
 from .davidson import DENSE_MAX_NG, davidson
 from . import graphs as fgraphs
-from .linalg import gram_tall
+from .linalg import _EIGH_SMALL_MAX as EIGH_SMALL_MAX, gram_tall
 
 
 class DIIS:
@@ -191,7 +191,7 @@ def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, 
               switch_tol=1e-5, verbose=False, rho0=None, polish=4, dav_max_iter=None, log=None,
               nbuffer=None, eigensolver="auto", cheb_degree=None, refine=True, reset_on_switch=True,
               refine_max_sub=None, cheb_passes=None, batch_k=True, cold_max_sub=None, dav_factor=0.1,
-              refine_dav_factor=None, cold_max_iter=4, polish_gain_max=None, psi0=None, graphs=True,
+              refine_dav_factor=None, cold_max_iter=4, polish_gain_max=None, psi0=None, graphs=False,
               cheb_rr_all=True, refine_max_iter=2, mix_history=8, pulay_gg0=None, refine_nl_bf16=True,
               refine_first_rr=False, rho_init="uniform", rho_sigma=1.2, gamma_real=True):
     """Run the SCF on engine's current geometry.  Returns (SCFState with
@@ -230,8 +230,14 @@ def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, 
     Warm start: rho0 (renormalised to the electron count) and psi0 (per k,
       stick order; missing rows filled with plane waves) replace the cold map.
     Batching: batch_k (all k-points as one block on the shared sphere) and
-      graphs (replay the batched Chebyshev filter as a captured graph,
-      solver/graphs.py).
+      graphs (off by default): each batched Chebyshev map -- potential,
+      filter, orthonormalisation, Rayleigh-Ritz, occupations, density -- as
+      one captured graph replayed for the later maps of its precision
+      (solver/graphs.py::MapGraph; where its conditions do not hold, the
+      filter alone).  Replays equal the eager maps, but the capture and
+      instantiation each SCF pays exceed the launch cost they save on these
+      device-bound maps (8-atom cell, 4x4x4 mesh, mixed form: 0.287 s per
+      geometry against 0.277 s eager).
     gamma_real (default on; applies at k = 0 without psi0 on spheres above the
       dense-solver size): keep the eigenvectors real in real space (cos/sin
       start, real projected matrices, projection of each correction block)
@@ -309,12 +315,27 @@ def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, 
     if cheb_degree is None:
         cheb_degree = 8 if batched else 14
     if cheb_passes is None:
-        cheb_passes = 3 if batched else 1
+        cheb_passes = 1
     ritz_b = [None]
 
-    # the batched Chebyshev filter as one captured graph per (precision, shape)
+    def cheb_for(fp32):
+        """Chebyshev filtering for the maps of this precision: where the block
+        is large (use_cheb), and for the FP32 maps of k-point batches of any
+        block size -- there one degree-8 filter per map costs less than the
+        Davidson steps (8-atom cell, 4x4x4 mesh, mixed form: 0.42 -> 0.28 s
+        per geometry); the FP64 maps of small blocks keep Davidson, which
+        reaches the tight final tolerances in fewer products."""
+        return use_cheb or (batched and fp32 and eigensolver == "auto")
+
+    # the batched Chebyshev maps as one captured graph per (precision, shape)
+    # where the block's small factorisations and eigensolves are the
+    # library's one-launch LDS kernels (n <= 64) and the occupations are
+    # smeared (the integer fill reads eigenvalues on the host); otherwise the
+    # filter alone is captured
     use_graphs = batched and graphs and fgraphs.enabled()
+    use_map_graphs = use_graphs and nbt <= EIGH_SMALL_MAX and st.sigma >= 1e-6
     graph_cache = {}
+    map_graphs = {}
 
     def filter_graph(Xb, vs, fp32):
         tag = "fp32" if fp32 else "fp64"
@@ -345,14 +366,14 @@ def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, 
                                               tol=tol_k,
                                               nconv=nb, weight=W, apply_lo=apply_lo,
                                               max_sub=(refine_max_sub or 2 * nbt))
-        elif use_cheb and not cold and ritz_b[0] is not None:
+        elif cheb_for(fp32) and not cold and ritz_b[0] is not None:
             key = ("batch", fp32)
             if key not in bounds:
                 bounds[key] = 1.1 * kbatch.upper_bound(apply, engine.nk, engine.ng, Xb.dtype, engine.dev)
             lam = ritz_b[0]
             nit, r = 0, float("inf")
             HXb = None
-            filt = filter_graph(Xb, vs, fp32) if use_graphs else None
+            filt = filter_graph(Xb, vs, fp32) if (use_graphs and not use_map_graphs) else None
             npass = 41 if tol_k <= 1e-9 else cheb_passes + 1
             for nit in range(1, npass):
                 rr = cheb_rr_all or npass > cheb_passes + 1 or nit == 1 or nit == npass - 1
@@ -366,7 +387,8 @@ def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, 
         else:
             lam, Xb, r, nit = kbatch.davidson(apply, Xb, nbt, diag,
                                               max_iter=(dav_max_iter or (cold_max_iter if cold else 25)),
-                                              tol=(max(tol_k, 1e-3) if (cold and use_cheb) else tol_k), nconv=nb,
+                                              tol=(max(tol_k, 1e-3) if (cold and cheb_for(fp32)) else tol_k),
+                                              nconv=nb,
                                               weight=W)
         dav_iters += nit
         ritz_b[0] = lam
@@ -456,7 +478,76 @@ def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, 
             w[:nb] = torch.clamp(f / torch.clamp(f.max(), min=1e-300), min=W_MIN)
         return w
 
+    def cheb_map_body(fp32):
+        """One batched Chebyshev map as a function of its inputs, free of
+        host synchronisation (captured by graphs.MapGraph): the same passes
+        as solve_batched's Chebyshev branch, all of them (no early exit)."""
+        key = ("batch", fp32)
+
+        def body(rho, Xb, lam, W, infos):
+            v, _ = engine.effective_potential(rho)
+            vs = engine.stage(v, mixed_stage=fp32)
+
+            def apply(p):
+                return engine.hpsi_batch(p, vs, fp32=fp32)
+            HXb, rn = None, None
+            for nit in range(1, cheb_passes + 1):
+                rr = cheb_rr_all or nit == 1 or nit == cheb_passes
+                lam, Xb, HXb, rn_i = kbatch.chfsi_step(apply, Xb, cheb_degree, bounds[key], lam, HX=HXb, rr=rr,
+                                                       infos=infos)
+                if rr:
+                    rn = rn_i
+            eig = lam[:, :nb].to(torch.float64).contiguous()
+            occ, mu = engine.occupations(eig)
+            rho_out = engine.density_batch(Xb[:, :nb], occ, fp32=fp32)
+            return {"rho_out": rho_out, "X": Xb, "lam": lam, "eig": eig, "occ": occ, "mu": mu,
+                    "r": (rn[:, :nb] * W[:, :nb]).max()}
+        return body
+
+    def graph_map(rho, X, fp32, dav_tol, cold):
+        """one_map through a captured graph, or None where the map is not a
+        plain batched Chebyshev map (first map, FP64 refinement maps of the
+        mixed form, the open-ended tight passes near convergence, before one
+        eager map of this precision has set the spectral bounds and the
+        tuning decisions) or where a factorisation or eigensolve of the
+        replay failed (the caller then runs the map eagerly)."""
+        nonlocal dav_iters
+        if not (use_map_graphs and cheb_for(fp32) and not cold and ritz_b[0] is not None and dav_tol > 1e-9
+                and ("batch", fp32) in bounds and not (engine.mixed and not fp32 and refine)):
+            return None
+        Xb = torch.stack(list(X))
+        W = torch.stack([weights(ik) for ik in range(engine.nk)])
+        tag = "fp32" if fp32 else "fp64"
+        mkey = (fp32, tuple(Xb.shape), Xb.dtype)
+        mg = map_graphs.get(mkey)
+        if mg is None:
+            before = engine.hpsi_bands[tag]
+            mg = fgraphs.MapGraph(cheb_map_body(fp32), rho, Xb, ritz_b[0], W,
+                                  held=lambda: engine.graph_buffers() + list(engine._work.values())
+                                  + [bounds[("batch", fp32)]])
+            mg.bands = engine.hpsi_bands[tag] - before
+            engine.hpsi_bands[tag] = before
+            map_graphs[mkey] = mg
+        ok, r, out = mg(rho, Xb, ritz_b[0], W)
+        if not ok:
+            return None
+        engine.hpsi_bands[tag] += mg.bands
+        dav_iters += cheb_passes
+        last["dav_res"], last["dav_it"] = r, cheb_passes
+        lam = out["lam"].clone()
+        Xb = out["X"].clone()
+        eig, occ, mu = out["eig"].clone(), out["occ"].clone(), out["mu"].clone()
+        ritz_b[0] = lam
+        for ik in range(engine.nk):
+            occ_prev[ik] = occ[ik].to(torch.float64)
+            ritz[ik] = lam[ik]
+        return out["rho_out"].clone(), list(Xb.unbind(0)), eig, occ, mu
+
     def one_map(rho, X, fp32, dav_tol, cold):
+        if batched:
+            got = graph_map(rho, X, fp32, dav_tol, cold)
+            if got is not None:
+                return got
         v, _ = engine.effective_potential(rho)
         vs = engine.stage(v, mixed_stage=fp32)
         vs_lo = engine.stage(v, mixed_stage=True) if (engine.mixed and not fp32 and refine) else None
@@ -598,6 +689,7 @@ def solve_scf(engine, tol=1e-10, max_iter=80, mixer="auto", alpha=0.7, gg0=0.5, 
     engine.scf_buffer = (float(eig.sum()), buffer, bool(real))
     info = {"iterations": it, "residual_history": hist, "davidson_iterations": dav_iters,
             "fp64_from_iteration": info_switch, "map_end_seconds": tstamps, "map_gain": gain,
+            "map_graph_replays": sum(g.replays for g in map_graphs.values()),
             "polish_steps": n_polish,
             "seconds": time.perf_counter() - t0}
     state = SCFState(psi=X, eig=eig, occ=occ, mu=float(mu), rho_r=rho, n_iter=it, residual=res)

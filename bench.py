@@ -672,6 +672,8 @@ def main():
         say(f"mixed warm-up step {i + 1}/{args.warmup}")
         one_geometry(addft, sys_m, args.mixed_tol, torch)
     times, last = [], None
+    from addft.solver import linalg as _linalg
+    ritz0 = dict(_linalg.RITZ_STATS)
     torch.cuda.synchronize()
     mark(torch)
     for i in range(args.steps):
@@ -707,7 +709,9 @@ def main():
                    "mixed_tol": args.mixed_tol, "fp64_tol": args.fp64_tol},
         "mixed": {"s_per_geometry": round(sec, 3), "scf_iterations": info["iterations"],
                   "fp64_from_iteration": info.get("fp64_from_iteration"),
-                  "hpsi_band_applications": bands, "F": F},
+                  "hpsi_band_applications": bands, "F": F,
+                  "chebyshev_rr_per_geometry": {k: (_linalg.RITZ_STATS[k] - ritz0[k]) / max(1, args.steps)
+                                                for k in ritz0}},
         "roofline": {"bound": "hbm", "achieved": round(f32["hpsi_GBps"], 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(f32["hpsi_GBps"] / HBM_PEAK_GBS, 4),
                      "traffic": traffic,

@@ -120,3 +120,57 @@ def test_lowest_split_eigh_refuses_overlapping_spectra():
     T = _davidson_like(40, 30, 4)
     T[:40, :40] += 0.05 * torch.ones(40, 40, dtype=T.dtype)
     assert lowest_split_eigh(T, 40) is None
+
+
+def _ritz_like(n, seed, off):
+    """A projected matrix in the previous Ritz basis: ascending diagonal with
+    the spacing of a large cell (a threefold multiplet, a nearly degenerate
+    pair, a cluster of eight values 1e-7 apart) plus symmetric off-diagonal
+    entries of size off."""
+    rng = np.random.default_rng(seed)
+    a = np.sort(rng.uniform(-0.4, 0.6, n))
+    a[50:53] = a[50]
+    a[80] = a[79] + 1e-10
+    a[120:128] = a[120] + 1e-7 * np.arange(8)
+    E = rng.standard_normal((n, n))
+    return torch.as_tensor(np.diag(a) + off * (E + E.T) / 2)
+
+
+@pytest.mark.parametrize("off", [1e-9, 1e-6, 1e-5])
+def test_ritz_eigh_refines_near_diagonal(off, monkeypatch):
+    """Chebyshev Rayleigh-Ritz matrices close to diagonal take the refinement
+    (no dense eigensolve): eigenvalues equal to LAPACK's to 1e-13, eigen-
+    residual and orthonormality at the rounding level, and the invariant
+    subspaces of the clustered values equal to LAPACK's."""
+    from addft.solver import davidson, linalg
+
+    def refuse(T):
+        raise AssertionError("dense eigensolve taken")
+    monkeypatch.setattr(davidson, "eigh", refuse)
+    T = _ritz_like(300, 0, off)
+    before = linalg.RITZ_STATS["refined"]
+    w, U = linalg.ritz_eigh(T)
+    assert linalg.RITZ_STATS["refined"] == before + 1
+    w0, U0 = torch.linalg.eigh(T)
+    assert (w - w0).abs().max().item() < 1e-13
+    assert (T @ U - U * w[None, :]).abs().max().item() < 1e-12 * T.abs().max().item()
+    assert (U.T @ U - torch.eye(300, dtype=U.dtype)).abs().max().item() < 1e-13
+    for lo, hi in ((50, 53), (79, 81), (120, 128)):
+        s = torch.linalg.svdvals(U0[:, lo:hi].T @ U[:, lo:hi])
+        assert (1.0 - s.min()).item() < 1e-10
+
+
+def test_ritz_eigh_far_from_diagonal_takes_dense(monkeypatch):
+    """A projected matrix with large off-diagonal entries goes to the dense
+    eigensolver unchanged."""
+    from addft.solver import davidson, linalg
+    calls = []
+
+    def dense(T):
+        calls.append(T.shape)
+        return torch.linalg.eigh(T)
+    monkeypatch.setattr(davidson, "eigh", dense)
+    T = _ritz_like(300, 1, 0.05)
+    w, U = linalg.ritz_eigh(T)
+    assert calls == [(300, 300)]
+    assert (w - torch.linalg.eigvalsh(T)).abs().max().item() < 1e-13
