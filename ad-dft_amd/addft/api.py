@@ -65,18 +65,45 @@ def _engine(static: Static, geo: Cell) -> Engine:
 def setup(lattice, frac, psp: Pseudo | None = None, ecut: float = 8.0, nband: int = 4,
           kmesh: KMesh | None = None, sigma: float = 0.05, density_factor: float = 4.0,
           eta_ewald: float = 0.9, spec_idx=None, precision: str = "fp64", shells: str = "union",
-          xc: str = "lda") -> System:
+          xc: str = "lda", symmetry: bool = False) -> System:
     """api.py:35-45.  shells="per_k": each k-point on its own sphere
     |k + G|^2 <= 2 ecut (unequal plane-wave counts, padded to the union
     shell in storage); "union" (default): the reference's union shell.
     xc="gga_model": the reference's model GGA (physics/xc.py:137-150) with
-    its potential from the spectral gradient and divergence."""
+    its potential from the spectral gradient and divergence.
+    symmetry=True: the k mesh reduced by the crystal's space group and time
+    reversal (geometry.symmetry; the reference has no symmetry), densities,
+    forces and stress symmetrised; the geometry must keep the symmetry
+    (delta_q_grad, whose perturbation breaks it, refuses such a system)."""
     geo = make_cell(lattice, frac)
     psp = synthetic() if psp is None else psp
-    static = make_static(geo, psp, kmesh or gamma(), ecut, nband, spec_idx=spec_idx, sigma=sigma,
+    kmesh = kmesh or gamma()
+    group = None
+    if symmetry:
+        if shells != "union":
+            raise ValueError("symmetry=True needs the union shell")
+        kmesh, group = _reduce_by_symmetry(geo, psp, kmesh, ecut, density_factor, spec_idx)
+    static = make_static(geo, psp, kmesh, ecut, nband, spec_idx=spec_idx, sigma=sigma,
                          density_factor=density_factor, eta_ewald=eta_ewald, precision=precision,
-                         shells=shells, xc=xc)
+                         shells=shells, xc=xc, symmetry=group)
     return System(geo=geo, static=static)
+
+
+def _reduce_by_symmetry(geo, psp, kmesh, ecut, density_factor, spec_idx):
+    """(irreducible mesh, group) with the operations that map the FFT box
+    of the resulting basis onto itself (the box depends on the closed shell,
+    which depends on the operations kept: repeated until it settles)."""
+    from .geometry import symmetry as sym
+    group = sym.find_space_group(geo.lattice, geo.frac, spec_idx)
+    for _ in range(4):
+        red = sym.reduce_mesh(kmesh, group, geo.lattice)
+        box = make_static(geo, psp, red, ecut, 1, spec_idx=spec_idx,
+                          density_factor=density_factor).basis.fft_shape
+        ok = sym.grid_compatible(group, box)
+        if ok.all():
+            return red, group
+        group = group.subset(ok)
+    raise RuntimeError("symmetry reduction did not settle on an FFT box")
 
 
 def setup_from_case(case_dir, psp=None, precision: str = "fp64") -> System:
@@ -152,6 +179,10 @@ def _gradient(s: _Solved, geo: Cell):
     eng = s.engine
     ep = eng.energy_parts(s.psi, s.occ, s.rho)
     dR, de, parts = eng.backward(s.psi, s.occ, ep["rho_psi"])
+    if eng.st.symmetry is not None:
+        from .geometry.symmetry import symmetrize_tensor, symmetrize_vectors
+        dR = symmetrize_vectors(dR, eng.st.symmetry, geo.lattice)
+        de = symmetrize_tensor(de, eng.st.symmetry, geo.lattice)
     A = np.asarray(geo.lattice)
     grad = Gradient(dR=dR, frac=dR @ A.T, lattice=np.linalg.solve(A.T, de), eps=de)
     return ep["F"], grad, parts
@@ -239,6 +270,9 @@ def delta_q_grad(system: System, state: SCFState | None = None, weight=None, rto
     precondition: right-precondition the adjoint BiCGSTAB with a Kerker
     filter (same solution, fewer response products)."""
     from .response import DeltaQGradient, delta_q_grad as _dq
+    if system.static.symmetry is not None:
+        raise ValueError("delta_q_grad: the response to a general perturbation needs the full k mesh "
+                         "(set up the system without symmetry=True)")
     s = _solved(system, state, scf_kwargs)
     eng = s.engine
     w_pl = eng.to_planar(torch.as_tensor(np.asarray(weight, dtype=np.float64)).to(eng.dev))

@@ -28,6 +28,9 @@ class EnvelopeEnergy(torch.autograd.Function):
     def forward(ctx, frac, eps, static, lattice, psi, occ):
         from ..api import _engine
         from ..geometry.cell import make_cell
+        if static.symmetry is not None:
+            raise ValueError("EnvelopeEnergy: displaced geometries break the symmetry that reduced the k mesh "
+                             "(set up the system without symmetry=True)")
         e = eps.detach().cpu().numpy().astype(np.float64)
         A = np.asarray(lattice, dtype=np.float64) @ (np.eye(3) + e)
         geo = make_cell(A, frac.detach().cpu().numpy())

@@ -140,6 +140,11 @@ class Engine:
             self.npw = km.sum(axis=1).astype(np.int64)
         else:
             self.npw = np.full(self.nk, self.ng, dtype=np.int64)
+        # space-group density symmetrisation: one gather map per operation
+        self.sym_maps = None
+        if static.symmetry is not None and static.symmetry.nops > 1:
+            from .geometry.symmetry import grid_maps
+            self.sym_maps = torch.as_tensor(grid_maps(static.symmetry, self.shape), device=self.dev)
         self._prep_psp()
         self._work = {}
         self.geo = None
@@ -529,6 +534,17 @@ class Engine:
         torch.baddbmm(out, self.dmul(becp_t), P, out=out)
         return self.restrict_all(out)
 
+    def symmetrize_density(self, rho):
+        """(1/|G|) sum_g rho(W n + N t) on the plane-major grid: the density
+        of the full k mesh from that of its irreducible points (identity
+        where the system has no symmetry reduction)."""
+        if self.sym_maps is None:
+            return rho
+        out = torch.zeros_like(rho)
+        for c in range(0, self.sym_maps.shape[0], 16):
+            out += rho[self.sym_maps[c:c + 16]].sum(dim=0)
+        return out / self.sym_maps.shape[0]
+
     def density_batch(self, psi3, occ, fp32=False):
         """rho for psi3 (nk, nb, nG) and occ (nk, nb) in one launch."""
         plan = self.plan32 if fp32 else self.plan64
@@ -539,7 +555,7 @@ class Engine:
         psi = psi.to(plan.cdtype) if psi.dtype != plan.cdtype else psi
         if fp32:
             w = w / self._norm2_fp32(psi)
-        return plan.density(psi.contiguous(), w)
+        return self.symmetrize_density(plan.density(psi.contiguous(), w))
 
     @staticmethod
     def _norm2_fp32(psi):
@@ -563,7 +579,7 @@ class Engine:
             if fp32:
                 w = w / self._norm2_fp32(psi)
             rho = plan.density(psi, w.contiguous(), out=rho, accumulate=ik > 0)
-        return rho
+        return self.symmetrize_density(rho)
 
     def density_half(self, h, occ, fp32=False):
         """density() for one k = 0 block of half rows (HalfRows) real in
@@ -577,7 +593,8 @@ class Engine:
             w = w / self._norm2_fp32(h)
         a = h * torch.sqrt(torch.clamp(w, min=0.0)).to(h.real.dtype)[:, None]
         pk = native.gamma_pack2_half(a.contiguous(), self.gplus_d, self.gminus_d, self.ng)
-        return plan.density(pk, torch.ones(pk.shape[0], dtype=torch.float64, device=h.device))
+        return self.symmetrize_density(plan.density(pk, torch.ones(pk.shape[0], dtype=torch.float64,
+                                                                   device=h.device)))
 
     def occupations(self, eig):
         """(f, mu) for eig (nk, nb) float64 (occupation.py:25-49)."""

@@ -40,6 +40,10 @@ class Static(NamedTuple):
     # exchange-correlation functional: "lda" (PZ81, xc.py:51-73) or
     # "gga_model" (n eps_LDA (1 + 0.1234 s^2), xc.py:137-150)
     xc: str = "lda"
+    # space group (geometry.symmetry.SpaceGroup) whose operations reduced the
+    # k mesh and symmetrise densities and gradients; None: no reduction
+    # beyond what the mesh itself holds (the reference rule)
+    symmetry: object = None
 
 
 class SCFState(NamedTuple):
@@ -54,7 +58,7 @@ class SCFState(NamedTuple):
 
 def make_static(cell: Cell, psp: Pseudo, kmesh: KMesh, ecut: float, nband: int, spec_idx=None,
                 sigma: float = 0.01, density_factor: float = 4.0, eta_ewald: float = 0.9,
-                precision: str = "fp64", shells: str = "union", xc: str = "lda") -> Static:
+                precision: str = "fp64", shells: str = "union", xc: str = "lda", symmetry=None) -> Static:
     """system.py:47-69.  shells="per_k" restricts each k-point to its own
     sphere |k + G|^2 <= 2 ecut inside the union shell (frozen with the basis:
     strained geometries keep the setup membership)."""
@@ -81,7 +85,8 @@ def make_static(cell: Cell, psp: Pseudo, kmesh: KMesh, ecut: float, nband: int, 
         raise ValueError("a per-k shell holds fewer plane waves than nband")
     return Static(basis=basis, kpts=kpts, kweights=np.asarray(kmesh.weights, dtype=np.float64), psp=psp,
                   spec_idx=spec_idx, nband=int(nband), spin_degeneracy=2.0, sigma=float(sigma),
-                  nelec=nelec, eta_ewald=float(eta_ewald), precision=precision, kshell=kshell, xc=xc)
+                  nelec=nelec, eta_ewald=float(eta_ewald), precision=precision, kshell=kshell, xc=xc,
+                  symmetry=symmetry)
 
 
 def per_k_shells(lattice, miller, kpts, ecut) -> np.ndarray:
