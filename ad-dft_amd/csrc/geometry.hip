@@ -75,11 +75,11 @@ __device__ __forceinline__ void sph_j(int l, double x, double* j, double* jp) {
   }
 }
 
-// beta~(q), d beta~/dq for every radial projector.  kind 1: f[ib][i] =
-// w_i rab_i betar_ib(r_i) r_i (Simpson weights folded on the host).
-__global__ void k_beta_table(int kind, int nrad, const int32_t* __restrict__ lval, const double* __restrict__ pw,
-                             const double* __restrict__ pb, int64_t mesh, const double* __restrict__ r,
-                             const double* __restrict__ f, const double* __restrict__ qn, int64_t nq,
+// beta~(q), d beta~/dq for every radial projector of analytic (kind 0,
+// Gaussian) radial forms; the tabulated forms (kind 1) are
+// k_beta_table_mesh below.
+__global__ void k_beta_table(int nrad, const int32_t* __restrict__ lval, const double* __restrict__ pw,
+                             const double* __restrict__ pb, const double* __restrict__ qn, int64_t nq,
                              double* __restrict__ beta, double* __restrict__ dbeta) {
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < nq * nrad;
        t += (int64_t)gridDim.x * blockDim.x) {
@@ -88,28 +88,51 @@ __global__ void k_beta_table(int kind, int nrad, const int32_t* __restrict__ lva
     const double q = qn[p];
     const int l = lval[ib];
     double b = 0.0, db = 0.0;
-    if (kind == 0) {
-      const double qq = fmax(q, 1e-12);
-      const double e = exp(-0.5 * (qq * pb[ib]) * (qq * pb[ib]));
-      const double ql = l == 0 ? 1.0 : (l == 1 ? qq : qq * qq);
-      b = pw[ib] * ql * e;
-      if (q > 1e-12) {
-        const double lq = l == 0 ? 0.0 : (l == 1 ? 1.0 : 2.0 * qq);
-        db = pw[ib] * (lq - ql * qq * pb[ib] * pb[ib]) * e;
-      }
-    } else {
-      const double* fi = f + ib * mesh;
-      for (int64_t i = 0; i < mesh; ++i) {
-        double j, jp;
-        sph_j(l, q * r[i], &j, &jp);
-        b += fi[i] * j;
-        db += fi[i] * r[i] * jp;
-      }
-      b *= kFourPi;
-      db *= kFourPi;
+    const double qq = fmax(q, 1e-12);
+    const double e = exp(-0.5 * (qq * pb[ib]) * (qq * pb[ib]));
+    const double ql = l == 0 ? 1.0 : (l == 1 ? qq : qq * qq);
+    b = pw[ib] * ql * e;
+    if (q > 1e-12) {
+      const double lq = l == 0 ? 0.0 : (l == 1 ? 1.0 : 2.0 * qq);
+      db = pw[ib] * (lq - ql * qq * pb[ib] * pb[ib]) * e;
     }
     beta[ib * nq + p] = b;
     dbeta[ib * nq + p] = db;
+  }
+}
+
+// kind 1, f[ib][i] = w_i rab_i betar_ib(r_i) r_i (Simpson weights folded on
+// the host), with one wavefront per (radial projector, q): the 64 lanes split
+// the radial mesh and the partial sums meet in a fixed butterfly, so a
+// table of a few thousand q (one k-point of a small cell) still spreads
+// over the chip instead of running one mesh loop per thread.
+__global__ void __launch_bounds__(256) k_beta_table_mesh(int nrad, const int32_t* __restrict__ lval, int64_t mesh,
+                                                         const double* __restrict__ r, const double* __restrict__ f,
+                                                         const double* __restrict__ qn, int64_t nq,
+                                                         double* __restrict__ beta, double* __restrict__ dbeta) {
+  const int lane = threadIdx.x & 63;
+  const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
+  for (int64_t t = blockIdx.x * (int64_t)(blockDim.x >> 6) + (threadIdx.x >> 6); t < nq * nrad; t += waves) {
+    const int ib = (int)(t / nq);
+    const int64_t p = t - ib * nq;
+    const double q = qn[p];
+    const int l = lval[ib];
+    const double* fi = f + ib * mesh;
+    double b = 0.0, db = 0.0;
+    for (int64_t i = lane; i < mesh; i += 64) {
+      double j, jp;
+      sph_j(l, q * r[i], &j, &jp);
+      b += fi[i] * j;
+      db += fi[i] * r[i] * jp;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      b += __shfl_xor(b, off, 64);
+      db += __shfl_xor(db, off, 64);
+    }
+    if (lane == 0) {
+      beta[ib * nq + p] = b * kFourPi;
+      dbeta[ib * nq + p] = db * kFourPi;
+    }
   }
 }
 
@@ -408,8 +431,12 @@ int addft_beta_table(int kind, int nrad, const int32_t* lval, const double* pw, 
                      void* stream) {
   ADDFT_CHECK_ARG(lval && qn && beta && dbeta && nrad > 0 && nq > 0, "bad argument");
   ADDFT_CHECK_ARG(kind == 0 ? (pw && pb) : (r && f && mesh > 0), "missing radial data");
-  hipLaunchKernelGGL(k_beta_table, dim3(nblk(nq * nrad)), dim3(256), 0, addft::as_stream(stream), kind, nrad, lval,
-                     pw, pb, mesh, r, f, qn, nq, beta, dbeta);
+  if (kind == 0)
+    hipLaunchKernelGGL(k_beta_table, dim3(nblk(nq * nrad)), dim3(256), 0, addft::as_stream(stream), nrad, lval, pw,
+                       pb, qn, nq, beta, dbeta);
+  else  // four (projector, q) pairs per 256-thread block
+    hipLaunchKernelGGL(k_beta_table_mesh, dim3(nblk(nq * nrad, 4)), dim3(256), 0, addft::as_stream(stream), nrad,
+                       lval, mesh, r, f, qn, nq, beta, dbeta);
   ADDFT_LAUNCH_CHECK();
   return ADDFT_OK;
 }
