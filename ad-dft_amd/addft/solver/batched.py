@@ -13,6 +13,8 @@ the batch rectangular -- and give the same converged eigenpairs.
 
 from __future__ import annotations
 
+import os
+
 import torch
 
 from .graphs import filter_recurrence
@@ -61,13 +63,41 @@ def chol_inverse(S, infos=None):
     return torch.linalg.solve_triangular(L, eye, upper=False)
 
 
+def _real_gram(A, B):
+    """A B^H for complex64 (nk, m, K), (nk, n, K) as one real GEMM over the
+    interleaved (re, im) views: with a = (ar, ai), b = (br, bi) per entry,
+    Re a conj(b) = ar br + ai bi and Im = ai br - ar bi, so A's view against
+    the rows of B's view and of B's view with each pair turned to (-bi, br)
+    gives [Re | Im] of the product.  rocBLAS runs the complex64 GEMM of these
+    thin blocks (tens of rows over a few thousand G per k-point) on mostly
+    empty tiles (the same real form as the batched <beta|psi> of
+    Engine.hpsi_batch; ADDFT_KBATCH_REAL_BECP=0 keeps the complex GEMMs)."""
+    nk, m, K = A.shape
+    n = B.shape[1]
+    ar = torch.view_as_real(A.contiguous()).reshape(nk, m, 2 * K)
+    br = torch.view_as_real(B.contiguous())
+    bs = torch.stack([-br[..., 1], br[..., 0]], dim=-1)
+    rows = torch.cat([br, bs], dim=1).reshape(nk, 2 * n, 2 * K)
+    out = torch.bmm(ar, rows.transpose(1, 2))
+    return torch.complex(out[:, :, :n], out[:, :, n:])
+
+
+def _real_form(A):
+    return (A.dtype == torch.complex64 and A.dim() == 3
+            and os.environ.get("ADDFT_KBATCH_REAL_BECP", "1") != "0")
+
+
 def _gram(A, B):
     """A B^H per k: (nk, m, n)."""
+    if _real_form(A):
+        return _real_gram(A, B)
     return torch.bmm(A, B.conj().transpose(1, 2))
 
 
 def _proj(A, B):
     """conj(A) B^T per k: entries <A_i|B_j>."""
+    if _real_form(A):
+        return _real_gram(A, B).conj()
     return torch.bmm(A.conj(), B.transpose(1, 2))
 
 
